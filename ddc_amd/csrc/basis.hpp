@@ -1,0 +1,264 @@
+// B-spline basis kernels shared by the host setup (matrix assembly) and the
+// device evaluators.  Every routine is a restatement of the reference's
+// UniformBSplines / NonUniformBSplines Impl methods and keeps their operation
+// order, so that cell indices are bit-identical and values match to rounding.
+// The whole library is compiled with -ffp-contract=off: no multiply-add is
+// fused, exactly as in a plain FP64 CPU build of the reference.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace ddcx {
+
+constexpr int kMaxDegree = 7;
+
+// Value-type description of a B-spline basis, passed by value to kernels.
+struct BSplinesView {
+    int degree;
+    int uniform;
+    int periodic;
+    int ncells;
+    // uniform knots: coordinate(k) = origin + k * step
+    // (UniformPointSampling::init_ghosted, uniform_point_sampling.hpp:163-192)
+    double origin;
+    double step;
+    double inv_step; // 1.0 / step, bsplines_uniform.hpp:357-360
+    double rmin;     // coordinate(break_point_domain.front())
+    double rmax;     // coordinate(break_point_domain.back())
+    double length;   // rmax - rmin
+    const double* knots; // non-uniform knots (ncells + 1 + 2*degree), device or host memory
+};
+
+__host__ __device__ inline double knot_coord(BSplinesView const& b, int k)
+{
+    if (b.uniform) return b.origin + static_cast<double>(k) * b.step;
+    return b.knots[k];
+}
+
+// UniformBSplines::Impl::get_icell_and_offset, bsplines_uniform.hpp:568-595
+__host__ __device__ inline int uniform_cell(BSplinesView const& b, double x, double& offset)
+{
+    int icell;
+    if (x <= b.rmin) {
+        icell = 0;
+        offset = 0.0;
+    } else if (x >= b.rmax) {
+        icell = b.ncells - 1;
+        offset = 1.0;
+    } else {
+        offset = (x - b.rmin) * b.inv_step;
+        icell = static_cast<int>(offset);
+        offset = offset - icell;
+        if (icell == b.ncells && offset == 0.0) {
+            icell = b.ncells - 1;
+            offset = 1.0;
+        }
+    }
+    return icell;
+}
+
+// NonUniformBSplines::Impl::find_cell_start, bsplines_non_uniform.hpp:668-695.
+// Returns the knot index of the cell start (break point domain starts at
+// knot index `degree`).
+__host__ __device__ inline int nonuniform_cell(BSplinesView const& b, const double* knots, double x)
+{
+    int const front = b.degree;
+    int const back = b.degree + b.ncells;
+    if (x <= knots[front]) return front;
+    if (x >= knots[back]) return back - 1;
+    int low = front;
+    int high = back;
+    int icell = low + (high - low) / 2;
+    while (x < knots[icell] || x >= knots[icell + 1]) {
+        if (x < knots[icell]) high = icell;
+        else low = icell;
+        icell = low + (high - low) / 2;
+    }
+    return icell;
+}
+
+// Cox-de Boor on the unit grid, UniformBSplines eval_basis,
+// bsplines_uniform.hpp:394-426.  NV values (NV = degree+1; integrals() uses
+// degree+2).
+template <int NV>
+__host__ __device__ inline void uniform_basis_from_offset(double offset, double* values)
+{
+    values[0] = 1.0;
+#pragma unroll
+    for (int j = 1; j < NV; ++j) {
+        double xx = -offset;
+        double saved = 0.0;
+#pragma unroll
+        for (int r = 0; r < j; ++r) {
+            xx += 1;
+            double const temp = values[r] / j;
+            values[r] = saved + xx * temp;
+            saved = (j - xx) * temp;
+        }
+        values[j] = saved;
+    }
+}
+
+// de Boor with left/right arrays, NonUniformBSplines eval_basis,
+// bsplines_non_uniform.hpp:461-497; `icell` is the knot index from
+// nonuniform_cell.
+template <int D>
+__host__ __device__ inline void nonuniform_basis(const double* knots, int icell, double x, double* values)
+{
+    double left[D > 0 ? D : 1];
+    double right[D > 0 ? D : 1];
+    values[0] = 1.0;
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+        left[j] = x - knots[icell - j];
+        right[j] = knots[icell + j + 1] - x;
+        double saved = 0.0;
+#pragma unroll
+        for (int r = 0; r < j + 1; ++r) {
+            double const temp = values[r] / (right[r] + left[j - r]);
+            values[r] = saved + right[r] * temp;
+            saved = left[j - r] * temp;
+        }
+        values[j + 1] = saved;
+    }
+}
+
+// eval_basis for either basis kind; returns jmin (first B-spline index).
+template <int D>
+__host__ __device__ inline int eval_basis(BSplinesView const& b, const double* knots, double x, double* values)
+{
+    if (b.uniform) {
+        double offset;
+        int const jmin = uniform_cell(b, x, offset);
+        uniform_basis_from_offset<D + 1>(offset, values);
+        return jmin;
+    }
+    int const icell = nonuniform_cell(b, knots, x);
+    nonuniform_basis<D>(knots, icell, x, values);
+    return icell - D;
+}
+
+// eval_basis_and_n_derivs (NURBS book A2.3), uniform bsplines_uniform.hpp:475-564
+// and non-uniform bsplines_non_uniform.hpp:561-664.  derivs is (D+1) x (N+1)
+// row-major.  Returns jmin.
+template <int D, int N>
+__host__ __device__ inline int eval_basis_and_n_derivs(BSplinesView const& b, const double* knots, double x,
+                                                       double* derivs)
+{
+    double ndu[D + 1][D + 1];
+    double a[D + 1][2];
+    int jmin;
+    bool const uni = b.uniform;
+    if (uni) {
+        double offset;
+        jmin = uniform_cell(b, x, offset);
+        ndu[0][0] = 1.0;
+#pragma unroll
+        for (int j = 1; j < D + 1; ++j) {
+            double xx = -offset;
+            double saved = 0.0;
+#pragma unroll
+            for (int r = 0; r < j; ++r) {
+                xx += 1.0;
+                double const temp = ndu[j - 1][r] / j;
+                ndu[j][r] = saved + xx * temp;
+                saved = (j - xx) * temp;
+            }
+            ndu[j][j] = saved;
+        }
+    } else {
+        double left[D > 0 ? D : 1];
+        double right[D > 0 ? D : 1];
+        int const icell = nonuniform_cell(b, knots, x);
+        jmin = icell - D;
+        ndu[0][0] = 1.0;
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            left[j] = x - knots[icell - j];
+            right[j] = knots[icell + j + 1] - x;
+            double saved = 0.0;
+#pragma unroll
+            for (int r = 0; r < j + 1; ++r) {
+                ndu[r][j + 1] = 1.0 / (right[r] + left[j - r]);
+                double const temp = ndu[j][r] * ndu[r][j + 1];
+                ndu[j + 1][r] = saved + right[r] * temp;
+                saved = left[j - r] * temp;
+            }
+            ndu[j + 1][j + 1] = saved;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < D + 1; ++i) derivs[i * (N + 1)] = ndu[D][i];
+#pragma unroll
+    for (int r = 0; r < D + 1; ++r) {
+        int s1 = 0;
+        int s2 = 1;
+        a[0][0] = 1.0;
+#pragma unroll
+        for (int k = 1; k < N + 1; ++k) {
+            double d = 0.0;
+            int const rk = r - k;
+            int const pk = D - k;
+            if (r >= k) {
+                a[0][s2] = uni ? a[0][s1] / (pk + 1) : a[0][s1] * ndu[rk][pk + 1];
+                d = a[0][s2] * ndu[pk][rk];
+            }
+            int const j1 = rk > -1 ? 1 : (-rk);
+            int const j2 = (r - 1) <= pk ? k : (D - r + 1);
+#pragma unroll
+            for (int j = 1; j < D + 1; ++j) {
+                if (j >= j1 && j < j2) {
+                    a[j][s2] = uni ? (a[j][s1] - a[j - 1][s1]) / (pk + 1)
+                                   : (a[j][s1] - a[j - 1][s1]) * ndu[rk + j][pk + 1];
+                    d += a[j][s2] * ndu[pk][rk + j];
+                }
+            }
+            if (r <= pk) {
+                a[k][s2] = uni ? -a[k - 1][s1] / (pk + 1) : -a[k - 1][s1] * ndu[r][pk + 1];
+                d += a[k][s2] * ndu[pk][r];
+            }
+            derivs[r * (N + 1) + k] = d;
+            int const t = s1;
+            s1 = s2;
+            s2 = t;
+        }
+    }
+    if (uni) {
+        double d = D * b.inv_step;
+#pragma unroll
+        for (int k = 1; k < N + 1; ++k) {
+#pragma unroll
+            for (int i = 0; i < D + 1; ++i) derivs[i * (N + 1) + k] *= d;
+            d *= (D - k) * b.inv_step;
+        }
+    } else {
+        int r = D;
+#pragma unroll
+        for (int k = 1; k < N + 1; ++k) {
+#pragma unroll
+            for (int i = 0; i < D + 1; ++i) derivs[i * (N + 1) + k] *= r;
+            r *= D - k;
+        }
+    }
+    return jmin;
+}
+
+// Dispatch a runtime degree to a compile-time one (host setup paths).
+template <class F>
+inline bool dispatch_degree(int degree, F&& f)
+{
+    switch (degree) {
+    case 1: f.template operator()<1>(); return true;
+    case 2: f.template operator()<2>(); return true;
+    case 3: f.template operator()<3>(); return true;
+    case 4: f.template operator()<4>(); return true;
+    case 5: f.template operator()<5>(); return true;
+    case 6: f.template operator()<6>(); return true;
+    case 7: f.template operator()<7>(); return true;
+    default: return false;
+    }
+}
+
+} // namespace ddcx

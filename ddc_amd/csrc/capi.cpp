@@ -1,0 +1,277 @@
+// extern "C" boundary of libddcx (declared in include/ddcx.h).  Converts C
+// handles and layouts to the internal plans, catches every exception and
+// returns a status with a thread-local message.
+#include <cstring>
+#include <string>
+
+#include "internal.hpp"
+
+struct ddcx_bsplines {
+    std::shared_ptr<ddcx::BSplinesPlan> p;
+};
+struct ddcx_builder {
+    std::unique_ptr<ddcx::BuilderPlan> p;
+};
+struct ddcx_evaluator {
+    std::shared_ptr<ddcx::BSplinesPlan> bs;
+    ddcx_extrap lower, upper;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <class F>
+ddcx_status guarded(F&& f)
+{
+    try {
+        f();
+        g_last_error.clear();
+        return DDCX_OK;
+    } catch (ddcx::Error const& e) {
+        g_last_error = e.what();
+        return e.status;
+    } catch (std::exception const& e) {
+        g_last_error = e.what();
+        return DDCX_ERR_INVALID;
+    } catch (...) {
+        g_last_error = "unknown error";
+        return DDCX_ERR_INVALID;
+    }
+}
+
+void require(bool cond, const char* msg)
+{
+    if (!cond) throw ddcx::Error(DDCX_ERR_INVALID, msg);
+}
+
+int rule_of(ddcx_extrap e)
+{
+    return e == DDCX_EXTRAP_CONSTANT ? DDCX_EXTRAP_CONSTANT : DDCX_EXTRAP_NULL;
+}
+
+} // namespace
+
+extern "C" {
+
+const char* ddcx_last_error(void) { return g_last_error.c_str(); }
+const char* ddcx_version(void) { return "ddcx 0.1.0 (gfx950)"; }
+
+ddcx_status ddcx_bsplines_create(const ddcx_bsplines_desc* desc, ddcx_bsplines** out)
+{
+    return guarded([&] {
+        require(desc && out, "null argument");
+        *out = nullptr;
+        auto h = new ddcx_bsplines{ddcx::make_bsplines(*desc)};
+        *out = h;
+    });
+}
+
+ddcx_status ddcx_bsplines_destroy(ddcx_bsplines* b)
+{
+    return guarded([&] { delete b; });
+}
+
+ddcx_status ddcx_bsplines_knots(const ddcx_bsplines* b, double* knots)
+{
+    return guarded([&] {
+        require(b && knots, "null argument");
+        std::memcpy(knots, b->p->knots_host.data(), b->p->knots_host.size() * sizeof(double));
+    });
+}
+
+ddcx_status ddcx_bsplines_integrals(const ddcx_bsplines* b, double* out)
+{
+    return guarded([&] {
+        require(b && out, "null argument");
+        std::memcpy(out, b->p->integrals.data(), b->p->integrals.size() * sizeof(double));
+    });
+}
+
+ddcx_status ddcx_builder_create(const ddcx_bsplines* b, const ddcx_builder_desc* desc, ddcx_builder** out)
+{
+    return guarded([&] {
+        require(b && desc && out, "null argument");
+        *out = nullptr;
+        auto h = new ddcx_builder{ddcx::make_builder(b->p, *desc)};
+        *out = h;
+    });
+}
+
+ddcx_status ddcx_builder_destroy(ddcx_builder* h)
+{
+    return guarded([&] { delete h; });
+}
+
+ddcx_status ddcx_builder_get_info(const ddcx_builder* h, ddcx_builder_info* info)
+{
+    return guarded([&] {
+        require(h && info, "null argument");
+        ddcx::BuilderPlan const& p = *h->p;
+        info->npoints = int(p.points.size());
+        info->nbasis = p.bs->nbasis();
+        info->size = p.bs->size();
+        info->offset = p.offset;
+        info->nbv_lower = p.nbv_l;
+        info->nbv_upper = p.nbv_r;
+        info->matrix_kind = int(p.prob.q.type);
+        info->blocked = p.prob.blocked ? (p.prob.top > 0 ? 2 : 1) : 0;
+        info->main_size = p.prob.q.n;
+        info->border = p.prob.nb();
+        info->dx = p.dx;
+    });
+}
+
+ddcx_status ddcx_builder_points(const ddcx_builder* h, double* points)
+{
+    return guarded([&] {
+        require(h && points, "null argument");
+        std::memcpy(points, h->p->points.data(), h->p->points.size() * sizeof(double));
+    });
+}
+
+ddcx_status ddcx_builder_build(const ddcx_builder* h, double* coef, ddcx_layout coef_layout, const double* vals,
+                               ddcx_layout vals_layout, const double* derivs_xmin, const double* derivs_xmax,
+                               ddcx_layout derivs_layout, int64_t nbatch, void* stream)
+{
+    return guarded([&] {
+        require(h, "null builder");
+        require(nbatch >= 0, "negative batch size");
+        if (nbatch == 0) return; // spline_builder.hpp:929-931
+        require(coef && vals, "null data pointer");
+        ddcx::BuilderPlan const& p = *h->p;
+        if (p.bcl == DDCX_HERMITE && p.nbv_l > 0) require(derivs_xmin != nullptr, "HERMITE lower closure needs derivs_xmin");
+        if (p.bcu == DDCX_HERMITE && p.nbv_r > 0) require(derivs_xmax != nullptr, "HERMITE upper closure needs derivs_xmax");
+        ddcx::launch_build(p, coef, ddcx::to_layout(coef_layout), vals, ddcx::to_layout(vals_layout), derivs_xmin,
+                           derivs_xmax, ddcx::to_layout(derivs_layout), nbatch, static_cast<hipStream_t>(stream));
+    });
+}
+
+ddcx_status ddcx_builder_build_vjp(const ddcx_builder* h, const double* gcoef, ddcx_layout coef_layout, double* gvals,
+                                   ddcx_layout vals_layout, double* gderivs_xmin, double* gderivs_xmax,
+                                   ddcx_layout derivs_layout, int64_t nbatch, void* stream)
+{
+    return guarded([&] {
+        require(h, "null builder");
+        require(nbatch >= 0, "negative batch size");
+        if (nbatch == 0) return;
+        require(gcoef && gvals, "null data pointer");
+        ddcx::BuilderPlan const& p = *h->p;
+        if (p.bcl == DDCX_HERMITE && p.nbv_l > 0) require(gderivs_xmin != nullptr, "HERMITE lower closure needs gderivs_xmin");
+        if (p.bcu == DDCX_HERMITE && p.nbv_r > 0) require(gderivs_xmax != nullptr, "HERMITE upper closure needs gderivs_xmax");
+        ddcx::launch_build_vjp(p, gcoef, ddcx::to_layout(coef_layout), gvals, ddcx::to_layout(vals_layout),
+                               gderivs_xmin, gderivs_xmax, ddcx::to_layout(derivs_layout), nbatch,
+                               static_cast<hipStream_t>(stream));
+    });
+}
+
+ddcx_status ddcx_builder_quadrature(const ddcx_builder* h, double* q_xmin, double* q_vals, double* q_xmax)
+{
+    return guarded([&] {
+        require(h && q_vals, "null argument");
+        ddcx::BuilderPlan const& p = *h->p;
+        ddcx::HostProblem const& pr = p.prob;
+        // spline_builder.hpp:1111-1234: solve A^T q = integrals (first size() rows)
+        std::vector<double> m(pr.rhs_rows(), 0.0);
+        for (int i = 0; i < pr.n; ++i) m[i] = p.bs->integrals[i];
+        pr.solve(m.data(), 1, true);
+        int const nv = p.bs->nbasis() - p.nbv_l - p.nbv_r;
+        auto dxp = [&](int k) {
+            double r = 1.0;
+            for (int j = 0; j < k; ++j) r *= p.dx;
+            return r;
+        };
+        for (int i = 0; i < p.nbv_l; ++i) q_xmin[i] = m[i] * dxp(i + p.odd);
+        for (int i = 0; i < nv; ++i) q_vals[i] = m[p.nbv_l + i];
+        for (int i = 0; i < p.nbv_r; ++i) q_xmax[i] = m[p.nbv_l + nv + i] * dxp(i + p.odd);
+    });
+}
+
+ddcx_status ddcx_evaluator_create(const ddcx_bsplines* b, ddcx_extrap lower, ddcx_extrap upper, ddcx_evaluator** out)
+{
+    return guarded([&] {
+        require(b && out, "null argument");
+        *out = new ddcx_evaluator{b->p, lower, upper};
+    });
+}
+
+ddcx_status ddcx_evaluator_destroy(ddcx_evaluator* h)
+{
+    return guarded([&] { delete h; });
+}
+
+ddcx_status ddcx_evaluator_eval(const ddcx_evaluator* h, double* out, const double* coords, ddcx_layout points_layout,
+                                int64_t npts, const double* coef, ddcx_layout coef_layout, int64_t nbatch, void* stream)
+{
+    return guarded([&] {
+        require(h, "null evaluator");
+        require(npts >= 0 && nbatch >= 0, "negative size");
+        if (npts * nbatch == 0) return;
+        require(out && coords && coef, "null data pointer");
+        ddcx::launch_eval(*h->bs, rule_of(h->lower), rule_of(h->upper), 0, out, coords, ddcx::to_layout(points_layout),
+                          npts, coef, ddcx::to_layout(coef_layout), nbatch, static_cast<hipStream_t>(stream));
+    });
+}
+
+ddcx_status ddcx_evaluator_deriv(const ddcx_evaluator* h, int order, double* out, const double* coords,
+                                 ddcx_layout points_layout, int64_t npts, const double* coef, ddcx_layout coef_layout,
+                                 int64_t nbatch, void* stream)
+{
+    return guarded([&] {
+        require(h, "null evaluator");
+        require(order > 0 && order <= h->bs->view.degree, "derivative order must be in [1, degree]");
+        require(npts >= 0 && nbatch >= 0, "negative size");
+        if (npts * nbatch == 0) return;
+        require(out && coords && coef, "null data pointer");
+        ddcx::launch_eval(*h->bs, rule_of(h->lower), rule_of(h->upper), order, out, coords,
+                          ddcx::to_layout(points_layout), npts, coef, ddcx::to_layout(coef_layout), nbatch,
+                          static_cast<hipStream_t>(stream));
+    });
+}
+
+ddcx_status ddcx_evaluator_integrate(const ddcx_evaluator* h, double* out, int64_t out_stride, const double* coef,
+                                     ddcx_layout coef_layout, int64_t nbatch, void* stream)
+{
+    return guarded([&] {
+        require(h, "null evaluator");
+        require(nbatch >= 0, "negative size");
+        if (nbatch == 0) return;
+        require(out && coef, "null data pointer");
+        ddcx::launch_integrate(*h->bs, out, out_stride, coef, ddcx::to_layout(coef_layout), nbatch,
+                               static_cast<hipStream_t>(stream));
+    });
+}
+
+ddcx_status ddcx_evaluator_eval_vjp(const ddcx_evaluator* h, const double* coords, const double* w, double* gx,
+                                    ddcx_layout points_layout, int64_t npts, const double* coef, double* gcoef,
+                                    ddcx_layout coef_layout, int64_t nbatch, void* stream)
+{
+    return guarded([&] {
+        require(h, "null evaluator");
+        require(npts >= 0 && nbatch >= 0, "negative size");
+        if (nbatch == 0) return;
+        require(gcoef && coef, "null data pointer");
+        if (npts > 0) require(coords && w && gx, "null data pointer");
+        ddcx::launch_eval_vjp(*h->bs, rule_of(h->lower), rule_of(h->upper), coords, w, gx,
+                              ddcx::to_layout(points_layout), npts, coef, gcoef, ddcx::to_layout(coef_layout), nbatch,
+                              static_cast<hipStream_t>(stream));
+    });
+}
+
+ddcx_status ddcx_evaluator2d_eval(const ddcx_evaluator* e1, const ddcx_evaluator* e2, int order1, int order2,
+                                  double* out, const double* coords, int64_t npts, const double* coef,
+                                  int64_t coef_batch_stride, int64_t st1, int64_t st2, int64_t nbatch, void* stream)
+{
+    return guarded([&] {
+        require(e1 && e2, "null evaluator");
+        require(order1 >= 0 && order2 >= 0, "negative derivative order");
+        require(npts >= 0 && nbatch >= 0, "negative size");
+        if (npts * nbatch == 0) return;
+        require(out && coords && coef, "null data pointer");
+        int const rules[4] = {rule_of(e1->lower), rule_of(e1->upper), rule_of(e2->lower), rule_of(e2->upper)};
+        ddcx::launch_eval2d(*e1->bs, *e2->bs, rules, order1, order2, out, coords, npts, coef, coef_batch_stride, st1,
+                            st2, nbatch, static_cast<hipStream_t>(stream));
+    });
+}
+
+} // extern "C"

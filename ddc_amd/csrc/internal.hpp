@@ -1,0 +1,197 @@
+// Internal types of libddcx: host-side spline plans (built and factored once,
+// in FP64, on the host) and the plain-old-data views of them that kernels take
+// by value.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/ddcx.h"
+#include "basis.hpp"
+
+namespace ddcx {
+
+// Error carried back across the C ABI as a status code plus message.
+struct Error : std::runtime_error {
+    ddcx_status status;
+    Error(ddcx_status s, std::string const& m) : std::runtime_error(m), status(s) {}
+};
+
+void hip_check(hipError_t e, const char* what);
+
+// RAII device buffer (hipMalloc'd once at plan creation).
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    DeviceBuffer() = default;
+    DeviceBuffer(DeviceBuffer const&) = delete;
+    DeviceBuffer& operator=(DeviceBuffer const&) = delete;
+    ~DeviceBuffer();
+    void upload(const void* host, size_t nbytes);
+    template <class T>
+    void upload(std::vector<T> const& v)
+    {
+        upload(v.data(), v.size() * sizeof(T));
+    }
+    template <class T>
+    T* as() const
+    {
+        return static_cast<T*>(ptr);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Host linear problems (mirror of SplinesLinearProblem*, splines_linear_problem*.cpp)
+// ---------------------------------------------------------------------------
+
+enum class QType : int { Dense = 0, Band = 1, PDSBand = 2, PDSTridiag = 3 };
+
+// A square banded or dense matrix, factored on the host with the LAPACK
+// algorithms the reference calls (dgbtrf/dpbtrf/dpttrf/dgetrf).
+struct HostQ {
+    QType type = QType::Dense;
+    int n = 0;
+    int kl = 0, ku = 0; // Band: kl, ku; PDSBand: kd in kl
+    std::vector<double> q; // storage as in the reference (row-major)
+    std::vector<int> ipiv; // 0-based
+    void init(QType t, int n_, int kl_, int ku_);
+    void set(int i, int j, double v);
+    double get(int i, int j) const;
+    int factor(); // returns LAPACK info
+    void solve(double* b, long ld, bool transpose) const; // one right-hand side, stride ld
+};
+
+// The spline matrix: a band (or dense) main block Q bordered by nb = n - nq
+// rows/columns (SplinesLinearProblem2x2Blocks / 3x3Blocks), or a plain dense
+// matrix when the maker selects SplinesLinearProblemDense.
+struct HostProblem {
+    int n = 0;
+    int top = 0;    // 3x3 top size (0 for 2x2)
+    bool blocked = false;
+    HostQ q;        // main block (whole matrix when !blocked)
+    HostQ schur;    // dense nb x nb Schur complement delta - lambda Q^-1 gamma
+    std::vector<double> gamma; // nq x nb: Q^-1 gamma after setup
+    std::vector<double> lambda; // nb x nq
+    // COO copies (|a| >= 1e-14, row-major scan) used by the solve
+    std::vector<int> tr_r, tr_c, bl_r, bl_c;
+    std::vector<double> tr_v, bl_v;
+
+    int nq() const { return blocked ? q.n : n; }
+    int nb() const { return blocked ? n - q.n : 0; }
+    int rhs_rows() const { return n + top; }
+    void set(int i, int j, double v);
+    double get(int i, int j) const;
+    void setup();
+    // Solve in the 3x3 index space on a column with rhs_rows() entries.
+    void solve(double* b, long ld, bool transpose) const;
+};
+
+// ---------------------------------------------------------------------------
+// Plans
+// ---------------------------------------------------------------------------
+
+// Device copies are made on first device use, so that host-only queries
+// (knots, points, quadrature weights, validation errors) work without a GPU.
+struct BSplinesPlan {
+    BSplinesView view{}; // host view (knots -> host vector)
+    std::vector<double> knots_host;
+    mutable DeviceBuffer knots_dev;
+    std::vector<double> integrals; // integrals() over the full spline domain (size() entries)
+    mutable DeviceBuffer integrals_dev;
+    mutable std::once_flag uploaded;
+    void ensure_device() const;
+    int nbasis() const { return view.ncells + (view.periodic ? 0 : view.degree); }
+    int size() const { return view.ncells + view.degree; }
+    BSplinesView device_view() const
+    {
+        BSplinesView v = view;
+        v.knots = view.uniform ? nullptr : knots_dev.as<double>();
+        return v;
+    }
+};
+
+// Device view of a factored spline matrix, passed by value to the build kernels.
+struct SolveView {
+    int qtype;       // QType
+    int nq, kl, ku;  // main block size and bandwidths
+    int nb, top;     // border size, 3x3 top size
+    int tr_nnz, bl_nnz;
+    const double* q;    // main block factor storage
+    const int* ipiv;    // main block pivots
+    const double* s_lu; // Schur LU (nb x nb row-major)
+    const int* s_ipiv;
+    const int* tr_r;
+    const int* tr_c;
+    const double* tr_v;
+    const int* bl_r;
+    const int* bl_c;
+    const double* bl_v;
+};
+
+struct BuildView {
+    SolveView sv;
+    int degree, periodic, nbasis, size, offset, npts;
+    int bcl, bcu, nbe_l, nbe_r, odd;
+    double dxpow[kMaxDegree + 1]; // dx^k, k = 0..degree
+};
+
+struct BuilderPlan {
+    std::shared_ptr<BSplinesPlan> bs;
+    ddcx_closure bcl, bcu;
+    ddcx_solver solver;
+    ddcx_precision precision;
+    int nbe_l = 0, nbe_r = 0, nbv_l = 0, nbv_r = 0, odd = 0;
+    int offset = 0;
+    double dx = 0;
+    std::vector<double> points;
+    HostProblem prob;
+    mutable DeviceBuffer d_q, d_ipiv, d_slu, d_sipiv, d_trr, d_trc, d_trv, d_blr, d_blc, d_blv;
+    mutable BuildView view{}; // device pointers filled by ensure_device()
+    mutable std::once_flag uploaded;
+    void ensure_device() const;
+};
+
+struct EvaluatorPlan {
+    std::shared_ptr<BSplinesPlan> bs;
+    ddcx_extrap lower, upper;
+};
+
+// Batched layout: element (row r, line l) at r*rs + (l % inner)*is + (l / inner)*os.
+struct Layout {
+    int64_t rs, inner, is, os;
+};
+
+inline Layout to_layout(ddcx_layout const& l)
+{
+    return Layout{l.row_stride, l.inner_count > 0 ? l.inner_count : 1, l.inner_stride, l.outer_stride};
+}
+
+// host setup entry points (host_setup.cpp)
+std::shared_ptr<BSplinesPlan> make_bsplines(ddcx_bsplines_desc const& d);
+std::unique_ptr<BuilderPlan> make_builder(std::shared_ptr<BSplinesPlan> bs, ddcx_builder_desc const& d);
+
+// kernel launchers (build_generic.hip / build_tiled.hip / eval_kernels.hip)
+void launch_build_generic(BuilderPlan const& p, double* coef, Layout cl, const double* vals, Layout vl,
+                          const double* dl, const double* dr, Layout dlay, int64_t nbatch, hipStream_t stream);
+void launch_build(BuilderPlan const& p, double* coef, Layout cl, const double* vals, Layout vl, const double* dl,
+                  const double* dr, Layout dlay, int64_t nbatch, hipStream_t stream);
+void launch_build_vjp(BuilderPlan const& p, const double* gcoef, Layout cl, double* gvals, Layout vl, double* gdl,
+                      double* gdr, Layout dlay, int64_t nbatch, hipStream_t stream);
+void launch_eval(BSplinesPlan const& bs, int lower, int upper, int order, double* out, const double* coords,
+                 Layout pl, int64_t npts, const double* coef, Layout cl, int64_t nbatch, hipStream_t stream);
+void launch_eval_vjp(BSplinesPlan const& bs, int lower, int upper, const double* coords, const double* w,
+                     double* gx, Layout pl, int64_t npts, const double* coef, double* gcoef, Layout cl,
+                     int64_t nbatch, hipStream_t stream);
+void launch_integrate(BSplinesPlan const& bs, double* out, int64_t ostride, const double* coef, Layout cl,
+                      int64_t nbatch, hipStream_t stream);
+void launch_eval2d(BSplinesPlan const& b1, BSplinesPlan const& b2, const int* rules, int d1, int d2, double* out,
+                   const double* coords, int64_t npts, const double* coef, int64_t coef_batch_stride, int64_t st1,
+                   int64_t st2, int64_t nbatch, hipStream_t stream);
+
+} // namespace ddcx

@@ -1,0 +1,395 @@
+"""Host-side mirror of DDC's spline interface over the libddcx C ABI.
+
+Class and method names follow the reference (src/ddc/kernels/splines/):
+``UniformBSplines`` / ``NonUniformBSplines`` (bsplines_uniform.hpp:48-370,
+bsplines_non_uniform.hpp:44-392), ``SplineBuilder`` (spline_builder.hpp:57-617),
+``SplineEvaluator`` (spline_evaluator.hpp:33-606), ``SplineBuilder2D`` /
+``SplineEvaluator2D`` (spline_builder_2d.hpp:27-651,
+spline_evaluator_2d.hpp:37-826), ``SplineBuilderClosure``, ``SplineSolver``,
+the extrapolation rules, ``GrevilleInterpolationPoints`` and ``integrals``.
+
+Arrays are torch tensors in device memory.  The reference names the
+interpolation dimension by type; here it is named by ``axis`` (default 0).
+Any other dimensions (at most two) are batch dimensions, in any order.  All
+calls are enqueued on torch's current stream.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import enum
+
+import numpy as np
+import torch
+
+from ._lib import (call, ddcx_builder_desc, ddcx_builder_info, ddcx_bsplines_desc, ddcx_layout, lib)
+
+
+class SplineBuilderClosure(enum.IntEnum):
+    """ddc::SplineBuilderClosure, spline_builder_closures.hpp:18-23."""
+
+    PERIODIC = 0
+    HERMITE = 1
+    HOMOGENEOUS_HERMITE = 2
+    GREVILLE = 3
+
+
+class SplineSolver(enum.IntEnum):
+    """ddc::SplineSolver, spline_builder.hpp:36-39 (GINKGO routes to the direct solver)."""
+
+    GINKGO = 0
+    LAPACK = 1
+
+
+class Precision(enum.IntEnum):
+    FP64 = 0
+    FP32_REFINE = 1
+
+
+class NullExtrapolationRule:
+    """null_extrapolation_rule.hpp:14-26: 0 outside the domain."""
+
+    code = 0
+
+
+class ConstantExtrapolationRule:
+    """constant_extrapolation_rule.hpp:27-73: value at the boundary outside the domain."""
+
+    code = 1
+
+    def __init__(self, eval_pos=None):
+        self.eval_pos = eval_pos
+
+
+class PeriodicExtrapolationRule:
+    """periodic_extrapolation_rule.hpp:12-24 (the periodic wrap applies instead)."""
+
+    code = 2
+
+
+def n_boundary_equations(bc, degree):
+    """spline_builder_closures.hpp:33-55."""
+    return degree // 2 if bc in (SplineBuilderClosure.HERMITE, SplineBuilderClosure.HOMOGENEOUS_HERMITE) else 0
+
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _check_f64_cuda(t, name):
+    if t.dtype != torch.float64:
+        raise TypeError(f"{name} must be float64")
+    if not t.is_cuda:
+        raise TypeError(f"{name} must be a device tensor")
+
+
+def batched_layout(t, axis):
+    """ddcx_layout of a tensor whose dimension `axis` is the row dimension and
+    whose other (at most two) dimensions are batch dimensions, enumerated in
+    row-major order.  Returns (layout, nbatch)."""
+    axis = axis % t.dim()
+    bdims = [d for d in range(t.dim()) if d != axis]
+    if len(bdims) > 2:
+        raise ValueError("at most two batch dimensions are supported")
+    rs = t.stride(axis)
+    if len(bdims) == 0:
+        return ddcx_layout(rs, 1, 0, 0), 1
+    if len(bdims) == 1:
+        d = bdims[0]
+        return ddcx_layout(rs, t.shape[d], t.stride(d), 0), t.shape[d]
+    d0, d1 = bdims
+    return ddcx_layout(rs, t.shape[d1], t.stride(d1), t.stride(d0)), t.shape[d0] * t.shape[d1]
+
+
+class _BSplinesBase:
+    def __init__(self, desc, keep=None):
+        h = C.c_void_p()
+        call("ddcx_bsplines_create", C.byref(desc), C.byref(h))
+        self._h = h
+        self._keep = keep
+        self._degree = desc.degree
+        self._ncells = desc.ncells
+        self._periodic = bool(desc.periodic)
+        k = np.empty(self._ncells + 1 + 2 * self._degree)
+        call("ddcx_bsplines_knots", self._h, C.c_void_p(k.ctypes.data))
+        self._knots = k
+
+    def __del__(self):
+        if getattr(self, "_h", None) and lib is not None:
+            try:
+                lib().ddcx_bsplines_destroy(self._h)
+            except Exception:
+                pass
+            self._h = None
+
+    def degree(self):
+        return self._degree
+
+    def is_periodic(self):
+        return self._periodic
+
+    def ncells(self):
+        return self._ncells
+
+    def nbasis(self):
+        return self._ncells + (0 if self._periodic else self._degree)
+
+    def size(self):
+        return self._ncells + self._degree
+
+    def knots(self):
+        return self._knots.copy()
+
+    def rmin(self):
+        return float(self._knots[self._degree])
+
+    def rmax(self):
+        return float(self._knots[self._degree + self._ncells])
+
+    def length(self):
+        return self.rmax() - self.rmin()
+
+
+class UniformBSplines(_BSplinesBase):
+    """ddc::UniformBSplines<CDim, D> on [rmin, rmax] with ncells cells."""
+
+    def __init__(self, degree, rmin, rmax, ncells, periodic=False):
+        super().__init__(ddcx_bsplines_desc(degree, 1, int(periodic), ncells, float(rmin), float(rmax), None))
+
+    def is_uniform(self):
+        return True
+
+
+class NonUniformBSplines(_BSplinesBase):
+    """ddc::NonUniformBSplines<CDim, D> on the given sorted break points."""
+
+    def __init__(self, degree, breaks, periodic=False):
+        br = np.ascontiguousarray(np.asarray(breaks, dtype=np.float64))
+        super().__init__(ddcx_bsplines_desc(degree, 0, int(periodic), br.size - 1, 0.0, 0.0,
+                                            C.c_void_p(br.ctypes.data)), keep=br)
+
+    def is_uniform(self):
+        return False
+
+
+def integrals(bsplines):
+    """ddc::integrals(), integrals.hpp:165-177 (host array of size() values)."""
+    out = np.empty(bsplines.size())
+    call("ddcx_bsplines_integrals", bsplines._h, C.c_void_p(out.ctypes.data))
+    return out
+
+
+class SplineBuilder:
+    """ddc::SplineBuilder<Exec, Mem, BSplines, DDimI, BCl, BCu, Solver>."""
+
+    def __init__(self, bsplines, lower=None, upper=None, solver=SplineSolver.LAPACK, points=None,
+                 precision=Precision.FP64):
+        if lower is None:
+            lower = SplineBuilderClosure.PERIODIC if bsplines.is_periodic() else SplineBuilderClosure.GREVILLE
+        if upper is None:
+            upper = lower
+        self.bsplines = bsplines
+        self.lower, self.upper = SplineBuilderClosure(lower), SplineBuilderClosure(upper)
+        pts = None
+        if points is not None:
+            pts = np.ascontiguousarray(np.asarray(points, dtype=np.float64))
+        desc = ddcx_builder_desc(int(self.lower), int(self.upper), int(solver), int(precision),
+                                 C.c_void_p(pts.ctypes.data) if pts is not None else None,
+                                 0 if pts is None else pts.size)
+        h = C.c_void_p()
+        call("ddcx_builder_create", bsplines._h, C.byref(desc), C.byref(h))
+        self._h = h
+        info = ddcx_builder_info()
+        call("ddcx_builder_get_info", self._h, C.byref(info))
+        self.info = info
+        self.s_odd = bsplines.degree() % 2
+        self.s_nbe_xmin = n_boundary_equations(self.lower, bsplines.degree())
+        self.s_nbe_xmax = n_boundary_equations(self.upper, bsplines.degree())
+        self.s_nbv_xmin = info.nbv_lower
+        self.s_nbv_xmax = info.nbv_upper
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            try:
+                lib().ddcx_builder_destroy(self._h)
+            except Exception:
+                pass
+            self._h = None
+
+    def interpolation_points(self):
+        p = np.empty(self.info.npoints)
+        call("ddcx_builder_points", self._h, C.c_void_p(p.ctypes.data))
+        return p
+
+    def npoints(self):
+        return self.info.npoints
+
+    def offset(self):
+        return self.info.offset
+
+    def __call__(self, spline, vals, derivs_xmin=None, derivs_xmax=None, axis=0):
+        """SplineBuilder::operator(), spline_builder.hpp:538-557."""
+        _check_f64_cuda(spline, "spline")
+        _check_f64_cuda(vals, "vals")
+        if vals.shape[axis] != self.info.npoints:
+            raise ValueError(f"vals has {vals.shape[axis]} interpolation points, expected {self.info.npoints}")
+        if spline.shape[axis] != self.info.size:
+            raise ValueError(f"spline has {spline.shape[axis]} coefficients, expected {self.info.size}")
+        vl, nbatch = batched_layout(vals, axis)
+        cl, nb2 = batched_layout(spline, axis)
+        if nb2 != nbatch:
+            raise ValueError("spline and vals batch domains differ")
+        dl = ddcx_layout(1, 1, 0, 0)
+        for d, name in ((derivs_xmin, "derivs_xmin"), (derivs_xmax, "derivs_xmax")):
+            if d is not None:
+                _check_f64_cuda(d, name)
+                dl, nb3 = batched_layout(d, axis)
+                if nb3 != nbatch:
+                    raise ValueError(f"{name} batch domain differs")
+        call("ddcx_builder_build", self._h, _ptr(spline), cl, _ptr(vals), vl, _ptr(derivs_xmin), _ptr(derivs_xmax),
+             dl, nbatch, _stream())
+        return spline
+
+    def build_vjp(self, gcoef, gvals, gderivs_xmin=None, gderivs_xmax=None, axis=0):
+        """Adjoint of __call__: gvals (and Hermite derivative gradients) from gcoef."""
+        cl, nbatch = batched_layout(gcoef, axis)
+        vl, _ = batched_layout(gvals, axis)
+        dl = ddcx_layout(1, 1, 0, 0)
+        for d in (gderivs_xmin, gderivs_xmax):
+            if d is not None:
+                dl, _ = batched_layout(d, axis)
+        call("ddcx_builder_build_vjp", self._h, _ptr(gcoef), cl, _ptr(gvals), vl, _ptr(gderivs_xmin),
+             _ptr(gderivs_xmax), dl, nbatch, _stream())
+        return gvals
+
+    def quadrature_coefficients(self):
+        """SplineBuilder::quadrature_coefficients(), spline_builder.hpp:1111-1234 (host arrays)."""
+        qa = np.empty(max(self.s_nbv_xmin, 1))
+        qv = np.empty(self.info.nbasis - self.s_nbv_xmin - self.s_nbv_xmax)
+        qb = np.empty(max(self.s_nbv_xmax, 1))
+        call("ddcx_builder_quadrature", self._h, C.c_void_p(qa.ctypes.data), C.c_void_p(qv.ctypes.data),
+             C.c_void_p(qb.ctypes.data))
+        return qa[: self.s_nbv_xmin], qv, qb[: self.s_nbv_xmax]
+
+
+class SplineEvaluator:
+    """ddc::SplineEvaluator<Exec, Mem, BSplines, EvalDDim, LowerRule, UpperRule>."""
+
+    def __init__(self, bsplines, lower_extrap_rule=None, upper_extrap_rule=None):
+        lo = lower_extrap_rule if lower_extrap_rule is not None else (
+            PeriodicExtrapolationRule() if bsplines.is_periodic() else NullExtrapolationRule())
+        up = upper_extrap_rule if upper_extrap_rule is not None else lo
+        self.bsplines = bsplines
+        self.lower_extrap_rule, self.upper_extrap_rule = lo, up
+        h = C.c_void_p()
+        call("ddcx_evaluator_create", bsplines._h, lo.code, up.code, C.byref(h))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            try:
+                lib().ddcx_evaluator_destroy(self._h)
+            except Exception:
+                pass
+            self._h = None
+
+    def _args(self, spline_eval, coords_eval, spline_coef, axis):
+        _check_f64_cuda(spline_eval, "spline_eval")
+        _check_f64_cuda(coords_eval, "coords_eval")
+        _check_f64_cuda(spline_coef, "spline_coef")
+        if spline_eval.shape != coords_eval.shape or spline_eval.stride() != coords_eval.stride():
+            raise ValueError("spline_eval and coords_eval must have the same shape and layout")
+        pl, nbatch = batched_layout(coords_eval, axis)
+        cl, nb2 = batched_layout(spline_coef, axis)
+        if nb2 != nbatch:
+            raise ValueError("coordinates and coefficients batch domains differ")
+        if spline_coef.shape[axis] != self.bsplines.size():
+            raise ValueError("spline_coef has the wrong number of coefficients")
+        return pl, coords_eval.shape[axis], cl, nbatch
+
+    def __call__(self, spline_eval, coords_eval, spline_coef, axis=0):
+        """SplineEvaluator::operator(), spline_evaluator.hpp:257-294."""
+        pl, npts, cl, nbatch = self._args(spline_eval, coords_eval, spline_coef, axis)
+        call("ddcx_evaluator_eval", self._h, _ptr(spline_eval), _ptr(coords_eval), pl, npts, _ptr(spline_coef), cl,
+             nbatch, _stream())
+        return spline_eval
+
+    def deriv(self, order, spline_eval, coords_eval, spline_coef, axis=0):
+        """SplineEvaluator::deriv(), spline_evaluator.hpp:357-430."""
+        pl, npts, cl, nbatch = self._args(spline_eval, coords_eval, spline_coef, axis)
+        call("ddcx_evaluator_deriv", self._h, int(order), _ptr(spline_eval), _ptr(coords_eval), pl, npts,
+             _ptr(spline_coef), cl, nbatch, _stream())
+        return spline_eval
+
+    def integrate(self, integrals_out, spline_coef, axis=0):
+        """SplineEvaluator::integrate(), spline_evaluator.hpp:493-526."""
+        _check_f64_cuda(integrals_out, "integrals")
+        cl, nbatch = batched_layout(spline_coef, axis)
+        if integrals_out.numel() != nbatch or not integrals_out.is_contiguous():
+            raise ValueError("integrals must be a contiguous tensor over the batch domain")
+        call("ddcx_evaluator_integrate", self._h, _ptr(integrals_out), 1, _ptr(spline_coef), cl, nbatch, _stream())
+        return integrals_out
+
+    def eval_vjp(self, coords_eval, w, spline_coef, gx, gcoef, axis=0):
+        """Adjoint of __call__: gcoef = dL/dcoef, gx = dL/dcoords for L = sum(w * f(coords))."""
+        pl, nbatch = batched_layout(coords_eval, axis)
+        cl, _ = batched_layout(spline_coef, axis)
+        call("ddcx_evaluator_eval_vjp", self._h, _ptr(coords_eval), _ptr(w), _ptr(gx), pl, coords_eval.shape[axis],
+             _ptr(spline_coef), _ptr(gcoef), cl, nbatch, _stream())
+        return gcoef, gx
+
+
+class SplineBuilder2D:
+    """ddc::SplineBuilder2D (spline_builder_2d.hpp:27-651): builds along dim 1
+    batched over (batch, dim 2) into a temporary, then along dim 2 batched over
+    (batch, bsplines 1).  Arrays are (batch, n1, n2) (or (n1, n2))."""
+
+    def __init__(self, builder1: SplineBuilder, builder2: SplineBuilder):
+        self.builder1, self.builder2 = builder1, builder2
+        if builder1.s_nbe_xmin or builder1.s_nbe_xmax or builder2.s_nbe_xmin or builder2.s_nbe_xmax:
+            raise NotImplementedError("2D Hermite closures are not part of this build")
+        self._tmp = None
+
+    def __call__(self, spline, vals):
+        squeeze = vals.dim() == 2
+        v = vals.unsqueeze(0) if squeeze else vals
+        s = spline.unsqueeze(0) if squeeze else spline
+        nb, n1, n2 = v.shape
+        shape1 = (nb, self.builder1.bsplines.size(), n2)
+        if self._tmp is None or tuple(self._tmp.shape) != shape1 or self._tmp.device != v.device:
+            self._tmp = torch.empty(shape1, dtype=torch.float64, device=v.device)
+        self.builder1(self._tmp, v, axis=1)
+        self.builder2(s, self._tmp, axis=2)
+        return spline
+
+
+class SplineEvaluator2D:
+    """ddc::SplineEvaluator2D (spline_evaluator_2d.hpp:37-826) on (batch, size1, size2) coefficients."""
+
+    def __init__(self, ev1: SplineEvaluator, ev2: SplineEvaluator):
+        self.ev1, self.ev2 = ev1, ev2
+
+    def __call__(self, spline_eval, coords_eval, spline_coef, order1=0, order2=0):
+        """coords_eval: (batch, npts, 2); spline_eval: (batch, npts); spline_coef: (batch, s1, s2)."""
+        c = coords_eval.contiguous()
+        squeeze = spline_coef.dim() == 2
+        coef = spline_coef.unsqueeze(0) if squeeze else spline_coef
+        nb = coef.shape[0]
+        npts = c.numel() // (2 * nb)
+        if not spline_eval.is_contiguous():
+            raise ValueError("spline_eval must be contiguous")
+        call("ddcx_evaluator2d_eval", self.ev1._h, self.ev2._h, int(order1), int(order2), _ptr(spline_eval), _ptr(c),
+             npts, _ptr(coef), coef.stride(0), coef.stride(1), coef.stride(2), nb, _stream())
+        return spline_eval
+
+    def deriv_dim_1(self, spline_eval, coords_eval, spline_coef):
+        return self(spline_eval, coords_eval, spline_coef, 1, 0)
+
+    def deriv_dim_2(self, spline_eval, coords_eval, spline_coef):
+        return self(spline_eval, coords_eval, spline_coef, 0, 1)
+
+    def deriv_1_and_2(self, spline_eval, coords_eval, spline_coef):
+        return self(spline_eval, coords_eval, spline_coef, 1, 1)

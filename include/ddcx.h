@@ -1,0 +1,198 @@
+/*
+ * ddcx.h -- C ABI of libddcx, the MI355X (gfx950) implementation of DDC's
+ * batched spline hot path: ddc::SplineBuilder / ddc::SplineEvaluator (1D and
+ * tensor-product 2D) with their closures, solver tags and extrapolation rules.
+ *
+ * All citations are to DDC v0.15.1 (src/ddc/kernels/splines/...).  The
+ * reference exposes this path as C++ templates, not as a C interface; each
+ * entry point below names the reference member it stands in for.
+ *
+ * Conventions
+ *  - Plain pointers and sizes only.  Array arguments of build/eval/integrate
+ *    calls are DEVICE pointers (hipMalloc'd or equivalent); the `stream`
+ *    argument is a hipStream_t (NULL = default stream).  Calls are
+ *    stream-ordered and asynchronous; nothing allocates device memory after
+ *    create (plans own all device data).
+ *  - A batched array is described by a ddcx_layout: element (row r, batch
+ *    line l) lives at  r*row_stride + (l % inner_count)*inner_stride
+ *    + (l / inner_count)*outer_stride  (units: elements).  This covers every
+ *    layout_right dimension order of a ddc::ChunkSpan with one interpolation
+ *    dimension and up to two batch dimensions.
+ *  - Every function returns a ddcx_status; on error ddcx_last_error() gives a
+ *    thread-local message (the facade rethrows std::runtime_error with it, as
+ *    the reference throws).
+ */
+#ifndef DDCX_H
+#define DDCX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum ddcx_status {
+    DDCX_OK = 0,
+    DDCX_ERR_INVALID = 1,    /* bad argument / shape (reference: assert) */
+    DDCX_ERR_GRID = 2,       /* check_valid_grid, spline_builder.hpp:1273-1317 */
+    DDCX_ERR_SINGULAR = 3,   /* LAPACK info != 0, splines_linear_problem_band.cpp:121-123 */
+    DDCX_ERR_HIP = 4,        /* HIP runtime error */
+    DDCX_ERR_UNSUPPORTED = 5 /* outside this build's range (e.g. degree > 7) */
+} ddcx_status;
+
+/* ddc::SplineBuilderClosure, spline_builder_closures.hpp:18-23 (same order) */
+typedef enum ddcx_closure {
+    DDCX_PERIODIC = 0,
+    DDCX_HERMITE = 1,
+    DDCX_HOMOGENEOUS_HERMITE = 2,
+    DDCX_GREVILLE = 3
+} ddcx_closure;
+
+/* ddc::SplineSolver, spline_builder.hpp:36-39.  GINKGO is accepted and routed
+ * to the direct (LAPACK-algorithm) solver. */
+typedef enum ddcx_solver { DDCX_SOLVER_GINKGO = 0, DDCX_SOLVER_LAPACK = 1 } ddcx_solver;
+
+/* Extrapolation rules: NullExtrapolationRule (null_extrapolation_rule.hpp:14-26),
+ * ConstantExtrapolationRule (constant_extrapolation_rule.hpp:27-73),
+ * PeriodicExtrapolationRule (periodic_extrapolation_rule.hpp:12-24). */
+typedef enum ddcx_extrap { DDCX_EXTRAP_NULL = 0, DDCX_EXTRAP_CONSTANT = 1, DDCX_EXTRAP_PERIODIC = 2 } ddcx_extrap;
+
+/* Arithmetic of the batched solve: plain FP64 (the drop-in), or FP32 sweeps
+ * with FP64 residual refinement.  No reference counterpart. */
+typedef enum ddcx_precision { DDCX_FP64 = 0, DDCX_FP32_REFINE = 1 } ddcx_precision;
+
+typedef struct ddcx_layout {
+    int64_t row_stride;
+    int64_t inner_count;
+    int64_t inner_stride;
+    int64_t outer_stride;
+} ddcx_layout;
+
+/* UniformBSplines<CDim, D>::Impl(rmin, rmax, ncells) (bsplines_uniform.hpp:135-148)
+ * or NonUniformBSplines<CDim, D>::Impl(breaks) (bsplines_non_uniform.hpp:414-457).
+ * periodic = CDim::PERIODIC.  breaks: host pointer, ncells+1 sorted values. */
+typedef struct ddcx_bsplines_desc {
+    int degree;
+    int uniform;
+    int periodic;
+    int ncells;
+    double rmin;
+    double rmax;
+    const double* breaks;
+} ddcx_bsplines_desc;
+
+typedef struct ddcx_bsplines ddcx_bsplines;
+typedef struct ddcx_builder ddcx_builder;
+typedef struct ddcx_evaluator ddcx_evaluator;
+
+/* SplineBuilder<Exec, Mem, BSplines, DDimI, BCl, BCu, Solver> constructor,
+ * spline_builder.hpp:227-260 (with check_valid_grid :1273-1317,
+ * compute_offset :627-658, allocate_matrix :719-774, build_matrix_system
+ * :784-886).  points: host pointer to the interpolation points (npoints of
+ * them), or NULL for GrevilleInterpolationPoints<BSplines, BCl, BCu>
+ * (greville_interpolation_points.hpp:153-269). */
+typedef struct ddcx_builder_desc {
+    ddcx_closure lower;
+    ddcx_closure upper;
+    ddcx_solver solver;
+    ddcx_precision precision;
+    const double* points;
+    int npoints;
+} ddcx_builder_desc;
+
+typedef struct ddcx_builder_info {
+    int npoints;      /* interpolation points (rows of vals) */
+    int nbasis;       /* UniformBSplines::nbasis(), bsplines_uniform.hpp:339-342 */
+    int size;         /* coefficients per line, BSplines::size() */
+    int offset;       /* periodic RHS rotation, spline_builder.hpp:627-658 */
+    int nbv_lower;    /* s_nbv_xmin: rows of derivs_xmin */
+    int nbv_upper;    /* s_nbv_xmax */
+    int matrix_kind;  /* 0 dense, 1 band, 2 pds band, 3 pds tridiag (main block) */
+    int blocked;      /* 0 plain, 1 2x2 blocks, 2 3x3 blocks */
+    int main_size;    /* size of the band main block */
+    int border;       /* nb = n - main_size */
+    double dx;        /* m_dx, average cell size */
+} ddcx_builder_info;
+
+ddcx_status ddcx_bsplines_create(const ddcx_bsplines_desc* desc, ddcx_bsplines** out);
+ddcx_status ddcx_bsplines_destroy(ddcx_bsplines* b);
+/* knots: host output of ncells+1+2*degree coordinates (ddc::coordinate of the knot domain). */
+ddcx_status ddcx_bsplines_knots(const ddcx_bsplines* b, double* knots);
+/* ddc::integrals(), integrals.hpp:165-177: host output of size() values. */
+ddcx_status ddcx_bsplines_integrals(const ddcx_bsplines* b, double* out);
+
+ddcx_status ddcx_builder_create(const ddcx_bsplines* b, const ddcx_builder_desc* desc, ddcx_builder** out);
+ddcx_status ddcx_builder_destroy(ddcx_builder* h);
+ddcx_status ddcx_builder_get_info(const ddcx_builder* h, ddcx_builder_info* info);
+/* Interpolation points (host output, npoints values). */
+ddcx_status ddcx_builder_points(const ddcx_builder* h, double* points);
+
+/* SplineBuilder::operator()(spline, vals, derivs_xmin, derivs_xmax),
+ * spline_builder.hpp:538-557 / :896-1101.  coef: size() rows per line;
+ * vals: npoints rows; derivs_*: nbv rows (NULL unless the closure is HERMITE),
+ * both described by derivs_layout. */
+ddcx_status ddcx_builder_build(const ddcx_builder* h, double* coef, ddcx_layout coef_layout, const double* vals,
+                               ddcx_layout vals_layout, const double* derivs_xmin, const double* derivs_xmax,
+                               ddcx_layout derivs_layout, int64_t nbatch, void* stream);
+
+/* Vector-Jacobian product of ddcx_builder_build (no reference counterpart;
+ * built on the transposed solve SplinesLinearProblem::solve(b, true),
+ * splines_linear_problem.hpp:77): given gcoef, writes gvals and the Hermite
+ * derivative gradients (NULL when not Hermite). */
+ddcx_status ddcx_builder_build_vjp(const ddcx_builder* h, const double* gcoef, ddcx_layout coef_layout,
+                                   double* gvals, ddcx_layout vals_layout, double* gderivs_xmin,
+                                   double* gderivs_xmax, ddcx_layout derivs_layout, int64_t nbatch, void* stream);
+
+/* SplineBuilder::quadrature_coefficients(), spline_builder.hpp:1111-1234.
+ * Host outputs: derivs_xmin (nbv_lower), values (nbasis - nbv_lower - nbv_upper;
+ * this equals npoints except for HOMOGENEOUS_HERMITE closures, where the
+ * reference also returns the weights of the zero derivative rows),
+ * derivs_xmax (nbv_upper). */
+ddcx_status ddcx_builder_quadrature(const ddcx_builder* h, double* q_xmin, double* q_vals, double* q_xmax);
+
+/* SplineEvaluator<Exec, Mem, BSplines, EvalDDim, LowerRule, UpperRule> ctor,
+ * spline_evaluator.hpp:140-153. */
+ddcx_status ddcx_evaluator_create(const ddcx_bsplines* b, ddcx_extrap lower, ddcx_extrap upper,
+                                  ddcx_evaluator** out);
+ddcx_status ddcx_evaluator_destroy(ddcx_evaluator* h);
+
+/* SplineEvaluator::operator()(spline_eval, coords_eval, spline_coef),
+ * spline_evaluator.hpp:257-294: out and coords share points_layout (npts rows). */
+ddcx_status ddcx_evaluator_eval(const ddcx_evaluator* h, double* out, const double* coords,
+                                ddcx_layout points_layout, int64_t npts, const double* coef,
+                                ddcx_layout coef_layout, int64_t nbatch, void* stream);
+
+/* SplineEvaluator::deriv(DiscreteElement<Deriv<X>> order, ...), spline_evaluator.hpp:357-430. */
+ddcx_status ddcx_evaluator_deriv(const ddcx_evaluator* h, int order, double* out, const double* coords,
+                                 ddcx_layout points_layout, int64_t npts, const double* coef,
+                                 ddcx_layout coef_layout, int64_t nbatch, void* stream);
+
+/* SplineEvaluator::integrate(integrals, spline_coef), spline_evaluator.hpp:493-526. */
+ddcx_status ddcx_evaluator_integrate(const ddcx_evaluator* h, double* out, int64_t out_stride, const double* coef,
+                                     ddcx_layout coef_layout, int64_t nbatch, void* stream);
+
+/* Vector-Jacobian product of ddcx_evaluator_eval (no reference counterpart):
+ * gcoef = sum_p w_p dB(x_p)/dcoef (segmented scatter-add, overwritten),
+ * gx_p = w_p f'(x_p) (deriv() order 1; 0 where an extrapolation rule applies). */
+ddcx_status ddcx_evaluator_eval_vjp(const ddcx_evaluator* h, const double* coords, const double* w, double* gx,
+                                    ddcx_layout points_layout, int64_t npts, const double* coef, double* gcoef,
+                                    ddcx_layout coef_layout, int64_t nbatch, void* stream);
+
+/* SplineEvaluator2D::operator() / deriv*(), spline_evaluator_2d.hpp:370-600
+ * (eval_no_bc :740-825): batched 2D evaluation.  coords: interleaved (x1, x2)
+ * pairs, point p of batch l at coords[2*(l*npts + p)], out[l*npts + p]; coef of
+ * batch l at coef + l*coef_batch_stride, element (i1, i2) at i1*st1 + i2*st2.
+ * rules: {lower1, upper1, lower2, upper2} extrapolation rules. */
+ddcx_status ddcx_evaluator2d_eval(const ddcx_evaluator* e1, const ddcx_evaluator* e2, int order1, int order2,
+                                  double* out, const double* coords, int64_t npts, const double* coef,
+                                  int64_t coef_batch_stride, int64_t st1, int64_t st2, int64_t nbatch,
+                                  void* stream);
+
+const char* ddcx_last_error(void);
+const char* ddcx_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DDCX_H */

@@ -1,0 +1,324 @@
+"""CPU oracle for DDC's batched spline path (test checker and CPU baseline).
+
+This package is test infrastructure.  Only ``tests/``, ``__graft_entry__.smoke()``
+and the ``cpu_baseline`` leg of ``bench.py`` may import it, and only to check or
+time the product; the product library never links to or calls it.
+
+``liboracle.so`` is built from ``ddc_oracle.c`` (a step-by-step FP64
+restatement of the reference algorithm, compiled with ``-ffp-contract=off``)
+by ``oracle/Makefile``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+PERIODIC, HERMITE, HOMOGENEOUS_HERMITE, GREVILLE = 0, 1, 2, 3
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+_L = C.c_long
+
+
+def build() -> str:
+    """Compile liboracle.so in place (gcc, OpenMP)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return os.path.join(_HERE, "liboracle.so")
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        L.or_last_error.restype = C.c_char_p
+        L.or_problem_create.restype = _vp
+        L.or_problem_create.argtypes = [C.c_int] * 7
+        L.or_problem_set.argtypes = [_vp, C.c_int, C.c_int, C.c_double]
+        L.or_problem_get.argtypes = [_vp, C.c_int, C.c_int]
+        L.or_problem_get.restype = C.c_double
+        L.or_problem_setup.argtypes = [_vp]
+        L.or_problem_type.argtypes = [_vp]
+        L.or_problem_rhs_rows.argtypes = [_vp]
+        L.or_problem_solve.argtypes = [_vp, _dp, C.c_int, C.c_int]
+        L.or_problem_destroy.argtypes = [_vp]
+        L.or_problem_q.argtypes = [_vp]
+        L.or_problem_q.restype = C.POINTER(C.c_double)
+        L.or_problem_ipiv.argtypes = [_vp]
+        L.or_problem_ipiv.restype = C.POINTER(C.c_int)
+        L.or_spline_create.restype = _vp
+        L.or_spline_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                       _vp, _vp, C.c_int]
+        for name in ["or_spline_npts", "or_spline_nbasis", "or_spline_size", "or_spline_offset",
+                     "or_spline_matrix_type", "or_spline_nknots"]:
+            getattr(L, name).argtypes = [_vp]
+        for name in ["or_spline_dx", "or_spline_rmin", "or_spline_rmax"]:
+            getattr(L, name).argtypes = [_vp]
+            getattr(L, name).restype = C.c_double
+        L.or_spline_problem.argtypes = [_vp]
+        L.or_spline_problem.restype = _vp
+        L.or_spline_points.argtypes = [_vp, _dp]
+        L.or_spline_knots.argtypes = [_vp, _dp]
+        L.or_spline_integrals.argtypes = [_vp, _dp]
+        L.or_spline_eval_basis.argtypes = [_vp, C.c_double, _dp]
+        L.or_spline_eval_basis_and_n_derivs.argtypes = [_vp, C.c_double, C.c_int, _dp]
+        L.or_spline_build.argtypes = [_vp, _dp, _L, _L, _L, _L, _dp, _L, _L, _L, _L, _vp, _vp, _L, _L, _L, _L, _L]
+        L.or_spline_build_vjp.argtypes = [_vp, _dp, _L, _L, _L, _L, _dp, _L, _L, _L, _L, _vp, _vp,
+                                          _L, _L, _L, _L, _L]
+        L.or_spline_eval.argtypes = [_vp, _dp, _dp, _L, _L, _L, _L, _L, _dp, _L, _L, _L, _L, _L, C.c_int,
+                                     C.c_int, C.c_int]
+        L.or_spline_eval_vjp.argtypes = [_vp, _dp, _dp, _dp, _L, _L, _L, _L, _L, _dp, _dp, _L, _L, _L, _L, _L,
+                                         C.c_int, C.c_int]
+        L.or_spline_integrate.argtypes = [_vp, _dp, _L, _L, _L, _L, _dp, _L, _L]
+        L.or_spline_quadrature.argtypes = [_vp, _dp]
+        L.or_spline2d_eval.argtypes = [_vp, _vp, _dp, _dp, _L, _dp, _L, _L, _L, _L, C.c_int, C.c_int,
+                                       np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")]
+        L.or_spline_destroy.argtypes = [_vp]
+        _LIB = L
+    return _LIB
+
+
+def _err():
+    return lib().or_last_error().decode()
+
+
+class Problem:
+    """A SplinesLinearProblem restated on the CPU (for solver known-answer tests)."""
+
+    KINDS = {"dense": 0, "band": 1, "block": 2, "periodic": 3, "dense2x2": 4, "dense3x3": 5}
+    TYPES = ["dense", "band", "pds_band", "pds_tridiag", "2x2", "3x3"]
+
+    def __init__(self, kind, n, kl=0, ku=0, pds=False, bottom=0, top=0):
+        self.h = lib().or_problem_create(self.KINDS[kind], n, kl, ku, int(pds), bottom, top)
+        self.n = n
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            try:
+                lib().or_problem_destroy(self.h)
+            except Exception:
+                pass
+            self.h = None
+
+    def set(self, i, j, v):
+        lib().or_problem_set(self.h, i, j, float(v))
+
+    def get(self, i, j):
+        return lib().or_problem_get(self.h, i, j)
+
+    @property
+    def type(self):
+        return self.TYPES[lib().or_problem_type(self.h)]
+
+    @property
+    def rhs_rows(self):
+        return lib().or_problem_rhs_rows(self.h)
+
+    def setup(self):
+        info = lib().or_problem_setup(self.h)
+        if info:
+            raise RuntimeError(f"factorisation failed with error code {info}")
+
+    def solve(self, b, transpose=False):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        assert b.shape[0] == self.rhs_rows
+        lib().or_problem_solve(self.h, b, b.shape[1], int(transpose))
+        return b
+
+
+class Spline:
+    """A 1D spline builder + evaluator (SplineBuilder/SplineEvaluator) on the CPU."""
+
+    def __init__(self, degree, uniform, bc_lower, bc_upper, ncells=None, rmin=0.0, rmax=1.0, breaks=None,
+                 points=None):
+        self._keep = []
+        bp = None
+        if not uniform:
+            br = np.ascontiguousarray(breaks, dtype=np.float64)
+            self._keep.append(br)
+            ncells = br.size - 1
+            bp = br.ctypes.data
+        pp, npts = None, 0
+        if points is not None:
+            pa = np.ascontiguousarray(points, dtype=np.float64)
+            self._keep.append(pa)
+            pp, npts = pa.ctypes.data, pa.size
+        self.h = lib().or_spline_create(degree, int(uniform), bc_lower, bc_upper, ncells, rmin, rmax, bp, pp, npts)
+        if not self.h:
+            raise RuntimeError(_err())
+        self.degree, self.uniform, self.bc_lower, self.bc_upper = degree, uniform, bc_lower, bc_upper
+        self.periodic = bc_lower == PERIODIC
+        self.ncells = ncells
+        L = lib()
+        self.npts = L.or_spline_npts(self.h)
+        self.nbasis = L.or_spline_nbasis(self.h)
+        self.size = L.or_spline_size(self.h)
+        self.offset = L.or_spline_offset(self.h)
+        self.dx = L.or_spline_dx(self.h)
+        self.rmin = L.or_spline_rmin(self.h)
+        self.rmax = L.or_spline_rmax(self.h)
+        self.matrix_type = Problem.TYPES[L.or_spline_matrix_type(self.h)]
+        nbe = lambda bc: degree // 2 if bc in (HERMITE, HOMOGENEOUS_HERMITE) else 0
+        self.nbe_l, self.nbe_r = nbe(bc_lower), nbe(bc_upper)
+        self.nbv_l = 0 if bc_lower == HOMOGENEOUS_HERMITE else self.nbe_l
+        self.nbv_r = 0 if bc_upper == HOMOGENEOUS_HERMITE else self.nbe_r
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            try:
+                lib().or_spline_destroy(self.h)
+            except Exception:
+                pass
+            self.h = None
+
+    def points(self):
+        out = np.empty(self.npts)
+        lib().or_spline_points(self.h, out)
+        return out
+
+    def knots(self):
+        out = np.empty(lib().or_spline_nknots(self.h))
+        lib().or_spline_knots(self.h, out)
+        return out
+
+    def integrals(self):
+        out = np.empty(self.size)
+        lib().or_spline_integrals(self.h, out)
+        return out
+
+    def eval_basis(self, x):
+        v = np.empty(self.degree + 1)
+        j = lib().or_spline_eval_basis(self.h, float(x), v)
+        return j, v
+
+    def eval_basis_and_n_derivs(self, x, n):
+        d = np.empty((self.degree + 1) * (n + 1))
+        j = lib().or_spline_eval_basis_and_n_derivs(self.h, float(x), n, d)
+        return j, d.reshape(self.degree + 1, n + 1)
+
+    def matrix(self):
+        p = lib().or_spline_problem(self.h)
+        n = self.nbasis
+        # the matrix is factored in place by setup; this returns the factored storage view
+        return p
+
+    @staticmethod
+    def _lay(a, axis_interp):
+        """Strides (in elements) of a 2-D C-contiguous array (rows along axis_interp)."""
+        s = [st // 8 for st in a.strides]
+        if axis_interp == 0:  # [interp][batch]
+            return s[0], a.shape[1], s[1], 0
+        return s[1], a.shape[0], s[0], 0
+
+    def build(self, vals, derivs_l=None, derivs_r=None, layout="batch_contiguous"):
+        """vals: (npts, B) for batch_contiguous or (B, npts) for interp_contiguous.
+        Returns coef in the same layout, with `size` coefficients per line."""
+        ax = 0 if layout == "batch_contiguous" else 1
+        vals = np.ascontiguousarray(vals, dtype=np.float64)
+        B = vals.shape[1 - ax]
+        coef = np.empty((self.size, B) if ax == 0 else (B, self.size))
+        vl, cl = self._lay(vals, ax), self._lay(coef, ax)
+        dlp = drp = None
+        dlay = (1, B, 0, 0)
+        if derivs_l is not None and self.nbv_l:
+            dl = np.ascontiguousarray(derivs_l, dtype=np.float64)
+            self._tmp_dl = dl
+            dlp = dl.ctypes.data
+            dlay = self._lay(dl, ax)
+        if derivs_r is not None and self.nbv_r:
+            dr = np.ascontiguousarray(derivs_r, dtype=np.float64)
+            self._tmp_dr = dr
+            drp = dr.ctypes.data
+            dlay = self._lay(dr, ax)
+        if (self.bc_lower == HERMITE and dlp is None) or (self.bc_upper == HERMITE and drp is None):
+            raise ValueError("Hermite closure needs boundary derivatives")
+        lib().or_spline_build(self.h, vals, *vl, coef, *cl, dlp, drp, *dlay, B)
+        return coef
+
+    def build_vjp(self, gcoef, layout="batch_contiguous"):
+        ax = 0 if layout == "batch_contiguous" else 1
+        gcoef = np.ascontiguousarray(gcoef, dtype=np.float64)
+        B = gcoef.shape[1 - ax]
+        gvals = np.empty((self.npts, B) if ax == 0 else (B, self.npts))
+        gdl = np.zeros((max(self.nbv_l, 1), B) if ax == 0 else (B, max(self.nbv_l, 1)))
+        gdr = np.zeros((max(self.nbv_r, 1), B) if ax == 0 else (B, max(self.nbv_r, 1)))
+        lib().or_spline_build_vjp(self.h, gcoef, *self._lay(gcoef, ax), gvals, *self._lay(gvals, ax),
+                                  gdl.ctypes.data, gdr.ctypes.data, *self._lay(gdl, ax), B)
+        if ax == 0:
+            return gvals, gdl[: self.nbv_l], gdr[: self.nbv_r]
+        return gvals, gdl[:, : self.nbv_l], gdr[:, : self.nbv_r]
+
+    def eval(self, coords, coef, deriv=0, layout="batch_contiguous", extrap=("null", "null")):
+        ax = 0 if layout == "batch_contiguous" else 1
+        coords = np.ascontiguousarray(coords, dtype=np.float64)
+        coef = np.ascontiguousarray(coef, dtype=np.float64)
+        B = coords.shape[1 - ax]
+        P = coords.shape[ax]
+        out = np.empty_like(coords)
+        rules = [0 if e == "null" else 1 for e in extrap]
+        lib().or_spline_eval(self.h, coords, out, *self._lay(coords, ax), P, coef, *self._lay(coef, ax), B,
+                             deriv, rules[0], rules[1])
+        return out
+
+    def eval_vjp(self, coords, w, coef, layout="batch_contiguous", extrap=("null", "null")):
+        ax = 0 if layout == "batch_contiguous" else 1
+        coords = np.ascontiguousarray(coords, dtype=np.float64)
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        coef = np.ascontiguousarray(coef, dtype=np.float64)
+        B = coords.shape[1 - ax]
+        P = coords.shape[ax]
+        gx = np.empty_like(coords)
+        gc = np.empty_like(coef)
+        rules = [0 if e == "null" else 1 for e in extrap]
+        lib().or_spline_eval_vjp(self.h, coords, w, gx, *self._lay(coords, ax), P, coef, gc,
+                                 *self._lay(coef, ax), B, rules[0], rules[1])
+        return gc, gx
+
+    def integrate(self, coef, layout="batch_contiguous"):
+        ax = 0 if layout == "batch_contiguous" else 1
+        coef = np.ascontiguousarray(coef, dtype=np.float64)
+        B = coef.shape[1 - ax]
+        out = np.empty(B)
+        lib().or_spline_integrate(self.h, coef, *self._lay(coef, ax), out, 1, B)
+        return out
+
+    def quadrature_coefficients(self):
+        out = np.empty(self.nbv_l + (self.nbasis - self.nbv_l - self.nbv_r) + self.nbv_r)
+        lib().or_spline_quadrature(self.h, out)
+        return (out[: self.nbv_l], out[self.nbv_l: out.size - self.nbv_r], out[out.size - self.nbv_r:])
+
+
+def spline2d_build(s1: Spline, s2: Spline, vals):
+    """SplineBuilder2D::operator() (spline_builder_2d.hpp:514-651) without Hermite
+    closures: vals (nbatch, n1, n2) -> coef (nbatch, size1, size2)."""
+    vals = np.ascontiguousarray(vals, dtype=np.float64)
+    nb, n1, n2 = vals.shape
+    L = lib()
+    spline1 = np.empty((nb, s1.size, n2))
+    # builder1 along dim 1 batched over (batch, dim2): row stride n2, inner = n2 (stride 1), outer stride n1*n2
+    L.or_spline_build(s1.h, vals, n2, n2, 1, n1 * n2, spline1, n2, n2, 1, s1.size * n2, None, None, 1, 1, 0, 0,
+                      nb * n2)
+    coef = np.empty((nb, s1.size, s2.size))
+    # builder2 along dim 2 batched over (batch, bsplines1)
+    L.or_spline_build(s2.h, spline1, 1, s1.size, n2, s1.size * n2, coef, 1, s1.size, s2.size,
+                      s1.size * s2.size, None, None, 1, 1, 0, 0, nb * s1.size)
+    return coef
+
+
+def spline2d_eval(s1: Spline, s2: Spline, coords, coef, d1=0, d2=0, rules=(0, 0, 0, 0)):
+    """coords (nbatch, npts, 2), coef (nbatch, size1, size2) -> (nbatch, npts)."""
+    coords = np.ascontiguousarray(coords, dtype=np.float64)
+    coef = np.ascontiguousarray(coef, dtype=np.float64)
+    nb, P, _ = coords.shape
+    out = np.empty((nb, P))
+    lib().or_spline2d_eval(s1.h, s2.h, coords, out, P, coef, s1.size * s2.size, s2.size, 1, nb, d1, d2,
+                           np.asarray(rules, dtype=np.int32))
+    return out

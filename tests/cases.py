@@ -1,0 +1,86 @@
+"""Shared spline configurations and synthetic data for the tests.
+
+Every case is built twice: by the oracle (``oracle.Spline``, the CPU
+restatement) and by the product (``ddc_amd``), from the same breakpoints.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+import oracle as O
+
+P, H, HH, G = O.PERIODIC, O.HERMITE, O.HOMOGENEOUS_HERMITE, O.GREVILLE
+BC_NAMES = {P: "P", H: "H", HH: "HH", G: "G"}
+
+
+def breaks_for(kind, ncells, seed=0, ratio=1.05):
+    if kind == "uniform":
+        return np.linspace(0.0, 1.0, ncells + 1)
+    if kind == "random":
+        rng = np.random.default_rng(seed)
+        inner = np.sort(rng.uniform(0.0, 1.0, ncells - 1))
+        return np.concatenate([[0.0], inner, [1.0]])
+    if kind == "geometric":
+        k = np.arange(ncells + 1, dtype=np.float64)
+        return (ratio ** k - 1.0) / (ratio ** ncells - 1.0)
+    raise ValueError(kind)
+
+
+class Case:
+    def __init__(self, degree, uniform, bcl, bcu=None, ncells=10, mesh="random", seed=0, ratio=1.05):
+        self.degree, self.uniform, self.bcl = degree, uniform, bcl
+        self.bcu = bcl if bcu is None else bcu
+        self.ncells, self.mesh, self.seed, self.ratio = ncells, mesh, seed, ratio
+        self.periodic = bcl == P
+        self.breaks = None if uniform else breaks_for(mesh, ncells, seed, ratio)
+
+    @property
+    def id(self):
+        kind = "U" if self.uniform else f"NU{self.mesh[0]}"
+        return f"d{self.degree}-{kind}-{BC_NAMES[self.bcl]}{BC_NAMES[self.bcu]}-n{self.ncells}"
+
+    def oracle(self):
+        if self.uniform:
+            return O.Spline(self.degree, True, self.bcl, self.bcu, ncells=self.ncells, rmin=0.0, rmax=1.0)
+        return O.Spline(self.degree, False, self.bcl, self.bcu, breaks=self.breaks)
+
+    def product(self):
+        import ddc_amd as D
+
+        if self.uniform:
+            bs = D.UniformBSplines(self.degree, 0.0, 1.0, self.ncells, periodic=self.periodic)
+        else:
+            bs = D.NonUniformBSplines(self.degree, self.breaks, periodic=self.periodic)
+        builder = D.SplineBuilder(bs, D.SplineBuilderClosure(self.bcl), D.SplineBuilderClosure(self.bcu))
+        return bs, builder
+
+
+def values(npts, nbatch, seed, x=None):
+    """f(x, y) = sin(2 pi x k_y) + 0.1 N(0, 1), k_y in {1..8} (one line per y)."""
+    rng = np.random.default_rng(seed)
+    if x is None:
+        x = np.linspace(0, 1, npts, endpoint=False)
+    k = rng.integers(1, 9, size=nbatch)
+    f = np.sin(2 * np.pi * x[:, None] * k[None, :]) + 0.1 * rng.standard_normal((npts, nbatch))
+    return f  # (npts, nbatch): batch-contiguous
+
+
+def derivs(nbv, nbatch, seed):
+    rng = np.random.default_rng(seed + 1000)
+    return rng.standard_normal((max(nbv, 0), nbatch))
+
+
+# Cases covering every matrix shape the maker produces for degrees 1..5 and
+# every closure kind (the reference tests degrees 1..5 in CI).
+ALL_CASES = []
+for d in (1, 2, 3, 4, 5):
+    ALL_CASES.append(Case(d, True, P, ncells=16))
+    ALL_CASES.append(Case(d, False, P, ncells=16, mesh="random"))
+    ALL_CASES.append(Case(d, True, G, ncells=16))
+    ALL_CASES.append(Case(d, False, G, ncells=16, mesh="random"))
+    if d >= 2:
+        ALL_CASES.append(Case(d, True, H, ncells=16))
+        ALL_CASES.append(Case(d, False, H, ncells=16, mesh="random"))
+        ALL_CASES.append(Case(d, False, HH, ncells=16, mesh="random"))
+        ALL_CASES.append(Case(d, False, H, G, ncells=16, mesh="random"))
+        ALL_CASES.append(Case(d, True, G, H, ncells=16))

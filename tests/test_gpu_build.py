@@ -1,0 +1,103 @@
+"""GPU parity of SplineBuilder::operator() against the CPU oracle (same inputs).
+
+Bar: relative error (max norm) <= 1e-13 for FP64 builds.
+"""
+import numpy as np
+import pytest
+
+from cases import ALL_CASES, Case, G, H, HH, P, derivs, values
+from conftest import rel_err
+
+TOL = 1e-13
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(case, nbatch, layout, cuda, seed=0):
+    import torch
+
+    orc = case.oracle()
+    bs, builder = case.product()
+    npts = orc.npts
+    assert builder.npoints() == npts
+    assert np.array_equal(builder.interpolation_points(), orc.points())
+    f = values(npts, nbatch, seed, x=orc.points())
+    dl = derivs(orc.nbv_l, nbatch, seed) if case.bcl == H else None
+    dr = derivs(orc.nbv_r, nbatch, seed + 7) if case.bcu == H else None
+    if layout == "batch_contiguous":
+        ref = orc.build(f, dl, dr, layout=layout)
+        tv = torch.from_numpy(f).to(cuda)
+        coef = torch.full((orc.size, nbatch), np.nan, dtype=torch.float64, device=cuda)
+        tdl = torch.from_numpy(dl).to(cuda) if dl is not None else None
+        tdr = torch.from_numpy(dr).to(cuda) if dr is not None else None
+        builder(coef, tv, tdl, tdr, axis=0)
+    else:
+        ref = orc.build(f.T, None if dl is None else dl.T, None if dr is None else dr.T, layout=layout)
+        tv = torch.from_numpy(np.ascontiguousarray(f.T)).to(cuda)
+        coef = torch.full((nbatch, orc.size), np.nan, dtype=torch.float64, device=cuda)
+        tdl = torch.from_numpy(np.ascontiguousarray(dl.T)).to(cuda) if dl is not None else None
+        tdr = torch.from_numpy(np.ascontiguousarray(dr.T)).to(cuda) if dr is not None else None
+        builder(coef, tv, tdl, tdr, axis=1)
+    torch.cuda.synchronize()
+    got = coef.cpu().numpy()
+    assert np.all(np.isfinite(got)), "coefficients not fully written"
+    return got, ref
+
+
+@pytest.mark.parametrize("case", ALL_CASES, ids=lambda c: c.id)
+@pytest.mark.parametrize("layout", ["batch_contiguous", "interp_contiguous"])
+def test_build_matches_oracle(case, layout, cuda):
+    got, ref = _run(case, 37, layout, cuda)
+    assert rel_err(got, ref) <= TOL
+
+
+@pytest.mark.parametrize("nbatch", [1, 63, 64, 65, 1000])
+def test_build_ragged_batches(nbatch, cuda):
+    got, ref = _run(Case(3, True, P, ncells=100), nbatch, "batch_contiguous", cuda)
+    assert rel_err(got, ref) <= TOL
+
+
+def test_build_empty_batch(cuda):
+    import torch
+
+    case = Case(3, True, P, ncells=16)
+    bs, builder = case.product()
+    coef = torch.empty((case.ncells + 3, 0), dtype=torch.float64, device=cuda)
+    vals = torch.empty((case.ncells, 0), dtype=torch.float64, device=cuda)
+    builder(coef, vals)  # no-op, spline_builder.hpp:929-931
+    torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("order", ["XB1B2", "B1XB2", "B1B2X"])
+def test_build_two_batch_dims(order, cuda):
+    """Batch-layout permutations of the reference tests (batched_spline_builder.cpp)."""
+    import torch
+
+    case = Case(3, False, G, ncells=12, mesh="random")
+    orc = case.oracle()
+    bs, builder = case.product()
+    n1, n2, m = 5, 7, orc.npts
+    rng = np.random.default_rng(3)
+    f = rng.standard_normal((m, n1 * n2))
+    ref = orc.build(f)  # (size, n1*n2), lines ordered (b1, b2)
+    f3 = f.reshape(m, n1, n2)
+    if order == "XB1B2":
+        fv, axis = f3, 0
+    elif order == "B1XB2":
+        fv, axis = np.transpose(f3, (1, 0, 2)), 1
+    else:
+        fv, axis = np.transpose(f3, (1, 2, 0)), 2
+    tv = torch.from_numpy(np.ascontiguousarray(fv)).to(cuda)
+    shape = list(tv.shape)
+    shape[axis] = orc.size
+    coef = torch.full(shape, np.nan, dtype=torch.float64, device=cuda)
+    builder(coef, tv, axis=axis)
+    got = coef.cpu().numpy()
+    got = np.moveaxis(got, axis, 0).reshape(orc.size, n1 * n2)
+    assert rel_err(got, ref) <= TOL
+
+
+def test_build_config1_cpu_case(cuda):
+    """Uniform periodic cubic, 1000 cells x 1000 lines, interpolation-contiguous."""
+    got, ref = _run(Case(3, True, P, ncells=1000), 1000, "interp_contiguous", cuda, seed=0xDDC0001)
+    assert rel_err(got, ref) <= TOL

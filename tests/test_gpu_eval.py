@@ -1,0 +1,123 @@
+"""GPU parity of SplineEvaluator (operator(), deriv(), integrate()) against the oracle.
+
+Bars: cell indices are bit-exact (checked through values at break points and
+near cell edges); values <= 1e-13 relative error (max norm).
+"""
+import numpy as np
+import pytest
+
+from cases import ALL_CASES, Case, G, H, P, values
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-13
+
+
+def _coef(case, nbatch, seed=0):
+    orc = case.oracle()
+    rng = np.random.default_rng(seed)
+    coef = rng.standard_normal((orc.size, nbatch))
+    if case.periodic:  # periodic coefficients repeat the first `degree` ones
+        coef[orc.nbasis:] = coef[: case.degree]
+    return orc, coef
+
+
+def _points(orc, npts, nbatch, seed, outside=0.0):
+    rng = np.random.default_rng(seed)
+    lo, hi = orc.rmin, orc.rmax
+    L = hi - lo
+    x = rng.uniform(lo - outside * L, hi + outside * L, (npts, nbatch))
+    # exact break points and their neighbours exercise the cell-edge rounding
+    br = orc.knots()[orc.degree: orc.degree + orc.ncells + 1]
+    k = min(npts, br.size)
+    x[:k, 0] = br[:k]
+    x[:k, -1] = np.nextafter(br[:k], np.inf)
+    x[:k, nbatch // 2] = np.nextafter(br[:k], -np.inf)
+    return x
+
+
+@pytest.mark.parametrize("case", ALL_CASES, ids=lambda c: c.id)
+@pytest.mark.parametrize("layout", ["batch_contiguous", "interp_contiguous"])
+def test_eval_and_deriv(case, layout, cuda):
+    import torch
+    import ddc_amd as D
+
+    orc, coef = _coef(case, 9)
+    bs, _ = case.product()
+    x = _points(orc, 50, 9, 1, outside=0.2)
+    rules = ("constant", "null")
+    ev = D.SplineEvaluator(bs, D.ConstantExtrapolationRule(), D.NullExtrapolationRule())
+    for order in range(0, case.degree + 1):
+        if order > 0:
+            xin = np.clip(x, orc.rmin, orc.rmax)
+        else:
+            xin = x
+        ref = orc.eval(xin, coef, deriv=order, extrap=rules)
+        if layout == "batch_contiguous":
+            tx = torch.from_numpy(xin).to(cuda)
+            tc = torch.from_numpy(coef).to(cuda)
+            out = torch.empty_like(tx)
+            (ev(out, tx, tc) if order == 0 else ev.deriv(order, out, tx, tc))
+            got = out.cpu().numpy()
+        else:
+            tx = torch.from_numpy(np.ascontiguousarray(xin.T)).to(cuda)
+            tc = torch.from_numpy(np.ascontiguousarray(coef.T)).to(cuda)
+            out = torch.empty_like(tx)
+            (ev(out, tx, tc, axis=1) if order == 0 else ev.deriv(order, out, tx, tc, axis=1))
+            got = out.cpu().numpy().T
+        scale = max(1.0, np.abs(ref).max())
+        assert np.abs(got - ref).max() / scale <= TOL * (10 ** order), f"order {order}"
+
+
+@pytest.mark.parametrize("case", [c for c in ALL_CASES if c.degree in (3, 5)], ids=lambda c: c.id)
+def test_integrate(case, cuda):
+    import torch
+    import ddc_amd as D
+
+    orc, coef = _coef(case, 11, seed=2)
+    bs, _ = case.product()
+    ev = D.SplineEvaluator(bs)
+    ref = orc.integrate(coef)
+    out = torch.empty(11, dtype=torch.float64, device=cuda)
+    ev.integrate(out, torch.from_numpy(coef).to(cuda))
+    assert rel_err(out.cpu().numpy(), ref) <= TOL
+
+
+def test_build_then_eval_roundtrip(cuda):
+    """Evaluating the built spline at the interpolation points returns the values."""
+    import torch
+    import ddc_amd as D
+
+    case = Case(3, False, G, ncells=200, mesh="random")
+    bs, builder = case.product()
+    x = builder.interpolation_points()
+    f = values(x.size, 64, 5, x=x)
+    tv = torch.from_numpy(f).to(cuda)
+    coef = torch.empty((bs.size(), 64), dtype=torch.float64, device=cuda)
+    builder(coef, tv)
+    ev = D.SplineEvaluator(bs)
+    tx = torch.from_numpy(np.repeat(x[:, None], 64, axis=1)).to(cuda)
+    out = torch.empty_like(tx)
+    ev(out, tx, coef)
+    assert rel_err(out.cpu().numpy(), f) <= 1e-13
+
+
+def test_periodicity(cuda):
+    """periodicity_spline_builder.cpp:170-202: f(x + 1.5) on an even function."""
+    import torch
+    import ddc_amd as D
+
+    case = Case(3, True, P, ncells=64)
+    orc = case.oracle()
+    bs, builder = case.product()
+    x = builder.interpolation_points()
+    f = np.cos(2 * np.pi * x)[:, None]
+    coef = torch.empty((bs.size(), 1), dtype=torch.float64, device=cuda)
+    builder(coef, torch.from_numpy(f).to(cuda))
+    ev = D.SplineEvaluator(bs)
+    xe = np.linspace(0, 1, 101)[:, None] + 1.5
+    out = torch.empty((101, 1), dtype=torch.float64, device=cuda)
+    ev(out, torch.from_numpy(xe).to(cuda), coef)
+    ref = orc.eval(xe, coef.cpu().numpy())
+    assert rel_err(out.cpu().numpy(), ref) <= TOL
+    assert np.abs(out.cpu().numpy()[:, 0] + np.cos(2 * np.pi * (xe[:, 0] - 1.5))).max() < 1e-5
