@@ -34,7 +34,8 @@ class ddcx_builder_desc(C.Structure):
 class ddcx_builder_info(C.Structure):
     _fields_ = [("npoints", C.c_int), ("nbasis", C.c_int), ("size", C.c_int), ("offset", C.c_int),
                 ("nbv_lower", C.c_int), ("nbv_upper", C.c_int), ("matrix_kind", C.c_int), ("blocked", C.c_int),
-                ("main_size", C.c_int), ("border", C.c_int), ("dx", C.c_double)]
+                ("main_size", C.c_int), ("border", C.c_int), ("dx", C.c_double), ("kernel", C.c_int),
+                ("pivoted", C.c_int)]
 
 
 # name -> (argtypes) ; all return ddcx_status (int) unless listed in _RESTYPES

@@ -304,6 +304,7 @@ void launch_build(BuilderPlan const& p, double* coef, Layout cl, const double* v
                   const double* dr, Layout dlay, int64_t nbatch, hipStream_t stream)
 {
     p.ensure_device();
+    if (launch_build_tiled(p, coef, cl, vals, vl, dl, dr, dlay, nbatch, stream)) return;
     launch_build_generic(p, coef, cl, vals, vl, dl, dr, dlay, nbatch, stream);
 }
 

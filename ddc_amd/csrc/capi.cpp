@@ -119,6 +119,10 @@ ddcx_status ddcx_builder_get_info(const ddcx_builder* h, ddcx_builder_info* info
         info->main_size = p.prob.q.n;
         info->border = p.prob.nb();
         info->dx = p.dx;
+        info->kernel = p.tiled.ok ? 1 : 0;
+        info->pivoted = 0;
+        for (size_t j = 0; j < p.prob.q.ipiv.size(); ++j)
+            if (p.prob.q.ipiv[j] != int(j) && p.prob.q.type == ddcx::QType::Band) info->pivoted = 1;
     });
 }
 

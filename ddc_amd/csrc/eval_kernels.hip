@@ -242,6 +242,7 @@ void launch_eval(BSplinesPlan const& bs, int lower, int upper, int order, double
 {
     if (npts * nbatch == 0) return;
     bs.ensure_device();
+    if (launch_eval_tiled(bs, lower, upper, order, out, coords, pl, npts, coef, cl, nbatch, stream)) return;
     BSplinesView const b = bs.device_view();
     bool const ok = dispatch_degree(b.degree, [&]<int D>() {
         eval_order<D>(order, b, lower, upper, out, coords, pl, npts, coef, cl, nbatch, stream);

@@ -1,0 +1,144 @@
+// LDS-staged batched evaluation for batch-contiguous layouts
+// (SplineEvaluator::operator() / deriv(), spline_evaluator.hpp:257-430).
+//
+// In a batch-contiguous layout the coefficients of one line are spread over
+// `size()` cache lines, and random evaluation points gather them at random.
+// A workgroup therefore takes LB consecutive lines, stages their
+// coefficient tile (size() x LB, coalesced rows of LB*8 bytes) and, for
+// non-uniform bases, the knot vector in LDS, and then evaluates all the
+// points of those lines with coalesced coordinate loads and output stores.
+// HBM sees every coefficient, coordinate and output once.
+#include "internal.hpp"
+
+namespace ddcx {
+
+namespace {
+
+__device__ __forceinline__ int64_t lay(Layout const& L, int64_t r, int64_t l)
+{
+    return r * L.rs + (l % L.inner) * L.is + (l / L.inner) * L.os;
+}
+
+template <int D, int ORDER>
+__device__ __forceinline__ double point_value(BSplinesView const& b, const double* kn, int lower, int upper, double x,
+                                              const double* tile, int pitch)
+{
+    double v[D + 1];
+    int jmin;
+    if constexpr (ORDER == 0) {
+        if (b.periodic) {
+            if (x < b.rmin || x > b.rmax) x -= floor((x - b.rmin) / b.length) * b.length;
+        } else if (x < b.rmin) {
+            if (lower == DDCX_EXTRAP_NULL) return 0.0;
+            x = b.rmin;
+        } else if (x > b.rmax) {
+            if (upper == DDCX_EXTRAP_NULL) return 0.0;
+            x = b.rmax;
+        }
+        jmin = eval_basis<D>(b, kn, x, v);
+    } else {
+        double der[(D + 1) * (ORDER + 1)];
+        jmin = eval_basis_and_n_derivs<D, ORDER>(b, kn, x, der);
+#pragma unroll
+        for (int i = 0; i < D + 1; ++i) v[i] = der[i * (ORDER + 1) + ORDER];
+    }
+    double y = 0.0;
+#pragma unroll
+    for (int i = 0; i < D + 1; ++i) y += tile[(jmin + i) * pitch] * v[i];
+    return y;
+}
+
+template <int D, int ORDER, int LB, int NT>
+__global__ void __launch_bounds__(NT) eval_tiled_kernel(BSplinesView b, int lower, int upper, double* __restrict__ out,
+                                                        const double* __restrict__ coords, Layout pl, int64_t npts,
+                                                        const double* __restrict__ coef, Layout cl, int64_t nbatch,
+                                                        int ncoef, int pitch, int nknots)
+{
+    extern __shared__ double smem[];
+    double* tile = smem;
+    double* kn = smem + size_t(ncoef) * pitch;
+    constexpr int ROWS = NT / LB;
+    constexpr int U = 4; // points in flight per thread
+    int const tid = threadIdx.x;
+    int const ln = tid % LB;
+    int const prow = tid / LB;
+    int64_t const line = int64_t(blockIdx.x) * LB + ln;
+    bool const active = line < nbatch;
+    int64_t const lc = active ? line : 0;
+    int64_t const cb = lay(cl, 0, lc);
+    for (int r = prow; r < ncoef; r += ROWS) tile[r * pitch + ln] = active ? coef[cb + int64_t(r) * cl.rs] : 0.0;
+    if (!b.uniform)
+        for (int k = tid; k < nknots; k += NT) kn[k] = b.knots[k];
+    __syncthreads();
+    if (!active) return;
+    const double* knots = b.uniform ? nullptr : kn;
+    const double* tl = tile + ln;
+    int64_t const pb = lay(pl, 0, lc);
+    int64_t i = prow;
+    for (; i + (U - 1) * ROWS < npts; i += U * ROWS) {
+        double x[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) x[u] = coords[pb + (i + u * ROWS) * pl.rs];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            out[pb + (i + u * ROWS) * pl.rs] = point_value<D, ORDER>(b, knots, lower, upper, x[u], tl, pitch);
+    }
+    for (; i < npts; i += ROWS) {
+        double const x = coords[pb + i * pl.rs];
+        out[pb + i * pl.rs] = point_value<D, ORDER>(b, knots, lower, upper, x, tl, pitch);
+    }
+}
+
+constexpr size_t kMaxLds = 160 * 1024;
+
+template <int D, int ORDER>
+bool launch_orders(BSplinesPlan const& bs, int lower, int upper, double* out, const double* coords, Layout pl,
+                   int64_t npts, const double* coef, Layout cl, int64_t nbatch, hipStream_t stream)
+{
+    int const ncoef = bs.size();
+    int const nknots = bs.view.uniform ? 0 : int(bs.knots_host.size());
+    BSplinesView const b = bs.device_view();
+    auto fits = [&](int lb, int pitch) { return (size_t(ncoef) * pitch + nknots) * sizeof(double) <= kMaxLds; };
+    auto go = [&]<int LB>(int pitch) {
+        constexpr int NT = 1024;
+        size_t const lds = (size_t(ncoef) * pitch + nknots) * sizeof(double);
+        auto kern = eval_tiled_kernel<D, ORDER, LB, NT>;
+        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)),
+                  "hipFuncSetAttribute");
+        int64_t const blocks = (nbatch + LB - 1) / LB;
+        hipLaunchKernelGGL(kern, dim3(unsigned(blocks)), dim3(NT), lds, stream, b, lower, upper, out, coords, pl, npts,
+                           coef, cl, nbatch, ncoef, pitch, nknots);
+        hip_check(hipGetLastError(), "eval_tiled_kernel launch");
+        return true;
+    };
+    if (fits(16, 17)) return go.template operator()<16>(17);
+    if (fits(16, 16)) return go.template operator()<16>(16);
+    if (fits(8, 9)) return go.template operator()<8>(9);
+    if (fits(8, 8)) return go.template operator()<8>(8);
+    return false;
+}
+
+} // namespace
+
+bool launch_eval_tiled(BSplinesPlan const& bs, int lower, int upper, int order, double* out, const double* coords,
+                       Layout pl, int64_t npts, const double* coef, Layout cl, int64_t nbatch, hipStream_t stream)
+{
+    if (pl.rs == 1 || npts < 64 || nbatch < 8) return false; // interpolation-contiguous or tiny: generic kernel
+    bool launched = false;
+    dispatch_degree(bs.view.degree, [&]<int D>() {
+        auto ord = [&]<int O>() {
+            if constexpr (O <= D) {
+                if (!launched && order == O)
+                    launched = launch_orders<D, O>(bs, lower, upper, out, coords, pl, npts, coef, cl, nbatch, stream);
+            }
+        };
+        ord.template operator()<0>();
+        ord.template operator()<1>();
+        ord.template operator()<2>();
+        ord.template operator()<3>();
+    });
+    return launched;
+}
+
+} // namespace ddcx

@@ -538,6 +538,39 @@ void BuilderPlan::ensure_device() const
         sv.bl_r = d_blr.as<int>();
         sv.bl_c = d_blc.as<int>();
         sv.bl_v = d_blv.as<double>();
+        if (tiled.ok) {
+            t_Lf.upload(tiled.Lf);
+            t_Df.upload(tiled.Df);
+            t_Ub.upload(tiled.Ub);
+            t_Db.upload(tiled.Db);
+            t_Mf.upload(tiled.Mf);
+            t_Mb.upload(tiled.Mb);
+            t_G.upload(tiled.G);
+            TiledView& tv = tview;
+            tv.K = tiled.K;
+            tv.L = tiled.L;
+            tv.kf = tiled.kf;
+            tv.kb = tiled.kb;
+            tv.wf = tiled.wf;
+            tv.wb = tiled.wb;
+            tv.nq = prob.q.n;
+            tv.nb = prob.nb();
+            tv.top = prob.top;
+            tv.q0 = bs->view.periodic ? 0 : prob.top - nbe_l;
+            tv.bl_nnz = int(prob.bl_v.size());
+            tv.Lf = t_Lf.as<double>();
+            tv.Df = t_Df.as<double>();
+            tv.Ub = t_Ub.as<double>();
+            tv.Db = t_Db.as<double>();
+            tv.Mf = t_Mf.as<double>();
+            tv.Mb = t_Mb.as<double>();
+            tv.G = t_G.as<double>();
+            tv.bl_r = sv.bl_r;
+            tv.bl_c = sv.bl_c;
+            tv.bl_v = sv.bl_v;
+            tv.s_lu = sv.s_lu;
+            tv.s_ipiv = sv.s_ipiv;
+        }
     });
 }
 
@@ -818,6 +851,7 @@ std::unique_ptr<BuilderPlan> make_builder(std::shared_ptr<BSplinesPlan> bs, ddcx
         for (int j = 0; j < k; ++j) r *= p->dx;
         bv.dxpow[k] = r;
     }
+    make_tiled(*p);
     return p;
 }
 

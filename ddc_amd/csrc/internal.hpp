@@ -141,6 +141,41 @@ struct BuildView {
     double dxpow[kMaxDegree + 1]; // dx^k, k = 0..degree
 };
 
+// Plan of the register-resident tiled build kernel (build_tiled.hip).  The
+// main block is swept as K segments of L rows per line; every sweep is in
+// row-oriented form (same operations and order as the netlib column-oriented
+// sweeps the reference runs) so that the state crossing a segment boundary is
+// just the last kf solution values (forward) or the next kb ones (backward).
+// Segment carries are exact: in(s) = sum_{w=1..W} P_{s,w} out_loc(s-w), with
+// the batch-independent transfer products P precomputed here and W chosen so
+// that the first neglected product is below 2^-60.
+struct TiledHost {
+    bool ok = false;
+    std::string why; // reason when !ok
+    int K = 0, L = 0, kf = 0, kb = 0, wf = 0, wb = 0;
+    int fwd_div = 0;   // forward rows divide by Df (PDSBand)
+    int bwd_divfirst = 0; // backward row form y/d - x e (PDSTridiag) instead of (y - sum)/d
+    std::vector<double> Lf, Df, Ub, Db; // padded to K*L rows
+    std::vector<double> Mf, Mb;         // K x W x k x k
+    std::vector<double> G;              // (K*L) x nb: Q^-1 gamma with |a| < 1e-14 zeroed
+};
+
+struct TiledView {
+    int K, L, kf, kb, wf, wb, nq, nb, top, q0, bl_nnz;
+    const double* Lf;
+    const double* Df;
+    const double* Ub;
+    const double* Db;
+    const double* Mf;
+    const double* Mb;
+    const double* G;
+    const int* bl_r;
+    const int* bl_c;
+    const double* bl_v;
+    const double* s_lu;
+    const int* s_ipiv;
+};
+
 struct BuilderPlan {
     std::shared_ptr<BSplinesPlan> bs;
     ddcx_closure bcl, bcu;
@@ -153,9 +188,14 @@ struct BuilderPlan {
     HostProblem prob;
     mutable DeviceBuffer d_q, d_ipiv, d_slu, d_sipiv, d_trr, d_trc, d_trv, d_blr, d_blc, d_blv;
     mutable BuildView view{}; // device pointers filled by ensure_device()
+    TiledHost tiled;
+    mutable DeviceBuffer t_Lf, t_Df, t_Ub, t_Db, t_Mf, t_Mb, t_G;
+    mutable TiledView tview{};
     mutable std::once_flag uploaded;
     void ensure_device() const;
 };
+
+void make_tiled(BuilderPlan& p);
 
 struct EvaluatorPlan {
     std::shared_ptr<BSplinesPlan> bs;
@@ -179,10 +219,14 @@ std::unique_ptr<BuilderPlan> make_builder(std::shared_ptr<BSplinesPlan> bs, ddcx
 // kernel launchers (build_generic.hip / build_tiled.hip / eval_kernels.hip)
 void launch_build_generic(BuilderPlan const& p, double* coef, Layout cl, const double* vals, Layout vl,
                           const double* dl, const double* dr, Layout dlay, int64_t nbatch, hipStream_t stream);
+bool launch_build_tiled(BuilderPlan const& p, double* coef, Layout cl, const double* vals, Layout vl,
+                        const double* dl, const double* dr, Layout dlay, int64_t nbatch, hipStream_t stream);
 void launch_build(BuilderPlan const& p, double* coef, Layout cl, const double* vals, Layout vl, const double* dl,
                   const double* dr, Layout dlay, int64_t nbatch, hipStream_t stream);
 void launch_build_vjp(BuilderPlan const& p, const double* gcoef, Layout cl, double* gvals, Layout vl, double* gdl,
                       double* gdr, Layout dlay, int64_t nbatch, hipStream_t stream);
+bool launch_eval_tiled(BSplinesPlan const& bs, int lower, int upper, int order, double* out, const double* coords,
+                       Layout pl, int64_t npts, const double* coef, Layout cl, int64_t nbatch, hipStream_t stream);
 void launch_eval(BSplinesPlan const& bs, int lower, int upper, int order, double* out, const double* coords,
                  Layout pl, int64_t npts, const double* coef, Layout cl, int64_t nbatch, hipStream_t stream);
 void launch_eval_vjp(BSplinesPlan const& bs, int lower, int upper, const double* coords, const double* w,

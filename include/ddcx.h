@@ -112,6 +112,8 @@ typedef struct ddcx_builder_info {
     int main_size;    /* size of the band main block */
     int border;       /* nb = n - main_size */
     double dx;        /* m_dx, average cell size */
+    int kernel;       /* build kernel used: 0 generic (one thread per line), 1 tiled (register-resident) */
+    int pivoted;      /* 1 if the band factorisation of the main block interchanged rows */
 } ddcx_builder_info;
 
 ddcx_status ddcx_bsplines_create(const ddcx_bsplines_desc* desc, ddcx_bsplines** out);

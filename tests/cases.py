@@ -84,3 +84,24 @@ for d in (1, 2, 3, 4, 5):
         ALL_CASES.append(Case(d, False, HH, ncells=16, mesh="random"))
         ALL_CASES.append(Case(d, False, H, G, ncells=16, mesh="random"))
         ALL_CASES.append(Case(d, True, G, H, ncells=16))
+
+# Larger cases that take the register-resident tiled build kernel (main block
+# of 256..2048 rows): every sweep shape (tridiagonal, PDS band, unpivoted band)
+# and both segment lengths.
+TILED_CASES = [
+    Case(3, True, P, ncells=1024),
+    Case(5, True, P, ncells=1024),
+    Case(2, True, P, ncells=600),
+    Case(4, True, G, ncells=700),
+    Case(3, True, H, ncells=300),
+    Case(3, False, G, ncells=1024, mesh="random"),
+    Case(2, False, G, ncells=500, mesh="random"),
+    Case(4, False, G, ncells=400, mesh="random"),
+    Case(5, False, G, ncells=1024, mesh="random", seed=3),
+    Case(3, False, P, ncells=800, mesh="random"),
+    Case(3, False, H, ncells=2048, mesh="geometric", ratio=1.002),
+    Case(5, False, H, ncells=2048, mesh="geometric", ratio=1.002),
+    Case(3, False, H, ncells=1500, mesh="random"),
+    Case(5, False, HH, ncells=1200, mesh="uniform"),
+    Case(3, True, P, ncells=1800),
+]

@@ -101,3 +101,21 @@ def test_build_config1_cpu_case(cuda):
     """Uniform periodic cubic, 1000 cells x 1000 lines, interpolation-contiguous."""
     got, ref = _run(Case(3, True, P, ncells=1000), 1000, "interp_contiguous", cuda, seed=0xDDC0001)
     assert rel_err(got, ref) <= TOL
+
+
+@pytest.mark.parametrize("case", __import__("cases").TILED_CASES, ids=lambda c: c.id)
+@pytest.mark.parametrize("layout", ["batch_contiguous", "interp_contiguous"])
+def test_build_tiled_matches_oracle(case, layout, cuda):
+    bs, builder = case.product()
+    got, ref = _run(case, 45, layout, cuda, seed=11)
+    assert rel_err(got, ref) <= TOL, f"kernel={builder.info.kernel}"
+
+
+def test_tiled_kernel_selected():
+    """The hot configurations run the register-resident kernel."""
+    from cases import TILED_CASES
+
+    for case in TILED_CASES:
+        bs, builder = case.product()
+        assert builder.info.pivoted == 0
+        assert builder.info.kernel == 1, case.id

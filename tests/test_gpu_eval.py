@@ -121,3 +121,29 @@ def test_periodicity(cuda):
     ref = orc.eval(xe, coef.cpu().numpy())
     assert rel_err(out.cpu().numpy(), ref) <= TOL
     assert np.abs(out.cpu().numpy()[:, 0] + np.cos(2 * np.pi * (xe[:, 0] - 1.5))).max() < 1e-5
+
+
+@pytest.mark.parametrize("case", [Case(3, True, P, ncells=1024), Case(5, True, P, ncells=300),
+                                  Case(3, False, G, ncells=1024, mesh="random"),
+                                  Case(5, False, H, ncells=2000, mesh="random"),
+                                  Case(4, True, G, ncells=100)], ids=lambda c: c.id)
+def test_eval_tiled_batch_contiguous(case, cuda):
+    """LDS-staged evaluator (batch-contiguous, >= 64 points, >= 8 lines)."""
+    import torch
+    import ddc_amd as D
+
+    nbatch = 37
+    orc, coef = _coef(case, nbatch, seed=4)
+    bs, _ = case.product()
+    x = _points(orc, 300, nbatch, 5, outside=0.1)
+    ev = D.SplineEvaluator(bs, D.ConstantExtrapolationRule(), D.NullExtrapolationRule())
+    tc = torch.from_numpy(coef).to(cuda)
+    for order in range(0, min(case.degree, 3) + 1):
+        xin = x if order == 0 else np.clip(x, orc.rmin, orc.rmax)
+        ref = orc.eval(xin, coef, deriv=order, extrap=("constant", "null"))
+        tx = torch.from_numpy(xin).to(cuda)
+        out = torch.empty_like(tx)
+        (ev(out, tx, tc) if order == 0 else ev.deriv(order, out, tx, tc))
+        got = out.cpu().numpy()
+        scale = max(1.0, np.abs(ref).max())
+        assert np.abs(got - ref).max() / scale <= TOL * (10 ** order), f"order {order}"
