@@ -137,6 +137,9 @@ def main():
     n_k = max(5, args.steps)
     t_build = timed(lambda: builder(coef, vals), n_k)
     t_eval = timed(lambda: ev(out, coords, coef), n_k)
+    # calibration on this box: a plain device copy of the same size as vals
+    t_copy = timed(lambda: out.copy_(vals), n_k)
+    copy_gbps = 2 * vals.numel() * 8 / t_copy / 1e9
     b_build, b_eval = algorithmic_bytes(m, B, ncoef, P)
     dominant = "build" if t_build >= t_eval else "eval"
     t_dom = max(t_build, t_eval)
@@ -160,7 +163,8 @@ def main():
                                f"(batch-contiguous), {P} eval points", "ncells": nc, "batch": B,
                    "eval_points": P, "precision": "fp64"},
         "kernels": {"build_us": round(t_build * 1e6, 2), "eval_us": round(t_eval * 1e6, 2),
-                    "build_GBps": round(b_build / t_build / 1e9, 1), "eval_GBps": round(b_eval / t_eval / 1e9, 1)},
+                    "build_GBps": round(b_build / t_build / 1e9, 1), "eval_GBps": round(b_eval / t_eval / 1e9, 1),
+                    "device_copy_GBps": round(copy_gbps, 1)},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic},
         "cpu_baseline": None,

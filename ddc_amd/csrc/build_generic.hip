@@ -308,10 +308,175 @@ void launch_build(BuilderPlan const& p, double* coef, Layout cl, const double* v
     launch_build_generic(p, coef, cl, vals, vl, dl, dr, dlay, nbatch, stream);
 }
 
-void launch_build_vjp(BuilderPlan const&, const double*, Layout, double*, Layout, double*, double*, Layout, int64_t,
-                      hipStream_t)
+namespace {
+
+// ---------------------------------------------------------------------------
+// Adjoint of the build: the transposed solve of the reference
+// (SplinesLinearProblem::solve(b, true); 2x2_blocks.cpp:292-297 order), one
+// thread per line, working in place in the gradient-of-values array.
+// ---------------------------------------------------------------------------
+
+struct VjpIO {
+    BuildView const& bv;
+    const double* gcoef;
+    Layout cl;
+    int64_t cb;
+    double* gvals;
+    int64_t vrs, vb;
+    int q0;
+    // folded gradient of the physical unknown i: the sum over every
+    // coefficient row j with (j - offset) mod nbasis == i, in increasing j
+    __device__ double fold(int i) const
+    {
+        if (!bv.periodic) return gcoef[cb + int64_t(i) * cl.rs];
+        int const j1 = i + bv.offset, n = bv.nbasis, N = bv.size;
+        double acc = 0.0;
+        if (j1 - n >= 0) acc += gcoef[cb + int64_t(j1 - n) * cl.rs];
+        acc += gcoef[cb + int64_t(j1) * cl.rs];
+        if (j1 + n < N) acc += gcoef[cb + int64_t(j1 + n) * cl.rs];
+        return acc;
+    }
+    __device__ double& w(int r) const { return gvals[vb + int64_t(q0 + r) * vrs]; }
+};
+
+__device__ void getrs_trans_local(int n, const double* a, const int* ipiv, double* b)
 {
-    throw Error(DDCX_ERR_UNSUPPORTED, "build_vjp not available");
+    for (int j = 0; j < n; ++j) {
+        double temp = b[j];
+        for (int i = 0; i < j; ++i) temp = temp - a[i * n + j] * b[i];
+        b[j] = temp / a[j * n + j];
+    }
+    for (int j = n - 1; j >= 0; --j) {
+        double temp = b[j];
+        for (int i = j + 1; i < n; ++i) temp = temp - a[i * n + j] * b[i];
+        b[j] = temp;
+    }
+    for (int j = n - 1; j >= 0; --j)
+        if (ipiv[j] != j) {
+            double const tmp = b[ipiv[j]];
+            b[ipiv[j]] = b[j];
+            b[j] = tmp;
+        }
+}
+
+// Q^T solve in place on the main-block rows (PDS types are symmetric: the
+// reference ignores `transpose` for them).
+__device__ void q_solve_trans(SolveView const& sv, VjpIO const& io)
+{
+    int const n = sv.nq;
+    switch (QType(sv.qtype)) {
+    case QType::PDSTridiag: {
+        const double* d = sv.q;
+        const double* e = sv.q + n;
+        if (n == 1) {
+            io.w(0) = io.w(0) * (1.0 / d[0]);
+            return;
+        }
+        for (int i = 1; i < n; ++i) io.w(i) = io.w(i) - io.w(i - 1) * e[i - 1];
+        io.w(n - 1) = io.w(n - 1) / d[n - 1];
+        for (int i = n - 2; i >= 0; --i) io.w(i) = io.w(i) / d[i] - io.w(i + 1) * e[i];
+        return;
+    }
+    case QType::PDSBand: {
+        int const k = sv.kl;
+        const double* q = sv.q;
+        for (int j = 0; j < n; ++j) {
+            double const xj = io.w(j) / q[j];
+            io.w(j) = xj;
+            for (int i = j + 1; i <= min(n - 1, j + k); ++i) io.w(i) = io.w(i) - xj * q[size_t(i - j) * n + j];
+        }
+        for (int j = n - 1; j >= 0; --j) {
+            double temp = io.w(j);
+            for (int i = min(n - 1, j + k); i >= j + 1; --i) temp = temp - q[size_t(i - j) * n + j] * io.w(i);
+            io.w(j) = temp / q[j];
+        }
+        return;
+    }
+    case QType::Band: {
+        int const kl = sv.kl, kd = sv.kl + sv.ku;
+        const double* q = sv.q;
+        auto AB = [&](int r, int c) { return q[size_t(r) * n + c]; };
+        for (int j = 0; j < n; ++j) { // dtbsv('U','T','N')
+            double temp = io.w(j);
+            for (int i = max(0, j - kd); i <= j - 1; ++i) temp = temp - AB(kd + i - j, j) * io.w(i);
+            io.w(j) = temp / AB(kd, j);
+        }
+        for (int j = n - 2; j >= 0; --j) { // L^T with the interchanges in reverse
+            int const lm = min(kl, n - 1 - j);
+            double temp = 0.0;
+            for (int i = 0; i < lm; ++i) temp = temp + AB(kd + 1 + i, j) * io.w(j + 1 + i);
+            io.w(j) = io.w(j) + -1.0 * temp;
+            int const l = sv.ipiv[j];
+            if (l != j) {
+                double const t = io.w(l);
+                io.w(l) = io.w(j);
+                io.w(j) = t;
+            }
+        }
+        return;
+    }
+    case QType::Dense: {
+        double b[kMaxDenseQ];
+        for (int i = 0; i < n; ++i) b[i] = io.w(i);
+        getrs_trans_local(n, sv.q, sv.ipiv, b);
+        for (int i = 0; i < n; ++i) io.w(i) = b[i];
+        return;
+    }
+    }
+}
+
+__global__ void __launch_bounds__(256) build_vjp_generic_kernel(BuildView bv, const double* __restrict__ gcoef,
+                                                                Layout cl, double* __restrict__ gvals, Layout vl,
+                                                                double* __restrict__ gdl, double* __restrict__ gdr,
+                                                                Layout dlay, int64_t nbatch)
+{
+    int64_t const l = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (l >= nbatch) return;
+    SolveView const& sv = bv.sv;
+    int const nq = sv.nq, nb = sv.nb, t = sv.top;
+    int const q0 = bv.periodic ? 0 : t - bv.nbe_l;
+    VjpIO io{bv, gcoef, cl, lay(cl, 0, l), gvals, vl.rs, lay(vl, 0, l), q0};
+    for (int r = 0; r < nq; ++r) io.w(r) = io.fold(t + r);
+    double b2[kMaxBorder];
+    for (int k = 0; k < nb; ++k) b2[k] = io.fold(k < t ? k : nq + k);
+    if (nb > 0) {
+        for (int z = 0; z < sv.tr_nnz; ++z) b2[sv.tr_c[z]] -= sv.tr_v[z] * io.w(sv.tr_r[z]);
+        getrs_trans_local(nb, sv.s_lu, sv.s_ipiv, b2);
+        for (int z = 0; z < sv.bl_nnz; ++z) io.w(sv.bl_c[z]) -= sv.bl_v[z] * b2[sv.bl_r[z]];
+    }
+    q_solve_trans(sv, io);
+    // border rows back to the value / derivative gradients they came from
+    int64_t const db = lay(dlay, 0, l);
+    int const nv_end = bv.nbasis - bv.nbe_r;
+    for (int k = 0; k < nb; ++k) {
+        int const i = k < t ? k : nq + k;
+        if (bv.periodic) {
+            io.gvals[io.vb + int64_t(i) * vl.rs] = b2[k];
+        } else if (i < bv.nbe_l) {
+            if (bv.bcl == DDCX_HERMITE) gdl[db + int64_t(i) * dlay.rs] = b2[k] * bv.dxpow[i + bv.odd];
+        } else if (i < nv_end) {
+            io.gvals[io.vb + int64_t(i - bv.nbe_l) * vl.rs] = b2[k];
+        } else if (bv.bcu == DDCX_HERMITE) {
+            gdr[db + int64_t(i - nv_end) * dlay.rs] = b2[k] * bv.dxpow[i - nv_end + bv.odd];
+        }
+    }
+}
+
+} // namespace
+
+void launch_build_vjp(BuilderPlan const& p, const double* gcoef, Layout cl, double* gvals, Layout vl, double* gdl,
+                      double* gdr, Layout dlay, int64_t nbatch, hipStream_t stream)
+{
+    p.ensure_device();
+    SolveView const& sv = p.view.sv;
+    if (sv.nb > kMaxBorder) throw Error(DDCX_ERR_UNSUPPORTED, "border block larger than 16 rows");
+    if (QType(sv.qtype) == QType::Dense && sv.nq > kMaxDenseQ)
+        throw Error(DDCX_ERR_UNSUPPORTED, "dense main block larger than 32 rows");
+    int const threads = 256;
+    int64_t const blocks = (nbatch + threads - 1) / threads;
+    hipLaunchKernelGGL(build_vjp_generic_kernel, dim3(unsigned(blocks)), dim3(threads), 0, stream, p.view, gcoef, cl,
+                       gvals, vl, gdl, gdr, dlay, nbatch);
+    hip_check(hipGetLastError(), "build_vjp_generic_kernel launch");
 }
 
 } // namespace ddcx

@@ -19,6 +19,8 @@
 //      registers;
 //   5. store, including the periodic duplicate rows (spline_builder.hpp:
 //      1074-1100).
+#include <algorithm>
+
 #include "internal.hpp"
 
 namespace ddcx {
@@ -27,6 +29,13 @@ namespace {
 
 constexpr int kNbMax = 8;
 constexpr int kLamMax = 64;
+
+// Keeps the scheduler from hoisting the table loads of a whole segment at
+// once (which would not fit in registers next to the segment's rows).
+__device__ __forceinline__ void chunk_fence(int i)
+{
+    if ((i & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+}
 
 __device__ __forceinline__ int64_t lay(Layout const& L, int64_t r, int64_t l)
 {
@@ -77,60 +86,104 @@ __device__ __forceinline__ void getrs_small(int nb, const double* a, const int* 
     }
 }
 
-template <int LB, int K, int L, int KF, int KB, bool FDIV, bool BDIVF>
+template <int LB, int K, int L, int KF, int KB, bool FDIV, bool BDIVF, bool TLDS>
 __global__ void __launch_bounds__(LB* K) build_tiled_kernel(TiledView tv, BuildView bv,
                                                            const double* __restrict__ vals, Layout vl,
                                                            const double* __restrict__ dl,
                                                            const double* __restrict__ dr, Layout dlay,
                                                            double* __restrict__ coef, Layout cl, int64_t nbatch)
 {
-    constexpr int KE = (KF > KB ? KF : KB) > 0 ? (KF > KB ? KF : KB) : 1;
-    constexpr int SF = KF > 0 ? KF : 1;
-    constexpr int SB = KB > 0 ? KB : 1;
+    constexpr int KE = KF > KB ? KF : KB;
+    constexpr int NT = LB * K;
+    // row-record field offsets: [Lf(KF) | gf(KF) | Ub(KB) | rDb | gb(KB) | rDf | G(nb)]
+    constexpr int oLf = 0, ogf = KF, oUb = 2 * KF, orDb = 2 * KF + KB, ogb = 2 * KF + KB + 1;
+    constexpr int orDf = 2 * KF + 2 * KB + 1, oG = 2 * KF + 2 * KB + 2;
     __shared__ double ex[KE][K][LB];
     __shared__ double lamx[kLamMax][LB];
     __shared__ double x2s[kNbMax][LB];
+    extern __shared__ double rt_lds[];
 
     int const tid = threadIdx.x;
     int const ln = tid % LB;
     int const seg = tid / LB;
-    int64_t const line = int64_t(blockIdx.x) * LB + ln;
+    int64_t const line0 = int64_t(blockIdx.x) * LB;
+    int64_t const line = line0 + ln;
     bool const active = line < nbatch;
-    int64_t const lc = active ? line : 0;
+    int64_t const lc = active ? line : line0;
     int64_t const vb = lay(vl, 0, lc);
     int64_t const cb = lay(cl, 0, lc);
     int const r0 = seg * L;
     int const nq = tv.nq;
+    int const RS = tv.rs;
+    // Workgroup-uniform bases plus 32-bit per-lane offsets (the host checks
+    // that every offset of a tile fits), so that each load/store needs one
+    // address register instead of two.
+    const double* __restrict__ vwg = vals + lay(vl, 0, line0);
+    double* __restrict__ cwg = coef + lay(cl, 0, line0);
+    uint32_t const vlo = uint32_t(vb - lay(vl, 0, line0)) + uint32_t(tv.q0 + r0) * uint32_t(vl.rs);
+    uint32_t const clo = uint32_t(cb - lay(cl, 0, line0));
+    uint32_t const vrs = uint32_t(vl.rs), crs = uint32_t(cl.rs);
 
     double y[L];
+    if (r0 + L <= nq) {
 #pragma unroll
-    for (int i = 0; i < L; ++i) {
-        int const row = r0 + i;
-        y[i] = (active && row < nq) ? vals[vb + int64_t(tv.q0 + row) * vl.rs] : 0.0;
-    }
-
-    // ---------------- forward sweep ----------------
-    if constexpr (KF > 0) {
-        double s[SF];
-#pragma unroll
-        for (int k = 0; k < SF; ++k) s[k] = 0.0;
+        for (int i = 0; i < L; ++i) y[i] = vwg[vlo + uint32_t(i) * vrs];
+    } else {
 #pragma unroll
         for (int i = 0; i < L; ++i) {
-            int const row = r0 + i;
+            double const v = vwg[vlo + uint32_t(min(i, max(nq - 1 - r0, 0))) * vrs];
+            y[i] = r0 + i < nq ? v : 0.0;
+        }
+    }
+
+    // per-row table: staged in LDS when it fits (one copy per workgroup)
+    const double* T;
+    if constexpr (TLDS) {
+        // RS <= 24 doubles per row, so at most 12 double2 per thread
+        int const n2 = (K * L * RS) / 2;
+        const double2* src = reinterpret_cast<const double2*>(tv.rt);
+        double2* dst = reinterpret_cast<double2*>(rt_lds);
+        double2 tmp[12];
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+            int const idx = tid + k * NT;
+            if (idx < n2) tmp[k] = src[idx];
+        }
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+            int const idx = tid + k * NT;
+            if (idx < n2) dst[idx] = tmp[k];
+        }
+        __syncthreads();
+        T = rt_lds + r0 * RS;
+    } else {
+        T = tv.rt + size_t(r0) * RS;
+    }
+
+    // ---------------- forward sweep: local pass, exchange, fixup ----------------
+    {
+        double s[KF];
+#pragma unroll
+        for (int k = 0; k < KF; ++k) s[k] = 0.0;
+#pragma unroll
+        for (int i = 0; i < L; ++i) {
+            const double* R = T + i * RS;
             double t = y[i];
 #pragma unroll
-            for (int k = KF; k >= 1; --k) t = t - s[k - 1] * tv.Lf[row * KF + (k - 1)];
-            if constexpr (FDIV) t = t / tv.Df[row];
+            for (int k = KF; k >= 1; --k) t = fma(-s[k - 1], R[oLf + k - 1], t);
+            if constexpr (FDIV) t = t * R[orDf];
+            y[i] = t;
 #pragma unroll
-            for (int k = SF - 1; k > 0; --k) s[k] = s[k - 1];
+            for (int k = KF - 1; k > 0; --k) s[k] = s[k - 1];
             s[0] = t;
+            chunk_fence(i);
         }
 #pragma unroll
         for (int k = 0; k < KF; ++k) ex[k][seg][ln] = s[k];
         __syncthreads();
-        double in[SF];
+        double in[KF];
 #pragma unroll
-        for (int k = 0; k < SF; ++k) in[k] = 0.0;
+        for (int k = 0; k < KF; ++k) in[k] = 0.0;
         for (int w = 1; w <= tv.wf; ++w) {
             int const src = seg - w;
             if (src >= 0) {
@@ -138,57 +191,48 @@ __global__ void __launch_bounds__(LB* K) build_tiled_kernel(TiledView tv, BuildV
 #pragma unroll
                 for (int a = 0; a < KF; ++a)
 #pragma unroll
-                    for (int b = 0; b < KF; ++b) in[a] += m[a * KF + b] * ex[b][src][ln];
+                    for (int b = 0; b < KF; ++b) in[a] = fma(m[a * KF + b], ex[b][src][ln], in[a]);
             }
         }
 #pragma unroll
-        for (int k = 0; k < SF; ++k) s[k] = in[k];
-#pragma unroll
         for (int i = 0; i < L; ++i) {
-            int const row = r0 + i;
-            double t = y[i];
+            const double* R = T + i * RS;
 #pragma unroll
-            for (int k = KF; k >= 1; --k) t = t - s[k - 1] * tv.Lf[row * KF + (k - 1)];
-            if constexpr (FDIV) t = t / tv.Df[row];
-            y[i] = t;
-#pragma unroll
-            for (int k = SF - 1; k > 0; --k) s[k] = s[k - 1];
-            s[0] = t;
+            for (int k = 0; k < KF; ++k) y[i] = fma(R[ogf + k], in[k], y[i]);
+            chunk_fence(i);
         }
         __syncthreads();
-    } else if constexpr (FDIV) {
-#pragma unroll
-        for (int i = 0; i < L; ++i) y[i] = y[i] / tv.Df[r0 + i];
     }
 
-    // ---------------- backward sweep ----------------
-    auto bwd_row = [&](int row, double yi, const double* s) {
-        if constexpr (BDIVF) {
-            return yi / tv.Db[row] - s[0] * tv.Ub[row * SB];
-        } else {
-            double t = yi;
+    // ---------------- backward sweep: local pass, exchange, fixup ----------------
+    {
+        double s[KB];
 #pragma unroll
-            for (int k = KB; k >= 1; --k) t = t - s[k - 1] * tv.Ub[row * KB + (k - 1)];
-            return t / tv.Db[row];
-        }
-    };
-    if constexpr (KB > 0) {
-        double s[SB];
-#pragma unroll
-        for (int k = 0; k < SB; ++k) s[k] = 0.0;
+        for (int k = 0; k < KB; ++k) s[k] = 0.0;
 #pragma unroll
         for (int i = L - 1; i >= 0; --i) {
-            double const t = bwd_row(r0 + i, y[i], s);
+            const double* R = T + i * RS;
+            double t;
+            if constexpr (BDIVF) {
+                t = fma(-s[0], R[oUb], y[i] * R[orDb]);
+            } else {
+                t = y[i];
 #pragma unroll
-            for (int k = SB - 1; k > 0; --k) s[k] = s[k - 1];
+                for (int k = KB; k >= 1; --k) t = fma(-s[k - 1], R[oUb + k - 1], t);
+                t = t * R[orDb];
+            }
+            y[i] = t;
+#pragma unroll
+            for (int k = KB - 1; k > 0; --k) s[k] = s[k - 1];
             s[0] = t;
+            chunk_fence(i);
         }
 #pragma unroll
         for (int k = 0; k < KB; ++k) ex[k][seg][ln] = s[k];
         __syncthreads();
-        double in[SB];
+        double in[KB];
 #pragma unroll
-        for (int k = 0; k < SB; ++k) in[k] = 0.0;
+        for (int k = 0; k < KB; ++k) in[k] = 0.0;
         for (int w = 1; w <= tv.wb; ++w) {
             int const src = seg + w;
             if (src < K) {
@@ -196,22 +240,16 @@ __global__ void __launch_bounds__(LB* K) build_tiled_kernel(TiledView tv, BuildV
 #pragma unroll
                 for (int a = 0; a < KB; ++a)
 #pragma unroll
-                    for (int b = 0; b < KB; ++b) in[a] += m[a * KB + b] * ex[b][src][ln];
+                    for (int b = 0; b < KB; ++b) in[a] = fma(m[a * KB + b], ex[b][src][ln], in[a]);
             }
         }
 #pragma unroll
-        for (int k = 0; k < SB; ++k) s[k] = in[k];
+        for (int i = 0; i < L; ++i) {
+            const double* R = T + i * RS;
 #pragma unroll
-        for (int i = L - 1; i >= 0; --i) {
-            double const t = bwd_row(r0 + i, y[i], s);
-            y[i] = t;
-#pragma unroll
-            for (int k = SB - 1; k > 0; --k) s[k] = s[k - 1];
-            s[0] = t;
+            for (int k = 0; k < KB; ++k) y[i] = fma(R[ogb + k], in[k], y[i]);
+            chunk_fence(i);
         }
-    } else {
-#pragma unroll
-        for (int i = 0; i < L; ++i) y[i] = y[i] / tv.Db[r0 + i];
     }
 
     // ---------------- border (2x2 blocks) ----------------
@@ -268,52 +306,72 @@ __global__ void __launch_bounds__(LB* K) build_tiled_kernel(TiledView tv, BuildV
         for (int k = 0; k < kNbMax; ++k) x2[k] = k < nb ? x2s[k][ln] : 0.0;
 #pragma unroll
         for (int i = 0; i < L; ++i) {
-            int const row = r0 + i;
+            const double* R = T + i * RS + oG;
             double t = y[i];
 #pragma unroll
             for (int k = 0; k < kNbMax; ++k)
-                if (k < nb) t = t - tv.G[row * nb + k] * x2[k];
+                if (k < nb) t = fma(-R[k], x2[k], t);
             y[i] = t;
+            chunk_fence(i);
         }
     }
 
     // ---------------- store ----------------
     if (active) {
+        uint32_t const c0 = clo + uint32_t(tv.top + r0 + off) * crs;
+        if (r0 + L <= nq) {
 #pragma unroll
-        for (int i = 0; i < L; ++i) {
-            int const row = r0 + i;
-            if (row < nq) {
-                int const prow = tv.top + row;
-                coef[cb + int64_t(prow + off) * cl.rs] = y[i];
-                if (bv.periodic) {
+            for (int i = 0; i < L; ++i) cwg[c0 + uint32_t(i) * crs] = y[i];
+        } else {
+#pragma unroll
+            for (int i = 0; i < L; ++i)
+                if (r0 + i < nq) cwg[c0 + uint32_t(i) * crs] = y[i];
+        }
+        if (bv.periodic && (r0 < bv.degree || r0 + L > nbas - off)) {
+#pragma unroll
+            for (int i = 0; i < L; ++i) {
+                int const prow = tv.top + r0 + i;
+                if (prow < nq) {
                     int const j0 = prow + off - nbas;
-                    if (j0 >= 0) coef[cb + int64_t(j0) * cl.rs] = y[i];
+                    if (j0 >= 0) cwg[clo + uint32_t(j0) * crs] = y[i];
                     int const j2 = prow + off + nbas;
-                    if (j2 < nbas + bv.degree) coef[cb + int64_t(j2) * cl.rs] = y[i];
+                    if (j2 < nbas + bv.degree) cwg[clo + uint32_t(j2) * crs] = y[i];
                 }
             }
         }
     }
 }
 
-template <int L, int KF, int KB, bool FDIV, bool BDIVF>
+template <int K, int L, int KF, int KB, bool FDIV, bool BDIVF>
 void launch_cfg(BuilderPlan const& p, double* coef, Layout cl, const double* vals, Layout vl, const double* dl,
                 const double* dr, Layout dlay, int64_t nbatch, hipStream_t stream)
 {
-    constexpr int LB = 16, K = 64;
+    constexpr int NT = 1024, LB = NT / K;
+    constexpr int KE = KF > KB ? KF : KB;
+    constexpr size_t static_lds = size_t(KE * K * LB + kLamMax * LB + kNbMax * LB) * sizeof(double);
+    size_t const table = size_t(K) * L * p.tiled.rs * sizeof(double);
     int64_t const blocks = (nbatch + LB - 1) / LB;
-    hipLaunchKernelGGL((build_tiled_kernel<LB, K, L, KF, KB, FDIV, BDIVF>), dim3(unsigned(blocks)), dim3(LB * K), 0,
-                       stream, p.tview, p.view, vals, vl, dl, dr, dlay, coef, cl, nbatch);
+    if (static_lds + table <= size_t(160 * 1024) && p.tiled.rs <= 24) {
+        auto kern = build_tiled_kernel<LB, K, L, KF, KB, FDIV, BDIVF, true>;
+        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      int(table)),
+                  "hipFuncSetAttribute");
+        hipLaunchKernelGGL(kern, dim3(unsigned(blocks)), dim3(NT), table, stream, p.tview, p.view, vals, vl, dl, dr,
+                           dlay, coef, cl, nbatch);
+    } else {
+        hipLaunchKernelGGL((build_tiled_kernel<LB, K, L, KF, KB, FDIV, BDIVF, false>), dim3(unsigned(blocks)),
+                           dim3(NT), 0, stream, p.tview, p.view, vals, vl, dl, dr, dlay, coef, cl, nbatch);
+    }
     hip_check(hipGetLastError(), "build_tiled_kernel launch");
 }
 
-template <int L>
-bool launch_L(BuilderPlan const& p, double* coef, Layout cl, const double* vals, Layout vl, const double* dl,
+template <int K, int L>
+bool launch_K(BuilderPlan const& p, double* coef, Layout cl, const double* vals, Layout vl, const double* dl,
               const double* dr, Layout dlay, int64_t nbatch, hipStream_t stream)
 {
     TiledHost const& t = p.tiled;
     auto go = [&]<int KF, int KB, bool FD, bool BD>() {
-        launch_cfg<L, KF, KB, FD, BD>(p, coef, cl, vals, vl, dl, dr, dlay, nbatch, stream);
+        launch_cfg<K, L, KF, KB, FD, BD>(p, coef, cl, vals, vl, dl, dr, dlay, nbatch, stream);
         return true;
     };
     if (t.bwd_divfirst) {
@@ -339,8 +397,17 @@ bool launch_build_tiled(BuilderPlan const& p, double* coef, Layout cl, const dou
                         const double* dl, const double* dr, Layout dlay, int64_t nbatch, hipStream_t stream)
 {
     if (!p.tiled.ok) return false;
-    if (p.tiled.L == 16) return launch_L<16>(p, coef, cl, vals, vl, dl, dr, dlay, nbatch, stream);
-    if (p.tiled.L == 32) return launch_L<32>(p, coef, cl, vals, vl, dl, dr, dlay, nbatch, stream);
+    // the kernel addresses a tile as a uniform base plus 32-bit offsets
+    auto fits = [&](Layout const& l, int64_t rows) {
+        if (l.rs < 0 || l.is < 0 || l.os < 0 || l.inner < 1) return false;
+        if (nbatch > l.inner && l.os < (l.inner - 1) * l.is + 1) return false;
+        int64_t const span = rows * l.rs + std::min<int64_t>(nbatch, l.inner) * l.is + l.os;
+        return span < (int64_t(1) << 31);
+    };
+    if (!fits(vl, p.tiled.K * p.tiled.L + p.tview.q0) || !fits(cl, p.view.size)) return false;
+    if (p.tiled.K == 32 && p.tiled.L == 32) return launch_K<32, 32>(p, coef, cl, vals, vl, dl, dr, dlay, nbatch, stream);
+    if (p.tiled.K == 64 && p.tiled.L == 16) return launch_K<64, 16>(p, coef, cl, vals, vl, dl, dr, dlay, nbatch, stream);
+    if (p.tiled.K == 64 && p.tiled.L == 32) return launch_K<64, 32>(p, coef, cl, vals, vl, dl, dr, dlay, nbatch, stream);
     return false;
 }
 

@@ -58,7 +58,8 @@ __global__ void __launch_bounds__(NT) eval_tiled_kernel(BSplinesView b, int lowe
     double* tile = smem;
     double* kn = smem + size_t(ncoef) * pitch;
     constexpr int ROWS = NT / LB;
-    constexpr int U = 4; // points in flight per thread
+    constexpr int U = 8;           // points per pipeline stage per thread
+    constexpr int TC = 16;         // tile rows per thread per load chunk
     int const tid = threadIdx.x;
     int const ln = tid % LB;
     int const prow = tid / LB;
@@ -66,26 +67,52 @@ __global__ void __launch_bounds__(NT) eval_tiled_kernel(BSplinesView b, int lowe
     bool const active = line < nbatch;
     int64_t const lc = active ? line : 0;
     int64_t const cb = lay(cl, 0, lc);
-    for (int r = prow; r < ncoef; r += ROWS) tile[r * pitch + ln] = active ? coef[cb + int64_t(r) * cl.rs] : 0.0;
+    int64_t const pb = lay(pl, 0, lc);
+    // first batch of coordinates, then the whole coefficient tile: every load
+    // of the tile is in flight before the first LDS write
+    double xa[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        int64_t const ii = prow + int64_t(u) * ROWS;
+        xa[u] = ii < npts ? coords[pb + ii * pl.rs] : 0.0;
+    }
+    // the tile is loaded in chunks of TC rows per thread, every load of a
+    // chunk in flight before its LDS writes
+    for (int c0 = prow; c0 < ncoef; c0 += TC * ROWS) {
+        double tv[TC];
+#pragma unroll
+        for (int k = 0; k < TC; ++k) {
+            int const r = c0 + k * ROWS;
+            tv[k] = r < ncoef ? coef[cb + int64_t(r) * cl.rs] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < TC; ++k) {
+            int const r = c0 + k * ROWS;
+            if (r < ncoef) tile[r * pitch + ln] = tv[k];
+        }
+    }
     if (!b.uniform)
         for (int k = tid; k < nknots; k += NT) kn[k] = b.knots[k];
     __syncthreads();
     if (!active) return;
     const double* knots = b.uniform ? nullptr : kn;
     const double* tl = tile + ln;
-    int64_t const pb = lay(pl, 0, lc);
-    int64_t i = prow;
-    for (; i + (U - 1) * ROWS < npts; i += U * ROWS) {
-        double x[U];
+    // software pipeline: the next batch of coordinates is in flight while the
+    // current one is evaluated and stored
+    for (int64_t i = prow; i < npts; i += U * ROWS) {
+        double xb[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) x[u] = coords[pb + (i + u * ROWS) * pl.rs];
+        for (int u = 0; u < U; ++u) {
+            int64_t const ii = i + int64_t(U + u) * ROWS;
+            xb[u] = ii < npts ? coords[pb + ii * pl.rs] : 0.0;
+        }
 #pragma unroll
-        for (int u = 0; u < U; ++u)
-            out[pb + (i + u * ROWS) * pl.rs] = point_value<D, ORDER>(b, knots, lower, upper, x[u], tl, pitch);
-    }
-    for (; i < npts; i += ROWS) {
-        double const x = coords[pb + i * pl.rs];
-        out[pb + i * pl.rs] = point_value<D, ORDER>(b, knots, lower, upper, x, tl, pitch);
+        for (int u = 0; u < U; ++u) {
+            int64_t const ii = i + int64_t(u) * ROWS;
+            if (ii < npts) out[pb + ii * pl.rs] = point_value<D, ORDER>(b, knots, lower, upper, xa[u], tl, pitch);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) xa[u] = xb[u];
     }
 }
 
@@ -112,6 +139,7 @@ bool launch_orders(BSplinesPlan const& bs, int lower, int upper, double* out, co
         hip_check(hipGetLastError(), "eval_tiled_kernel launch");
         return true;
     };
+    if (ncoef > 2560) return false;
     if (fits(16, 17)) return go.template operator()<16>(17);
     if (fits(16, 16)) return go.template operator()<16>(16);
     if (fits(8, 9)) return go.template operator()<8>(9);

@@ -540,9 +540,11 @@ void BuilderPlan::ensure_device() const
         sv.bl_v = d_blv.as<double>();
         if (tiled.ok) {
             t_Lf.upload(tiled.Lf);
-            t_Df.upload(tiled.Df);
+            t_rDf.upload(tiled.rDf);
+            t_gf.upload(tiled.gf);
             t_Ub.upload(tiled.Ub);
-            t_Db.upload(tiled.Db);
+            t_rDb.upload(tiled.rDb);
+            t_gb.upload(tiled.gb);
             t_Mf.upload(tiled.Mf);
             t_Mb.upload(tiled.Mb);
             t_G.upload(tiled.G);
@@ -559,12 +561,17 @@ void BuilderPlan::ensure_device() const
             tv.q0 = bs->view.periodic ? 0 : prob.top - nbe_l;
             tv.bl_nnz = int(prob.bl_v.size());
             tv.Lf = t_Lf.as<double>();
-            tv.Df = t_Df.as<double>();
+            tv.rDf = t_rDf.as<double>();
+            tv.gf = t_gf.as<double>();
             tv.Ub = t_Ub.as<double>();
-            tv.Db = t_Db.as<double>();
+            tv.rDb = t_rDb.as<double>();
+            tv.gb = t_gb.as<double>();
             tv.Mf = t_Mf.as<double>();
             tv.Mb = t_Mb.as<double>();
             tv.G = t_G.as<double>();
+            t_rt.upload(tiled.rt);
+            tv.rt = t_rt.as<double>();
+            tv.rs = tiled.rs;
             tv.bl_r = sv.bl_r;
             tv.bl_c = sv.bl_c;
             tv.bl_v = sv.bl_v;

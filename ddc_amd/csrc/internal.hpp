@@ -142,30 +142,43 @@ struct BuildView {
 };
 
 // Plan of the register-resident tiled build kernel (build_tiled.hip).  The
-// main block is swept as K segments of L rows per line; every sweep is in
-// row-oriented form (same operations and order as the netlib column-oriented
-// sweeps the reference runs) so that the state crossing a segment boundary is
-// just the last kf solution values (forward) or the next kb ones (backward).
-// Segment carries are exact: in(s) = sum_{w=1..W} P_{s,w} out_loc(s-w), with
-// the batch-independent transfer products P precomputed here and W chosen so
-// that the first neglected product is below 2^-60.
+// main block is swept as K segments of L rows per line.  Every sweep is in
+// row-oriented form (the same row operations as the netlib column-oriented
+// sweeps the reference runs), so the state crossing a segment boundary is the
+// last kf solution values (forward) or the next kb ones (backward).  Each
+// segment is swept once with a zero incoming state; the true solution is the
+// local one plus the segment's response to its incoming state (gf / gb, per
+// row), and the incoming state itself is the short sum
+//   in(s) = sum_{w=1..W} P_{s,w} out_loc(s -/+ w)
+// of batch-independent transfer products, W chosen so that the first
+// neglected product is below 2^-60.  Divisions by pivots are multiplications
+// by reciprocals computed here.
 struct TiledHost {
     bool ok = false;
     std::string why; // reason when !ok
     int K = 0, L = 0, kf = 0, kb = 0, wf = 0, wb = 0;
-    int fwd_div = 0;   // forward rows divide by Df (PDSBand)
-    int bwd_divfirst = 0; // backward row form y/d - x e (PDSTridiag) instead of (y - sum)/d
-    std::vector<double> Lf, Df, Ub, Db; // padded to K*L rows
-    std::vector<double> Mf, Mb;         // K x W x k x k
-    std::vector<double> G;              // (K*L) x nb: Q^-1 gamma with |a| < 1e-14 zeroed
+    int fwd_div = 0;      // forward rows are scaled by rDf (PDSBand)
+    int bwd_divfirst = 0; // backward row form y*rD - x*e (PDSTridiag) instead of (y - sum)*rD
+    // per row (padded to K*L rows): coefficients, reciprocal pivots, responses
+    std::vector<double> Lf, rDf, gf; // Lf, gf: kf per row
+    std::vector<double> Ub, rDb, gb; // Ub, gb: kb per row
+    std::vector<double> Mf, Mb;      // K x W x k x k transfer products
+    std::vector<double> G;           // (K*L) x nb: Q^-1 gamma with |a| < 1e-14 zeroed
+    // The same tables interleaved per row, rs doubles per row:
+    // [Lf(kf) | gf(kf) | Ub(kb) | rDb | gb(kb) | rDf | G(nb) | pad]
+    int rs = 0;
+    std::vector<double> rt;
 };
 
 struct TiledView {
-    int K, L, kf, kb, wf, wb, nq, nb, top, q0, bl_nnz;
+    int K, L, kf, kb, wf, wb, nq, nb, top, q0, bl_nnz, rs;
+    const double* rt;
     const double* Lf;
-    const double* Df;
+    const double* rDf;
+    const double* gf;
     const double* Ub;
-    const double* Db;
+    const double* rDb;
+    const double* gb;
     const double* Mf;
     const double* Mb;
     const double* G;
@@ -189,7 +202,7 @@ struct BuilderPlan {
     mutable DeviceBuffer d_q, d_ipiv, d_slu, d_sipiv, d_trr, d_trc, d_trv, d_blr, d_blc, d_blv;
     mutable BuildView view{}; // device pointers filled by ensure_device()
     TiledHost tiled;
-    mutable DeviceBuffer t_Lf, t_Df, t_Ub, t_Db, t_Mf, t_Mb, t_G;
+    mutable DeviceBuffer t_Lf, t_rDf, t_gf, t_Ub, t_rDb, t_gb, t_Mf, t_Mb, t_G, t_rt;
     mutable TiledView tview{};
     mutable std::once_flag uploaded;
     void ensure_device() const;

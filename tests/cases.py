@@ -20,6 +20,11 @@ def breaks_for(kind, ncells, seed=0, ratio=1.05):
         rng = np.random.default_rng(seed)
         inner = np.sort(rng.uniform(0.0, 1.0, ncells - 1))
         return np.concatenate([[0.0], inner, [1.0]])
+    if kind == "jitter":
+        rng = np.random.default_rng(seed)
+        x = np.linspace(0.0, 1.0, ncells + 1)
+        x[1:-1] += rng.uniform(-0.3, 0.3, ncells - 1) / ncells
+        return x
     if kind == "geometric":
         k = np.arange(ncells + 1, dtype=np.float64)
         return (ratio ** k - 1.0) / (ratio ** ncells - 1.0)

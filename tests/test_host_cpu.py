@@ -1,0 +1,108 @@
+"""Host-side checks of the product library (no GPU compute): the C ABI exports
+every entry point of include/ddcx.h, and the FP64 host setup (knots,
+interpolation points, integrals, offsets, matrix shapes, quadrature weights,
+validation errors) agrees with the oracle.
+"""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle as O
+from cases import ALL_CASES, TILED_CASES, Case, breaks_for
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    text = open(os.path.join(ROOT, "include", "ddcx.h")).read()
+    return sorted(set(re.findall(r"\b(ddcx_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_header_symbol():
+    import ddc_amd
+
+    lib = C.CDLL(ddc_amd.LIB_PATH)
+    syms = header_symbols()
+    assert len(syms) >= 20
+    for s in syms:
+        assert hasattr(lib, s), s
+
+
+def test_no_oracle_in_product():
+    """The product library neither links to nor names the oracle."""
+    import ddc_amd
+
+    blob = open(ddc_amd.LIB_PATH, "rb").read()
+    assert b"liboracle" not in blob and b"or_spline" not in blob
+
+
+def test_version_string():
+    import ddc_amd
+
+    assert b"gfx950" in ddc_amd.lib().ddcx_version()
+
+
+@pytest.mark.parametrize("case", ALL_CASES + TILED_CASES, ids=lambda c: c.id)
+def test_host_setup_matches_oracle(case):
+    orc = case.oracle()
+    bs, builder = case.product()
+    assert np.array_equal(bs.knots(), orc.knots())
+    assert np.array_equal(builder.interpolation_points(), orc.points())
+    import ddc_amd as D
+
+    assert np.array_equal(D.integrals(bs), orc.integrals())
+    assert builder.offset() == orc.offset
+    assert builder.info.size == orc.size and builder.info.nbasis == orc.nbasis
+    kinds = {0: "dense", 1: "band", 2: "pds_band", 3: "pds_tridiag"}
+    blocked = {0: None, 1: "2x2", 2: "3x3"}[builder.info.blocked]
+    assert (blocked or kinds[builder.info.matrix_kind]) == orc.matrix_type
+    q1 = builder.quadrature_coefficients()
+    q2 = orc.quadrature_coefficients()
+    for a, b in zip(q1, q2):
+        assert np.array_equal(a, b)
+
+
+def test_errors_mirror_reference():
+    import ddc_amd as D
+
+    # check_valid_grid (spline_builder.hpp:1273-1317)
+    bs = D.UniformBSplines(2, 0.0, 1.0, 5, periodic=True)
+    for pts in ([0.1, 0.3, 0.5, 0.7], [0.1, 0.3, 0.5, 0.7, 0.9, 0.95], [0.1, 0.11, 0.12, 0.13, 0.14]):
+        with pytest.raises(D.DDCXError) as e:
+            D.SplineBuilder(bs, points=pts)
+        assert e.value.status == 2
+    # unsorted break points (non_uniform_point_sampling.hpp:88-90)
+    with pytest.raises(D.DDCXError):
+        D.NonUniformBSplines(3, [0.0, 0.5, 0.4, 1.0])
+    # closures must both be periodic or both not
+    with pytest.raises(D.DDCXError):
+        D.SplineBuilder(bs, D.SplineBuilderClosure.PERIODIC, D.SplineBuilderClosure.GREVILLE)
+    # degree outside the compiled range
+    with pytest.raises(D.DDCXError) as e:
+        D.UniformBSplines(9, 0.0, 1.0, 10)
+    assert e.value.status == 5
+
+
+def test_ginkgo_tag_routes_to_direct_solver():
+    import ddc_amd as D
+
+    bs = D.UniformBSplines(3, 0.0, 1.0, 32, periodic=True)
+    b1 = D.SplineBuilder(bs, solver=D.SplineSolver.GINKGO)
+    b2 = D.SplineBuilder(bs, solver=D.SplineSolver.LAPACK)
+    assert b1.info.matrix_kind == b2.info.matrix_kind
+    assert np.array_equal(b1.quadrature_coefficients()[1], b2.quadrature_coefficients()[1])
+
+
+def test_batched_layout_helper():
+    import torch
+
+    import ddc_amd as D
+
+    t = torch.empty((5, 1024, 7), dtype=torch.float64)
+    L, nb = D.batched_layout(t, 1)
+    assert (L.row_stride, L.inner_count, L.inner_stride, L.outer_stride, nb) == (7, 7, 1, 1024 * 7, 35)
+    L, nb = D.batched_layout(t.permute(1, 0, 2), 0)
+    assert nb == 35 and L.row_stride == 7
