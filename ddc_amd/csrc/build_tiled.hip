@@ -406,7 +406,6 @@ bool launch_build_tiled(BuilderPlan const& p, double* coef, Layout cl, const dou
     };
     if (!fits(vl, p.tiled.K * p.tiled.L + p.tview.q0) || !fits(cl, p.view.size)) return false;
     if (p.tiled.K == 32 && p.tiled.L == 32) return launch_K<32, 32>(p, coef, cl, vals, vl, dl, dr, dlay, nbatch, stream);
-    if (p.tiled.K == 64 && p.tiled.L == 16) return launch_K<64, 16>(p, coef, cl, vals, vl, dl, dr, dlay, nbatch, stream);
     if (p.tiled.K == 64 && p.tiled.L == 32) return launch_K<64, 32>(p, coef, cl, vals, vl, dl, dr, dlay, nbatch, stream);
     return false;
 }

@@ -75,11 +75,11 @@ void make_tiled(BuilderPlan& p)
     if (nq > 2048) { t.why = "main block larger than 2048 rows"; return; }
     if (pr.nb() > kMaxBorderTiled) { t.why = "border larger than 8"; return; }
     if (int(pr.bl_v.size()) > kMaxLambda) { t.why = "too many border couplings"; return; }
-    // tile shape: K segments of L rows (K*L >= nq), 1024 threads, LB = 1024/K lines
+    // tile shape: K segments of L rows (K*L >= nq), 1024 threads, LB = 1024/K
+    // lines (32 lines give 256-byte rows in batch-contiguous layouts)
     if (nq <= 1024) {
-        char const* env = std::getenv("DDCX_TILE_K");
-        t.K = (env && std::atoi(env) == 64) ? 64 : 32;
-        t.L = 1024 / t.K;
+        t.K = 32;
+        t.L = 32;
     } else {
         t.K = 64;
         t.L = 32;
