@@ -28,14 +28,14 @@ class ddcx_bsplines_desc(C.Structure):
 
 class ddcx_builder_desc(C.Structure):
     _fields_ = [("lower", C.c_int), ("upper", C.c_int), ("solver", C.c_int), ("precision", C.c_int),
-                ("points", C.c_void_p), ("npoints", C.c_int)]
+                ("points", C.c_void_p), ("npoints", C.c_int), ("refine_steps", C.c_int)]
 
 
 class ddcx_builder_info(C.Structure):
     _fields_ = [("npoints", C.c_int), ("nbasis", C.c_int), ("size", C.c_int), ("offset", C.c_int),
                 ("nbv_lower", C.c_int), ("nbv_upper", C.c_int), ("matrix_kind", C.c_int), ("blocked", C.c_int),
                 ("main_size", C.c_int), ("border", C.c_int), ("dx", C.c_double), ("kernel", C.c_int),
-                ("pivoted", C.c_int)]
+                ("pivoted", C.c_int), ("refine_steps", C.c_int), ("refine_contraction", C.c_double)]
 
 
 # name -> (argtypes) ; all return ddcx_status (int) unless listed in _RESTYPES

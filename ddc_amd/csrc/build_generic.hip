@@ -304,6 +304,8 @@ void launch_build(BuilderPlan const& p, double* coef, Layout cl, const double* v
                   const double* dr, Layout dlay, int64_t nbatch, hipStream_t stream)
 {
     p.ensure_device();
+    if (p.precision == DDCX_FP32_REFINE && launch_build_refine(p, coef, cl, vals, vl, dl, dr, dlay, nbatch, stream))
+        return;
     if (launch_build_tiled(p, coef, cl, vals, vl, dl, dr, dlay, nbatch, stream)) return;
     launch_build_generic(p, coef, cl, vals, vl, dl, dr, dlay, nbatch, stream);
 }

@@ -21,70 +21,13 @@
 //      1074-1100).
 #include <algorithm>
 
-#include "internal.hpp"
+#include "tiled_common.hpp"
 
 namespace ddcx {
 
 namespace {
 
-constexpr int kNbMax = 8;
-constexpr int kLamMax = 64;
-
-// Keeps the scheduler from hoisting the table loads of a whole segment at
-// once (which would not fit in registers next to the segment's rows).
-__device__ __forceinline__ void chunk_fence(int i)
-{
-    if ((i & 7) == 7) __builtin_amdgcn_sched_barrier(0);
-}
-
-__device__ __forceinline__ int64_t lay(Layout const& L, int64_t r, int64_t l)
-{
-    return r * L.rs + (l % L.inner) * L.is + (l / L.inner) * L.os;
-}
-
-// physical right-hand-side row i of one line (spline_builder.hpp:949-1033)
-__device__ __forceinline__ double rhs_src(BuildView const& bv, const double* vals, Layout const& vl, int64_t vb,
-                                          const double* dl, const double* dr, Layout const& dlay, int64_t db, int i)
-{
-    if (bv.periodic) return vals[vb + int64_t(i) * vl.rs];
-    if (i < bv.nbe_l) return bv.bcl == DDCX_HERMITE ? dl[db + int64_t(i) * dlay.rs] * bv.dxpow[i + bv.odd] : 0.0;
-    int const nv_end = bv.nbasis - bv.nbe_r;
-    if (i < nv_end) return vals[vb + int64_t(i - bv.nbe_l) * vl.rs];
-    return bv.bcu == DDCX_HERMITE ? dr[db + int64_t(i - nv_end) * dlay.rs] * bv.dxpow[i - nv_end + bv.odd] : 0.0;
-}
-
-// dgetrs on an nb x nb LU (nb <= kNbMax) held in registers
-__device__ __forceinline__ void getrs_small(int nb, const double* a, const int* ipiv, double* b)
-{
-#pragma unroll
-    for (int j = 0; j < kNbMax; ++j) {
-        if (j < nb) {
-            int const p = ipiv[j];
-            double bp = 0.0;
-#pragma unroll
-            for (int k = 0; k < kNbMax; ++k)
-                if (k == p) bp = b[k];
-            double const bj = b[j];
-#pragma unroll
-            for (int k = 0; k < kNbMax; ++k)
-                if (k == p) b[k] = bj;
-            b[j] = bp;
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < kNbMax; ++j)
-#pragma unroll
-        for (int i = j + 1; i < kNbMax; ++i)
-            if (i < nb) b[i] = b[i] - b[j] * a[i * nb + j];
-#pragma unroll
-    for (int j = kNbMax - 1; j >= 0; --j) {
-        if (j < nb) {
-            b[j] = b[j] / a[j * nb + j];
-#pragma unroll
-            for (int i = 0; i < j; ++i) b[i] = b[i] - b[j] * a[i * nb + j];
-        }
-    }
-}
+using namespace tiled;
 
 template <int LB, int K, int L, int KF, int KB, bool FDIV, bool BDIVF, bool TLDS>
 __global__ void __launch_bounds__(LB* K) build_tiled_kernel(TiledView tv, BuildView bv,
@@ -129,9 +72,11 @@ __global__ void __launch_bounds__(LB* K) build_tiled_kernel(TiledView tv, BuildV
 #pragma unroll
         for (int i = 0; i < L; ++i) y[i] = vwg[vlo + uint32_t(i) * vrs];
     } else {
+        // last segments: rows at or past nq read row nq - 1 (in bounds) and are zeroed
+        uint32_t const vl0 = vlo - uint32_t(r0) * vrs; // row q0 of this line
 #pragma unroll
         for (int i = 0; i < L; ++i) {
-            double const v = vwg[vlo + uint32_t(min(i, max(nq - 1 - r0, 0))) * vrs];
+            double const v = vwg[vl0 + uint32_t(min(r0 + i, nq - 1)) * vrs];
             y[i] = r0 + i < nq ? v : 0.0;
         }
     }
@@ -397,14 +342,8 @@ bool launch_build_tiled(BuilderPlan const& p, double* coef, Layout cl, const dou
                         const double* dl, const double* dr, Layout dlay, int64_t nbatch, hipStream_t stream)
 {
     if (!p.tiled.ok) return false;
-    // the kernel addresses a tile as a uniform base plus 32-bit offsets
-    auto fits = [&](Layout const& l, int64_t rows) {
-        if (l.rs < 0 || l.is < 0 || l.os < 0 || l.inner < 1) return false;
-        if (nbatch > l.inner && l.os < (l.inner - 1) * l.is + 1) return false;
-        int64_t const span = rows * l.rs + std::min<int64_t>(nbatch, l.inner) * l.is + l.os;
-        return span < (int64_t(1) << 31);
-    };
-    if (!fits(vl, p.tiled.K * p.tiled.L + p.tview.q0) || !fits(cl, p.view.size)) return false;
+    if (!offsets_fit(vl, p.tiled.K * p.tiled.L + p.tview.q0, nbatch) || !offsets_fit(cl, p.view.size, nbatch))
+        return false;
     if (p.tiled.K == 32 && p.tiled.L == 32) return launch_K<32, 32>(p, coef, cl, vals, vl, dl, dr, dlay, nbatch, stream);
     if (p.tiled.K == 64 && p.tiled.L == 32) return launch_K<64, 32>(p, coef, cl, vals, vl, dl, dr, dlay, nbatch, stream);
     return false;

@@ -119,7 +119,9 @@ ddcx_status ddcx_builder_get_info(const ddcx_builder* h, ddcx_builder_info* info
         info->main_size = p.prob.q.n;
         info->border = p.prob.nb();
         info->dx = p.dx;
-        info->kernel = p.tiled.ok ? 1 : 0;
+        info->kernel = p.refine.ok ? 2 : p.tiled.ok ? 1 : 0;
+        info->refine_steps = p.refine.ok ? p.refine.steps : 0;
+        info->refine_contraction = p.refine.ok ? p.refine.rho : 0.0;
         info->pivoted = 0;
         for (size_t j = 0; j < p.prob.q.ipiv.size(); ++j)
             if (p.prob.q.ipiv[j] != int(j) && p.prob.q.type == ddcx::QType::Band) info->pivoted = 1;

@@ -363,6 +363,11 @@ void HostProblem::setup()
                         : q.type == QType::Band ? "LAPACKE_dgbtrf"
                         : q.type == QType::PDSBand ? "LAPACKE_dpbtrf"
                                                    : "LAPACKE_dpttrf";
+    q_orig = q;
+    if (blocked) {
+        gamma_orig = gamma;
+        delta_orig = schur.q;
+    }
     check(q.factor(), qname);
     if (!blocked) return;
     int const nq_ = q.n, nb_ = n - q.n;
@@ -578,6 +583,48 @@ void BuilderPlan::ensure_device() const
             tv.s_lu = sv.s_lu;
             tv.s_ipiv = sv.s_ipiv;
         }
+        if (refine.ok) {
+            RefineHost const& R = refine;
+            r_rt.upload(R.rt32);
+            r_Mf.upload(R.Mf32);
+            r_Mb.upload(R.Mb32);
+            r_rq.upload(R.rq);
+            r_gr.upload(R.g_r);
+            r_gc.upload(R.g_c);
+            r_gv.upload(R.g_v);
+            r_gseg.upload(R.g_seg);
+            r_delta.upload(R.delta);
+            RefineView& rv = rview;
+            rv.K = R.t.K;
+            rv.L = R.t.L;
+            rv.kf = R.t.kf;
+            rv.kb = R.t.kb;
+            rv.wf = R.t.wf;
+            rv.wb = R.t.wb;
+            rv.nq = prob.q.n;
+            rv.nb = prob.nb();
+            rv.top = prob.top;
+            rv.q0 = bs->view.periodic ? 0 : prob.top - nbe_l;
+            rv.bl_nnz = int(prob.bl_v.size());
+            rv.rs = R.rs;
+            rv.kq = R.kq;
+            rv.steps = R.steps;
+            rv.shift = R.shift;
+            rv.rt = r_rt.as<float>();
+            rv.Mf = r_Mf.as<float>();
+            rv.Mb = r_Mb.as<float>();
+            rv.rq = r_rq.as<double>();
+            rv.g_r = r_gr.as<int>();
+            rv.g_c = r_gc.as<int>();
+            rv.g_v = r_gv.as<double>();
+            rv.g_seg = r_gseg.as<int>();
+            rv.delta = r_delta.as<double>();
+            rv.bl_r = sv.bl_r;
+            rv.bl_c = sv.bl_c;
+            rv.bl_v = sv.bl_v;
+            rv.s_lu = sv.s_lu;
+            rv.s_ipiv = sv.s_ipiv;
+        }
     });
 }
 
@@ -709,6 +756,9 @@ std::unique_ptr<BuilderPlan> make_builder(std::shared_ptr<BSplinesPlan> bs, ddcx
     p->bcu = d.upper;
     p->solver = d.solver;
     p->precision = d.precision;
+    if (d.precision != DDCX_FP64 && d.precision != DDCX_FP32_REFINE) throw Error(DDCX_ERR_INVALID, "unknown precision");
+    if (d.refine_steps < 0 || d.refine_steps > 2) throw Error(DDCX_ERR_INVALID, "refine_steps must be 0, 1 or 2");
+    p->refine_request = d.refine_steps;
     p->nbe_l = n_boundary_equations(d.lower, D);
     p->nbe_r = n_boundary_equations(d.upper, D);
     p->nbv_l = d.lower == DDCX_HOMOGENEOUS_HERMITE ? 0 : p->nbe_l;
@@ -859,6 +909,7 @@ std::unique_ptr<BuilderPlan> make_builder(std::shared_ptr<BSplinesPlan> bs, ddcx
         bv.dxpow[k] = r;
     }
     make_tiled(*p);
+    if (p->precision == DDCX_FP32_REFINE) make_refine(*p);
     return p;
 }
 

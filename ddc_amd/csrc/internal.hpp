@@ -81,6 +81,10 @@ struct HostProblem {
     // COO copies (|a| >= 1e-14, row-major scan) used by the solve
     std::vector<int> tr_r, tr_c, bl_r, bl_c;
     std::vector<double> tr_v, bl_v;
+    // the assembled matrix before setup(): main block, gamma (nq x nb) and
+    // delta (nb x nb), for residuals b - A x
+    HostQ q_orig;
+    std::vector<double> gamma_orig, delta_orig;
 
     int nq() const { return blocked ? q.n : n; }
     int nb() const { return blocked ? n - q.n : 0; }
@@ -189,11 +193,59 @@ struct TiledView {
     const int* s_ipiv;
 };
 
+// Plan of the mixed-precision build (build_refine.hip): the tiled sweeps run
+// in FP32 on a K x L tiling with FP32 tables (carries to 2^-30), the border
+// and Schur steps in FP64, and each line is refined with FP64 residuals
+// r = b - A x of the assembled matrix,
+//   x_0 = S(b),  x_{k+1} = x_k + S(r_k),  k < steps,
+// where S is the approximate solve.  The right-hand side of each S is scaled
+// per line by a power of two, 2^(k*shift - e) in step k with e the exponent of
+// max |b| over the line and shift = floor(-log2 rho), so the FP32 sweeps see
+// values near magnitude 1 whatever the data's range.  steps is the smallest
+// count (<= 2) for which the contraction rho measured on the host, with a
+// margin of 2, gives (2 rho)^(steps+1) <= 1e-12; a matrix that needs more
+// is built by the FP64 kernels.
+struct RefineHost {
+    bool ok = false;
+    std::string why;
+    TiledHost t;                // tables in FP64 before rounding
+    int rs = 0;                 // floats per row of rt32 (the TiledHost row layout)
+    std::vector<float> rt32;    // interleaved row table rounded to FP32
+    std::vector<float> Mf32, Mb32;
+    int kq = 0;                 // half bandwidth of the main block (kl == ku)
+    std::vector<double> rq;     // (K*L) x (2kq+1): main block rows A(u, u-kq+j), zero padded
+    std::vector<int> g_r, g_c;  // gamma couplings, COO sorted by row
+    std::vector<double> g_v;
+    std::vector<int> g_seg;     // K+1 offsets into the gamma COO, by segment
+    std::vector<double> delta;  // nb x nb, row-major
+    int steps = 0, shift = 0;
+    double rho = 0;
+};
+
+struct RefineView {
+    int K, L, kf, kb, wf, wb, nq, nb, top, q0, bl_nnz, rs, kq, steps, shift;
+    const float* rt;
+    const float* Mf;
+    const float* Mb;
+    const double* rq;
+    const int* g_r;
+    const int* g_c;
+    const double* g_v;
+    const int* g_seg;
+    const double* delta;
+    const int* bl_r;
+    const int* bl_c;
+    const double* bl_v;
+    const double* s_lu;
+    const int* s_ipiv;
+};
+
 struct BuilderPlan {
     std::shared_ptr<BSplinesPlan> bs;
     ddcx_closure bcl, bcu;
     ddcx_solver solver;
     ddcx_precision precision;
+    int refine_request = 0; // requested refinement steps, 0 = automatic
     int nbe_l = 0, nbe_r = 0, nbv_l = 0, nbv_r = 0, odd = 0;
     int offset = 0;
     double dx = 0;
@@ -204,11 +256,15 @@ struct BuilderPlan {
     TiledHost tiled;
     mutable DeviceBuffer t_Lf, t_rDf, t_gf, t_Ub, t_rDb, t_gb, t_Mf, t_Mb, t_G, t_rt;
     mutable TiledView tview{};
+    RefineHost refine;
+    mutable DeviceBuffer r_rt, r_Mf, r_Mb, r_rq, r_gr, r_gc, r_gv, r_gseg, r_delta;
+    mutable RefineView rview{};
     mutable std::once_flag uploaded;
     void ensure_device() const;
 };
 
 void make_tiled(BuilderPlan& p);
+void make_refine(BuilderPlan& p);
 
 struct EvaluatorPlan {
     std::shared_ptr<BSplinesPlan> bs;
@@ -234,6 +290,8 @@ void launch_build_generic(BuilderPlan const& p, double* coef, Layout cl, const d
                           const double* dl, const double* dr, Layout dlay, int64_t nbatch, hipStream_t stream);
 bool launch_build_tiled(BuilderPlan const& p, double* coef, Layout cl, const double* vals, Layout vl,
                         const double* dl, const double* dr, Layout dlay, int64_t nbatch, hipStream_t stream);
+bool launch_build_refine(BuilderPlan const& p, double* coef, Layout cl, const double* vals, Layout vl,
+                         const double* dl, const double* dr, Layout dlay, int64_t nbatch, hipStream_t stream);
 void launch_build(BuilderPlan const& p, double* coef, Layout cl, const double* vals, Layout vl, const double* dl,
                   const double* dr, Layout dlay, int64_t nbatch, hipStream_t stream);
 void launch_build_vjp(BuilderPlan const& p, const double* gcoef, Layout cl, double* gvals, Layout vl, double* gdl,

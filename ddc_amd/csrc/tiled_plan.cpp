@@ -61,29 +61,28 @@ Mat identity(int k)
     return m;
 }
 
-} // namespace
-
-void make_tiled(BuilderPlan& p)
+// Reason the tiled kernels cannot take this matrix, or nullptr.
+const char* tiled_unsupported(HostProblem const& pr)
 {
-    TiledHost& t = p.tiled;
+    int const nq = pr.q.n;
+    if (!pr.blocked) return "plain dense matrix";
+    if (nq < 256) return "main block too small";
+    if (nq > 2048) return "main block larger than 2048 rows";
+    if (pr.nb() > kMaxBorderTiled) return "border larger than 8";
+    if (int(pr.bl_v.size()) > kMaxLambda) return "too many border couplings";
+    return nullptr;
+}
+
+// Sweep tables of the factored main block on a K x L tiling; the carry
+// windows are cut where the transfer products fall below tol.
+void tile_tables(HostProblem const& pr, int K, int L, double tol, TiledHost& t)
+{
     t = TiledHost{};
-    HostProblem const& pr = p.prob;
+    if (char const* why = tiled_unsupported(pr)) { t.why = why; return; }
     HostQ const& q = pr.q;
     int const nq = q.n;
-    if (!pr.blocked) { t.why = "plain dense matrix"; return; }
-    if (nq < 256) { t.why = "main block too small"; return; }
-    if (nq > 2048) { t.why = "main block larger than 2048 rows"; return; }
-    if (pr.nb() > kMaxBorderTiled) { t.why = "border larger than 8"; return; }
-    if (int(pr.bl_v.size()) > kMaxLambda) { t.why = "too many border couplings"; return; }
-    // tile shape: K segments of L rows (K*L >= nq), 1024 threads, LB = 1024/K
-    // lines (32 lines give 256-byte rows in batch-contiguous layouts)
-    if (nq <= 1024) {
-        t.K = 32;
-        t.L = 32;
-    } else {
-        t.K = 64;
-        t.L = 32;
-    }
+    t.K = K;
+    t.L = L;
     int const npad = t.K * t.L;
     switch (q.type) {
     case QType::PDSTridiag:
@@ -187,7 +186,7 @@ void make_tiled(BuilderPlan& p)
                 int const src = forward ? s - w : s + w;
                 if (src < 0 || src >= t.K) break;
                 P = matmul(P, T[src], k);
-                if (maxabs(P) < kCarryTol) break;
+                if (maxabs(P) < tol) break;
                 ++w;
                 if (w > kMaxWindow) return false;
             }
@@ -230,6 +229,165 @@ void make_tiled(BuilderPlan& p)
         for (int c = 0; c < nbd; ++c) r[2 * kf + 2 * kb + 2 + c] = t.G[size_t(i) * nbd + c];
     }
     t.ok = true;
+}
+
+} // namespace
+
+void make_tiled(BuilderPlan& p)
+{
+    // K segments of L rows (K*L >= nq) on 1024 threads, LB = 1024/K lines
+    // (32 lines give 256-byte rows in batch-contiguous layouts)
+    int const K = p.prob.q.n <= 1024 ? 32 : 64;
+    tile_tables(p.prob, K, 32, kCarryTol, p.tiled);
+}
+
+namespace {
+
+// Host model of one approximate solve S of the refinement (sequential
+// sweeps in FP32 with the FP32 tables, border steps in FP64), in the 2x2
+// index space: b1 (nq) and b2 (nb) in, x1 and x2 out.
+void approx_solve(RefineHost const& R, HostProblem const& pr, std::vector<double> const& b1,
+                  std::vector<double> const& b2, std::vector<double>& x1, std::vector<double>& x2)
+{
+    TiledHost const& t = R.t;
+    int const nq = pr.q.n, nb = pr.nb(), kf = t.kf, kb = t.kb, rs = R.rs;
+    auto at = [&](int i, int f) { return R.rt32[size_t(i) * rs + f]; };
+    int const oLf = 0, oUb = 2 * kf, orDb = 2 * kf + kb, orDf = 2 * kf + 2 * kb + 1, oG = 2 * kf + 2 * kb + 2;
+    std::vector<float> y(nq);
+    for (int i = 0; i < nq; ++i) {
+        float tt = float(b1[i]);
+        for (int k = kf; k >= 1; --k)
+            if (i - k >= 0) tt = std::fma(-y[i - k], at(i, oLf + k - 1), tt);
+        if (t.fwd_div) tt = tt * at(i, orDf);
+        y[i] = tt;
+    }
+    for (int i = nq - 1; i >= 0; --i) {
+        float tt;
+        if (t.bwd_divfirst) {
+            tt = y[i] * at(i, orDb);
+            if (i + 1 < nq) tt = std::fma(-y[i + 1], at(i, oUb), tt);
+        } else {
+            tt = y[i];
+            for (int k = kb; k >= 1; --k)
+                if (i + k < nq) tt = std::fma(-y[i + k], at(i, oUb + k - 1), tt);
+            tt = tt * at(i, orDb);
+        }
+        y[i] = tt;
+    }
+    x2.assign(b2.begin(), b2.end());
+    for (size_t z = 0; z < pr.bl_v.size(); ++z) x2[pr.bl_r[z]] -= pr.bl_v[z] * double(y[pr.bl_c[z]]);
+    if (nb > 0) pr.schur.solve(x2.data(), 1, false);
+    x1.resize(nq);
+    for (int i = 0; i < nq; ++i) {
+        float tt = y[i];
+        for (int c = 0; c < nb; ++c) tt = std::fma(-at(i, oG + c), float(x2[c]), tt);
+        x1[i] = double(tt);
+    }
+}
+
+// r = b - A x with the assembled matrix, in the 2x2 index space.
+void residual(RefineHost const& R, HostProblem const& pr, std::vector<double> const& b1,
+              std::vector<double> const& b2, std::vector<double> const& x1, std::vector<double> const& x2,
+              std::vector<double>& r1, std::vector<double>& r2)
+{
+    int const nq = pr.q.n, nb = pr.nb(), kq = R.kq, w = 2 * kq + 1;
+    r1.resize(nq);
+    for (int i = 0; i < nq; ++i) {
+        double tt = b1[i];
+        for (int j = 0; j < w; ++j) {
+            int const c = i - kq + j;
+            if (c >= 0 && c < nq) tt = std::fma(-R.rq[size_t(i) * w + j], x1[c], tt);
+        }
+        r1[i] = tt;
+    }
+    for (size_t z = 0; z < R.g_v.size(); ++z) r1[R.g_r[z]] -= R.g_v[z] * x2[R.g_c[z]];
+    r2.assign(b2.begin(), b2.end());
+    for (size_t z = 0; z < pr.bl_v.size(); ++z) r2[pr.bl_r[z]] -= pr.bl_v[z] * x1[pr.bl_c[z]];
+    for (int k = 0; k < nb; ++k)
+        for (int c = 0; c < nb; ++c) r2[k] -= R.delta[size_t(k) * nb + c] * x2[c];
+}
+
+} // namespace
+
+void make_refine(BuilderPlan& p)
+{
+    RefineHost& R = p.refine;
+    R = RefineHost{};
+    HostProblem const& pr = p.prob;
+    // L = 16 rows per thread leaves registers for the FP64 right-hand side
+    // and solution next to the FP32 work rows
+    int const K = pr.q.n <= 1024 ? 64 : 128;
+    tile_tables(pr, K, 16, 0x1p-30, R.t);
+    if (!R.t.ok) { R.why = R.t.why; return; }
+    TiledHost const& t = R.t;
+    HostQ const& q0 = pr.q_orig;
+    if (q0.kl != q0.ku || q0.kl != t.kf) { R.why = "unsymmetric band"; return; }
+    int const nq = q0.n, nb = pr.nb(), npad = t.K * t.L;
+    R.rs = t.rs;
+    R.rt32.assign(t.rt.begin(), t.rt.end());
+    R.Mf32.assign(t.Mf.begin(), t.Mf.end());
+    R.Mb32.assign(t.Mb.begin(), t.Mb.end());
+    R.kq = q0.kl;
+    int const w = 2 * R.kq + 1;
+    R.rq.assign(size_t(npad) * w, 0.0);
+    for (int i = 0; i < nq; ++i)
+        for (int j = 0; j < w; ++j) {
+            int const c = i - R.kq + j;
+            if (c >= 0 && c < nq) R.rq[size_t(i) * w + j] = q0.get(i, c);
+        }
+    for (int i = 0; i < nq; ++i)
+        for (int c = 0; c < nb; ++c) {
+            double const a = pr.gamma_orig[size_t(i) * nb + c];
+            if (a != 0.0) {
+                R.g_r.push_back(i);
+                R.g_c.push_back(c);
+                R.g_v.push_back(a);
+            }
+        }
+    R.g_seg.assign(t.K + 1, 0);
+    for (int s = 0, z = 0; s <= t.K; ++s) {
+        while (z < int(R.g_r.size()) && R.g_r[z] < s * t.L) ++z;
+        R.g_seg[s] = z;
+    }
+    R.delta = pr.delta_orig;
+    // contraction of the refinement on random right-hand sides
+    uint64_t st = 0x9E3779B97F4A7C15ull;
+    auto rnd = [&] {
+        st ^= st << 13;
+        st ^= st >> 7;
+        st ^= st << 17;
+        return double(int64_t(st >> 11) - (int64_t(1) << 52)) * 0x1p-52;
+    };
+    std::vector<double> b1(nq), b2(nb), x1, x2, r1, r2, d1, d2;
+    double rho = 0.0;
+    for (int trial = 0; trial < 3; ++trial) {
+        for (double& v : b1) v = rnd();
+        for (double& v : b2) v = rnd();
+        approx_solve(R, pr, b1, b2, x1, x2);
+        residual(R, pr, b1, b2, x1, x2, r1, r2);
+        approx_solve(R, pr, r1, r2, d1, d2);
+        double num = 0.0, den = 0.0;
+        for (int i = 0; i < nq; ++i) {
+            num = std::max(num, std::fabs(d1[i]));
+            den = std::max(den, std::fabs(x1[i]));
+        }
+        for (int k = 0; k < nb; ++k) {
+            num = std::max(num, std::fabs(d2[k]));
+            den = std::max(den, std::fabs(x2[k]));
+        }
+        rho = std::max(rho, den > 0.0 ? num / den : 1.0);
+    }
+    R.rho = rho;
+    R.shift = rho > 0.0 ? std::clamp(int(std::floor(-std::log2(rho))), 0, 60) : 60;
+    R.steps = 0;
+    for (int k = 1; k <= 2; ++k)
+        if (std::pow(2.0 * rho, k + 1) <= 1e-12) {
+            R.steps = k;
+            break;
+        }
+    if (R.steps == 0) { R.why = "FP32 sweeps do not contract enough for 2 refinement steps"; return; }
+    if (p.refine_request > 0) R.steps = std::max(R.steps, p.refine_request);
+    R.ok = true;
 }
 
 } // namespace ddcx

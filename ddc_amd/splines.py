@@ -186,7 +186,7 @@ class SplineBuilder:
     """ddc::SplineBuilder<Exec, Mem, BSplines, DDimI, BCl, BCu, Solver>."""
 
     def __init__(self, bsplines, lower=None, upper=None, solver=SplineSolver.LAPACK, points=None,
-                 precision=Precision.FP64):
+                 precision=Precision.FP64, refine_steps=0):
         if lower is None:
             lower = SplineBuilderClosure.PERIODIC if bsplines.is_periodic() else SplineBuilderClosure.GREVILLE
         if upper is None:
@@ -198,7 +198,7 @@ class SplineBuilder:
             pts = np.ascontiguousarray(np.asarray(points, dtype=np.float64))
         desc = ddcx_builder_desc(int(self.lower), int(self.upper), int(solver), int(precision),
                                  C.c_void_p(pts.ctypes.data) if pts is not None else None,
-                                 0 if pts is None else pts.size)
+                                 0 if pts is None else pts.size, int(refine_steps))
         h = C.c_void_p()
         call("ddcx_builder_create", bsplines._h, C.byref(desc), C.byref(h))
         self._h = h

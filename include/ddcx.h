@@ -98,6 +98,10 @@ typedef struct ddcx_builder_desc {
     ddcx_precision precision;
     const double* points;
     int npoints;
+    /* DDCX_FP32_REFINE: refinement steps after the first FP32 solve; 0 picks
+     * the smallest count (1 or 2) whose error bound from the contraction
+     * measured at create time is <= 1e-12, 1 or 2 requests that count. */
+    int refine_steps;
 } ddcx_builder_desc;
 
 typedef struct ddcx_builder_info {
@@ -112,8 +116,11 @@ typedef struct ddcx_builder_info {
     int main_size;    /* size of the band main block */
     int border;       /* nb = n - main_size */
     double dx;        /* m_dx, average cell size */
-    int kernel;       /* build kernel used: 0 generic (one thread per line), 1 tiled (register-resident) */
+    int kernel;       /* build kernel used: 0 generic (one thread per line), 1 tiled (register-resident),
+                         2 tiled FP32 sweeps with FP64 refinement (precision DDCX_FP32_REFINE) */
     int pivoted;      /* 1 if the band factorisation of the main block interchanged rows */
+    int refine_steps; /* kernel 2: refinement steps after the first FP32 solve (1 or 2), else 0 */
+    double refine_contraction; /* kernel 2: relative error of one FP32 solve, measured at create */
 } ddcx_builder_info;
 
 ddcx_status ddcx_bsplines_create(const ddcx_bsplines_desc* desc, ddcx_bsplines** out);

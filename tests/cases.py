@@ -49,14 +49,15 @@ class Case:
             return O.Spline(self.degree, True, self.bcl, self.bcu, ncells=self.ncells, rmin=0.0, rmax=1.0)
         return O.Spline(self.degree, False, self.bcl, self.bcu, breaks=self.breaks)
 
-    def product(self):
+    def product(self, precision=0, refine_steps=0):
         import ddc_amd as D
 
         if self.uniform:
             bs = D.UniformBSplines(self.degree, 0.0, 1.0, self.ncells, periodic=self.periodic)
         else:
             bs = D.NonUniformBSplines(self.degree, self.breaks, periodic=self.periodic)
-        builder = D.SplineBuilder(bs, D.SplineBuilderClosure(self.bcl), D.SplineBuilderClosure(self.bcu))
+        builder = D.SplineBuilder(bs, D.SplineBuilderClosure(self.bcl), D.SplineBuilderClosure(self.bcu),
+                                  precision=D.Precision(precision), refine_steps=refine_steps)
         return bs, builder
 
 
@@ -110,3 +111,20 @@ TILED_CASES = [
     Case(5, False, HH, ncells=1200, mesh="uniform"),
     Case(3, True, P, ncells=1800),
 ]
+
+# The precision-form workload: Hermite closures on a geometrically stretched
+# mesh (ratio 1.05 over 2048 cells, smallest cell ~2e-45 of the domain).
+STRETCHED_CASES = [
+    Case(3, False, H, ncells=2048, mesh="geometric", ratio=1.05),
+    Case(5, False, H, ncells=2048, mesh="geometric", ratio=1.05),
+]
+
+
+def box_filtered_normal(npts, nbatch, seed, width=9):
+    """N(0, 1) values smoothed along the interpolation axis by a width-point
+    box filter with clamped ends (one line per column)."""
+    rng = np.random.default_rng(seed)
+    raw = rng.standard_normal((npts, nbatch))
+    h = width // 2
+    idx = np.clip(np.arange(npts)[:, None] + np.arange(-h, h + 1)[None, :], 0, npts - 1)
+    return raw[idx].mean(axis=1)
