@@ -213,6 +213,7 @@ __global__ void __launch_bounds__(LB* K) build_refine_kernel(RefineView rv, Buil
                 for (int k = 0; k < kNbMax; ++k)
                     if (k < nb) r2s[k][ln] = t2[k];
             }
+            __syncthreads(); // halo reads of xch end before the sweeps reuse it
         }
 
         // ---------------- FP32 forward sweep ----------------

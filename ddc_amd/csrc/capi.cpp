@@ -280,4 +280,21 @@ ddcx_status ddcx_evaluator2d_eval(const ddcx_evaluator* e1, const ddcx_evaluator
     });
 }
 
+ddcx_status ddcx_evaluator2d_eval_vjp(const ddcx_evaluator* e1, const ddcx_evaluator* e2, const double* coords,
+                                      const double* w, double* gx, int64_t npts, const double* coef, double* gcoef,
+                                      int64_t coef_batch_stride, int64_t st1, int64_t st2, int64_t nbatch,
+                                      void* stream)
+{
+    return guarded([&] {
+        require(e1 && e2, "null evaluator");
+        require(npts >= 0 && nbatch >= 0, "negative size");
+        if (nbatch == 0) return;
+        require(coef && gcoef, "null data pointer");
+        require(npts == 0 || (coords && w && gx), "null data pointer");
+        int const rules[4] = {rule_of(e1->lower), rule_of(e1->upper), rule_of(e2->lower), rule_of(e2->upper)};
+        ddcx::launch_eval2d_vjp(*e1->bs, *e2->bs, rules, coords, w, gx, npts, coef, gcoef, coef_batch_stride, st1,
+                                st2, nbatch, static_cast<hipStream_t>(stream));
+    });
+}
+
 } // extern "C"

@@ -201,6 +201,81 @@ __global__ void __launch_bounds__(256) eval2d_kernel(BSplinesView b1, BSplinesVi
     out[t] = y;
 }
 
+// Adjoint of the 2D value form: gx = w * (df/dx1, df/dx2) where the
+// coordinate reaches the spline (a coordinate clamped by the constant rule
+// has zero gradient, the Null rule zeroes both), and the coefficient
+// gradient w * B1_i(x1) B2_j(x2) scattered with atomics.
+template <int D1, int D2>
+__global__ void __launch_bounds__(256) eval2d_vjp_kernel(BSplinesView b1, BSplinesView b2, int4 rules,
+                                                         const double* __restrict__ coords,
+                                                         const double* __restrict__ w, double* __restrict__ gx,
+                                                         int64_t npts, const double* __restrict__ coef,
+                                                         double* __restrict__ gcoef, int64_t cbs, int64_t st1,
+                                                         int64_t st2, int64_t nbatch)
+{
+    int64_t const t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= npts * nbatch) return;
+    int64_t const l = t / npts;
+    double x1 = coords[2 * t];
+    double x2 = coords[2 * t + 1];
+    double const wt = w[t];
+    bool in1 = true, in2 = true;
+    if (b1.periodic && (x1 < b1.rmin || x1 > b1.rmax)) x1 -= floor((x1 - b1.rmin) / b1.length) * b1.length;
+    if (b2.periodic && (x2 < b2.rmin || x2 > b2.rmax)) x2 -= floor((x2 - b2.rmin) / b2.length) * b2.length;
+    auto clamp2 = [](double v, BSplinesView const& bb, bool& in) {
+        if (v < bb.rmin || v > bb.rmax) in = false;
+        return v < bb.rmin ? bb.rmin : (v > bb.rmax ? bb.rmax : v);
+    };
+    if (!b1.periodic && (x1 < b1.rmin || x1 > b1.rmax)) {
+        int const rule = x1 < b1.rmin ? rules.x : rules.y;
+        if (rule == DDCX_EXTRAP_NULL) {
+            gx[2 * t] = 0.0;
+            gx[2 * t + 1] = 0.0;
+            return;
+        }
+        x1 = clamp2(x1, b1, in1);
+        if (!b2.periodic) x2 = clamp2(x2, b2, in2);
+    }
+    if (!b2.periodic && (x2 < b2.rmin || x2 > b2.rmax)) {
+        int const rule = x2 < b2.rmin ? rules.z : rules.w;
+        if (rule == DDCX_EXTRAP_NULL) {
+            gx[2 * t] = 0.0;
+            gx[2 * t + 1] = 0.0;
+            return;
+        }
+        x2 = clamp2(x2, b2, in2);
+        if (!b1.periodic) x1 = clamp2(x1, b1, in1);
+    }
+    double e1[(D1 + 1) * 2], e2[(D2 + 1) * 2];
+    int const j1 = eval_basis_and_n_derivs<D1, 1>(b1, b1.knots, x1, e1);
+    int const j2 = eval_basis_and_n_derivs<D2, 1>(b2, b2.knots, x2, e2);
+    const double* c = coef + l * cbs;
+    double* gc = gcoef + l * cbs;
+    double f1 = 0.0, f2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < D1 + 1; ++i)
+#pragma unroll
+        for (int j = 0; j < D2 + 1; ++j) {
+            int64_t const o = int64_t(j1 + i) * st1 + int64_t(j2 + j) * st2;
+            double const cij = c[o];
+            f1 += cij * e1[i * 2 + 1] * e2[j * 2];
+            f2 += cij * e1[i * 2] * e2[j * 2 + 1];
+            atomicAdd(&gc[o], wt * e1[i * 2] * e2[j * 2]);
+        }
+    gx[2 * t] = in1 ? wt * f1 : 0.0;
+    gx[2 * t + 1] = in2 ? wt * f2 : 0.0;
+}
+
+__global__ void zero_coef2d_kernel(double* __restrict__ g, int s1, int s2, int64_t cbs, int64_t st1, int64_t st2,
+                                   int64_t nbatch)
+{
+    int64_t const t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    int64_t const per = int64_t(s1) * s2;
+    if (t >= per * nbatch) return;
+    int64_t const l = t / per, r = t % per;
+    g[l * cbs + (r / s2) * st1 + (r % s2) * st2] = 0.0;
+}
+
 unsigned grid_for(int64_t n, int threads)
 {
     int64_t const b = (n + threads - 1) / threads;
@@ -334,6 +409,48 @@ void launch_eval2d(BSplinesPlan const& bp1, BSplinesPlan const& bp2, const int* 
     }
     if (!launched) throw Error(DDCX_ERR_UNSUPPORTED, "2D evaluation supports degrees 1..5 and derivative orders <= 2");
     hip_check(hipGetLastError(), "eval2d_kernel launch");
+}
+
+void launch_eval2d_vjp(BSplinesPlan const& bp1, BSplinesPlan const& bp2, const int* rules, const double* coords,
+                       const double* w, double* gx, int64_t npts, const double* coef, double* gcoef, int64_t cbs,
+                       int64_t st1, int64_t st2, int64_t nbatch, hipStream_t stream)
+{
+    if (nbatch == 0) return;
+    bp1.ensure_device();
+    bp2.ensure_device();
+    BSplinesView const b1 = bp1.device_view(), b2 = bp2.device_view();
+    hipLaunchKernelGGL(zero_coef2d_kernel, dim3(grid_for(int64_t(bp1.size()) * bp2.size() * nbatch, 256)), dim3(256),
+                       0, stream, gcoef, bp1.size(), bp2.size(), cbs, st1, st2, nbatch);
+    hip_check(hipGetLastError(), "zero_coef2d_kernel launch");
+    if (npts == 0) return;
+    int4 const r{rules[0], rules[1], rules[2], rules[3]};
+    unsigned const grid = grid_for(npts * nbatch, 256);
+    bool launched = false;
+    auto go = [&]<int D1, int D2>() {
+        hipLaunchKernelGGL((eval2d_vjp_kernel<D1, D2>), dim3(grid), dim3(256), 0, stream, b1, b2, r, coords, w, gx,
+                           npts, coef, gcoef, cbs, st1, st2, nbatch);
+        launched = true;
+    };
+    auto outer = [&]<int D1>() {
+        switch (b2.degree) {
+        case 1: go.template operator()<D1, 1>(); break;
+        case 2: go.template operator()<D1, 2>(); break;
+        case 3: go.template operator()<D1, 3>(); break;
+        case 4: go.template operator()<D1, 4>(); break;
+        case 5: go.template operator()<D1, 5>(); break;
+        default: break;
+        }
+    };
+    switch (b1.degree) {
+    case 1: outer.template operator()<1>(); break;
+    case 2: outer.template operator()<2>(); break;
+    case 3: outer.template operator()<3>(); break;
+    case 4: outer.template operator()<4>(); break;
+    case 5: outer.template operator()<5>(); break;
+    default: break;
+    }
+    if (!launched) throw Error(DDCX_ERR_UNSUPPORTED, "2D adjoint supports degrees 1..5");
+    hip_check(hipGetLastError(), "eval2d_vjp_kernel launch");
 }
 
 } // namespace ddcx

@@ -308,5 +308,8 @@ void launch_integrate(BSplinesPlan const& bs, double* out, int64_t ostride, cons
 void launch_eval2d(BSplinesPlan const& b1, BSplinesPlan const& b2, const int* rules, int d1, int d2, double* out,
                    const double* coords, int64_t npts, const double* coef, int64_t coef_batch_stride, int64_t st1,
                    int64_t st2, int64_t nbatch, hipStream_t stream);
+void launch_eval2d_vjp(BSplinesPlan const& b1, BSplinesPlan const& b2, const int* rules, const double* coords,
+                       const double* w, double* gx, int64_t npts, const double* coef, double* gcoef,
+                       int64_t coef_batch_stride, int64_t st1, int64_t st2, int64_t nbatch, hipStream_t stream);
 
 } // namespace ddcx

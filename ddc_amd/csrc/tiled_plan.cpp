@@ -95,17 +95,18 @@ void tile_tables(HostProblem const& pr, int K, int L, double tol, TiledHost& t)
         for (int j = 0; j < nq; ++j)
             if (q.ipiv[j] != j) { t.why = "pivoted band factor"; return; }
         // without interchanges the U factor keeps the upper bandwidth ku
+        // (the sweep uses ku even where the factor's outer diagonals are zero)
         int const kd = q.kl + q.ku;
-        int kbe = 0;
         for (int c = 0; c < nq; ++c)
-            for (int k = 1; k <= kd && c - k >= 0; ++k)
-                if (q.q[size_t(kd - k) * nq + c] != 0.0) kbe = std::max(kbe, k);
-        t.kf = q.kl; t.kb = kbe; t.fwd_div = 0; t.bwd_divfirst = 0;
+            for (int k = q.ku + 1; k <= kd && c - k >= 0; ++k)
+                if (q.q[size_t(kd - k) * nq + c] != 0.0) { t.why = "U factor wider than ku"; return; }
+        t.kf = q.kl; t.kb = q.ku; t.fwd_div = 0; t.bwd_divfirst = 0;
         break;
     }
     default: t.why = "dense main block"; return;
     }
     if (t.kf < 1 || t.kb < 1) { t.why = "diagonal main block"; return; }
+    if (t.kf > 4 || t.kb > 4) { t.why = "bandwidth above 4"; return; }
     int const kf = t.kf, kb = t.kb;
     t.Lf.assign(size_t(npad) * kf, 0.0);
     t.rDf.assign(npad, 1.0);

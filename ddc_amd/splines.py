@@ -365,6 +365,19 @@ class SplineBuilder2D:
         self.builder2(s, self._tmp, axis=2)
         return spline
 
+    def build_vjp(self, gspline, gvals):
+        """Adjoint of __call__: the two transposed 1D solves in reverse order
+        (dim 2, then dim 1).  gspline: (batch, size1, size2); gvals: (batch, n1, n2)."""
+        squeeze = gspline.dim() == 2
+        g = gspline.unsqueeze(0) if squeeze else gspline
+        gv = gvals.unsqueeze(0) if squeeze else gvals
+        nb, n1, n2 = gv.shape
+        shape1 = (nb, self.builder1.bsplines.size(), n2)
+        tmp = torch.empty(shape1, dtype=torch.float64, device=g.device)
+        self.builder2.build_vjp(g, tmp, axis=2)
+        self.builder1.build_vjp(tmp, gv, axis=1)
+        return gvals
+
 
 class SplineEvaluator2D:
     """ddc::SplineEvaluator2D (spline_evaluator_2d.hpp:37-826) on (batch, size1, size2) coefficients."""
@@ -384,6 +397,24 @@ class SplineEvaluator2D:
         call("ddcx_evaluator2d_eval", self.ev1._h, self.ev2._h, int(order1), int(order2), _ptr(spline_eval), _ptr(c),
              npts, _ptr(coef), coef.stride(0), coef.stride(1), coef.stride(2), nb, _stream())
         return spline_eval
+
+    def eval_vjp(self, coords_eval, w, spline_coef, gx, gcoef):
+        """Adjoint of the value form: gcoef (overwritten, same shape as spline_coef)
+        and gx (same shape as coords_eval) from the weights w (batch, npts)."""
+        c = coords_eval.contiguous()
+        ww = w.contiguous()
+        squeeze = spline_coef.dim() == 2
+        coef = spline_coef.unsqueeze(0) if squeeze else spline_coef
+        gc = gcoef.unsqueeze(0) if squeeze else gcoef
+        if tuple(gc.stride()) != tuple(coef.stride()):
+            raise ValueError("gcoef must have the strides of spline_coef")
+        if not gx.is_contiguous():
+            raise ValueError("gx must be contiguous")
+        nb = coef.shape[0]
+        npts = c.numel() // (2 * nb)
+        call("ddcx_evaluator2d_eval_vjp", self.ev1._h, self.ev2._h, _ptr(c), _ptr(ww), _ptr(gx), npts, _ptr(coef),
+             _ptr(gc), coef.stride(0), coef.stride(1), coef.stride(2), nb, _stream())
+        return gx, gcoef
 
     def deriv_dim_1(self, spline_eval, coords_eval, spline_coef):
         return self(spline_eval, coords_eval, spline_coef, 1, 0)

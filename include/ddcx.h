@@ -197,6 +197,16 @@ ddcx_status ddcx_evaluator2d_eval(const ddcx_evaluator* e1, const ddcx_evaluator
                                   int64_t coef_batch_stride, int64_t st1, int64_t st2, int64_t nbatch,
                                   void* stream);
 
+/* Vector-Jacobian product of ddcx_evaluator2d_eval (value form; no reference
+ * counterpart): gcoef (same strides as coef, overwritten) = sum_p w_p
+ * B1(x1_p) (x) B2(x2_p) (tensor scatter-add); gx (interleaved pairs like
+ * coords) = w_p (df/dx1, df/dx2), 0 for a coordinate clamped by the constant
+ * rule and for both under the Null rule. */
+ddcx_status ddcx_evaluator2d_eval_vjp(const ddcx_evaluator* e1, const ddcx_evaluator* e2, const double* coords,
+                                      const double* w, double* gx, int64_t npts, const double* coef, double* gcoef,
+                                      int64_t coef_batch_stride, int64_t st1, int64_t st2, int64_t nbatch,
+                                      void* stream);
+
 const char* ddcx_last_error(void);
 const char* ddcx_version(void);
 

@@ -73,11 +73,14 @@ def test_refine_matches_oracle(case, cuda):
 
 
 def test_refine_line_scaling(cuda):
-    """Lines scaled by 2^-1000 .. 2^1000 and 1e-30 / 1e30, and an all-zero line."""
+    """Lines scaled by 2^-1000 .. 2^700 and 1e-30 / 1e30, and an all-zero line.
+    (On this mesh the Hermite border entries reach ~2^140, so the FP64 solve
+    itself overflows for data near 2^900; 2^700 is the top of the range.)"""
     case = STRETCHED_CASES[0]
     nbatch = 9
-    scales = np.array([2.0 ** -1000, 2.0 ** -300, 1e-30, 1.0, 1e30, 2.0 ** 300, 2.0 ** 1000, 0.0, 3.0])
+    scales = np.array([2.0 ** -1000, 2.0 ** -300, 1e-30, 1.0, 1e30, 2.0 ** 300, 2.0 ** 700, 0.0, 3.0])
     got, ref, ax, _ = _run(case, nbatch, "batch_contiguous", cuda, seed=5, scales=scales, smooth=True)
+    assert np.all(np.isfinite(ref))
     assert np.all(got[:, 7] == 0.0)
     assert _line_err(got, ref, ax) <= TOL
 
