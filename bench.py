@@ -27,11 +27,12 @@ def algorithmic_bytes(m, B, nc, P, nbe=0):
     return build, ev
 
 
-def read_traffic(name):
-    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
-    scaled by a copy-kernel calibration), recorded in profiles/<name>.md as
-    a line 'traffic_bytes_per_launch: <int>'."""
-    path = os.path.join(ROOT, "profiles", name + ".md")
+def read_traffic(kernel):
+    """HBM bytes per launch of `kernel` on the headline workload, measured with
+    rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel
+    calibration) and recorded in profiles/<kernel>__uniform_periodic_d3_1024x65536.md
+    as a line 'traffic_bytes_per_launch: <int>'."""
+    path = os.path.join(ROOT, "profiles", kernel + "__uniform_periodic_d3_1024x65536.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -70,6 +71,99 @@ def cpu_baseline(ncells, B_sample, P_line, seed):
                       f"{P_line} points/line, {reps} reps over {dt * reps:.1f} s"}
 
 
+def _timer(stream):
+    import torch
+
+    def timed(fn, n):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(n):
+            fn()
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / n * 1e-3
+
+    return timed
+
+
+def _line_values(xg, B, gen, axis_interp_first=True):
+    """f(x, y) = sin(2 pi x k_y) + 0.1 N(0, 1), k_y in {1..8}."""
+    import numpy as np
+    import torch
+
+    dev = xg.device
+    k = torch.randint(1, 9, (B,), device=dev, generator=gen).to(torch.float64)
+    v = torch.sin(2 * np.pi * xg[:, None] * k[None, :]) + 0.1 * torch.randn((xg.numel(), B), device=dev,
+                                                                           dtype=torch.float64, generator=gen)
+    return v if axis_interp_first else v.T.contiguous()
+
+
+def measure_config(D, bs, builder, B, layout, timed, n, gen, derivs=False):
+    """Build, eval and deriv(1) times for one 1D configuration (1024 uniform
+    random points per line, the interpolation extent)."""
+    import torch
+
+    dev = torch.device("cuda:0")
+    m, nc = builder.npoints(), bs.size()
+    xg = torch.from_numpy(builder.interpolation_points()).to(dev)
+    bc = layout == "batch_contiguous"
+    ax = 0 if bc else 1
+    vals = _line_values(xg, B, gen, bc)
+    coef = torch.empty((nc, B) if bc else (B, nc), device=dev, dtype=torch.float64)
+    P_line = 1024
+    lo, hi = bs.rmin(), bs.rmax()
+    coords = lo + (hi - lo) * torch.rand((P_line, B) if bc else (B, P_line), device=dev, dtype=torch.float64,
+                                         generator=gen)
+    out = torch.empty_like(coords)
+    ev = D.SplineEvaluator(bs, D.ConstantExtrapolationRule(), D.ConstantExtrapolationRule())
+    for _ in range(2):
+        builder(coef, vals, axis=ax)
+        ev(out, coords, coef, axis=ax)
+    t_b = timed(lambda: builder(coef, vals, axis=ax), n)
+    t_e = timed(lambda: ev(out, coords, coef, axis=ax), n)
+    t_d = timed(lambda: ev.deriv(1, out, coords, coef, axis=ax), n)
+    P = P_line * B
+    bb, be = algorithmic_bytes(m, B, nc, P)
+    return {"layout": layout, "build_us": round(t_b * 1e6, 1), "eval_us": round(t_e * 1e6, 1),
+            "deriv_us": round(t_d * 1e6, 1), "build_GBps": round(bb / t_b / 1e9, 1),
+            "eval_GBps": round(be / t_e / 1e9, 1), "deriv_GBps": round(be / t_d / 1e9, 1),
+            "build_eval_Mpts_s": round((m * B + P) / (t_b + t_e) / 1e6, 1)}
+
+
+def measure_precision_form(D, degree, B, timed, n, gen):
+    """Hermite builder on the geometric mesh x_k = (1.05^k - 1)/(1.05^2048 - 1):
+    FP64 and FP32-sweep + FP64-refinement builds on the same inputs.  The
+    error is measured against the FP64 GPU build, line by line."""
+    import numpy as np
+    import torch
+
+    dev = torch.device("cuda:0")
+    nc = 2048
+    br = (1.05 ** np.arange(nc + 1, dtype=np.float64) - 1.0) / (1.05 ** nc - 1.0)
+    bs = D.NonUniformBSplines(degree, br)
+    H = D.SplineBuilderClosure.HERMITE
+    b64 = D.SplineBuilder(bs, H, H)
+    b32 = D.SplineBuilder(bs, H, H, precision=D.Precision.FP32_REFINE)
+    m, nbv, size = b64.npoints(), b64.s_nbv_xmin, bs.size()
+    raw = torch.randn((m + 8, B), device=dev, dtype=torch.float64, generator=gen)
+    vals = torch.nn.functional.avg_pool1d(raw.T.unsqueeze(1), 9, stride=1).squeeze(1).T.contiguous()
+    dl = torch.randn((nbv, B), device=dev, dtype=torch.float64, generator=gen)
+    dr = torch.randn((nbv, B), device=dev, dtype=torch.float64, generator=gen)
+    c64 = torch.empty((size, B), device=dev, dtype=torch.float64)
+    c32 = torch.empty_like(c64)
+    for _ in range(2):
+        b64(c64, vals, dl, dr)
+        b32(c32, vals, dl, dr)
+    t64 = timed(lambda: b64(c64, vals, dl, dr), n)
+    t32 = timed(lambda: b32(c32, vals, dl, dr), n)
+    err = ((c32 - c64).abs().amax(0) / c64.abs().amax(0)).max().item()
+    by = 8 * B * (m + 2 * nbv) + 8 * B * size
+    return {"degree": degree, "fp64_build_us": round(t64 * 1e6, 1), "fp32_refine_build_us": round(t32 * 1e6, 1),
+            "fp64_GBps": round(by / t64 / 1e9, 1), "fp32_refine_GBps": round(by / t32 / 1e9, 1),
+            "speedup_fp32_refine_vs_fp64": round(t64 / t32, 3), "refine_steps": b32.info.refine_steps,
+            "max_line_rel_err_vs_fp64": float(f"{err:.3e}"), "fp32_refine_kernel": b32.info.kernel}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,6 +172,7 @@ def main():
     ap.add_argument("--ncells", type=int, default=1024)
     ap.add_argument("--batch", type=int, default=65536)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--headline-only", action="store_true", help="skip the other workloads")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the spline path shards by batch line with no communication; "
@@ -99,9 +194,7 @@ def main():
     g = torch.Generator(device=dev)
     g.manual_seed(0xDDC0002)
     xg = torch.from_numpy(builder.interpolation_points()).to(dev)
-    k = torch.randint(1, 9, (B,), device=dev, generator=g).to(torch.float64)
-    vals = torch.sin(2 * np.pi * xg[:, None] * k[None, :]) + 0.1 * torch.randn((m, B), device=dev, dtype=torch.float64,
-                                                                                 generator=g)
+    vals = _line_values(xg, B, g)
     coef = torch.empty((ncoef, B), device=dev, dtype=torch.float64)
     P_line = m
     coords = torch.rand((P_line, B), device=dev, dtype=torch.float64, generator=g)
@@ -123,17 +216,7 @@ def main():
     ms = (time.perf_counter() - t0) * 1e3 / args.steps
 
     # per-kernel durations with HIP events on the launch stream
-    stream = torch.cuda.current_stream()
-
-    def timed(fn, n):
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        for _ in range(n):
-            fn()
-        e1.record(stream)
-        e1.synchronize()
-        return e0.elapsed_time(e1) / n * 1e-3
-
+    timed = _timer(torch.cuda.current_stream())
     n_k = max(5, args.steps)
     t_build = timed(lambda: builder(coef, vals), n_k)
     t_eval = timed(lambda: ev(out, coords, coef), n_k)
@@ -144,7 +227,7 @@ def main():
     dominant = "build" if t_build >= t_eval else "eval"
     t_dom = max(t_build, t_eval)
     ach = (b_build if dominant == "build" else b_eval) / t_dom / 1e9
-    traffic = read_traffic(f"{dominant}_uniform_periodic_d3_1024x65536")
+    traffic = read_traffic("build_tiled_kernel" if dominant == "build" else "eval_tiled_kernel")
     value = (m * B + P) / (ms * 1e-3) / 1e6
     res = {
         "metric": "spline build+eval Mpts/s, %HBM peak; FP32+refine vs FP64 at <=1e-12 rel err",
@@ -169,6 +252,26 @@ def main():
                      "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic},
         "cpu_baseline": None,
     }
+    if not args.headline_only:
+        del vals, coef, coords, out
+        torch.cuda.empty_cache()
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(0xDDC0003)
+        other = {}
+        other["2a_interp_contiguous"] = measure_config(D, bs, builder, B, "interp_contiguous", timed, n_k, gen)
+        bs5 = D.UniformBSplines(5, 0.0, 1.0, nc, periodic=True)
+        other["2b_uniform_periodic_d5"] = measure_config(D, bs5, D.SplineBuilder(bs5), B, "batch_contiguous", timed,
+                                                         n_k, gen)
+        rng = np.random.default_rng(0xDDC0002)
+        br = np.concatenate([[0.0], np.sort(rng.uniform(0.0, 1.0, nc - 1)), [1.0]])
+        bsn = D.NonUniformBSplines(3, br)
+        G = D.SplineBuilderClosure.GREVILLE
+        other["2c_nonuniform_greville_d3"] = measure_config(D, bsn, D.SplineBuilder(bsn, G, G), B,
+                                                            "batch_contiguous", timed, n_k, gen)
+        torch.cuda.empty_cache()
+        other["3_precision_form_d3"] = measure_precision_form(D, 3, 32768, timed, n_k, gen)
+        other["3_precision_form_d5"] = measure_precision_form(D, 5, 32768, timed, n_k, gen)
+        res["workloads"] = other
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(nc, 4096, P_line, 0xDDC0002)
     print(json.dumps(res))

@@ -44,9 +44,8 @@ __global__ void __launch_bounds__(LB* K) build_refine_kernel(RefineView rv, Buil
     constexpr int KX = 2 * KQ > KE ? 2 * KQ : KE;
     constexpr int NT = LB * K;
     constexpr int W = 2 * KQ + 1;
-    // row-record field offsets: [Lf(KF) | gf(KF) | Ub(KB) | rDb | gb(KB) | rDf | G(nb)]
-    constexpr int oLf = 0, ogf = KF, oUb = 2 * KF, orDb = 2 * KF + KB, ogb = 2 * KF + KB + 1;
-    constexpr int orDf = 2 * KF + 2 * KB + 1, oG = 2 * KF + 2 * KB + 2;
+    // row-record field offsets: [Lf(KF) | Ub(KB) | rDb | rDf (PDS band only)]
+    constexpr int oLf = 0, oUb = KF, orDb = KF + KB, orDf = KF + KB + 1;
     __shared__ double xch[KX][K][LB]; // sweep states / solution halo
     __shared__ double lamx[kLamMax][LB];
     __shared__ double b2s[kNbMax][LB]; // border right-hand side
@@ -91,21 +90,11 @@ __global__ void __launch_bounds__(LB* K) build_refine_kernel(RefineView rv, Buil
 
     const float* T;
     if constexpr (TLDS) {
-        // at most 12 float4 per thread (RS <= 24, L = 16, LB >= 8)
+        // a few float4 per thread (RS <= 10, L = 16, K * L <= 2 * NT rows)
         int const n4 = (K * L * RS) / 4;
         const float4* src = reinterpret_cast<const float4*>(rv.rt);
         float4* dst = reinterpret_cast<float4*>(rt32_lds);
-        float4 tmp[12];
-#pragma unroll
-        for (int k = 0; k < 12; ++k) {
-            int const idx = tid + k * NT;
-            if (idx < n4) tmp[k] = src[idx];
-        }
-#pragma unroll
-        for (int k = 0; k < 12; ++k) {
-            int const idx = tid + k * NT;
-            if (idx < n4) dst[idx] = tmp[k];
-        }
+        for (int idx = tid; idx < n4; idx += NT) dst[idx] = src[idx];
         T = rt32_lds + r0 * RS;
     } else {
         T = rv.rt + size_t(r0) * RS;
@@ -174,7 +163,7 @@ __global__ void __launch_bounds__(LB* K) build_refine_kernel(RefineView rv, Buil
                 }
             }
             __syncthreads();
-            int const g0 = rv.g_seg[seg], g1 = rv.g_seg[seg + 1];
+            int const g0 = rv.gam_seg[seg], g1 = rv.gam_seg[seg + 1];
 #pragma unroll
             for (int i = 0; i < L; ++i) {
                 const double* Q = rv.rq + size_t(r0 + i) * W;
@@ -189,7 +178,7 @@ __global__ void __launch_bounds__(LB* K) build_refine_kernel(RefineView rv, Buil
                     t = fma(-Q[j], v, t);
                 }
                 for (int z = g0; z < g1; ++z)
-                    if (rv.g_r[z] == r0 + i) t = fma(-rv.g_v[z], x2s[rv.g_c[z]][ln], t);
+                    if (rv.gam_r[z] == r0 + i) t = fma(-rv.gam_v[z], x2s[rv.gam_c[z]][ln], t);
                 d[i] = float(t * sc);
                 chunk_fence<2>(i);
             }
@@ -250,12 +239,20 @@ __global__ void __launch_bounds__(LB* K) build_refine_kernel(RefineView rv, Buil
                         for (int b = 0; b < KF; ++b) in[a] = fmaf(mm[a * KF + b], float(xch[b][src][ln]), in[a]);
                 }
             }
+            if (seg > 0) {
 #pragma unroll
-            for (int i = 0; i < L; ++i) {
-                const float* R = T + i * RS;
+                for (int i = 0; i < L; ++i) {
+                    const float* R = T + i * RS;
+                    float t = 0.0f;
 #pragma unroll
-                for (int k = 0; k < KF; ++k) d[i] = fmaf(R[ogf + k], in[k], d[i]);
-                chunk_fence<4>(i);
+                    for (int k = KF; k >= 1; --k) t = fmaf(-in[k - 1], R[oLf + k - 1], t);
+                    if constexpr (FDIV) t = t * R[orDf];
+                    d[i] += t;
+#pragma unroll
+                    for (int k = KF - 1; k > 0; --k) in[k] = in[k - 1];
+                    in[0] = t;
+                    chunk_fence<4>(i);
+                }
             }
             __syncthreads();
         }
@@ -299,12 +296,25 @@ __global__ void __launch_bounds__(LB* K) build_refine_kernel(RefineView rv, Buil
                         for (int b = 0; b < KB; ++b) in[a] = fmaf(mm[a * KB + b], float(xch[b][src][ln]), in[a]);
                 }
             }
+            if (seg + 1 < K) {
 #pragma unroll
-            for (int i = 0; i < L; ++i) {
-                const float* R = T + i * RS;
+                for (int i = L - 1; i >= 0; --i) {
+                    const float* R = T + i * RS;
+                    float t;
+                    if constexpr (BDIVF) {
+                        t = -in[0] * R[oUb];
+                    } else {
+                        t = 0.0f;
 #pragma unroll
-                for (int k = 0; k < KB; ++k) d[i] = fmaf(R[ogb + k], in[k], d[i]);
-                chunk_fence<4>(i);
+                        for (int k = KB; k >= 1; --k) t = fmaf(-in[k - 1], R[oUb + k - 1], t);
+                        t = t * R[orDb];
+                    }
+                    d[i] += t;
+#pragma unroll
+                    for (int k = KB - 1; k > 0; --k) in[k] = in[k - 1];
+                    in[0] = t;
+                    chunk_fence<4>(i);
+                }
             }
         }
 
@@ -341,19 +351,23 @@ __global__ void __launch_bounds__(LB* K) build_refine_kernel(RefineView rv, Buil
             }
         }
         __syncthreads();
-        float e2f[kNbMax];
+        int const gblk = rv.gseg[seg];
+        if (gblk >= 0) { // Q^-1 gamma is nonzero only in the segments near the block ends
+            float e2f[kNbMax];
 #pragma unroll
-        for (int k = 0; k < kNbMax; ++k) e2f[k] = k < nb ? float(e2s[k][ln]) : 0.0f;
+            for (int k = 0; k < kNbMax; ++k) e2f[k] = k < nb ? float(e2s[k][ln]) : 0.0f;
+            const float* Gb = rv.Gblk + size_t(gblk) * L * nb;
 #pragma unroll
-        for (int i = 0; i < L; ++i) {
-            const float* R = T + i * RS + oG;
-            float t = d[i];
+            for (int i = 0; i < L; ++i) {
+                float t = d[i];
 #pragma unroll
-            for (int k = 0; k < kNbMax; ++k)
-                if (k < nb) t = fmaf(-R[k], e2f[k], t);
-            x[i] = x[i] + double(t) * isc;
-            chunk_fence<4>(i);
+                for (int k = 0; k < kNbMax; ++k)
+                    if (k < nb) t = fmaf(-Gb[i * nb + k], e2f[k], t);
+                d[i] = t;
+            }
         }
+#pragma unroll
+        for (int i = 0; i < L; ++i) x[i] = x[i] + double(d[i]) * isc;
     }
 
     // ---------------- store ----------------
@@ -425,7 +439,7 @@ void launch_cfg(BuilderPlan const& p, double* coef, Layout cl, const double* val
     constexpr size_t static_lds =
         size_t(KX * K * LB + kLamMax * LB + 4 * kNbMax * LB) * sizeof(double) + LB * sizeof(int);
     size_t const table = size_t(K) * L * p.refine.rs * sizeof(float);
-    bool const tlds = static_lds + table <= size_t(160 * 1024) && p.refine.rs <= 24;
+    bool const tlds = static_lds + table <= size_t(160 * 1024) && p.refine.rs <= 10;
     auto go = [&]<bool TL, int S>() {
         launch_one<K, L, KF, KB, FDIV, BDIVF, TL, S>(p, TL ? table : 0, coef, cl, vals, vl, dl, dr, dlay, nbatch,
                                                      stream);

@@ -9,9 +9,9 @@
 //      batch-contiguous layout);
 //   2. forward sweep, local pass with a zero incoming state -> outgoing
 //      state, exchanged through LDS; the exact incoming state of every
-//      segment is a short sum of precomputed transfer products; second pass
-//      with the true state (the same row operations, in the same order, as
-//      the sequential LAPACK sweep);
+//      segment is a short sum of precomputed transfer products, and the
+//      segment's rows are corrected by the homogeneous recurrence of the
+//      sweep started from that state;
 //   3. the same for the backward sweep;
 //   4. 2x2-block border (splines_linear_problem_2x2_blocks.cpp:275-298):
 //      lambda couplings gathered through LDS, the small Schur system solved
@@ -38,9 +38,8 @@ __global__ void __launch_bounds__(LB* K) build_tiled_kernel(TiledView tv, BuildV
 {
     constexpr int KE = KF > KB ? KF : KB;
     constexpr int NT = LB * K;
-    // row-record field offsets: [Lf(KF) | gf(KF) | Ub(KB) | rDb | gb(KB) | rDf | G(nb)]
-    constexpr int oLf = 0, ogf = KF, oUb = 2 * KF, orDb = 2 * KF + KB, ogb = 2 * KF + KB + 1;
-    constexpr int orDf = 2 * KF + 2 * KB + 1, oG = 2 * KF + 2 * KB + 2;
+    // row-record field offsets: [Lf(KF) | Ub(KB) | rDb | rDf (PDS band only)]
+    constexpr int oLf = 0, oUb = KF, orDb = KF + KB, orDf = KF + KB + 1;
     __shared__ double ex[KE][K][LB];
     __shared__ double lamx[kLamMax][LB];
     __shared__ double x2s[kNbMax][LB];
@@ -84,21 +83,11 @@ __global__ void __launch_bounds__(LB* K) build_tiled_kernel(TiledView tv, BuildV
     // per-row table: staged in LDS when it fits (one copy per workgroup)
     const double* T;
     if constexpr (TLDS) {
-        // RS <= 24 doubles per row, so at most 12 double2 per thread
+        // a few double2 per thread (RS <= 10 doubles per row, K * L <= 2 * NT rows)
         int const n2 = (K * L * RS) / 2;
         const double2* src = reinterpret_cast<const double2*>(tv.rt);
         double2* dst = reinterpret_cast<double2*>(rt_lds);
-        double2 tmp[12];
-#pragma unroll
-        for (int k = 0; k < 12; ++k) {
-            int const idx = tid + k * NT;
-            if (idx < n2) tmp[k] = src[idx];
-        }
-#pragma unroll
-        for (int k = 0; k < 12; ++k) {
-            int const idx = tid + k * NT;
-            if (idx < n2) dst[idx] = tmp[k];
-        }
+        for (int idx = tid; idx < n2; idx += NT) dst[idx] = src[idx];
         __syncthreads();
         T = rt_lds + r0 * RS;
     } else {
@@ -139,12 +128,21 @@ __global__ void __launch_bounds__(LB* K) build_tiled_kernel(TiledView tv, BuildV
                     for (int b = 0; b < KF; ++b) in[a] = fma(m[a * KF + b], ex[b][src][ln], in[a]);
             }
         }
+        if (seg > 0) {
+            // rows += homogeneous forward recurrence started from the incoming state
 #pragma unroll
-        for (int i = 0; i < L; ++i) {
-            const double* R = T + i * RS;
+            for (int i = 0; i < L; ++i) {
+                const double* R = T + i * RS;
+                double t = 0.0;
 #pragma unroll
-            for (int k = 0; k < KF; ++k) y[i] = fma(R[ogf + k], in[k], y[i]);
-            chunk_fence(i);
+                for (int k = KF; k >= 1; --k) t = fma(-in[k - 1], R[oLf + k - 1], t);
+                if constexpr (FDIV) t = t * R[orDf];
+                y[i] += t;
+#pragma unroll
+                for (int k = KF - 1; k > 0; --k) in[k] = in[k - 1];
+                in[0] = t;
+                chunk_fence(i);
+            }
         }
         __syncthreads();
     }
@@ -188,12 +186,26 @@ __global__ void __launch_bounds__(LB* K) build_tiled_kernel(TiledView tv, BuildV
                     for (int b = 0; b < KB; ++b) in[a] = fma(m[a * KB + b], ex[b][src][ln], in[a]);
             }
         }
+        if (seg + 1 < K) {
+            // rows += homogeneous backward recurrence started from the incoming state
 #pragma unroll
-        for (int i = 0; i < L; ++i) {
-            const double* R = T + i * RS;
+            for (int i = L - 1; i >= 0; --i) {
+                const double* R = T + i * RS;
+                double t;
+                if constexpr (BDIVF) {
+                    t = -in[0] * R[oUb];
+                } else {
+                    t = 0.0;
 #pragma unroll
-            for (int k = 0; k < KB; ++k) y[i] = fma(R[ogb + k], in[k], y[i]);
-            chunk_fence(i);
+                    for (int k = KB; k >= 1; --k) t = fma(-in[k - 1], R[oUb + k - 1], t);
+                    t = t * R[orDb];
+                }
+                y[i] += t;
+#pragma unroll
+                for (int k = KB - 1; k > 0; --k) in[k] = in[k - 1];
+                in[0] = t;
+                chunk_fence(i);
+            }
         }
     }
 
@@ -246,18 +258,21 @@ __global__ void __launch_bounds__(LB* K) build_tiled_kernel(TiledView tv, BuildV
             }
         }
         __syncthreads();
-        double x2[kNbMax];
+        int const gblk = tv.gseg[seg];
+        if (gblk >= 0) { // Q^-1 gamma is nonzero only in the segments near the block ends
+            double x2[kNbMax];
 #pragma unroll
-        for (int k = 0; k < kNbMax; ++k) x2[k] = k < nb ? x2s[k][ln] : 0.0;
+            for (int k = 0; k < kNbMax; ++k) x2[k] = k < nb ? x2s[k][ln] : 0.0;
+            const double* Gb = tv.Gblk + size_t(gblk) * L * nb;
 #pragma unroll
-        for (int i = 0; i < L; ++i) {
-            const double* R = T + i * RS + oG;
-            double t = y[i];
+            for (int i = 0; i < L; ++i) {
+                double t = y[i];
 #pragma unroll
-            for (int k = 0; k < kNbMax; ++k)
-                if (k < nb) t = fma(-R[k], x2[k], t);
-            y[i] = t;
-            chunk_fence(i);
+                for (int k = 0; k < kNbMax; ++k)
+                    if (k < nb) t = fma(-Gb[i * nb + k], x2[k], t);
+                y[i] = t;
+                chunk_fence(i);
+            }
         }
     }
 
@@ -296,7 +311,7 @@ void launch_cfg(BuilderPlan const& p, double* coef, Layout cl, const double* val
     constexpr size_t static_lds = size_t(KE * K * LB + kLamMax * LB + kNbMax * LB) * sizeof(double);
     size_t const table = size_t(K) * L * p.tiled.rs * sizeof(double);
     int64_t const blocks = (nbatch + LB - 1) / LB;
-    if (static_lds + table <= size_t(160 * 1024) && p.tiled.rs <= 24) {
+    if (static_lds + table <= size_t(160 * 1024) && p.tiled.rs <= 10) {
         auto kern = build_tiled_kernel<LB, K, L, KF, KB, FDIV, BDIVF, true>;
         hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                       int(table)),

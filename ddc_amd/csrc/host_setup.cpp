@@ -544,15 +544,11 @@ void BuilderPlan::ensure_device() const
         sv.bl_c = d_blc.as<int>();
         sv.bl_v = d_blv.as<double>();
         if (tiled.ok) {
-            t_Lf.upload(tiled.Lf);
-            t_rDf.upload(tiled.rDf);
-            t_gf.upload(tiled.gf);
-            t_Ub.upload(tiled.Ub);
-            t_rDb.upload(tiled.rDb);
-            t_gb.upload(tiled.gb);
             t_Mf.upload(tiled.Mf);
             t_Mb.upload(tiled.Mb);
-            t_G.upload(tiled.G);
+            t_rt.upload(tiled.rt);
+            t_gseg.upload(tiled.gseg);
+            t_Gblk.upload(tiled.Gblk);
             TiledView& tv = tview;
             tv.K = tiled.K;
             tv.L = tiled.L;
@@ -565,18 +561,12 @@ void BuilderPlan::ensure_device() const
             tv.top = prob.top;
             tv.q0 = bs->view.periodic ? 0 : prob.top - nbe_l;
             tv.bl_nnz = int(prob.bl_v.size());
-            tv.Lf = t_Lf.as<double>();
-            tv.rDf = t_rDf.as<double>();
-            tv.gf = t_gf.as<double>();
-            tv.Ub = t_Ub.as<double>();
-            tv.rDb = t_rDb.as<double>();
-            tv.gb = t_gb.as<double>();
+            tv.rs = tiled.rs;
+            tv.rt = t_rt.as<double>();
             tv.Mf = t_Mf.as<double>();
             tv.Mb = t_Mb.as<double>();
-            tv.G = t_G.as<double>();
-            t_rt.upload(tiled.rt);
-            tv.rt = t_rt.as<double>();
-            tv.rs = tiled.rs;
+            tv.gseg = t_gseg.as<int>();
+            tv.Gblk = t_Gblk.as<double>();
             tv.bl_r = sv.bl_r;
             tv.bl_c = sv.bl_c;
             tv.bl_v = sv.bl_v;
@@ -588,11 +578,13 @@ void BuilderPlan::ensure_device() const
             r_rt.upload(R.rt32);
             r_Mf.upload(R.Mf32);
             r_Mb.upload(R.Mb32);
+            r_gseg_blk.upload(R.t.gseg);
+            r_Gblk.upload(R.Gblk32);
             r_rq.upload(R.rq);
-            r_gr.upload(R.g_r);
-            r_gc.upload(R.g_c);
-            r_gv.upload(R.g_v);
-            r_gseg.upload(R.g_seg);
+            r_gr.upload(R.gam_r);
+            r_gc.upload(R.gam_c);
+            r_gv.upload(R.gam_v);
+            r_gamseg.upload(R.gam_seg);
             r_delta.upload(R.delta);
             RefineView& rv = rview;
             rv.K = R.t.K;
@@ -613,11 +605,13 @@ void BuilderPlan::ensure_device() const
             rv.rt = r_rt.as<float>();
             rv.Mf = r_Mf.as<float>();
             rv.Mb = r_Mb.as<float>();
+            rv.gseg = r_gseg_blk.as<int>();
+            rv.Gblk = r_Gblk.as<float>();
             rv.rq = r_rq.as<double>();
-            rv.g_r = r_gr.as<int>();
-            rv.g_c = r_gc.as<int>();
-            rv.g_v = r_gv.as<double>();
-            rv.g_seg = r_gseg.as<int>();
+            rv.gam_r = r_gr.as<int>();
+            rv.gam_c = r_gc.as<int>();
+            rv.gam_v = r_gv.as<double>();
+            rv.gam_seg = r_gamseg.as<int>();
             rv.delta = r_delta.as<double>();
             rv.bl_r = sv.bl_r;
             rv.bl_c = sv.bl_c;

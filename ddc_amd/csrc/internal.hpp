@@ -150,42 +150,42 @@ struct BuildView {
 // row-oriented form (the same row operations as the netlib column-oriented
 // sweeps the reference runs), so the state crossing a segment boundary is the
 // last kf solution values (forward) or the next kb ones (backward).  Each
-// segment is swept once with a zero incoming state; the true solution is the
-// local one plus the segment's response to its incoming state (gf / gb, per
-// row), and the incoming state itself is the short sum
+// segment is swept once with a zero incoming state; the incoming state is then
+// the short sum
 //   in(s) = sum_{w=1..W} P_{s,w} out_loc(s -/+ w)
 // of batch-independent transfer products, W chosen so that the first
-// neglected product is below 2^-60.  Divisions by pivots are multiplications
-// by reciprocals computed here.
+// neglected product is below a tolerance (2^-60 for the FP64 kernel), and the
+// true rows are the local ones plus the homogeneous recurrence of the sweep
+// started from that state.  Divisions by pivots are multiplications by
+// reciprocals computed here.
 struct TiledHost {
     bool ok = false;
     std::string why; // reason when !ok
     int K = 0, L = 0, kf = 0, kb = 0, wf = 0, wb = 0;
     int fwd_div = 0;      // forward rows are scaled by rDf (PDSBand)
     int bwd_divfirst = 0; // backward row form y*rD - x*e (PDSTridiag) instead of (y - sum)*rD
-    // per row (padded to K*L rows): coefficients, reciprocal pivots, responses
-    std::vector<double> Lf, rDf, gf; // Lf, gf: kf per row
-    std::vector<double> Ub, rDb, gb; // Ub, gb: kb per row
-    std::vector<double> Mf, Mb;      // K x W x k x k transfer products
-    std::vector<double> G;           // (K*L) x nb: Q^-1 gamma with |a| < 1e-14 zeroed
-    // The same tables interleaved per row, rs doubles per row:
-    // [Lf(kf) | gf(kf) | Ub(kb) | rDb | gb(kb) | rDf | G(nb) | pad]
+    // per row (padded to K*L rows): coefficients and reciprocal pivots
+    std::vector<double> Lf, rDf; // Lf: kf per row
+    std::vector<double> Ub, rDb; // Ub: kb per row
+    std::vector<double> Mf, Mb;  // K x W x k x k transfer products
+    // The sweep tables interleaved per row, rs doubles per row:
+    // [Lf(kf) | Ub(kb) | rDb | rDf]
     int rs = 0;
     std::vector<double> rt;
+    // Q^-1 gamma (|a| < 1e-14 zeroed) is nonzero only near the ends of the
+    // main block: gseg[s] is the index of segment s's L x nb block in Gblk,
+    // or -1 when the segment has no nonzero entry.
+    std::vector<int> gseg;
+    std::vector<double> Gblk;
 };
 
 struct TiledView {
     int K, L, kf, kb, wf, wb, nq, nb, top, q0, bl_nnz, rs;
     const double* rt;
-    const double* Lf;
-    const double* rDf;
-    const double* gf;
-    const double* Ub;
-    const double* rDb;
-    const double* gb;
     const double* Mf;
     const double* Mb;
-    const double* G;
+    const int* gseg;
+    const double* Gblk;
     const int* bl_r;
     const int* bl_c;
     const double* bl_v;
@@ -211,12 +211,12 @@ struct RefineHost {
     TiledHost t;                // tables in FP64 before rounding
     int rs = 0;                 // floats per row of rt32 (the TiledHost row layout)
     std::vector<float> rt32;    // interleaved row table rounded to FP32
-    std::vector<float> Mf32, Mb32;
+    std::vector<float> Mf32, Mb32, Gblk32;
     int kq = 0;                 // half bandwidth of the main block (kl == ku)
     std::vector<double> rq;     // (K*L) x (2kq+1): main block rows A(u, u-kq+j), zero padded
-    std::vector<int> g_r, g_c;  // gamma couplings, COO sorted by row
-    std::vector<double> g_v;
-    std::vector<int> g_seg;     // K+1 offsets into the gamma COO, by segment
+    std::vector<int> gam_r, gam_c;  // gamma couplings, COO sorted by row
+    std::vector<double> gam_v;
+    std::vector<int> gam_seg;     // K+1 offsets into the gamma COO, by segment
     std::vector<double> delta;  // nb x nb, row-major
     int steps = 0, shift = 0;
     double rho = 0;
@@ -227,11 +227,13 @@ struct RefineView {
     const float* rt;
     const float* Mf;
     const float* Mb;
+    const int* gseg;
+    const float* Gblk;
     const double* rq;
-    const int* g_r;
-    const int* g_c;
-    const double* g_v;
-    const int* g_seg;
+    const int* gam_r;
+    const int* gam_c;
+    const double* gam_v;
+    const int* gam_seg;
     const double* delta;
     const int* bl_r;
     const int* bl_c;
@@ -254,10 +256,10 @@ struct BuilderPlan {
     mutable DeviceBuffer d_q, d_ipiv, d_slu, d_sipiv, d_trr, d_trc, d_trv, d_blr, d_blc, d_blv;
     mutable BuildView view{}; // device pointers filled by ensure_device()
     TiledHost tiled;
-    mutable DeviceBuffer t_Lf, t_rDf, t_gf, t_Ub, t_rDb, t_gb, t_Mf, t_Mb, t_G, t_rt;
+    mutable DeviceBuffer t_Mf, t_Mb, t_rt, t_gseg, t_Gblk;
     mutable TiledView tview{};
     RefineHost refine;
-    mutable DeviceBuffer r_rt, r_Mf, r_Mb, r_rq, r_gr, r_gc, r_gv, r_gseg, r_delta;
+    mutable DeviceBuffer r_rt, r_Mf, r_Mb, r_gseg_blk, r_Gblk, r_rq, r_gr, r_gc, r_gv, r_gamseg, r_delta;
     mutable RefineView rview{};
     mutable std::once_flag uploaded;
     void ensure_device() const;

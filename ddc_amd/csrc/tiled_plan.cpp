@@ -144,9 +144,8 @@ void tile_tables(HostProblem const& pr, int K, int L, double tol, TiledHost& t)
         default: break;
         }
     }
-    // ---- per-row responses to the incoming state and segment transfer maps
-    t.gf.assign(size_t(npad) * kf, 0.0);
-    t.gb.assign(size_t(npad) * kb, 0.0);
+    // ---- segment transfer maps (responses of each row to a unit incoming state)
+    std::vector<double> gf(size_t(npad) * kf, 0.0), gb(size_t(npad) * kb, 0.0);
     std::vector<Mat> Tf(t.K), Tb(t.K);
     for (int s = 0; s < t.K; ++s) {
         int const r0 = s * t.L;
@@ -156,7 +155,7 @@ void tile_tables(HostProblem const& pr, int K, int L, double tol, TiledHost& t)
             st[j] = 1.0;
             for (int i = r0; i < r0 + t.L; ++i) {
                 double const y = fwd_row(t, i, 0.0, st.data());
-                t.gf[size_t(i) * kf + j] = y;
+                gf[size_t(i) * kf + j] = y;
                 for (int k = kf - 1; k > 0; --k) st[k] = st[k - 1];
                 st[0] = y;
             }
@@ -169,7 +168,7 @@ void tile_tables(HostProblem const& pr, int K, int L, double tol, TiledHost& t)
             st[j] = 1.0;
             for (int i = r0 + t.L - 1; i >= r0; --i) {
                 double const x = bwd_row(t, i, 0.0, st.data());
-                t.gb[size_t(i) * kb + j] = x;
+                gb[size_t(i) * kb + j] = x;
                 for (int k = kb - 1; k > 0; --k) st[k] = st[k - 1];
                 st[0] = x;
             }
@@ -207,27 +206,32 @@ void tile_tables(HostProblem const& pr, int K, int L, double tol, TiledHost& t)
     };
     if (!window(kf, Tf, true, t.Mf, t.wf)) { t.why = "forward carries decay too slowly"; return; }
     if (!window(kb, Tb, false, t.Mb, t.wb)) { t.why = "backward carries decay too slowly"; return; }
-    // ---- border table
+    // ---- Q^-1 gamma by segment (|a| < 1e-14 zeroed, as the reference's COO copy)
     int const nbd = pr.nb();
-    t.G.assign(size_t(npad) * std::max(nbd, 1), 0.0);
-    for (int i = 0; i < nq; ++i)
-        for (int c = 0; c < nbd; ++c) {
-            double const a = pr.gamma[size_t(i) * nbd + c];
-            t.G[size_t(i) * nbd + c] = std::fabs(a) >= 1e-14 ? a : 0.0;
-        }
-    // ---- interleaved row table
-    t.rs = 2 * kf + 2 * kb + 2 + nbd;
-    t.rs += t.rs & 1; // 16-byte aligned rows
+    t.gseg.assign(t.K, -1);
+    t.Gblk.clear();
+    for (int s = 0; s < t.K; ++s) {
+        bool any = false;
+        for (int i = s * t.L; i < std::min((s + 1) * t.L, nq) && !any; ++i)
+            for (int c = 0; c < nbd; ++c)
+                if (std::fabs(pr.gamma[size_t(i) * nbd + c]) >= 1e-14) any = true;
+        if (!any) continue;
+        t.gseg[s] = int(t.Gblk.size() / (size_t(t.L) * nbd));
+        for (int i = s * t.L; i < (s + 1) * t.L; ++i)
+            for (int c = 0; c < nbd; ++c) {
+                double const a = i < nq ? pr.gamma[size_t(i) * nbd + c] : 0.0;
+                t.Gblk.push_back(std::fabs(a) >= 1e-14 ? a : 0.0);
+            }
+    }
+    // ---- interleaved row table [Lf(kf) | Ub(kb) | rDb | rDf (PDS band only)]
+    t.rs = kf + kb + 1 + t.fwd_div;
     t.rt.assign(size_t(npad) * t.rs, 0.0);
     for (int i = 0; i < npad; ++i) {
         double* r = t.rt.data() + size_t(i) * t.rs;
         for (int k = 0; k < kf; ++k) r[k] = t.Lf[size_t(i) * kf + k];
-        for (int k = 0; k < kf; ++k) r[kf + k] = t.gf[size_t(i) * kf + k];
-        for (int k = 0; k < kb; ++k) r[2 * kf + k] = t.Ub[size_t(i) * kb + k];
-        r[2 * kf + kb] = t.rDb[i];
-        for (int k = 0; k < kb; ++k) r[2 * kf + kb + 1 + k] = t.gb[size_t(i) * kb + k];
-        r[2 * kf + 2 * kb + 1] = t.rDf[i];
-        for (int c = 0; c < nbd; ++c) r[2 * kf + 2 * kb + 2 + c] = t.G[size_t(i) * nbd + c];
+        for (int k = 0; k < kb; ++k) r[kf + k] = t.Ub[size_t(i) * kb + k];
+        r[kf + kb] = t.rDb[i];
+        if (t.fwd_div) r[kf + kb + 1] = t.rDf[i];
     }
     t.ok = true;
 }
@@ -253,7 +257,11 @@ void approx_solve(RefineHost const& R, HostProblem const& pr, std::vector<double
     TiledHost const& t = R.t;
     int const nq = pr.q.n, nb = pr.nb(), kf = t.kf, kb = t.kb, rs = R.rs;
     auto at = [&](int i, int f) { return R.rt32[size_t(i) * rs + f]; };
-    int const oLf = 0, oUb = 2 * kf, orDb = 2 * kf + kb, orDf = 2 * kf + 2 * kb + 1, oG = 2 * kf + 2 * kb + 2;
+    int const oLf = 0, oUb = kf, orDb = kf + kb, orDf = kf + kb + 1;
+    auto G = [&](int i, int c) {
+        int const b = t.gseg[i / t.L];
+        return b < 0 ? 0.0f : R.Gblk32[(size_t(b) * t.L + i % t.L) * nb + c];
+    };
     std::vector<float> y(nq);
     for (int i = 0; i < nq; ++i) {
         float tt = float(b1[i]);
@@ -281,7 +289,7 @@ void approx_solve(RefineHost const& R, HostProblem const& pr, std::vector<double
     x1.resize(nq);
     for (int i = 0; i < nq; ++i) {
         float tt = y[i];
-        for (int c = 0; c < nb; ++c) tt = std::fma(-at(i, oG + c), float(x2[c]), tt);
+        for (int c = 0; c < nb; ++c) tt = std::fma(-G(i, c), float(x2[c]), tt);
         x1[i] = double(tt);
     }
 }
@@ -301,7 +309,7 @@ void residual(RefineHost const& R, HostProblem const& pr, std::vector<double> co
         }
         r1[i] = tt;
     }
-    for (size_t z = 0; z < R.g_v.size(); ++z) r1[R.g_r[z]] -= R.g_v[z] * x2[R.g_c[z]];
+    for (size_t z = 0; z < R.gam_v.size(); ++z) r1[R.gam_r[z]] -= R.gam_v[z] * x2[R.gam_c[z]];
     r2.assign(b2.begin(), b2.end());
     for (size_t z = 0; z < pr.bl_v.size(); ++z) r2[pr.bl_r[z]] -= pr.bl_v[z] * x1[pr.bl_c[z]];
     for (int k = 0; k < nb; ++k)
@@ -328,6 +336,7 @@ void make_refine(BuilderPlan& p)
     R.rt32.assign(t.rt.begin(), t.rt.end());
     R.Mf32.assign(t.Mf.begin(), t.Mf.end());
     R.Mb32.assign(t.Mb.begin(), t.Mb.end());
+    R.Gblk32.assign(t.Gblk.begin(), t.Gblk.end());
     R.kq = q0.kl;
     int const w = 2 * R.kq + 1;
     R.rq.assign(size_t(npad) * w, 0.0);
@@ -340,15 +349,15 @@ void make_refine(BuilderPlan& p)
         for (int c = 0; c < nb; ++c) {
             double const a = pr.gamma_orig[size_t(i) * nb + c];
             if (a != 0.0) {
-                R.g_r.push_back(i);
-                R.g_c.push_back(c);
-                R.g_v.push_back(a);
+                R.gam_r.push_back(i);
+                R.gam_c.push_back(c);
+                R.gam_v.push_back(a);
             }
         }
-    R.g_seg.assign(t.K + 1, 0);
+    R.gam_seg.assign(t.K + 1, 0);
     for (int s = 0, z = 0; s <= t.K; ++s) {
-        while (z < int(R.g_r.size()) && R.g_r[z] < s * t.L) ++z;
-        R.g_seg[s] = z;
+        while (z < int(R.gam_r.size()) && R.gam_r[z] < s * t.L) ++z;
+        R.gam_seg[s] = z;
     }
     R.delta = pr.delta_orig;
     // contraction of the refinement on random right-hand sides
