@@ -106,3 +106,72 @@ def test_batched_layout_helper():
     assert (L.row_stride, L.inner_count, L.inner_stride, L.outer_stride, nb) == (7, 7, 1, 1024 * 7, 35)
     L, nb = D.batched_layout(t.permute(1, 0, 2), 0)
     assert nb == 35 and L.row_stride == 7
+
+
+CPP_FACADE_PROGRAM = r"""
+#include <cmath>
+#include <cstdio>
+#include "ddcx.hpp"
+int main()
+{
+    std::vector<double> br{0.0, 0.05, 0.2, 0.3, 0.45, 0.5, 0.7, 0.8, 0.95, 1.0};
+    ddcx::NonUniformBSplines bs(3, br, false);
+    ddcx::SplineBuilder b(bs, ddcx::SplineBuilderClosure::GREVILLE, ddcx::SplineBuilderClosure::GREVILLE);
+    auto pts = b.interpolation_points();
+    auto q = b.quadrature_coefficients();
+    double qs = 0.0;
+    for (double v : q[1]) qs += v;
+    std::printf("%d %d %.17g %.17g %.17g\n", b.npoints(), bs.size(), pts[1], pts[b.npoints() - 2], qs);
+    ddcx::UniformBSplines ub(3, 0.0, 2.0, 16, true);
+    ddcx::SplineBuilder pb(ub, ddcx::SplineBuilderClosure::PERIODIC, ddcx::SplineBuilderClosure::PERIODIC);
+    std::printf("%d %d %d\n", pb.npoints(), pb.offset(), pb.info().kernel);
+    try {
+        std::vector<double> bad{0.0, 0.5, 0.4, 1.0};
+        ddcx::NonUniformBSplines nb(3, bad, false);
+        std::printf("no error\n");
+    } catch (std::runtime_error const& e) {
+        std::printf("error: %s\n", e.what());
+    }
+    int64_t nbatch = 0;
+    ddcx_layout l = ddcx::batched_layout({5, 7, 11}, 1, &nbatch);
+    std::printf("%lld %lld %lld %lld %lld\n", (long long)l.row_stride, (long long)l.inner_count,
+                (long long)l.inner_stride, (long long)l.outer_stride, (long long)nbatch);
+    return 0;
+}
+"""
+
+
+def test_cpp_facade(tmp_path):
+    """include/ddcx.hpp (the C++20 facade with the reference's class names)
+    compiles against the C ABI and answers host-side queries like the Python
+    mirror; errors surface as std::runtime_error."""
+    import shutil
+    import subprocess
+
+    import ddc_amd as D
+
+    cxx = shutil.which("g++")
+    if cxx is None:
+        pytest.skip("no host C++ compiler")
+    src = tmp_path / "facade.cpp"
+    src.write_text(CPP_FACADE_PROGRAM)
+    exe = tmp_path / "facade"
+    libdir = os.path.dirname(D.LIB_PATH)
+    subprocess.run([cxx, "-std=c++20", "-O1", str(src), "-I", os.path.join(ROOT, "include"), "-L", libdir,
+                    "-lddcx", f"-Wl,-rpath,{libdir}", "-Wl,--allow-shlib-undefined", "-o", str(exe)],
+                   check=True, capture_output=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines()
+    br = np.array([0.0, 0.05, 0.2, 0.3, 0.45, 0.5, 0.7, 0.8, 0.95, 1.0])
+    bs = D.NonUniformBSplines(3, br)
+    b = D.SplineBuilder(bs, D.SplineBuilderClosure.GREVILLE, D.SplineBuilderClosure.GREVILLE)
+    pts = b.interpolation_points()
+    ql, qv, qr = b.quadrature_coefficients()
+    f = out[0].split()
+    assert int(f[0]) == b.npoints() and int(f[1]) == bs.size()
+    assert float(f[2]) == pts[1] and float(f[3]) == pts[-2]
+    assert abs(float(f[4]) - qv.sum()) <= 1e-15
+    ub = D.UniformBSplines(3, 0.0, 2.0, 16, periodic=True)
+    pb = D.SplineBuilder(ub)
+    assert out[1].split() == [str(pb.npoints()), str(pb.offset()), str(pb.info.kernel)]
+    assert out[2].startswith("error:")
+    assert out[3].split() == ["11", "11", "1", "77", "55"]  # same as splines.batched_layout

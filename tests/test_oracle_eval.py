@@ -1,0 +1,127 @@
+"""Pins the oracle's evaluator and tensor-product restatements (CPU only).
+
+- deriv(k) against scipy.interpolate.BSpline derivatives on the same knots
+  and coefficients, every degree and order (SplineEvaluator::deriv,
+  spline_evaluator.hpp:395-478).
+- Extrapolation rules by their definitions: Null gives 0 outside the domain
+  (null_extrapolation_rule.hpp:14-26), Constant the boundary value
+  (constant_extrapolation_rule.hpp:27-73), periodic splines repeat with the
+  domain length (spline_evaluator.hpp:529-543).
+- 2D build and evaluation (spline_builder_2d.hpp:514-651,
+  spline_evaluator_2d.hpp:370-825): interpolation of the grid values, the
+  separable case c = c1 (x) c2 giving f1(x) f2(y), and partial derivatives
+  against central differences.
+"""
+import numpy as np
+import pytest
+import scipy.interpolate as si
+
+import oracle as O
+from cases import breaks_for
+
+
+def _coef(orc, rng, periodic):
+    c = rng.standard_normal((orc.size, 1))
+    if periodic:
+        c[orc.nbasis:] = c[: orc.degree]
+    return c
+
+
+@pytest.mark.parametrize("degree", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("uniform", [True, False])
+def test_derivatives_vs_scipy(degree, uniform):
+    if uniform:
+        orc = O.Spline(degree, True, O.GREVILLE, O.GREVILLE, ncells=14)
+    else:
+        orc = O.Spline(degree, False, O.GREVILLE, O.GREVILLE, breaks=breaks_for("random", 14, seed=degree))
+    rng = np.random.default_rng(10 + degree)
+    c = _coef(orc, rng, False)
+    x = rng.uniform(orc.rmin, orc.rmax, (200, 1))
+    spl = si.BSpline(orc.knots(), c[:, 0], degree, extrapolate=False)
+    for k in range(degree + 1):
+        got = orc.eval(x, c, deriv=k)[:, 0]
+        ref = spl.derivative(k)(x[:, 0]) if k else spl(x[:, 0])
+        scale = max(1.0, np.abs(ref).max())
+        assert np.abs(got - ref).max() <= 1e-12 * scale, k
+
+
+@pytest.mark.parametrize("uniform", [True, False])
+def test_extrapolation_rules(uniform):
+    if uniform:
+        orc = O.Spline(3, True, O.GREVILLE, O.GREVILLE, ncells=10)
+    else:
+        orc = O.Spline(3, False, O.GREVILLE, O.GREVILLE, breaks=breaks_for("random", 10, seed=2))
+    rng = np.random.default_rng(3)
+    c = _coef(orc, rng, False)
+    L = orc.rmax - orc.rmin
+    below = rng.uniform(orc.rmin - L, orc.rmin, (20, 1))
+    above = np.nextafter(rng.uniform(orc.rmax, orc.rmax + L, (20, 1)), np.inf)
+    x = np.concatenate([below - 1e-9, above])
+    at = orc.eval(np.array([[orc.rmin], [orc.rmax]]), c)[:, 0]
+    null = orc.eval(x, c, extrap=("null", "null"))[:, 0]
+    assert np.all(null == 0.0)
+    const = orc.eval(x, c, extrap=("constant", "constant"))[:, 0]
+    assert np.all(const[:20] == at[0]) and np.all(const[20:] == at[1])
+    mixed = orc.eval(x, c, extrap=("constant", "null"))[:, 0]
+    assert np.all(mixed[:20] == at[0]) and np.all(mixed[20:] == 0.0)
+
+
+@pytest.mark.parametrize("uniform", [True, False])
+def test_periodic_repeats(uniform):
+    if uniform:
+        orc = O.Spline(3, True, O.PERIODIC, O.PERIODIC, ncells=12)
+    else:
+        orc = O.Spline(3, False, O.PERIODIC, O.PERIODIC, breaks=breaks_for("jitter", 12, seed=4))
+    rng = np.random.default_rng(5)
+    c = _coef(orc, rng, True)
+    L = orc.rmax - orc.rmin
+    x = rng.uniform(orc.rmin, orc.rmax, (50, 1))
+    base = orc.eval(x, c)[:, 0]
+    for shift in (-2, -1, 1, 3):
+        got = orc.eval(x + shift * L, c)[:, 0]
+        assert np.abs(got - base).max() <= 1e-12 * max(1.0, np.abs(base).max())
+
+
+def _pair2d():
+    o1 = O.Spline(3, False, O.GREVILLE, O.GREVILLE, breaks=breaks_for("random", 9, seed=7))
+    o2 = O.Spline(3, True, O.PERIODIC, O.PERIODIC, ncells=8)
+    return o1, o2
+
+
+def test_2d_interpolates_grid():
+    o1, o2 = _pair2d()
+    rng = np.random.default_rng(6)
+    vals = rng.standard_normal((2, o1.npts, o2.npts))
+    coef = O.spline2d_build(o1, o2, vals)
+    X, Y = np.meshgrid(o1.points(), o2.points(), indexing="ij")
+    coords = np.broadcast_to(np.stack([X.ravel(), Y.ravel()], -1), (2, X.size, 2)).copy()
+    got = O.spline2d_eval(o1, o2, coords, coef)
+    assert np.abs(got - vals.reshape(2, -1)).max() <= 1e-13 * np.abs(vals).max()
+
+
+def test_2d_separable_and_partials():
+    o1, o2 = _pair2d()
+    rng = np.random.default_rng(8)
+    c1 = _coef(o1, rng, False)
+    c2 = _coef(o2, rng, True)
+    coef = (c1[:, 0][:, None] * c2[:, 0][None, :])[None]
+    P = 300
+    xs = rng.uniform(o1.rmin, o1.rmax, P)
+    ys = rng.uniform(o2.rmin, o2.rmax, P)
+    coords = np.stack([xs, ys], -1)[None]
+    f1 = o1.eval(xs[:, None], c1)[:, 0]
+    f2 = o2.eval(ys[:, None], c2)[:, 0]
+    g1 = o1.eval(xs[:, None], c1, deriv=1)[:, 0]
+    g2 = o2.eval(ys[:, None], c2, deriv=1)[:, 0]
+    for (d1, d2), ref in {(0, 0): f1 * f2, (1, 0): g1 * f2, (0, 1): f1 * g2, (1, 1): g1 * g2}.items():
+        got = O.spline2d_eval(o1, o2, coords, coef, d1, d2)[0]
+        assert np.abs(got - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max()), (d1, d2)
+    # central differences of the value form, h = 1e-6
+    h = 1e-6
+    inner = (xs > o1.rmin + 2 * h) & (xs < o1.rmax - 2 * h)
+    for k, ref in ((0, g1 * f2), (1, f1 * g2)):
+        cp, cm = coords.copy(), coords.copy()
+        cp[..., k] += h
+        cm[..., k] -= h
+        fd = (O.spline2d_eval(o1, o2, cp, coef) - O.spline2d_eval(o1, o2, cm, coef))[0] / (2 * h)
+        assert np.abs(fd - ref)[inner].max() <= 1e-6 * max(1.0, np.abs(ref).max())
