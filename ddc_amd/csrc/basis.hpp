@@ -29,6 +29,12 @@ struct BSplinesView {
     double rmax;     // coordinate(break_point_domain.back())
     double length;   // rmax - rmin
     const double* knots; // non-uniform knots (ncells + 1 + 2*degree), device or host memory
+    // Non-uniform cell search brackets: nbucket equal buckets over [rmin, rmax];
+    // cells[k] packs the first (low 16 bits) and last (high 16 bits) cell that
+    // bucket k overlaps.  nullptr: plain binary search over all cells.
+    const uint32_t* cells;
+    int nbucket;
+    double inv_bucket; // nbucket / (rmax - rmin)
 };
 
 __host__ __device__ inline double knot_coord(BSplinesView const& b, int k)
@@ -70,6 +76,19 @@ __host__ __device__ inline int nonuniform_cell(BSplinesView const& b, const doub
     if (x >= knots[back]) return back - 1;
     int low = front;
     int high = back;
+    if (b.cells) {
+        // Start the same search from the bracket of x's bucket.  The result is
+        // the unique cell with knots[icell] <= x < knots[icell + 1], so it is
+        // the reference's whatever bracket holds it; a bracket missed through
+        // rounding of the bucket index falls back to the full range.
+        int k = int((x - b.rmin) * b.inv_bucket);
+        k = k < 0 ? 0 : (k >= b.nbucket ? b.nbucket - 1 : k);
+        uint32_t const e = b.cells[k];
+        int const lo = front + int(e & 0xffffu);
+        int const hi = front + int(e >> 16) + 1;
+        if (x >= knots[lo]) low = lo;
+        if (x < knots[hi]) high = hi;
+    }
     int icell = low + (high - low) / 2;
     while (x < knots[icell] || x >= knots[icell + 1]) {
         if (x < knots[icell]) high = icell;

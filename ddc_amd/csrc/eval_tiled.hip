@@ -1,13 +1,17 @@
-// LDS-staged batched evaluation for batch-contiguous layouts
-// (SplineEvaluator::operator() / deriv(), spline_evaluator.hpp:257-430).
+// LDS-staged batched evaluation (SplineEvaluator::operator() / deriv(),
+// spline_evaluator.hpp:263-478).
 //
-// In a batch-contiguous layout the coefficients of one line are spread over
-// `size()` cache lines, and random evaluation points gather them at random.
-// A workgroup therefore takes LB consecutive lines, stages their
-// coefficient tile (size() x LB, coalesced rows of LB*8 bytes) and, for
-// non-uniform bases, the knot vector in LDS, and then evaluates all the
-// points of those lines with coalesced coordinate loads and output stores.
-// HBM sees every coefficient, coordinate and output once.
+// Random evaluation points gather the coefficients of their line at random,
+// so a workgroup takes LB consecutive lines, stages their coefficient tile
+// (size() x LB) and, for non-uniform bases, the knot vector and the cell-search
+// brackets in LDS, and then evaluates all the points of those lines with
+// coalesced coordinate loads and output stores.  HBM sees every coefficient,
+// coordinate and output once.  Two thread mappings:
+//  - batch-contiguous (LINE_MAJOR = false): lanes take consecutive lines, so
+//    tile rows and point rows are read as LB*8-byte runs;
+//  - interpolation-contiguous (LINE_MAJOR = true): the tile is read line by
+//    line (size() contiguous doubles each) and the NT/LB threads of a line
+//    take consecutive points of it.
 #include "internal.hpp"
 
 namespace ddcx {
@@ -48,7 +52,7 @@ __device__ __forceinline__ double point_value(BSplinesView const& b, const doubl
     return y;
 }
 
-template <int D, int ORDER, int LB, int NT>
+template <int D, int ORDER, int LB, int NT, bool LINE_MAJOR>
 __global__ void __launch_bounds__(NT) eval_tiled_kernel(BSplinesView b, int lower, int upper, double* __restrict__ out,
                                                         const double* __restrict__ coords, Layout pl, int64_t npts,
                                                         const double* __restrict__ coef, Layout cl, int64_t nbatch,
@@ -57,13 +61,15 @@ __global__ void __launch_bounds__(NT) eval_tiled_kernel(BSplinesView b, int lowe
     extern __shared__ double smem[];
     double* tile = smem;
     double* kn = smem + size_t(ncoef) * pitch;
+    uint32_t* cells = reinterpret_cast<uint32_t*>(kn + nknots);
     constexpr int ROWS = NT / LB;
     constexpr int U = 8;           // points per pipeline stage per thread
     constexpr int TC = 16;         // tile rows per thread per load chunk
     int const tid = threadIdx.x;
-    int const ln = tid % LB;
-    int const prow = tid / LB;
-    int64_t const line = int64_t(blockIdx.x) * LB + ln;
+    int const ln = LINE_MAJOR ? tid / ROWS : tid % LB;
+    int const prow = LINE_MAJOR ? tid % ROWS : tid / LB;
+    int64_t const line0 = int64_t(blockIdx.x) * LB;
+    int64_t const line = line0 + ln;
     bool const active = line < nbatch;
     int64_t const lc = active ? line : 0;
     int64_t const cb = lay(cl, 0, lc);
@@ -76,26 +82,50 @@ __global__ void __launch_bounds__(NT) eval_tiled_kernel(BSplinesView b, int lowe
         int64_t const ii = prow + int64_t(u) * ROWS;
         xa[u] = ii < npts ? coords[pb + ii * pl.rs] : 0.0;
     }
-    // the tile is loaded in chunks of TC rows per thread, every load of a
+    // the tile is loaded in chunks of TC elements per thread, every load of a
     // chunk in flight before its LDS writes
-    for (int c0 = prow; c0 < ncoef; c0 += TC * ROWS) {
-        double tv[TC];
+    if constexpr (LINE_MAJOR) {
+        int const nl = int(min<int64_t>(LB, nbatch - line0));
+        int const nel = nl * ncoef;
+        for (int e0 = tid; e0 < nel; e0 += TC * NT) {
+            double tv[TC];
 #pragma unroll
-        for (int k = 0; k < TC; ++k) {
-            int const r = c0 + k * ROWS;
-            tv[k] = r < ncoef ? coef[cb + int64_t(r) * cl.rs] : 0.0;
+            for (int k = 0; k < TC; ++k) {
+                int const e = e0 + k * NT;
+                int const l = e / ncoef, r = e - l * ncoef;
+                tv[k] = e < nel ? coef[lay(cl, r, line0 + l)] : 0.0;
+            }
+#pragma unroll
+            for (int k = 0; k < TC; ++k) {
+                int const e = e0 + k * NT;
+                int const l = e / ncoef, r = e - l * ncoef;
+                if (e < nel) tile[r * pitch + l] = tv[k];
+            }
         }
+    } else {
+        for (int c0 = prow; c0 < ncoef; c0 += TC * ROWS) {
+            double tv[TC];
 #pragma unroll
-        for (int k = 0; k < TC; ++k) {
-            int const r = c0 + k * ROWS;
-            if (r < ncoef) tile[r * pitch + ln] = tv[k];
+            for (int k = 0; k < TC; ++k) {
+                int const r = c0 + k * ROWS;
+                tv[k] = r < ncoef ? coef[cb + int64_t(r) * cl.rs] : 0.0;
+            }
+#pragma unroll
+            for (int k = 0; k < TC; ++k) {
+                int const r = c0 + k * ROWS;
+                if (r < ncoef) tile[r * pitch + ln] = tv[k];
+            }
         }
     }
-    if (!b.uniform)
+    if (!b.uniform) {
         for (int k = tid; k < nknots; k += NT) kn[k] = b.knots[k];
+        for (int k = tid; k < b.nbucket; k += NT) cells[k] = b.cells[k];
+    }
     __syncthreads();
     if (!active) return;
     const double* knots = b.uniform ? nullptr : kn;
+    BSplinesView bl = b;
+    bl.cells = b.nbucket > 0 ? cells : nullptr;
     const double* tl = tile + ln;
     // software pipeline: the next batch of coordinates is in flight while the
     // current one is evaluated and stored
@@ -109,7 +139,7 @@ __global__ void __launch_bounds__(NT) eval_tiled_kernel(BSplinesView b, int lowe
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             int64_t const ii = i + int64_t(u) * ROWS;
-            if (ii < npts) out[pb + ii * pl.rs] = point_value<D, ORDER>(b, knots, lower, upper, xa[u], tl, pitch);
+            if (ii < npts) out[pb + ii * pl.rs] = point_value<D, ORDER>(bl, knots, lower, upper, xa[u], tl, pitch);
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) xa[u] = xb[u];
@@ -125,11 +155,13 @@ bool launch_orders(BSplinesPlan const& bs, int lower, int upper, double* out, co
     int const ncoef = bs.size();
     int const nknots = bs.view.uniform ? 0 : int(bs.knots_host.size());
     BSplinesView const b = bs.device_view();
-    auto fits = [&](int lb, int pitch) { return (size_t(ncoef) * pitch + nknots) * sizeof(double) <= kMaxLds; };
+    size_t const extra = nknots * sizeof(double) + size_t(b.nbucket) * sizeof(uint32_t);
+    auto fits = [&](int lb, int pitch) { return size_t(ncoef) * pitch * sizeof(double) + extra <= kMaxLds; };
+    bool const line_major = pl.rs == 1;
     auto go = [&]<int LB>(int pitch) {
         constexpr int NT = 1024;
-        size_t const lds = (size_t(ncoef) * pitch + nknots) * sizeof(double);
-        auto kern = eval_tiled_kernel<D, ORDER, LB, NT>;
+        size_t const lds = size_t(ncoef) * pitch * sizeof(double) + extra;
+        auto kern = line_major ? eval_tiled_kernel<D, ORDER, LB, NT, true> : eval_tiled_kernel<D, ORDER, LB, NT, false>;
         hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)),
                   "hipFuncSetAttribute");
@@ -152,7 +184,10 @@ bool launch_orders(BSplinesPlan const& bs, int lower, int upper, double* out, co
 bool launch_eval_tiled(BSplinesPlan const& bs, int lower, int upper, int order, double* out, const double* coords,
                        Layout pl, int64_t npts, const double* coef, Layout cl, int64_t nbatch, hipStream_t stream)
 {
-    if (pl.rs == 1 || npts < 64 || nbatch < 8) return false; // interpolation-contiguous or tiny: generic kernel
+    // tiny batches, and mixed layouts (points and coefficients not both
+    // batch-contiguous or both interpolation-contiguous), take the generic kernel
+    if (npts < 64 || nbatch < 8) return false;
+    if ((pl.rs == 1) != (cl.rs == 1)) return false;
     bool launched = false;
     dispatch_degree(bs.view.degree, [&]<int D>() {
         auto ord = [&]<int O>() {

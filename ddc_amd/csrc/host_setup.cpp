@@ -478,6 +478,24 @@ std::shared_ptr<BSplinesPlan> make_bsplines(ddcx_bsplines_desc const& d)
     }
     v.length = v.rmax - v.rmin;
     v.knots = p->knots_host.data();
+    v.cells = nullptr;
+    v.nbucket = 0;
+    v.inv_bucket = 0.0;
+    if (!v.uniform && d.ncells < 65536) {
+        // one bucket per cell on average; bucket k covers
+        // [rmin + k w, rmin + (k+1) w), w = (rmax - rmin) / nbucket
+        int const nbk = d.ncells;
+        double const w = v.length / nbk;
+        p->inv_bucket = double(nbk) / v.length;
+        p->cells_host.resize(nbk);
+        for (int k = 0; k < nbk; ++k) {
+            double const e0 = v.rmin + double(k) * w;
+            double const e1 = std::min(v.rmin + double(k + 1) * w, v.rmax);
+            int const lo = nonuniform_cell(v, p->knots_host.data(), e0) - D;
+            int const hi = nonuniform_cell(v, p->knots_host.data(), e1) - D;
+            p->cells_host[k] = uint32_t(lo) | (uint32_t(hi) << 16);
+        }
+    }
 
     // integrals(), integrals.hpp:30-153
     int const sz = p->size(), nb = p->nbasis();
@@ -514,6 +532,7 @@ void BSplinesPlan::ensure_device() const
 {
     std::call_once(uploaded, [&] {
         if (!view.uniform) knots_dev.upload(knots_host);
+        if (!cells_host.empty()) cells_dev.upload(cells_host);
         integrals_dev.upload(integrals);
     });
 }

@@ -106,6 +106,12 @@ struct BSplinesPlan {
     BSplinesView view{}; // host view (knots -> host vector)
     std::vector<double> knots_host;
     mutable DeviceBuffer knots_dev;
+    // cell-search brackets of the device view (non-uniform bases with fewer
+    // than 65536 cells; see BSplinesView::cells); the host view searches the
+    // whole range
+    std::vector<uint32_t> cells_host;
+    double inv_bucket = 0.0;
+    mutable DeviceBuffer cells_dev;
     std::vector<double> integrals; // integrals() over the full spline domain (size() entries)
     mutable DeviceBuffer integrals_dev;
     mutable std::once_flag uploaded;
@@ -116,6 +122,9 @@ struct BSplinesPlan {
     {
         BSplinesView v = view;
         v.knots = view.uniform ? nullptr : knots_dev.as<double>();
+        v.cells = cells_host.empty() ? nullptr : cells_dev.as<uint32_t>();
+        v.nbucket = int(cells_host.size());
+        v.inv_bucket = inv_bucket;
         return v;
     }
 };

@@ -127,8 +127,9 @@ def test_periodicity(cuda):
                                   Case(3, False, G, ncells=1024, mesh="random"),
                                   Case(5, False, H, ncells=2000, mesh="random"),
                                   Case(4, True, G, ncells=100)], ids=lambda c: c.id)
-def test_eval_tiled_batch_contiguous(case, cuda):
-    """LDS-staged evaluator (batch-contiguous, >= 64 points, >= 8 lines)."""
+@pytest.mark.parametrize("layout", ["batch_contiguous", "interp_contiguous"])
+def test_eval_tiled(case, layout, cuda):
+    """LDS-staged evaluator (>= 64 points, >= 8 lines), both thread mappings."""
     import torch
     import ddc_amd as D
 
@@ -137,13 +138,53 @@ def test_eval_tiled_batch_contiguous(case, cuda):
     bs, _ = case.product()
     x = _points(orc, 300, nbatch, 5, outside=0.1)
     ev = D.SplineEvaluator(bs, D.ConstantExtrapolationRule(), D.NullExtrapolationRule())
-    tc = torch.from_numpy(coef).to(cuda)
+    bc = layout == "batch_contiguous"
+    ax = 0 if bc else 1
+    tc = torch.from_numpy(coef if bc else np.ascontiguousarray(coef.T)).to(cuda)
     for order in range(0, min(case.degree, 3) + 1):
         xin = x if order == 0 else np.clip(x, orc.rmin, orc.rmax)
         ref = orc.eval(xin, coef, deriv=order, extrap=("constant", "null"))
-        tx = torch.from_numpy(xin).to(cuda)
+        tx = torch.from_numpy(xin if bc else np.ascontiguousarray(xin.T)).to(cuda)
         out = torch.empty_like(tx)
-        (ev(out, tx, tc) if order == 0 else ev.deriv(order, out, tx, tc))
-        got = out.cpu().numpy()
+        (ev(out, tx, tc, axis=ax) if order == 0 else ev.deriv(order, out, tx, tc, axis=ax))
+        got = out.cpu().numpy() if bc else out.cpu().numpy().T
         scale = max(1.0, np.abs(ref).max())
         assert np.abs(got - ref).max() / scale <= TOL * (10 ** order), f"order {order}"
+
+
+@pytest.mark.parametrize("mesh,ncells,ratio", [("random", 300, 1.05), ("random", 1000, 1.05), ("jitter", 500, 1.05),
+                                               ("geometric", 2048, 1.05), ("geometric", 700, 1.01)])
+@pytest.mark.parametrize("layout", ["batch_contiguous", "interp_contiguous"])
+def test_nonuniform_cell_search_at_edges(mesh, ncells, ratio, layout, cuda):
+    """deriv(degree) is piecewise constant with a jump at every knot, so it
+    exposes any wrong cell index: evaluate it at every break point, every
+    search-bucket edge rmin + k (rmax - rmin) / ncells, and their floating-
+    point neighbours, against the oracle's binary search."""
+    import torch
+    import ddc_amd as D
+
+    case = Case(3, False, G, ncells=ncells, mesh=mesh, ratio=ratio, seed=11)
+    orc, coef1 = _coef(case, 1, seed=3)
+    bs, _ = case.product()
+    br = orc.knots()[3: 3 + ncells + 1]
+    edges = orc.rmin + np.arange(ncells + 1) * ((orc.rmax - orc.rmin) / ncells)
+    base = np.concatenate([br, edges])
+    x = np.concatenate([base, np.nextafter(base, np.inf), np.nextafter(base, -np.inf)])
+    x = np.clip(x, orc.rmin, orc.rmax)
+    nbatch = 8
+    xs = np.repeat(x[:, None], nbatch, axis=1)
+    coef = np.repeat(coef1, nbatch, axis=1)
+    ref = orc.eval(xs, coef, deriv=3)
+    ev = D.SplineEvaluator(bs, D.NullExtrapolationRule(), D.NullExtrapolationRule())
+    if layout == "batch_contiguous":
+        tx, tc = torch.from_numpy(xs).to(cuda), torch.from_numpy(coef).to(cuda)
+        out = torch.empty_like(tx)
+        ev.deriv(3, out, tx, tc, axis=0)
+        got = out.cpu().numpy()
+    else:
+        tx = torch.from_numpy(np.ascontiguousarray(xs.T)).to(cuda)
+        tc = torch.from_numpy(np.ascontiguousarray(coef.T)).to(cuda)
+        out = torch.empty_like(tx)
+        ev.deriv(3, out, tx, tc, axis=1)
+        got = out.cpu().numpy().T
+    assert rel_err(got, ref) <= TOL
