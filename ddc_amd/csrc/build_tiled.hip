@@ -29,7 +29,13 @@ namespace {
 
 using namespace tiled;
 
-template <int LB, int K, int L, int KF, int KB, bool FDIV, bool BDIVF, bool TLDS>
+// Interpolation-contiguous layouts (IC) stage the tile through LDS in chunks
+// of kStageRows rows of all LB lines: each line's rows are read and written
+// as contiguous runs, and every thread takes its L rows from LDS.
+constexpr int kStageRows = 256;
+constexpr int kStagePitch = kStageRows + 1; // doubles; odd pitch spreads the LDS banks
+
+template <int LB, int K, int L, int KF, int KB, bool FDIV, bool BDIVF, bool TLDS, bool IC>
 __global__ void __launch_bounds__(LB* K) build_tiled_kernel(TiledView tv, BuildView bv,
                                                            const double* __restrict__ vals, Layout vl,
                                                            const double* __restrict__ dl,
@@ -66,8 +72,42 @@ __global__ void __launch_bounds__(LB* K) build_tiled_kernel(TiledView tv, BuildV
     uint32_t const clo = uint32_t(cb - lay(cl, 0, line0));
     uint32_t const vrs = uint32_t(vl.rs), crs = uint32_t(cl.rs);
 
+    // staging area after the row table: LB x kStagePitch doubles, then the
+    // per-line offsets of vals and coef
+    double* stg = rt_lds + (TLDS ? K * L * RS : 0);
+    uint32_t* loff = reinterpret_cast<uint32_t*>(stg + LB * kStagePitch);
+    int const nl = int(min<int64_t>(LB, nbatch - line0));
+
     double y[L];
-    if (r0 + L <= nq) {
+    if constexpr (IC) {
+        constexpr int PER = LB * kStageRows / NT; // staged elements per thread and chunk
+        if (tid < LB) {
+            int64_t const l = line0 + min(tid, nl - 1);
+            loff[tid] = uint32_t(lay(vl, 0, l) - lay(vl, 0, line0));
+            loff[LB + tid] = uint32_t(lay(cl, 0, l) - lay(cl, 0, line0));
+        }
+        __syncthreads();
+        for (int rbase = 0; rbase < K * L; rbase += kStageRows) {
+            double tmp[PER];
+#pragma unroll
+            for (int k = 0; k < PER; ++k) {
+                int const e = tid + k * NT;
+                int const l = e / kStageRows, u = rbase + e % kStageRows;
+                tmp[k] = (l < nl && u < nq) ? vwg[loff[l] + uint32_t(tv.q0 + u)] : 0.0;
+            }
+#pragma unroll
+            for (int k = 0; k < PER; ++k) {
+                int const e = tid + k * NT;
+                stg[(e / kStageRows) * kStagePitch + e % kStageRows] = tmp[k];
+            }
+            __syncthreads();
+            if (r0 >= rbase && r0 < rbase + kStageRows) {
+#pragma unroll
+                for (int i = 0; i < L; ++i) y[i] = stg[ln * kStagePitch + (r0 - rbase) + i];
+            }
+            __syncthreads();
+        }
+    } else if (r0 + L <= nq) {
 #pragma unroll
         for (int i = 0; i < L; ++i) y[i] = vwg[vlo + uint32_t(i) * vrs];
     } else {
@@ -277,9 +317,29 @@ __global__ void __launch_bounds__(LB* K) build_tiled_kernel(TiledView tv, BuildV
     }
 
     // ---------------- store ----------------
+    if constexpr (IC) {
+        constexpr int PER = LB * kStageRows / NT;
+        for (int rbase = 0; rbase < K * L; rbase += kStageRows) {
+            if (r0 >= rbase && r0 < rbase + kStageRows) {
+#pragma unroll
+                for (int i = 0; i < L; ++i) stg[ln * kStagePitch + (r0 - rbase) + i] = y[i];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < PER; ++k) {
+                int const e = tid + k * NT;
+                int const l = e / kStageRows, u = rbase + e % kStageRows;
+                if (l < nl && u < nq)
+                    cwg[loff[LB + l] + uint32_t(tv.top + u + off)] = stg[l * kStagePitch + e % kStageRows];
+            }
+            __syncthreads();
+        }
+    }
     if (active) {
         uint32_t const c0 = clo + uint32_t(tv.top + r0 + off) * crs;
-        if (r0 + L <= nq) {
+        if constexpr (IC) {
+            // rows stored above
+        } else if (r0 + L <= nq) {
 #pragma unroll
             for (int i = 0; i < L; ++i) cwg[c0 + uint32_t(i) * crs] = y[i];
         } else {
@@ -302,6 +362,22 @@ __global__ void __launch_bounds__(LB* K) build_tiled_kernel(TiledView tv, BuildV
     }
 }
 
+template <int K, int L, int KF, int KB, bool FDIV, bool BDIVF, bool TLDS, bool IC>
+void launch_one(BuilderPlan const& p, size_t lds, double* coef, Layout cl, const double* vals, Layout vl,
+                const double* dl, const double* dr, Layout dlay, int64_t nbatch, hipStream_t stream)
+{
+    constexpr int NT = 1024, LB = NT / K;
+    auto kern = build_tiled_kernel<LB, K, L, KF, KB, FDIV, BDIVF, TLDS, IC>;
+    if (lds > 0)
+        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      int(lds)),
+                  "hipFuncSetAttribute");
+    int64_t const blocks = (nbatch + LB - 1) / LB;
+    hipLaunchKernelGGL(kern, dim3(unsigned(blocks)), dim3(NT), lds, stream, p.tview, p.view, vals, vl, dl, dr, dlay,
+                       coef, cl, nbatch);
+    hip_check(hipGetLastError(), "build_tiled_kernel launch");
+}
+
 template <int K, int L, int KF, int KB, bool FDIV, bool BDIVF>
 void launch_cfg(BuilderPlan const& p, double* coef, Layout cl, const double* vals, Layout vl, const double* dl,
                 const double* dr, Layout dlay, int64_t nbatch, hipStream_t stream)
@@ -309,20 +385,21 @@ void launch_cfg(BuilderPlan const& p, double* coef, Layout cl, const double* val
     constexpr int NT = 1024, LB = NT / K;
     constexpr int KE = KF > KB ? KF : KB;
     constexpr size_t static_lds = size_t(KE * K * LB + kLamMax * LB + kNbMax * LB) * sizeof(double);
+    bool const ic = vl.rs == 1 && cl.rs == 1;
+    size_t const stage = ic ? size_t(LB) * kStagePitch * sizeof(double) + 2 * LB * sizeof(uint32_t) : 0;
     size_t const table = size_t(K) * L * p.tiled.rs * sizeof(double);
-    int64_t const blocks = (nbatch + LB - 1) / LB;
-    if (static_lds + table <= size_t(160 * 1024) && p.tiled.rs <= 10) {
-        auto kern = build_tiled_kernel<LB, K, L, KF, KB, FDIV, BDIVF, true>;
-        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      int(table)),
-                  "hipFuncSetAttribute");
-        hipLaunchKernelGGL(kern, dim3(unsigned(blocks)), dim3(NT), table, stream, p.tview, p.view, vals, vl, dl, dr,
-                           dlay, coef, cl, nbatch);
+    bool const tlds = static_lds + table + stage <= size_t(160 * 1024) && p.tiled.rs <= 10;
+    size_t const lds = (tlds ? table : 0) + stage;
+    auto go = [&]<bool TL, bool I>() {
+        launch_one<K, L, KF, KB, FDIV, BDIVF, TL, I>(p, lds, coef, cl, vals, vl, dl, dr, dlay, nbatch, stream);
+    };
+    if (ic) {
+        if (tlds) go.template operator()<true, true>();
+        else go.template operator()<false, true>();
     } else {
-        hipLaunchKernelGGL((build_tiled_kernel<LB, K, L, KF, KB, FDIV, BDIVF, false>), dim3(unsigned(blocks)),
-                           dim3(NT), 0, stream, p.tview, p.view, vals, vl, dl, dr, dlay, coef, cl, nbatch);
+        if (tlds) go.template operator()<true, false>();
+        else go.template operator()<false, false>();
     }
-    hip_check(hipGetLastError(), "build_tiled_kernel launch");
 }
 
 template <int K, int L>
