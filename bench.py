@@ -164,6 +164,78 @@ def measure_precision_form(D, degree, B, timed, n, gen):
             "max_line_rel_err_vs_fp64": float(f"{err:.3e}"), "fp32_refine_kernel": b32.info.kernel}
 
 
+def measure_advection(D, timed, n):
+    """One periodic advection step on 512 x 512 (uniform periodic cubic along
+    x, batched over y): build, eval at the feet x - sin(2 pi y) dt, and the
+    adjoint (eval_vjp then build_vjp).  The data (~6 MB) stays in cache, so
+    the figures are microseconds, not a fraction of HBM bandwidth."""
+    import numpy as np
+    import torch
+
+    dev = torch.device("cuda:0")
+    N = 512
+    bs = D.UniformBSplines(3, 0.0, 1.0, N, periodic=True)
+    ev = D.SplineEvaluator(bs, D.PeriodicExtrapolationRule(), D.PeriodicExtrapolationRule())
+    x = torch.from_numpy(D.SplineBuilder(bs).interpolation_points()).to(dev)
+    y = torch.arange(N, device=dev, dtype=torch.float64) / N
+    vals = torch.sin(2 * np.pi * x)[:, None] * torch.cos(2 * np.pi * y)[None, :]
+    feet = x[:, None] - torch.sin(2 * np.pi * y)[None, :] * 0.01
+    w = torch.ones_like(feet)
+    coef = torch.empty((bs.size(), N), dtype=torch.float64, device=dev)
+    out, gx, gc, gv = torch.empty_like(feet), torch.empty_like(feet), torch.empty_like(coef), torch.empty_like(vals)
+    res = {}
+    for name, prec in (("fp64", D.Precision.FP64), ("fp32_refine", D.Precision.FP32_REFINE)):
+        b = D.SplineBuilder(bs, precision=prec)
+        b(coef, vals)
+        ev(out, feet, coef)
+        ev.eval_vjp(feet, w, coef, gx, gc)
+        b.build_vjp(gc, gv)
+        t_b = timed(lambda: b(coef, vals), n)
+        t_e = timed(lambda: ev(out, feet, coef), n)
+        t_a = timed(lambda: (ev.eval_vjp(feet, w, coef, gx, gc), b.build_vjp(gc, gv)), n)
+        res[name] = {"build_us": round(t_b * 1e6, 1), "eval_us": round(t_e * 1e6, 1),
+                     "adjoint_us": round(t_a * 1e6, 1), "step_us": round((t_b + t_e) * 1e6, 1)}
+    return res
+
+
+def measure_2d(D, timed, gen):
+    """Tensor-product 2D: x non-uniform cubic (1024 cells, Greville closures,
+    1027 points), y uniform periodic cubic (1024 cells), batch 16; build, and
+    eval plus its adjoint at 2^24 scattered points per batch."""
+    import numpy as np
+    import torch
+
+    dev = torch.device("cuda:0")
+    nb, P = 16, 1 << 24
+    rng = np.random.default_rng(0xDDC0005)
+    br = np.concatenate([[0.0], np.sort(rng.uniform(0.0, 1.0, 1023)), [1.0]])
+    bs1 = D.NonUniformBSplines(3, br)
+    bs2 = D.UniformBSplines(3, 0.0, 1.0, 1024, periodic=True)
+    G = D.SplineBuilderClosure.GREVILLE
+    b2d = D.SplineBuilder2D(D.SplineBuilder(bs1, G, G), D.SplineBuilder(bs2))
+    ev = D.SplineEvaluator2D(D.SplineEvaluator(bs1, D.NullExtrapolationRule(), D.NullExtrapolationRule()),
+                             D.SplineEvaluator(bs2, D.PeriodicExtrapolationRule(), D.PeriodicExtrapolationRule()))
+    n1, n2 = b2d.builder1.npoints(), b2d.builder2.npoints()
+    vals = torch.randn((nb, n1, n2), dtype=torch.float64, device=dev, generator=gen)
+    coef = torch.empty((nb, bs1.size(), bs2.size()), dtype=torch.float64, device=dev)
+    b2d(coef, vals)
+    t_b = timed(lambda: b2d(coef, vals), 5)
+    coords = torch.rand((nb, P, 2), dtype=torch.float64, device=dev, generator=gen)
+    out = torch.empty((nb, P), dtype=torch.float64, device=dev)
+    ev(out, coords, coef)
+    t_e = timed(lambda: ev(out, coords, coef), 3)
+    w = torch.randn((nb, P), dtype=torch.float64, device=dev, generator=gen)
+    gx, gc = torch.empty_like(coords), torch.empty_like(coef)
+    ev.eval_vjp(coords, w, coef, gx, gc)
+    t_a = timed(lambda: ev.eval_vjp(coords, w, coef, gx, gc), 2)
+    by_b = 8 * nb * (n1 * n2 + bs1.size() * bs2.size())
+    by_e = 24 * nb * P + 8 * coef.numel()
+    return {"build_us": round(t_b * 1e6, 1), "build_GBps": round(by_b / t_b / 1e9, 1),
+            "eval_us": round(t_e * 1e6, 1), "eval_GBps": round(by_e / t_e / 1e9, 1),
+            "eval_Mpts_s": round(nb * P / t_e / 1e6, 1), "eval_vjp_us": round(t_a * 1e6, 1),
+            "points": nb * P}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -271,6 +343,9 @@ def main():
         torch.cuda.empty_cache()
         other["3_precision_form_d3"] = measure_precision_form(D, 3, 32768, timed, n_k, gen)
         other["3_precision_form_d5"] = measure_precision_form(D, 5, 32768, timed, n_k, gen)
+        torch.cuda.empty_cache()
+        other["4_advection_512x512"] = measure_advection(D, timed, n_k)
+        other["5_tensor_2d_1024x1024x16"] = measure_2d(D, timed, gen)
         res["workloads"] = other
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(nc, 4096, P_line, 0xDDC0002)

@@ -470,6 +470,7 @@ void launch_build_vjp(BuilderPlan const& p, const double* gcoef, Layout cl, doub
                       double* gdr, Layout dlay, int64_t nbatch, hipStream_t stream)
 {
     p.ensure_device();
+    if (launch_build_vjp_tiled(p, gcoef, cl, gvals, vl, gdl, gdr, dlay, nbatch, stream)) return;
     SolveView const& sv = p.view.sv;
     if (sv.nb > kMaxBorder) throw Error(DDCX_ERR_UNSUPPORTED, "border block larger than 16 rows");
     if (QType(sv.qtype) == QType::Dense && sv.nq > kMaxDenseQ)

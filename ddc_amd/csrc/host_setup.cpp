@@ -592,6 +592,37 @@ void BuilderPlan::ensure_device() const
             tv.s_lu = sv.s_lu;
             tv.s_ipiv = sv.s_ipiv;
         }
+        if (tiled_t.ok) {
+            tt_Mf.upload(tiled_t.Mf);
+            tt_Mb.upload(tiled_t.Mb);
+            tt_rt.upload(tiled_t.rt);
+            tt_gseg.upload(tiled_t.gseg);
+            tt_Gblk.upload(tiled_t.Gblk);
+            TiledView& tv = tview_t;
+            tv = tview; // sizes and border data are those of the solve
+            tv.K = tiled_t.K;
+            tv.L = tiled_t.L;
+            tv.kf = tiled_t.kf;
+            tv.kb = tiled_t.kb;
+            tv.wf = tiled_t.wf;
+            tv.wb = tiled_t.wb;
+            tv.nq = prob.q.n;
+            tv.nb = prob.nb();
+            tv.top = prob.top;
+            tv.q0 = bs->view.periodic ? 0 : prob.top - nbe_l;
+            tv.bl_nnz = int(prob.bl_v.size());
+            tv.rs = tiled_t.rs;
+            tv.rt = tt_rt.as<double>();
+            tv.Mf = tt_Mf.as<double>();
+            tv.Mb = tt_Mb.as<double>();
+            tv.gseg = tt_gseg.as<int>();
+            tv.Gblk = tt_Gblk.as<double>();
+            tv.bl_r = sv.bl_r;
+            tv.bl_c = sv.bl_c;
+            tv.bl_v = sv.bl_v;
+            tv.s_lu = sv.s_lu;
+            tv.s_ipiv = sv.s_ipiv;
+        }
         if (refine.ok) {
             RefineHost const& R = refine;
             r_rt.upload(R.rt32);

@@ -267,12 +267,19 @@ struct BuilderPlan {
     TiledHost tiled;
     mutable DeviceBuffer t_Mf, t_Mb, t_rt, t_gseg, t_Gblk;
     mutable TiledView tview{};
+    // tables of the transposed solve (build_vjp_tiled.hip)
+    TiledHost tiled_t;
+    mutable DeviceBuffer tt_Mf, tt_Mb, tt_rt, tt_gseg, tt_Gblk;
+    mutable TiledView tview_t{};
     RefineHost refine;
     mutable DeviceBuffer r_rt, r_Mf, r_Mb, r_gseg_blk, r_Gblk, r_rq, r_gr, r_gc, r_gv, r_gamseg, r_delta;
     mutable RefineView rview{};
     mutable std::once_flag uploaded;
     void ensure_device() const;
 };
+
+// segments holding Q^-1 gamma blocks that the tiled adjoint can reduce over
+constexpr int kMaxGBlocksVjp = 8;
 
 void make_tiled(BuilderPlan& p);
 void make_refine(BuilderPlan& p);
@@ -303,6 +310,8 @@ bool launch_build_tiled(BuilderPlan const& p, double* coef, Layout cl, const dou
                         const double* dl, const double* dr, Layout dlay, int64_t nbatch, hipStream_t stream);
 bool launch_build_refine(BuilderPlan const& p, double* coef, Layout cl, const double* vals, Layout vl,
                          const double* dl, const double* dr, Layout dlay, int64_t nbatch, hipStream_t stream);
+bool launch_build_vjp_tiled(BuilderPlan const& p, const double* gcoef, Layout cl, double* gvals, Layout vl,
+                            double* gdl, double* gdr, Layout dlay, int64_t nbatch, hipStream_t stream);
 void launch_build(BuilderPlan const& p, double* coef, Layout cl, const double* vals, Layout vl, const double* dl,
                   const double* dr, Layout dlay, int64_t nbatch, hipStream_t stream);
 void launch_build_vjp(BuilderPlan const& p, const double* gcoef, Layout cl, double* gvals, Layout vl, double* gdl,

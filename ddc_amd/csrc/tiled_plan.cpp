@@ -74,8 +74,11 @@ const char* tiled_unsupported(HostProblem const& pr)
 }
 
 // Sweep tables of the factored main block on a K x L tiling; the carry
-// windows are cut where the transfer products fall below tol.
-void tile_tables(HostProblem const& pr, int K, int L, double tol, TiledHost& t)
+// windows are cut where the transfer products fall below tol.  transpose:
+// tables of the solve with Q^T (the PDS blocks are symmetric; for the
+// unpivoted band Q^T = U^T L^T is a forward sweep with U^T, scaled by the
+// reciprocal pivots, and a unit backward sweep with L^T).
+void tile_tables(HostProblem const& pr, int K, int L, double tol, TiledHost& t, bool transpose = false)
 {
     t = TiledHost{};
     if (char const* why = tiled_unsupported(pr)) { t.why = why; return; }
@@ -100,7 +103,11 @@ void tile_tables(HostProblem const& pr, int K, int L, double tol, TiledHost& t)
         for (int c = 0; c < nq; ++c)
             for (int k = q.ku + 1; k <= kd && c - k >= 0; ++k)
                 if (q.q[size_t(kd - k) * nq + c] != 0.0) { t.why = "U factor wider than ku"; return; }
-        t.kf = q.kl; t.kb = q.ku; t.fwd_div = 0; t.bwd_divfirst = 0;
+        if (transpose) {
+            t.kf = q.ku; t.kb = q.kl; t.fwd_div = 1; t.bwd_divfirst = 0;
+        } else {
+            t.kf = q.kl; t.kb = q.ku; t.fwd_div = 0; t.bwd_divfirst = 0;
+        }
         break;
     }
     default: t.why = "dense main block"; return;
@@ -134,11 +141,21 @@ void tile_tables(HostProblem const& pr, int K, int L, double tol, TiledHost& t)
             break;
         case QType::Band: {
             int const kd = q.kl + q.ku;
-            for (int k = 1; k <= kf; ++k)
-                if (i - k >= 0) t.Lf[size_t(i) * kf + k - 1] = AB(kd + k, i - k);
-            for (int k = 1; k <= kb; ++k)
-                if (i + k < n) t.Ub[size_t(i) * kb + k - 1] = AB(kd - k, i + k);
-            t.rDb[i] = 1.0 / AB(kd, i);
+            if (transpose) {
+                // U(i-k, i) and L(i+k, i) in the band storage
+                for (int k = 1; k <= kf; ++k)
+                    if (i - k >= 0) t.Lf[size_t(i) * kf + k - 1] = AB(kd - k, i);
+                t.rDf[i] = 1.0 / AB(kd, i);
+                for (int k = 1; k <= kb; ++k)
+                    if (i + k < n) t.Ub[size_t(i) * kb + k - 1] = AB(kd + k, i);
+                t.rDb[i] = 1.0;
+            } else {
+                for (int k = 1; k <= kf; ++k)
+                    if (i - k >= 0) t.Lf[size_t(i) * kf + k - 1] = AB(kd + k, i - k);
+                for (int k = 1; k <= kb; ++k)
+                    if (i + k < n) t.Ub[size_t(i) * kb + k - 1] = AB(kd - k, i + k);
+                t.rDb[i] = 1.0 / AB(kd, i);
+            }
             break;
         }
         default: break;
@@ -244,6 +261,15 @@ void make_tiled(BuilderPlan& p)
     // (32 lines give 256-byte rows in batch-contiguous layouts)
     int const K = p.prob.q.n <= 1024 ? 32 : 64;
     tile_tables(p.prob, K, 32, kCarryTol, p.tiled);
+    tile_tables(p.prob, K, 32, kCarryTol, p.tiled_t, true);
+    if (p.tiled_t.ok) {
+        int nblk = 0;
+        for (int g : p.tiled_t.gseg) nblk += g >= 0;
+        if (nblk > kMaxGBlocksVjp) {
+            p.tiled_t.ok = false;
+            p.tiled_t.why = "border couplings spread over too many segments";
+        }
+    }
 }
 
 namespace {
