@@ -122,11 +122,20 @@ def measure_config(D, bs, builder, B, layout, timed, n, gen, derivs=False):
     t_b = timed(lambda: builder(coef, vals, axis=ax), n)
     t_e = timed(lambda: ev(out, coords, coef, axis=ax), n)
     t_d = timed(lambda: ev.deriv(1, out, coords, coef, axis=ax), n)
+    # adjoints: evaluator (w = out as weights) then builder
+    gx, gc, gv = torch.empty_like(coords), torch.empty_like(coef), torch.empty_like(vals)
+    ev.eval_vjp(coords, out, coef, gx, gc, axis=ax)
+    builder.build_vjp(gc, gv, axis=ax)
+    t_ev = timed(lambda: ev.eval_vjp(coords, out, coef, gx, gc, axis=ax), n)
+    t_bv = timed(lambda: builder.build_vjp(gc, gv, axis=ax), n)
     P = P_line * B
     bb, be = algorithmic_bytes(m, B, nc, P)
+    bev = 24 * P + 16 * B * nc  # coords, w, gx; coef read, gcoef written
     return {"layout": layout, "build_us": round(t_b * 1e6, 1), "eval_us": round(t_e * 1e6, 1),
             "deriv_us": round(t_d * 1e6, 1), "build_GBps": round(bb / t_b / 1e9, 1),
             "eval_GBps": round(be / t_e / 1e9, 1), "deriv_GBps": round(be / t_d / 1e9, 1),
+            "eval_vjp_us": round(t_ev * 1e6, 1), "eval_vjp_GBps": round(bev / t_ev / 1e9, 1),
+            "build_vjp_us": round(t_bv * 1e6, 1), "build_vjp_GBps": round(bb / t_bv / 1e9, 1),
             "build_eval_Mpts_s": round((m * B + P) / (t_b + t_e) / 1e6, 1)}
 
 

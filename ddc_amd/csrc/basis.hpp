@@ -98,6 +98,29 @@ __host__ __device__ inline int nonuniform_cell(BSplinesView const& b, const doub
     return icell;
 }
 
+// SplineEvaluator::eval, spline_evaluator.hpp:529-554: periodic wrap or
+// extrapolation (rules 0 = Null, 1 = Constant).  Returns false when the Null
+// rule gives 0; sets x to the position actually evaluated (the bound, for the
+// Constant rule) and `outside` when a rule applied.
+__host__ __device__ inline bool apply_bc(BSplinesView const& b, int lower, int upper, double& x, bool& outside)
+{
+    outside = false;
+    if (b.periodic) {
+        if (x < b.rmin || x > b.rmax) x -= floor((x - b.rmin) / b.length) * b.length;
+        return true;
+    }
+    if (x < b.rmin) {
+        outside = true;
+        if (lower == 0) return false;
+        x = b.rmin;
+    } else if (x > b.rmax) {
+        outside = true;
+        if (upper == 0) return false;
+        x = b.rmax;
+    }
+    return true;
+}
+
 // Cox-de Boor on the unit grid, UniformBSplines eval_basis,
 // bsplines_uniform.hpp:394-426.  NV values (NV = degree+1; integrals() uses
 // degree+2).

@@ -15,28 +15,6 @@ __device__ __forceinline__ int64_t lay(Layout const& L, int64_t r, int64_t l)
     return r * L.rs + (l % L.inner) * L.is + (l / L.inner) * L.os;
 }
 
-// SplineEvaluator::eval, spline_evaluator.hpp:529-554: periodic wrap or
-// extrapolation.  Returns false when a null rule gives 0; sets x to the
-// position actually evaluated (the bound, for the constant rule).
-__device__ __forceinline__ bool apply_bc(BSplinesView const& b, int lower, int upper, double& x, bool& outside)
-{
-    outside = false;
-    if (b.periodic) {
-        if (x < b.rmin || x > b.rmax) x -= floor((x - b.rmin) / b.length) * b.length;
-        return true;
-    }
-    if (x < b.rmin) {
-        outside = true;
-        if (lower == DDCX_EXTRAP_NULL) return false;
-        x = b.rmin;
-    } else if (x > b.rmax) {
-        outside = true;
-        if (upper == DDCX_EXTRAP_NULL) return false;
-        x = b.rmax;
-    }
-    return true;
-}
-
 // eval_no_bc, spline_evaluator.hpp:556-605
 template <int D, int ORDER>
 __device__ __forceinline__ double eval_no_bc(BSplinesView const& b, double x, const double* __restrict__ c, int64_t rs)
@@ -331,6 +309,7 @@ void launch_eval_vjp(BSplinesPlan const& bs, int lower, int upper, const double*
                      hipStream_t stream)
 {
     bs.ensure_device();
+    if (launch_eval_vjp_tiled(bs, lower, upper, coords, w, gx, pl, npts, coef, gcoef, cl, nbatch, stream)) return;
     BSplinesView const b = bs.device_view();
     int const size = bs.size();
     if (int64_t(size) * nbatch > 0)

@@ -318,6 +318,9 @@ void launch_build_vjp(BuilderPlan const& p, const double* gcoef, Layout cl, doub
                       double* gdr, Layout dlay, int64_t nbatch, hipStream_t stream);
 bool launch_eval_tiled(BSplinesPlan const& bs, int lower, int upper, int order, double* out, const double* coords,
                        Layout pl, int64_t npts, const double* coef, Layout cl, int64_t nbatch, hipStream_t stream);
+bool launch_eval_vjp_tiled(BSplinesPlan const& bs, int lower, int upper, const double* coords, const double* w,
+                           double* gx, Layout pl, int64_t npts, const double* coef, double* gcoef, Layout cl,
+                           int64_t nbatch, hipStream_t stream);
 void launch_eval(BSplinesPlan const& bs, int lower, int upper, int order, double* out, const double* coords,
                  Layout pl, int64_t npts, const double* coef, Layout cl, int64_t nbatch, hipStream_t stream);
 void launch_eval_vjp(BSplinesPlan const& bs, int lower, int upper, const double* coords, const double* w,

@@ -80,3 +80,41 @@ def test_eval_vjp_matches_oracle(case, cuda):
     ev.eval_vjp(tx, tw, tc, gx, gc)
     assert rel_err(gx.cpu().numpy(), gx_ref) <= 1e-13
     assert rel_err(gc.cpu().numpy(), gc_ref) <= 1e-12
+
+
+@pytest.mark.parametrize("case", [Case(3, True, P, ncells=1024), Case(5, True, P, ncells=300),
+                                  Case(3, False, G, ncells=1024, mesh="random"),
+                                  Case(4, False, H, ncells=500, mesh="random"), Case(2, True, G, ncells=200)],
+                         ids=lambda c: c.id)
+@pytest.mark.parametrize("layout", ["batch_contiguous", "interp_contiguous"])
+def test_eval_vjp_tiled(case, layout, cuda):
+    """LDS-accumulated evaluator adjoint (>= 8 lines, >= 64 points), with
+    points outside the domain under the constant and null rules."""
+    import torch
+    import ddc_amd as D
+
+    orc = case.oracle()
+    bs, _ = case.product()
+    rng = np.random.default_rng(9)
+    nbatch, npts = 21, 300
+    coef = rng.standard_normal((orc.size, nbatch))
+    if case.periodic:
+        coef[orc.nbasis:] = coef[: case.degree]
+    L = orc.rmax - orc.rmin
+    x = rng.uniform(orc.rmin - 0.1 * L, orc.rmax + 0.1 * L, (npts, nbatch))
+    w = rng.standard_normal((npts, nbatch))
+    gc_ref, gx_ref = orc.eval_vjp(x, w, coef, extrap=("constant", "null"))
+    ev = D.SplineEvaluator(bs, D.ConstantExtrapolationRule(), D.NullExtrapolationRule())
+    if layout == "batch_contiguous":
+        tx, tw, tc = (torch.from_numpy(a).to(cuda) for a in (x, w, coef))
+        gx, gc = torch.full_like(tx, np.nan), torch.full_like(tc, np.nan)
+        ev.eval_vjp(tx, tw, tc, gx, gc, axis=0)
+        gx, gc = gx.cpu().numpy(), gc.cpu().numpy()
+    else:
+        tx, tw, tc = (torch.from_numpy(np.ascontiguousarray(a.T)).to(cuda) for a in (x, w, coef))
+        gx, gc = torch.full_like(tx, np.nan), torch.full_like(tc, np.nan)
+        ev.eval_vjp(tx, tw, tc, gx, gc, axis=1)
+        gx, gc = gx.cpu().numpy().T, gc.cpu().numpy().T
+    assert np.all(np.isfinite(gc)) and np.all(np.isfinite(gx))
+    assert rel_err(gx, gx_ref) <= 1e-13
+    assert rel_err(gc, gc_ref) <= 1e-12
