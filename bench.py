@@ -139,6 +139,32 @@ def measure_config(D, bs, builder, B, layout, timed, n, gen, derivs=False):
             "build_eval_Mpts_s": round((m * B + P) / (t_b + t_e) / 1e6, 1)}
 
 
+def measure_fused(D, bs, builder, B, timed, n, gen):
+    """The headline workload through ddcx_build_eval: build + eval in one pass,
+    with the coefficients stored and kept on chip."""
+    import torch
+
+    dev = torch.device("cuda:0")
+    m, nc = builder.npoints(), bs.size()
+    xg = torch.from_numpy(builder.interpolation_points()).to(dev)
+    vals = _line_values(xg, B, gen)
+    coords = torch.rand((m, B), device=dev, dtype=torch.float64, generator=gen)
+    out = torch.empty_like(coords)
+    coef = torch.empty((nc, B), device=dev, dtype=torch.float64)
+    ev = D.SplineEvaluator(bs)
+    D.build_eval(builder, ev, out, coords, vals, coef)
+    D.build_eval(builder, ev, out, coords, vals, None)
+    t_s = timed(lambda: D.build_eval(builder, ev, out, coords, vals, coef), n)
+    t_c = timed(lambda: D.build_eval(builder, ev, out, coords, vals, None), n)
+    P = m * B
+    by_s = 8 * B * m + 8 * B * nc + 16 * P
+    by_c = 8 * B * m + 16 * P
+    return {"with_coef_us": round(t_s * 1e6, 1), "with_coef_GBps": round(by_s / t_s / 1e9, 1),
+            "with_coef_Mpts_s": round((m * B + P) / t_s / 1e6, 1),
+            "on_chip_us": round(t_c * 1e6, 1), "on_chip_GBps": round(by_c / t_c / 1e9, 1),
+            "on_chip_Mpts_s": round((m * B + P) / t_c / 1e6, 1)}
+
+
 def measure_precision_form(D, degree, B, timed, n, gen):
     """Hermite builder on the geometric mesh x_k = (1.05^k - 1)/(1.05^2048 - 1):
     FP64 and FP32-sweep + FP64-refinement builds on the same inputs.  The
@@ -339,6 +365,8 @@ def main():
         gen = torch.Generator(device=dev)
         gen.manual_seed(0xDDC0003)
         other = {}
+        other["2a_fused_build_eval"] = measure_fused(D, bs, builder, B, timed, n_k, gen)
+        torch.cuda.empty_cache()
         other["2a_interp_contiguous"] = measure_config(D, bs, builder, B, "interp_contiguous", timed, n_k, gen)
         bs5 = D.UniformBSplines(5, 0.0, 1.0, nc, periodic=True)
         other["2b_uniform_periodic_d5"] = measure_config(D, bs5, D.SplineBuilder(bs5), B, "batch_contiguous", timed,

@@ -19,6 +19,7 @@ from .splines import (  # noqa: F401
     SplineSolver,
     UniformBSplines,
     batched_layout,
+    build_eval,
     integrals,
     n_boundary_equations,
 )

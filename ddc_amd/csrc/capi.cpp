@@ -153,6 +153,38 @@ ddcx_status ddcx_builder_build(const ddcx_builder* h, double* coef, ddcx_layout 
     });
 }
 
+ddcx_status ddcx_build_eval(const ddcx_builder* h, const ddcx_evaluator* e, double* coef, ddcx_layout coef_layout,
+                            const double* vals, ddcx_layout vals_layout, const double* derivs_xmin,
+                            const double* derivs_xmax, ddcx_layout derivs_layout, double* out, const double* coords,
+                            ddcx_layout points_layout, int64_t npts, int64_t nbatch, void* stream)
+{
+    return guarded([&] {
+        require(h && e, "null handle");
+        require(npts >= 0 && nbatch >= 0, "negative size");
+        if (nbatch == 0) return;
+        require(vals && (npts == 0 || (out && coords)), "null data pointer");
+        ddcx::BuilderPlan const& p = *h->p;
+        if (p.bcl == DDCX_HERMITE && p.nbv_l > 0) require(derivs_xmin != nullptr, "HERMITE lower closure needs derivs_xmin");
+        if (p.bcu == DDCX_HERMITE && p.nbv_r > 0) require(derivs_xmax != nullptr, "HERMITE upper closure needs derivs_xmax");
+        require(p.bs.get() == e->bs.get(), "builder and evaluator must share one B-spline basis");
+        auto const cl = ddcx::to_layout(coef_layout), vl = ddcx::to_layout(vals_layout),
+                   dl = ddcx::to_layout(derivs_layout), pl = ddcx::to_layout(points_layout);
+        auto const st = static_cast<hipStream_t>(stream);
+        p.ensure_device();
+        e->bs->ensure_device();
+        if (p.precision == DDCX_FP64 && npts > 0 &&
+            ddcx::launch_build_eval_fused(p, *e->bs, rule_of(e->lower), rule_of(e->upper), coef, cl, vals, vl,
+                                          derivs_xmin, derivs_xmax, dl, out, coords, pl, npts, nbatch, st))
+            return;
+        require(coef != nullptr,
+                "the fused kernel does not cover this case (precision, layout or basis); pass a coefficient array");
+        ddcx::launch_build(p, coef, cl, vals, vl, derivs_xmin, derivs_xmax, dl, nbatch, st);
+        if (npts > 0)
+            ddcx::launch_eval(*e->bs, rule_of(e->lower), rule_of(e->upper), 0, out, coords, pl, npts, coef, cl, nbatch,
+                              st);
+    });
+}
+
 ddcx_status ddcx_builder_build_vjp(const ddcx_builder* h, const double* gcoef, ddcx_layout coef_layout, double* gvals,
                                    ddcx_layout vals_layout, double* gderivs_xmin, double* gderivs_xmax,
                                    ddcx_layout derivs_layout, int64_t nbatch, void* stream)

@@ -592,6 +592,37 @@ void BuilderPlan::ensure_device() const
             tv.s_lu = sv.s_lu;
             tv.s_ipiv = sv.s_ipiv;
         }
+        auto view_of = [&](TiledHost const& th, DeviceBuffer& mf, DeviceBuffer& mb, DeviceBuffer& rt,
+                           DeviceBuffer& gs, DeviceBuffer& gb, TiledView& tv) {
+            mf.upload(th.Mf);
+            mb.upload(th.Mb);
+            rt.upload(th.rt);
+            gs.upload(th.gseg);
+            gb.upload(th.Gblk);
+            tv.K = th.K;
+            tv.L = th.L;
+            tv.kf = th.kf;
+            tv.kb = th.kb;
+            tv.wf = th.wf;
+            tv.wb = th.wb;
+            tv.nq = prob.q.n;
+            tv.nb = prob.nb();
+            tv.top = prob.top;
+            tv.q0 = bs->view.periodic ? 0 : prob.top - nbe_l;
+            tv.bl_nnz = int(prob.bl_v.size());
+            tv.rs = th.rs;
+            tv.rt = rt.as<double>();
+            tv.Mf = mf.as<double>();
+            tv.Mb = mb.as<double>();
+            tv.gseg = gs.as<int>();
+            tv.Gblk = gb.as<double>();
+            tv.bl_r = sv.bl_r;
+            tv.bl_c = sv.bl_c;
+            tv.bl_v = sv.bl_v;
+            tv.s_lu = sv.s_lu;
+            tv.s_ipiv = sv.s_ipiv;
+        };
+        if (tiled_f.ok) view_of(tiled_f, tf_Mf, tf_Mb, tf_rt, tf_gseg, tf_Gblk, tview_f);
         if (tiled_t.ok) {
             tt_Mf.upload(tiled_t.Mf);
             tt_Mb.upload(tiled_t.Mb);

@@ -267,6 +267,10 @@ struct BuilderPlan {
     TiledHost tiled;
     mutable DeviceBuffer t_Mf, t_Mb, t_rt, t_gseg, t_Gblk;
     mutable TiledView tview{};
+    // tables of the fused build + eval kernel (fused_build_eval.hip)
+    TiledHost tiled_f;
+    mutable DeviceBuffer tf_Mf, tf_Mb, tf_rt, tf_gseg, tf_Gblk;
+    mutable TiledView tview_f{};
     // tables of the transposed solve (build_vjp_tiled.hip)
     TiledHost tiled_t;
     mutable DeviceBuffer tt_Mf, tt_Mb, tt_rt, tt_gseg, tt_Gblk;
@@ -312,6 +316,12 @@ bool launch_build_refine(BuilderPlan const& p, double* coef, Layout cl, const do
                          const double* dl, const double* dr, Layout dlay, int64_t nbatch, hipStream_t stream);
 bool launch_build_vjp_tiled(BuilderPlan const& p, const double* gcoef, Layout cl, double* gvals, Layout vl,
                             double* gdl, double* gdr, Layout dlay, int64_t nbatch, hipStream_t stream);
+// Fused build + value-form evaluation; coef may be null (coefficients kept in
+// LDS only).  Returns false when the fused kernel does not cover the case.
+bool launch_build_eval_fused(BuilderPlan const& p, BSplinesPlan const& bs, int lower, int upper, double* coef,
+                             Layout cl, const double* vals, Layout vl, const double* dl, const double* dr,
+                             Layout dlay, double* out, const double* coords, Layout pl, int64_t npts, int64_t nbatch,
+                             hipStream_t stream);
 void launch_build(BuilderPlan const& p, double* coef, Layout cl, const double* vals, Layout vl, const double* dl,
                   const double* dr, Layout dlay, int64_t nbatch, hipStream_t stream);
 void launch_build_vjp(BuilderPlan const& p, const double* gcoef, Layout cl, double* gvals, Layout vl, double* gdl,

@@ -262,6 +262,9 @@ void make_tiled(BuilderPlan& p)
     int const K = p.prob.q.n <= 1024 ? 32 : 64;
     tile_tables(p.prob, K, 32, kCarryTol, p.tiled);
     tile_tables(p.prob, K, 32, kCarryTol, p.tiled_t, true);
+    // the fused build + eval kernel keeps 16 (nq <= 1024) or 8 lines per
+    // workgroup so that their coefficient tile fits in LDS: 16-row segments
+    tile_tables(p.prob, p.prob.q.n <= 1024 ? 64 : 128, 16, kCarryTol, p.tiled_f);
     if (p.tiled_t.ok) {
         int nblk = 0;
         for (int g : p.tiled_t.gseg) nblk += g >= 0;

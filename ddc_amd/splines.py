@@ -342,6 +342,25 @@ class SplineEvaluator:
         return gcoef, gx
 
 
+def build_eval(builder, evaluator, spline_eval, coords_eval, vals, spline=None, derivs_xmin=None, derivs_xmax=None,
+               axis=0):
+    """builder(spline, vals, ...) then evaluator(spline_eval, coords_eval, spline)
+    in one pass (ddcx_build_eval).  With spline=None the coefficients stay on
+    chip; that needs the fused kernel (FP64 builder, degree 2..5, batch-
+    contiguous arrays)."""
+    _check_f64_cuda(vals, "vals")
+    vl, nbatch = batched_layout(vals, axis)
+    pl, _ = batched_layout(coords_eval, axis)
+    cl = batched_layout(spline, axis)[0] if spline is not None else ddcx_layout(1, 1, 0, 0)
+    dl = ddcx_layout(1, 1, 0, 0)
+    for d in (derivs_xmin, derivs_xmax):
+        if d is not None:
+            dl, _ = batched_layout(d, axis)
+    call("ddcx_build_eval", builder._h, evaluator._h, _ptr(spline), cl, _ptr(vals), vl, _ptr(derivs_xmin),
+         _ptr(derivs_xmax), dl, _ptr(spline_eval), _ptr(coords_eval), pl, coords_eval.shape[axis], nbatch, _stream())
+    return spline_eval
+
+
 class SplineBuilder2D:
     """ddc::SplineBuilder2D (spline_builder_2d.hpp:27-651): builds along dim 1
     batched over (batch, dim 2) into a temporary, then along dim 2 batched over

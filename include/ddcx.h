@@ -144,6 +144,17 @@ ddcx_status ddcx_builder_build(const ddcx_builder* h, double* coef, ddcx_layout 
                                ddcx_layout vals_layout, const double* derivs_xmin, const double* derivs_xmax,
                                ddcx_layout derivs_layout, int64_t nbatch, void* stream);
 
+/* SplineBuilder::operator() then SplineEvaluator::operator() on the same lines
+ * (spline_builder.hpp:905, spline_evaluator.hpp:263) in one pass (no reference
+ * counterpart as one call).  The evaluator must use the builder's basis.  coef
+ * may be NULL: the coefficients are then kept on chip only.  The fused kernel
+ * covers FP64 builders with a tiled plan, degrees 2..5, and batch-contiguous
+ * vals, coef and points; other cases run the two calls and need coef. */
+ddcx_status ddcx_build_eval(const ddcx_builder* h, const ddcx_evaluator* e, double* coef, ddcx_layout coef_layout,
+                            const double* vals, ddcx_layout vals_layout, const double* derivs_xmin,
+                            const double* derivs_xmax, ddcx_layout derivs_layout, double* out, const double* coords,
+                            ddcx_layout points_layout, int64_t npts, int64_t nbatch, void* stream);
+
 /* Vector-Jacobian product of ddcx_builder_build (no reference counterpart;
  * built on the transposed solve SplinesLinearProblem::solve(b, true),
  * splines_linear_problem.hpp:77): given gcoef, writes gvals and the Hermite
