@@ -3,7 +3,7 @@
 // dgbtrs / dgetrs restated, then the 2x2-block corrections of
 // splines_linear_problem_2x2_blocks.cpp:275-298).  It handles every matrix
 // shape the maker can produce and every layout; the register-resident tiled
-// kernel (build_tiled.hip) takes the hot configurations.
+// kernel (build_tiled_kernel.hpp) takes the hot configurations.
 //
 // Data flow per line (spline_builder.hpp:896-1101 without the separate copy,
 // transpose and duplication passes): the right-hand side is read straight

@@ -3,7 +3,7 @@
 // (SplineBuilder::operator(), spline_builder.hpp:896-1101;
 // splines_linear_problem_2x2_blocks.cpp:275-298 for the border steps).
 //
-// The tiling is the one of build_tiled.hip (thread = L rows of one line, LB
+// The tiling is the one of build_tiled_kernel.hpp (thread = L rows of one line, LB
 // lines per workgroup) with L = 16, so that a thread keeps its rows of the
 // line's FP64 right-hand side b and FP64 solution x in registers next to the
 // FP32 work rows d:

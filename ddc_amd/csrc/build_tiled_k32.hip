@@ -1,0 +1,24 @@
+// FP64 tiled build on the 32 x 32 tiling (main blocks of up to 1024 rows) and
+// the launcher that picks the tiling.
+#include "build_tiled_kernel.hpp"
+
+namespace ddcx {
+
+bool launch_build_tiled_k64(BuilderPlan const& p, double* coef, Layout cl, const double* vals, Layout vl,
+                            const double* dl, const double* dr, Layout dlay, int64_t nbatch, hipStream_t stream);
+
+bool launch_build_tiled(BuilderPlan const& p, double* coef, Layout cl, const double* vals, Layout vl,
+                        const double* dl, const double* dr, Layout dlay, int64_t nbatch, hipStream_t stream)
+{
+    if (!p.tiled.ok) return false;
+    if (!tiled::offsets_fit(vl, p.tiled.K * p.tiled.L + p.tview.q0, nbatch) ||
+        !tiled::offsets_fit(cl, p.view.size, nbatch))
+        return false;
+    if (p.tiled.K == 32 && p.tiled.L == 32)
+        return tiled_build::launch_K<32, 32>(p, coef, cl, vals, vl, dl, dr, dlay, nbatch, stream);
+    if (p.tiled.K == 64 && p.tiled.L == 32)
+        return launch_build_tiled_k64(p, coef, cl, vals, vl, dl, dr, dlay, nbatch, stream);
+    return false;
+}
+
+} // namespace ddcx

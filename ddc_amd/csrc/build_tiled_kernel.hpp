@@ -19,13 +19,14 @@
 //      registers;
 //   5. store, including the periodic duplicate rows (spline_builder.hpp:
 //      1074-1100).
+#pragma once
+
 #include <algorithm>
 
 #include "tiled_common.hpp"
 
 namespace ddcx {
-
-namespace {
+namespace tiled_build {
 
 using namespace tiled;
 
@@ -313,17 +314,5 @@ bool launch_K(BuilderPlan const& p, double* coef, Layout cl, const double* vals,
     return false;
 }
 
-} // namespace
-
-bool launch_build_tiled(BuilderPlan const& p, double* coef, Layout cl, const double* vals, Layout vl,
-                        const double* dl, const double* dr, Layout dlay, int64_t nbatch, hipStream_t stream)
-{
-    if (!p.tiled.ok) return false;
-    if (!offsets_fit(vl, p.tiled.K * p.tiled.L + p.tview.q0, nbatch) || !offsets_fit(cl, p.view.size, nbatch))
-        return false;
-    if (p.tiled.K == 32 && p.tiled.L == 32) return launch_K<32, 32>(p, coef, cl, vals, vl, dl, dr, dlay, nbatch, stream);
-    if (p.tiled.K == 64 && p.tiled.L == 32) return launch_K<64, 32>(p, coef, cl, vals, vl, dl, dr, dlay, nbatch, stream);
-    return false;
-}
-
+} // namespace tiled_build
 } // namespace ddcx

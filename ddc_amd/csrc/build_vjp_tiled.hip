@@ -2,7 +2,7 @@
 // of SplinesLinearProblem2x2Blocks::solve(b, true)
 // (splines_linear_problem_2x2_blocks.cpp:275-298: the direct order reversed)
 // after the fold of the periodic duplicate rows (the transpose of
-// spline_builder.hpp:1074-1100), on the tiling of build_tiled.hip:
+// spline_builder.hpp:1074-1100), on the tiling of build_tiled_kernel.hpp:
 //   1. y = folded gradient of the main-block rows (L per thread), b2 = the
 //      folded gradient of the border rows;
 //   2. b2 -= (Q^-1 gamma)^T y: partial sums of the segments that hold Q^-1

@@ -1,7 +1,7 @@
 // Fused batched build + evaluation: SplineBuilder::operator()
 // (spline_builder.hpp:905) followed by SplineEvaluator::operator()
 // (spline_evaluator.hpp:263) on the same lines, in one kernel.  A workgroup
-// solves LB lines with the register-resident tiled sweeps (build_tiled.hip,
+// solves LB lines with the register-resident tiled sweeps (build_tiled_kernel.hpp,
 // 16-row segments), writes the coefficients into an LDS tile that reuses the
 // row table's space, and evaluates every point of those lines from it
 // (eval_tiled.hip's batch-contiguous mapping).  The coefficients go to HBM only

@@ -154,7 +154,7 @@ struct BuildView {
     double dxpow[kMaxDegree + 1]; // dx^k, k = 0..degree
 };
 
-// Plan of the register-resident tiled build kernel (build_tiled.hip).  The
+// Plan of the register-resident tiled build kernel (build_tiled_kernel.hpp).  The
 // main block is swept as K segments of L rows per line.  Every sweep is in
 // row-oriented form (the same row operations as the netlib column-oriented
 // sweeps the reference runs), so the state crossing a segment boundary is the
@@ -307,7 +307,7 @@ inline Layout to_layout(ddcx_layout const& l)
 std::shared_ptr<BSplinesPlan> make_bsplines(ddcx_bsplines_desc const& d);
 std::unique_ptr<BuilderPlan> make_builder(std::shared_ptr<BSplinesPlan> bs, ddcx_builder_desc const& d);
 
-// kernel launchers (build_generic.hip / build_tiled.hip / eval_kernels.hip)
+// kernel launchers (build_generic.hip / build_tiled_kernel.hpp / eval_kernels.hip)
 void launch_build_generic(BuilderPlan const& p, double* coef, Layout cl, const double* vals, Layout vl,
                           const double* dl, const double* dr, Layout dlay, int64_t nbatch, hipStream_t stream);
 bool launch_build_tiled(BuilderPlan const& p, double* coef, Layout cl, const double* vals, Layout vl,

@@ -1,5 +1,6 @@
 // Device helpers shared by the register-resident build kernels
-// (build_tiled.hip, build_refine.hip).
+// (build_tiled_kernel.hpp, build_refine_kernel.hpp, build_vjp_tiled.hip,
+// fused_build_eval.hip).
 #pragma once
 
 #include <algorithm>
