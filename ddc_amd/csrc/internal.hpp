@@ -202,7 +202,7 @@ struct TiledView {
     const int* s_ipiv;
 };
 
-// Plan of the mixed-precision build (build_refine.hip): the tiled sweeps run
+// Plan of the mixed-precision build (build_refine_kernel.hpp): the tiled sweeps run
 // in FP32 on a K x L tiling with FP32 tables (carries to 2^-30), the border
 // and Schur steps in FP64, and each line is refined with FP64 residuals
 // r = b - A x of the assembled matrix,
