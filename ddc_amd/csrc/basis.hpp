@@ -121,10 +121,25 @@ __host__ __device__ inline bool apply_bc(BSplinesView const& b, int lower, int u
     return true;
 }
 
+// 1/d for the evaluation kernels: the hardware reciprocal refined by two
+// Newton-Raphson steps, r <- r + r (1 - d r), within a couple of ulps of the
+// correctly rounded quotient.  Cell indices never depend on it; the host
+// setup (matrix assembly, integrals) keeps exact divisions.
+__device__ __forceinline__ double fast_rcp(double d)
+{
+    double r = __builtin_amdgcn_rcp(d);
+    double e = fma(-d, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-d, r, 1.0);
+    return fma(r, e, r);
+}
+
 // Cox-de Boor on the unit grid, UniformBSplines eval_basis,
 // bsplines_uniform.hpp:394-426.  NV values (NV = degree+1; integrals() uses
 // degree+2).
-template <int NV>
+// FAST (device evaluation only): multiplications by the constants 1/j
+// instead of divisions by j.
+template <int NV, bool FAST = false>
 __host__ __device__ inline void uniform_basis_from_offset(double offset, double* values)
 {
     values[0] = 1.0;
@@ -135,7 +150,7 @@ __host__ __device__ inline void uniform_basis_from_offset(double offset, double*
 #pragma unroll
         for (int r = 0; r < j; ++r) {
             xx += 1;
-            double const temp = values[r] / j;
+            double const temp = FAST ? values[r] * (1.0 / j) : values[r] / j;
             values[r] = saved + xx * temp;
             saved = (j - xx) * temp;
         }
@@ -146,7 +161,7 @@ __host__ __device__ inline void uniform_basis_from_offset(double offset, double*
 // de Boor with left/right arrays, NonUniformBSplines eval_basis,
 // bsplines_non_uniform.hpp:461-497; `icell` is the knot index from
 // nonuniform_cell.
-template <int D>
+template <int D, bool FAST = false>
 __host__ __device__ inline void nonuniform_basis(const double* knots, int icell, double x, double* values)
 {
     double left[D > 0 ? D : 1];
@@ -159,7 +174,9 @@ __host__ __device__ inline void nonuniform_basis(const double* knots, int icell,
         double saved = 0.0;
 #pragma unroll
         for (int r = 0; r < j + 1; ++r) {
-            double const temp = values[r] / (right[r] + left[j - r]);
+            double temp;
+            if constexpr (FAST) temp = values[r] * fast_rcp(right[r] + left[j - r]);
+            else temp = values[r] / (right[r] + left[j - r]);
             values[r] = saved + right[r] * temp;
             saved = left[j - r] * temp;
         }
@@ -168,24 +185,24 @@ __host__ __device__ inline void nonuniform_basis(const double* knots, int icell,
 }
 
 // eval_basis for either basis kind; returns jmin (first B-spline index).
-template <int D>
+template <int D, bool FAST = false>
 __host__ __device__ inline int eval_basis(BSplinesView const& b, const double* knots, double x, double* values)
 {
     if (b.uniform) {
         double offset;
         int const jmin = uniform_cell(b, x, offset);
-        uniform_basis_from_offset<D + 1>(offset, values);
+        uniform_basis_from_offset<D + 1, FAST>(offset, values);
         return jmin;
     }
     int const icell = nonuniform_cell(b, knots, x);
-    nonuniform_basis<D>(knots, icell, x, values);
+    nonuniform_basis<D, FAST>(knots, icell, x, values);
     return icell - D;
 }
 
 // eval_basis_and_n_derivs (NURBS book A2.3), uniform bsplines_uniform.hpp:475-564
 // and non-uniform bsplines_non_uniform.hpp:561-664.  derivs is (D+1) x (N+1)
 // row-major.  Returns jmin.
-template <int D, int N>
+template <int D, int N, bool FAST = false>
 __host__ __device__ inline int eval_basis_and_n_derivs(BSplinesView const& b, const double* knots, double x,
                                                        double* derivs)
 {
@@ -204,7 +221,7 @@ __host__ __device__ inline int eval_basis_and_n_derivs(BSplinesView const& b, co
 #pragma unroll
             for (int r = 0; r < j; ++r) {
                 xx += 1.0;
-                double const temp = ndu[j - 1][r] / j;
+                double const temp = FAST ? ndu[j - 1][r] * (1.0 / j) : ndu[j - 1][r] / j;
                 ndu[j][r] = saved + xx * temp;
                 saved = (j - xx) * temp;
             }
@@ -223,7 +240,8 @@ __host__ __device__ inline int eval_basis_and_n_derivs(BSplinesView const& b, co
             double saved = 0.0;
 #pragma unroll
             for (int r = 0; r < j + 1; ++r) {
-                ndu[r][j + 1] = 1.0 / (right[r] + left[j - r]);
+                if constexpr (FAST) ndu[r][j + 1] = fast_rcp(right[r] + left[j - r]);
+                else ndu[r][j + 1] = 1.0 / (right[r] + left[j - r]);
                 double const temp = ndu[j][r] * ndu[r][j + 1];
                 ndu[j + 1][r] = saved + right[r] * temp;
                 saved = left[j - r] * temp;
@@ -244,7 +262,8 @@ __host__ __device__ inline int eval_basis_and_n_derivs(BSplinesView const& b, co
             int const rk = r - k;
             int const pk = D - k;
             if (r >= k) {
-                a[0][s2] = uni ? a[0][s1] / (pk + 1) : a[0][s1] * ndu[rk][pk + 1];
+                a[0][s2] = uni ? (FAST ? a[0][s1] * (1.0 / (pk + 1)) : a[0][s1] / (pk + 1))
+                               : a[0][s1] * ndu[rk][pk + 1];
                 d = a[0][s2] * ndu[pk][rk];
             }
             int const j1 = rk > -1 ? 1 : (-rk);
@@ -252,13 +271,15 @@ __host__ __device__ inline int eval_basis_and_n_derivs(BSplinesView const& b, co
 #pragma unroll
             for (int j = 1; j < D + 1; ++j) {
                 if (j >= j1 && j < j2) {
-                    a[j][s2] = uni ? (a[j][s1] - a[j - 1][s1]) / (pk + 1)
+                    a[j][s2] = uni ? (FAST ? (a[j][s1] - a[j - 1][s1]) * (1.0 / (pk + 1))
+                                          : (a[j][s1] - a[j - 1][s1]) / (pk + 1))
                                    : (a[j][s1] - a[j - 1][s1]) * ndu[rk + j][pk + 1];
                     d += a[j][s2] * ndu[pk][rk + j];
                 }
             }
             if (r <= pk) {
-                a[k][s2] = uni ? -a[k - 1][s1] / (pk + 1) : -a[k - 1][s1] * ndu[r][pk + 1];
+                a[k][s2] = uni ? (FAST ? -a[k - 1][s1] * (1.0 / (pk + 1)) : -a[k - 1][s1] / (pk + 1))
+                               : -a[k - 1][s1] * ndu[r][pk + 1];
                 d += a[k][s2] * ndu[pk][r];
             }
             derivs[r * (N + 1) + k] = d;

@@ -22,10 +22,10 @@ __device__ __forceinline__ double eval_no_bc(BSplinesView const& b, double x, co
     double v[D + 1];
     int jmin;
     if constexpr (ORDER == 0) {
-        jmin = eval_basis<D>(b, b.knots, x, v);
+        jmin = eval_basis<D, true>(b, b.knots, x, v);
     } else {
         double der[(D + 1) * (ORDER + 1)];
-        jmin = eval_basis_and_n_derivs<D, ORDER>(b, b.knots, x, der);
+        jmin = eval_basis_and_n_derivs<D, ORDER, true>(b, b.knots, x, der);
 #pragma unroll
         for (int i = 0; i < D + 1; ++i) v[i] = der[i * (ORDER + 1) + ORDER];
     }
@@ -82,7 +82,7 @@ __global__ void __launch_bounds__(256) eval_vjp_kernel(BSplinesView b, int lower
         return;
     }
     double der[(D + 1) * 2];
-    int const jmin = eval_basis_and_n_derivs<D, 1>(b, b.knots, x, der);
+    int const jmin = eval_basis_and_n_derivs<D, 1, true>(b, b.knots, x, der);
     const double* c = coef + lay(cl, 0, l);
     double* gc = gcoef + lay(cl, 0, l);
     double dy = 0.0;
@@ -156,17 +156,17 @@ __global__ void __launch_bounds__(256) eval2d_kernel(BSplinesView b1, BSplinesVi
     }
     double v1[D1 + 1], v2[D2 + 1];
     int j1, j2;
-    if constexpr (O1 == 0) j1 = eval_basis<D1>(b1, b1.knots, x1, v1);
+    if constexpr (O1 == 0) j1 = eval_basis<D1, true>(b1, b1.knots, x1, v1);
     else {
         double der[(D1 + 1) * (O1 + 1)];
-        j1 = eval_basis_and_n_derivs<D1, O1>(b1, b1.knots, x1, der);
+        j1 = eval_basis_and_n_derivs<D1, O1, true>(b1, b1.knots, x1, der);
 #pragma unroll
         for (int i = 0; i < D1 + 1; ++i) v1[i] = der[i * (O1 + 1) + O1];
     }
-    if constexpr (O2 == 0) j2 = eval_basis<D2>(b2, b2.knots, x2, v2);
+    if constexpr (O2 == 0) j2 = eval_basis<D2, true>(b2, b2.knots, x2, v2);
     else {
         double der[(D2 + 1) * (O2 + 1)];
-        j2 = eval_basis_and_n_derivs<D2, O2>(b2, b2.knots, x2, der);
+        j2 = eval_basis_and_n_derivs<D2, O2, true>(b2, b2.knots, x2, der);
 #pragma unroll
         for (int i = 0; i < D2 + 1; ++i) v2[i] = der[i * (O2 + 1) + O2];
     }
@@ -225,8 +225,8 @@ __global__ void __launch_bounds__(256) eval2d_vjp_kernel(BSplinesView b1, BSplin
         if (!b1.periodic) x1 = clamp2(x1, b1, in1);
     }
     double e1[(D1 + 1) * 2], e2[(D2 + 1) * 2];
-    int const j1 = eval_basis_and_n_derivs<D1, 1>(b1, b1.knots, x1, e1);
-    int const j2 = eval_basis_and_n_derivs<D2, 1>(b2, b2.knots, x2, e2);
+    int const j1 = eval_basis_and_n_derivs<D1, 1, true>(b1, b1.knots, x1, e1);
+    int const j2 = eval_basis_and_n_derivs<D2, 1, true>(b2, b2.knots, x2, e2);
     const double* c = coef + l * cbs;
     double* gc = gcoef + l * cbs;
     double f1 = 0.0, f2 = 0.0;

@@ -39,10 +39,10 @@ __device__ __forceinline__ double point_value(BSplinesView const& b, const doubl
             if (upper == DDCX_EXTRAP_NULL) return 0.0;
             x = b.rmax;
         }
-        jmin = eval_basis<D>(b, kn, x, v);
+        jmin = eval_basis<D, true>(b, kn, x, v);
     } else {
         double der[(D + 1) * (ORDER + 1)];
-        jmin = eval_basis_and_n_derivs<D, ORDER>(b, kn, x, der);
+        jmin = eval_basis_and_n_derivs<D, ORDER, true>(b, kn, x, der);
 #pragma unroll
         for (int i = 0; i < D + 1; ++i) v[i] = der[i * (ORDER + 1) + ORDER];
     }
@@ -240,7 +240,7 @@ __global__ void __launch_bounds__(NT) eval_vjp_tiled_kernel(BSplinesView b, int 
                     continue;
                 }
                 double der[(D + 1) * 2];
-                int const jmin = eval_basis_and_n_derivs<D, 1>(bl, knots, x, der);
+                int const jmin = eval_basis_and_n_derivs<D, 1, true>(bl, knots, x, der);
                 double dy = 0.0;
 #pragma unroll
                 for (int k = 0; k < D + 1; ++k) {

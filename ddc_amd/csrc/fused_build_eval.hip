@@ -186,7 +186,7 @@ __global__ void __launch_bounds__(LB* K) fused_build_eval_kernel(TiledView tv, B
                 double yv = 0.0;
                 if (apply_bc(b, lower, upper, x, outside)) {
                     double v[D + 1];
-                    int const jmin = eval_basis<D>(b, b.knots, x, v);
+                    int const jmin = eval_basis<D, true>(b, b.knots, x, v);
 #pragma unroll
                     for (int k = 0; k < D + 1; ++k) yv += tl[(jmin + k) * LB] * v[k];
                 }
