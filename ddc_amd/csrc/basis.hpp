@@ -184,11 +184,20 @@ __host__ __device__ inline void nonuniform_basis(const double* knots, int icell,
     }
 }
 
+// Basis kind of a kernel instantiation: decided by BSplinesView::uniform at
+// run time, or fixed at compile time so that the other branch is not emitted.
+enum BasisKind : int { kAnyBasis = -1, kNonUniformBasis = 0, kUniformBasis = 1 };
+
+__host__ __device__ constexpr bool is_uniform_kind(int kind, int uniform)
+{
+    return kind == kAnyBasis ? uniform != 0 : kind == kUniformBasis;
+}
+
 // eval_basis for either basis kind; returns jmin (first B-spline index).
-template <int D, bool FAST = false>
+template <int D, bool FAST = false, int KIND = kAnyBasis>
 __host__ __device__ inline int eval_basis(BSplinesView const& b, const double* knots, double x, double* values)
 {
-    if (b.uniform) {
+    if (is_uniform_kind(KIND, b.uniform)) {
         double offset;
         int const jmin = uniform_cell(b, x, offset);
         uniform_basis_from_offset<D + 1, FAST>(offset, values);
@@ -202,14 +211,14 @@ __host__ __device__ inline int eval_basis(BSplinesView const& b, const double* k
 // eval_basis_and_n_derivs (NURBS book A2.3), uniform bsplines_uniform.hpp:475-564
 // and non-uniform bsplines_non_uniform.hpp:561-664.  derivs is (D+1) x (N+1)
 // row-major.  Returns jmin.
-template <int D, int N, bool FAST = false>
+template <int D, int N, bool FAST = false, int KIND = kAnyBasis>
 __host__ __device__ inline int eval_basis_and_n_derivs(BSplinesView const& b, const double* knots, double x,
                                                        double* derivs)
 {
     double ndu[D + 1][D + 1];
     double a[D + 1][2];
     int jmin;
-    bool const uni = b.uniform;
+    bool const uni = is_uniform_kind(KIND, b.uniform);
     if (uni) {
         double offset;
         jmin = uniform_cell(b, x, offset);

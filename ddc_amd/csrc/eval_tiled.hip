@@ -12,6 +12,9 @@
 //  - interpolation-contiguous (LINE_MAJOR = true): the tile is read line by
 //    line (size() contiguous doubles each) and the NT/LB threads of a line
 //    take consecutive points of it.
+// Values on plain batch-contiguous arrays take eval_stream_kernel, a
+// persistent form that loads the next tile while the current one is
+// evaluated.
 #include "internal.hpp"
 
 namespace ddcx {
@@ -23,7 +26,7 @@ __device__ __forceinline__ int64_t lay(Layout const& L, int64_t r, int64_t l)
     return r * L.rs + (l % L.inner) * L.is + (l / L.inner) * L.os;
 }
 
-template <int D, int ORDER>
+template <int D, int ORDER, int KIND>
 __device__ __forceinline__ double point_value(BSplinesView const& b, const double* kn, int lower, int upper, double x,
                                               const double* tile, int pitch)
 {
@@ -39,10 +42,10 @@ __device__ __forceinline__ double point_value(BSplinesView const& b, const doubl
             if (upper == DDCX_EXTRAP_NULL) return 0.0;
             x = b.rmax;
         }
-        jmin = eval_basis<D, true>(b, kn, x, v);
+        jmin = eval_basis<D, true, KIND>(b, kn, x, v);
     } else {
         double der[(D + 1) * (ORDER + 1)];
-        jmin = eval_basis_and_n_derivs<D, ORDER, true>(b, kn, x, der);
+        jmin = eval_basis_and_n_derivs<D, ORDER, true, KIND>(b, kn, x, der);
 #pragma unroll
         for (int i = 0; i < D + 1; ++i) v[i] = der[i * (ORDER + 1) + ORDER];
     }
@@ -52,7 +55,7 @@ __device__ __forceinline__ double point_value(BSplinesView const& b, const doubl
     return y;
 }
 
-template <int D, int ORDER, int LB, int NT, bool LINE_MAJOR>
+template <int D, int ORDER, int LB, int NT, bool LINE_MAJOR, int KIND>
 __global__ void __launch_bounds__(NT) eval_tiled_kernel(BSplinesView b, int lower, int upper, double* __restrict__ out,
                                                         const double* __restrict__ coords, Layout pl, int64_t npts,
                                                         const double* __restrict__ coef, Layout cl, int64_t nbatch,
@@ -63,8 +66,12 @@ __global__ void __launch_bounds__(NT) eval_tiled_kernel(BSplinesView b, int lowe
     double* kn = smem + size_t(ncoef) * pitch;
     uint32_t* cells = reinterpret_cast<uint32_t*>(kn + nknots);
     constexpr int ROWS = NT / LB;
-    constexpr int U = 8;           // points per pipeline stage per thread
-    constexpr int TC = 16;         // tile rows per thread per load chunk
+    // points per pipeline stage per thread: as many as the registers hold
+    // without spilling (derivative tables of non-uniform bases are the largest)
+    constexpr int U = ORDER == 0 ? 8
+                      : KIND == kUniformBasis ? (D <= 4 ? 8 : 4)
+                                              : (D <= 1 ? 8 : (D <= 3 ? 4 : 2));
+    constexpr int TC = 20;         // tile rows per thread per load chunk (one chunk up to 20 * ROWS rows)
     int const tid = threadIdx.x;
     int const ln = LINE_MAJOR ? tid / ROWS : tid % LB;
     int const prow = LINE_MAJOR ? tid % ROWS : tid / LB;
@@ -74,14 +81,21 @@ __global__ void __launch_bounds__(NT) eval_tiled_kernel(BSplinesView b, int lowe
     int64_t const lc = active ? line : 0;
     int64_t const cb = lay(cl, 0, lc);
     int64_t const pb = lay(pl, 0, lc);
+    // Point i of the line is at cp[i * prs].  Indices past the end are clamped
+    // to the last point: such a slot loads the last coordinate and stores the
+    // last point's own value again, so every load and store of the loop below
+    // is unconditional and the loads of the next stage stay in flight while
+    // the current stage is evaluated.
+    int const last = int(npts) - 1;
+    uint32_t const prs = uint32_t(pl.rs);
+    const double* __restrict__ cp = coords + pb;
+    double* __restrict__ op = out + pb;
+    auto at = [&](int ii) { return uint64_t(uint32_t(min(ii, last))) * prs; };
     // first batch of coordinates, then the whole coefficient tile: every load
     // of the tile is in flight before the first LDS write
     double xa[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-        int64_t const ii = prow + int64_t(u) * ROWS;
-        xa[u] = ii < npts ? coords[pb + ii * pl.rs] : 0.0;
-    }
+    for (int u = 0; u < U; ++u) xa[u] = cp[at(prow + u * ROWS)];
     // the tile is loaded in chunks of TC elements per thread, every load of a
     // chunk in flight before its LDS writes
     if constexpr (LINE_MAJOR) {
@@ -117,32 +131,169 @@ __global__ void __launch_bounds__(NT) eval_tiled_kernel(BSplinesView b, int lowe
             }
         }
     }
-    if (!b.uniform) {
+    if (!is_uniform_kind(KIND, b.uniform)) {
         for (int k = tid; k < nknots; k += NT) kn[k] = b.knots[k];
         for (int k = tid; k < b.nbucket; k += NT) cells[k] = b.cells[k];
     }
     __syncthreads();
     if (!active) return;
-    const double* knots = b.uniform ? nullptr : kn;
+    const double* knots = is_uniform_kind(KIND, b.uniform) ? nullptr : kn;
     BSplinesView bl = b;
     bl.cells = b.nbucket > 0 ? cells : nullptr;
     const double* tl = tile + ln;
     // software pipeline: the next batch of coordinates is in flight while the
     // current one is evaluated and stored
-    for (int64_t i = prow; i < npts; i += U * ROWS) {
+    for (int i = prow; i <= last; i += U * ROWS) {
         double xb[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            int64_t const ii = i + int64_t(U + u) * ROWS;
-            xb[u] = ii < npts ? coords[pb + ii * pl.rs] : 0.0;
-        }
+        for (int u = 0; u < U; ++u) xb[u] = cp[at(i + (U + u) * ROWS)];
+        double y[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            int64_t const ii = i + int64_t(u) * ROWS;
-            if (ii < npts) out[pb + ii * pl.rs] = point_value<D, ORDER>(bl, knots, lower, upper, xa[u], tl, pitch);
-        }
+        for (int u = 0; u < U; ++u) y[u] = point_value<D, ORDER, KIND>(bl, knots, lower, upper, xa[u], tl, pitch);
+#pragma unroll
+        for (int u = 0; u < U; ++u) op[at(i + u * ROWS)] = y[u];
 #pragma unroll
         for (int u = 0; u < U; ++u) xa[u] = xb[u];
+    }
+}
+
+// Buffer resources (raw buffers, stride 0) for the streaming kernel: a load
+// at a byte offset at or past num_records returns 0 and a store there is
+// dropped, which is how the kernel handles the points past the end of a line.
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+constexpr int kBufferWord3 = 0x00020000; // 32-bit data format, gfx9 family
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_of(const double* p, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(p), 0, int(bytes), kBufferWord3);
+}
+
+__device__ __forceinline__ double buffer_load(__amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, int(off), 0, 0));
+}
+
+__device__ __forceinline__ void buffer_store(double v, __amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, int(off), 0, 0);
+}
+
+// Persistent form of eval_tiled_kernel for values with plain batch-contiguous
+// arrays (element (row, line) at row * rs + line, 16-line tiles, 1024
+// threads).  Each workgroup walks the tiles blockIdx.x, blockIdx.x +
+// gridDim.x, ...: while the points of one tile are evaluated, the coefficients
+// of its next tile are loaded into registers (PR rows per thread, one slice per
+// stage of U points), and they go to LDS once the current tile has been read.
+// Coefficient loads, point loads and stores therefore overlap inside every CU
+// instead of alternating in lockstep across the chip.  Every load and store of
+// a tile is unconditional, so the compiler keeps each stage's loads in flight
+// with exact vmcnt waits: points past the end of a line fall outside the
+// buffers (loads give 0, stores are dropped), rows past the end of the tile
+// repeat the last row, and lines past the batch repeat the last line.
+// Addresses are 32-bit byte offsets into buffers of at most 4 GiB.
+template <int D, int KIND, int NT, int U, int NST, int PR>
+__global__ void __launch_bounds__(NT) eval_stream_kernel(BSplinesView b, int lower, int upper,
+                                                           double* __restrict__ out,
+                                                           const double* __restrict__ coords, uint32_t prs,
+                                                           uint32_t pbytes, const double* __restrict__ coef,
+                                                           uint32_t crs, uint32_t cbytes, int64_t nbatch, int ncoef,
+                                                           int pitch, int nknots, int64_t ntiles, int npts)
+{
+    constexpr int LB = 16, ROWS = NT / LB;
+    constexpr int PS = (PR + NST - 1) / NST; // prefetch rows per stage
+    extern __shared__ double smem[];
+    double* tile = smem;
+    double* kn = smem + size_t(ncoef) * pitch;
+    uint32_t* cells = reinterpret_cast<uint32_t*>(kn + nknots);
+    int const tid = threadIdx.x;
+    int const ln = tid % LB;
+    int const prow = tid / LB;
+    uint32_t const lastr = uint32_t(ncoef - 1);
+    __amdgpu_buffer_rsrc_t const rc = buffer_of(coords, pbytes);
+    __amdgpu_buffer_rsrc_t const ro = buffer_of(out, pbytes);
+    __amdgpu_buffer_rsrc_t const rk = buffer_of(coef, cbytes);
+    // byte offsets of point slot p and coefficient row j of line l.  Slot p
+    // of tile t is point (p + rot(t)) mod npts, slots past npts stay outside
+    // the buffer; the rotation of 64 points per tile spreads the rows that the
+    // workgroups read at the same time over more DRAM banks.
+    int rot = 0;
+    auto pt = [&](int p, uint32_t l) {
+        int q = p + rot;
+        q = q >= npts ? q - npts : q;
+        q = p < npts ? q : p;
+        return (uint32_t(q) * prs + l) * 8u;
+    };
+    auto cr = [&](int j, uint32_t l) { return (min(uint32_t(prow + j * ROWS), lastr) * crs + l) * 8u; };
+    auto line_of = [&](int64_t t) { return uint32_t(min(t * LB + ln, nbatch - 1)); };
+
+    if (!is_uniform_kind(KIND, b.uniform)) {
+        for (int k = tid; k < nknots; k += NT) kn[k] = b.knots[k];
+        for (int k = tid; k < b.nbucket; k += NT) cells[k] = b.cells[k];
+    }
+    const double* knots = is_uniform_kind(KIND, b.uniform) ? nullptr : kn;
+    BSplinesView bl = b;
+    bl.cells = b.nbucket > 0 ? cells : nullptr;
+
+    int64_t const G = gridDim.x;
+    int64_t t = blockIdx.x;
+    uint32_t lt = line_of(t);
+    auto rot_of = [&](int64_t tt) { return int((tt * 64) % npts); };
+    rot = rot_of(t);
+    double xa[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) xa[u] = buffer_load(rc, pt(prow + u * ROWS, lt));
+    {
+        double tv[PR];
+#pragma unroll
+        for (int j = 0; j < PR; ++j) tv[j] = buffer_load(rk, cr(j, lt));
+        int const col = int(int64_t(lt) - t * LB);
+#pragma unroll
+        for (int j = 0; j < PR; ++j) {
+            int const r = prow + j * ROWS;
+            if (r < ncoef) tile[r * pitch + col] = tv[j];
+        }
+    }
+    while (true) {
+        int64_t const tn = t + G;
+        bool const more = tn < ntiles;
+        uint32_t const ltn = line_of(more ? tn : t);
+        const double* tl = tile + (int64_t(lt) - t * LB);
+        __syncthreads(); // tile t is in LDS
+        double pf[PR];
+#pragma unroll
+        for (int s = 0; s < NST; ++s) {
+            double xb[U];
+            if (s + 1 < NST) {
+#pragma unroll
+                for (int u = 0; u < U; ++u) xb[u] = buffer_load(rc, pt((s + 1) * U * ROWS + prow + u * ROWS, lt));
+            } else {
+                int const r0 = rot;
+                rot = rot_of(more ? tn : t);
+#pragma unroll
+                for (int u = 0; u < U; ++u) xb[u] = buffer_load(rc, pt(prow + u * ROWS, ltn));
+                rot = r0;
+            }
+#pragma unroll
+            for (int j = s * PS; j < (s + 1) * PS && j < PR; ++j) pf[j] = buffer_load(rk, cr(j, ltn));
+            double y[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) y[u] = point_value<D, 0, KIND>(bl, knots, lower, upper, xa[u], tl, pitch);
+#pragma unroll
+            for (int u = 0; u < U; ++u) buffer_store(y[u], ro, pt(s * U * ROWS + prow + u * ROWS, lt));
+#pragma unroll
+            for (int u = 0; u < U; ++u) xa[u] = xb[u];
+        }
+        __syncthreads(); // every read of tile t is done
+        if (!more) break;
+        int const col = int(int64_t(ltn) - tn * LB);
+#pragma unroll
+        for (int j = 0; j < PR; ++j) {
+            int const r = prow + j * ROWS;
+            if (r < ncoef) tile[r * pitch + col] = pf[j];
+        }
+        t = tn;
+        lt = ltn;
+        rot = rot_of(t);
     }
 }
 
@@ -264,6 +415,66 @@ __global__ void __launch_bounds__(NT) eval_vjp_tiled_kernel(BSplinesView b, int 
 
 constexpr size_t kMaxLds = 160 * 1024;
 
+int cu_count()
+{
+    static int cached[64] = {};
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= 64) dev = 0;
+    if (cached[dev] == 0) {
+        int n = 0;
+        hip_check(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+        cached[dev] = n > 0 ? n : 1;
+    }
+    return cached[dev];
+}
+
+// eval_stream_kernel for the value form when its shape fits: plain
+// batch-contiguous points and coefficients (element (row, line) at row * rs +
+// line), a uniform basis, npts <= 1024, size() <= 1056, buffers below 4 GiB
+// including the offsets of the point slots past the end of a line.
+template <int D>
+bool launch_stream(BSplinesPlan const& bs, int lower, int upper, double* out, const double* coords, Layout pl,
+                   int64_t npts, const double* coef, Layout cl, int64_t nbatch, hipStream_t stream)
+{
+    int const ncoef = bs.size();
+    int const nknots = bs.view.uniform ? 0 : int(bs.knots_host.size());
+    BSplinesView const b = bs.device_view();
+    constexpr int LB = 16, NT = 512, ROWS = NT / LB, U = 8, pitch = 17;
+    auto plain = [&](Layout const& L) { return L.is == 1 && L.inner >= nbatch && L.rs >= nbatch; };
+    if (!plain(pl) || !plain(cl)) return false;
+    int const nst = npts <= 2 * U * ROWS ? 2 : 4;
+    // the instantiations below hold their prefetch rows and 8-point stages in
+    // registers without spilling.  Non-uniform bases keep eval_tiled_kernel
+    // (their cell search runs better with 16 waves per CU), and so do batches
+    // of fewer than two tiles per CU, where nothing is left to prefetch.
+    if (npts > 4 * U * ROWS || ncoef > 33 * ROWS) return false;
+    if (!b.uniform) return false;
+    int64_t const ntiles = (nbatch + LB - 1) / LB;
+    if (ntiles < 2 * int64_t(cu_count())) return false;
+    uint64_t const lim = uint64_t(1) << 32;
+    // byte offsets up to the last slot of the last stage must fit 32 bits
+    if ((uint64_t(nst) * U * ROWS * uint64_t(pl.rs) + uint64_t(nbatch)) * 8 >= lim) return false;
+    if ((uint64_t(ncoef) * uint64_t(cl.rs) + uint64_t(nbatch)) * 8 >= lim) return false;
+    uint32_t const pbytes = uint32_t((uint64_t(npts - 1) * uint64_t(pl.rs) + uint64_t(nbatch)) * 8);
+    uint32_t const cbytes = uint32_t((uint64_t(ncoef - 1) * uint64_t(cl.rs) + uint64_t(nbatch)) * 8);
+    size_t const lds = size_t(ncoef) * pitch * sizeof(double) + nknots * sizeof(double) +
+                       size_t(b.nbucket) * sizeof(uint32_t);
+    if (lds > kMaxLds) return false;
+    unsigned const grid = unsigned(std::min<int64_t>(ntiles, cu_count()));
+    auto go = [&]<int KIND, int NST, int PR>() {
+        auto kern = eval_stream_kernel<D, KIND, NT, U, NST, PR>;
+        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)),
+                  "hipFuncSetAttribute");
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), lds, stream, b, lower, upper, out, coords, uint32_t(pl.rs),
+                           pbytes, coef, uint32_t(cl.rs), cbytes, nbatch, ncoef, pitch, nknots, ntiles, int(npts));
+        hip_check(hipGetLastError(), "eval_stream_kernel launch");
+        return true;
+    };
+    return nst == 2 ? go.template operator()<kUniformBasis, 2, 33>() : go.template operator()<kUniformBasis, 4, 33>();
+}
+
 template <int D, int ORDER>
 bool launch_orders(BSplinesPlan const& bs, int lower, int upper, double* out, const double* coords, Layout pl,
                    int64_t npts, const double* coef, Layout cl, int64_t nbatch, hipStream_t stream)
@@ -274,10 +485,12 @@ bool launch_orders(BSplinesPlan const& bs, int lower, int upper, double* out, co
     size_t const extra = nknots * sizeof(double) + size_t(b.nbucket) * sizeof(uint32_t);
     auto fits = [&](int lb, int pitch) { return size_t(ncoef) * pitch * sizeof(double) + extra <= kMaxLds; };
     bool const line_major = pl.rs == 1;
-    auto go = [&]<int LB>(int pitch) {
-        constexpr int NT = 1024;
+    auto go = [&]<int LB, int NT>(int pitch) {
         size_t const lds = size_t(ncoef) * pitch * sizeof(double) + extra;
-        auto kern = line_major ? eval_tiled_kernel<D, ORDER, LB, NT, true> : eval_tiled_kernel<D, ORDER, LB, NT, false>;
+        auto kern = b.uniform ? (line_major ? eval_tiled_kernel<D, ORDER, LB, NT, true, kUniformBasis>
+                                            : eval_tiled_kernel<D, ORDER, LB, NT, false, kUniformBasis>)
+                              : (line_major ? eval_tiled_kernel<D, ORDER, LB, NT, true, kNonUniformBasis>
+                                            : eval_tiled_kernel<D, ORDER, LB, NT, false, kNonUniformBasis>);
         hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)),
                   "hipFuncSetAttribute");
@@ -288,10 +501,13 @@ bool launch_orders(BSplinesPlan const& bs, int lower, int upper, double* out, co
         return true;
     };
     if (ncoef > 2560) return false;
-    if (fits(16, 17)) return go.template operator()<16>(17);
-    if (fits(16, 16)) return go.template operator()<16>(16);
-    if (fits(8, 9)) return go.template operator()<8>(9);
-    if (fits(8, 8)) return go.template operator()<8>(8);
+    if constexpr (ORDER == 0) {
+        if (launch_stream<D>(bs, lower, upper, out, coords, pl, npts, coef, cl, nbatch, stream)) return true;
+    }
+    if (fits(16, 17)) return go.template operator()<16, 1024>(17);
+    if (fits(16, 16)) return go.template operator()<16, 1024>(16);
+    if (fits(8, 9)) return go.template operator()<8, 1024>(9);
+    if (fits(8, 8)) return go.template operator()<8, 1024>(8);
     return false;
 }
 
@@ -304,6 +520,8 @@ bool launch_eval_tiled(BSplinesPlan const& bs, int lower, int upper, int order, 
     // batch-contiguous or both interpolation-contiguous), take the generic kernel
     if (npts < 64 || nbatch < 8) return false;
     if ((pl.rs == 1) != (cl.rs == 1)) return false;
+    // 32-bit point indices and point strides in the kernel
+    if (npts >= (int64_t(1) << 30) || pl.rs >= (int64_t(1) << 32)) return false;
     bool launched = false;
     dispatch_degree(bs.view.degree, [&]<int D>() {
         auto ord = [&]<int O>() {

@@ -152,6 +152,35 @@ def test_eval_tiled(case, layout, cuda):
         assert np.abs(got - ref).max() / scale <= TOL * (10 ** order), f"order {order}"
 
 
+@pytest.mark.parametrize("case", [Case(3, True, P, ncells=1024), Case(5, True, P, ncells=300),
+                                  Case(1, True, G, ncells=200), Case(4, True, G, ncells=100)],
+                         ids=lambda c: c.id)
+@pytest.mark.parametrize("npts", [300, 1000])
+def test_eval_stream(case, npts, cuda):
+    """Persistent evaluator for uniform bases on large batch-contiguous batches
+    (at least two 16-line tiles per CU): 8200 lines, so the last tile is
+    ragged, and point counts that leave the last stage of each tile partly
+    empty; extrapolated points on both sides.  Checked on 192 lines spread
+    over the batch (both ends included)."""
+    import torch
+    import ddc_amd as D
+
+    nbatch = 8200
+    orc, coef = _coef(case, nbatch, seed=6)
+    bs, _ = case.product()
+    x = _points(orc, npts, nbatch, 7, outside=0.1)
+    ev = D.SplineEvaluator(bs, D.ConstantExtrapolationRule(), D.NullExtrapolationRule())
+    out = torch.full((npts, nbatch), np.nan, dtype=torch.float64, device=cuda)
+    ev(out, torch.from_numpy(x).to(cuda), torch.from_numpy(coef).to(cuda))
+    got = out.cpu().numpy()
+    rng = np.random.default_rng(8)
+    lines = np.unique(np.concatenate([np.arange(64), np.arange(nbatch - 64, nbatch),
+                                      rng.choice(nbatch, 64, replace=False)]))
+    ref = orc.eval(x[:, lines], coef[:, lines], extrap=("constant", "null"))
+    assert not np.isnan(got).any()
+    assert rel_err(got[:, lines], ref) <= TOL
+
+
 @pytest.mark.parametrize("mesh,ncells,ratio", [("random", 300, 1.05), ("random", 1000, 1.05), ("jitter", 500, 1.05),
                                                ("geometric", 2048, 1.05), ("geometric", 700, 1.01)])
 @pytest.mark.parametrize("layout", ["batch_contiguous", "interp_contiguous"])
