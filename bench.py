@@ -334,7 +334,7 @@ def main():
     dominant = "build" if t_build >= t_eval else "eval"
     t_dom = max(t_build, t_eval)
     ach = (b_build if dominant == "build" else b_eval) / t_dom / 1e9
-    traffic = read_traffic("build_tiled_kernel" if dominant == "build" else "eval_tiled_kernel")
+    traffic = read_traffic("build_tiled_kernel" if dominant == "build" else "eval_stream_kernel")
     value = (m * B + P) / (ms * 1e-3) / 1e6
     res = {
         "metric": "spline build+eval Mpts/s, %HBM peak; FP32+refine vs FP64 at <=1e-12 rel err",

@@ -257,6 +257,10 @@ __global__ void __launch_bounds__(NT) eval_stream_kernel(BSplinesView b, int low
         int64_t const tn = t + G;
         bool const more = tn < ntiles;
         uint32_t const ltn = line_of(more ? tn : t);
+        // after the last tile the prefetch reads empty buffers: every access
+        // is out of range, so it costs no memory traffic
+        __amdgpu_buffer_rsrc_t const rcn = buffer_of(coords, more ? pbytes : 0u);
+        __amdgpu_buffer_rsrc_t const rkn = buffer_of(coef, more ? cbytes : 0u);
         const double* tl = tile + (int64_t(lt) - t * LB);
         __syncthreads(); // tile t is in LDS
         double pf[PR];
@@ -270,11 +274,11 @@ __global__ void __launch_bounds__(NT) eval_stream_kernel(BSplinesView b, int low
                 int const r0 = rot;
                 rot = rot_of(more ? tn : t);
 #pragma unroll
-                for (int u = 0; u < U; ++u) xb[u] = buffer_load(rc, pt(prow + u * ROWS, ltn));
+                for (int u = 0; u < U; ++u) xb[u] = buffer_load(rcn, pt(prow + u * ROWS, ltn));
                 rot = r0;
             }
 #pragma unroll
-            for (int j = s * PS; j < (s + 1) * PS && j < PR; ++j) pf[j] = buffer_load(rk, cr(j, ltn));
+            for (int j = s * PS; j < (s + 1) * PS && j < PR; ++j) pf[j] = buffer_load(rkn, cr(j, ltn));
             double y[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) y[u] = point_value<D, 0, KIND>(bl, knots, lower, upper, xa[u], tl, pitch);
