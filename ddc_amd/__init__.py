@@ -13,9 +13,12 @@ from .splines import (  # noqa: F401
     Precision,
     SplineBuilder,
     SplineBuilder2D,
+    SplineBuilder3D,
     SplineBuilderClosure,
     SplineEvaluator,
     SplineEvaluator2D,
+    SplineEvaluator3D,
+    SplineEvaluatorND,
     SplineSolver,
     UniformBSplines,
     batched_layout,

@@ -1,6 +1,7 @@
 // extern "C" boundary of libddcx (declared in include/ddcx.h).  Converts C
 // handles and layouts to the internal plans, catches every exception and
 // returns a status with a thread-local message.
+#include <algorithm>
 #include <cstring>
 #include <string>
 
@@ -326,6 +327,59 @@ ddcx_status ddcx_evaluator2d_eval_vjp(const ddcx_evaluator* e1, const ddcx_evalu
         int const rules[4] = {rule_of(e1->lower), rule_of(e1->upper), rule_of(e2->lower), rule_of(e2->upper)};
         ddcx::launch_eval2d_vjp(*e1->bs, *e2->bs, rules, coords, w, gx, npts, coef, gcoef, coef_batch_stride, st1,
                                 st2, nbatch, static_cast<hipStream_t>(stream));
+    });
+}
+
+ddcx_status ddcx_evaluatornd_eval(const ddcx_evaluator* const* evs, int nd, const int* orders, double* out,
+                                  const double* coords, int64_t npts, const double* coef, int64_t coef_batch_stride,
+                                  const int64_t* coef_strides, int64_t nbatch, void* stream)
+{
+    return guarded([&] {
+        require(evs && nd >= 1 && nd <= ddcx::kMaxNd, "nd must be between 1 and 4");
+        require(coef_strides, "null coefficient strides");
+        require(npts >= 0 && nbatch >= 0, "negative size");
+        ddcx::BSplinesPlan const* bs[ddcx::kMaxNd];
+        int lower[ddcx::kMaxNd], upper[ddcx::kMaxNd];
+        for (int k = 0; k < nd; ++k) {
+            require(evs[k], "null evaluator");
+            bs[k] = evs[k]->bs.get();
+            lower[k] = rule_of(evs[k]->lower);
+            upper[k] = rule_of(evs[k]->upper);
+            if (!bs[k]->view.periodic && (lower[k] == DDCX_EXTRAP_CONSTANT || upper[k] == DDCX_EXTRAP_CONSTANT))
+                throw ddcx::Error(DDCX_ERR_UNSUPPORTED,
+                                  "the N-dimensional evaluator takes Null or Periodic extrapolation rules");
+            if (bs[k]->view.degree > 5)
+                throw ddcx::Error(DDCX_ERR_UNSUPPORTED, "N-dimensional evaluation supports degrees 1..5");
+            if (orders) {
+                require(orders[k] >= 0, "negative derivative order");
+                if (orders[k] > std::min(bs[k]->view.degree, 3))
+                    throw ddcx::Error(DDCX_ERR_UNSUPPORTED, "derivative order above min(degree, 3)");
+            }
+        }
+        if (npts * nbatch == 0) return;
+        require(out && coords && coef, "null data pointer");
+        ddcx::launch_evalnd(bs, nd, orders, lower, upper, out, coords, npts, coef, coef_batch_stride, coef_strides,
+                            nbatch, static_cast<hipStream_t>(stream));
+    });
+}
+
+ddcx_status ddcx_evaluatornd_integrate(const ddcx_evaluator* const* evs, int nd, double* out, int64_t out_stride,
+                                       const double* coef, int64_t coef_batch_stride, const int64_t* coef_strides,
+                                       int64_t nbatch, void* stream)
+{
+    return guarded([&] {
+        require(evs && nd >= 1 && nd <= ddcx::kMaxNd, "nd must be between 1 and 4");
+        require(coef_strides, "null coefficient strides");
+        require(nbatch >= 0, "negative size");
+        ddcx::BSplinesPlan const* bs[ddcx::kMaxNd];
+        for (int k = 0; k < nd; ++k) {
+            require(evs[k], "null evaluator");
+            bs[k] = evs[k]->bs.get();
+        }
+        if (nbatch == 0) return;
+        require(out && coef, "null data pointer");
+        ddcx::launch_integratend(bs, nd, out, out_stride, coef, coef_batch_stride, coef_strides, nbatch,
+                                 static_cast<hipStream_t>(stream));
     });
 }
 

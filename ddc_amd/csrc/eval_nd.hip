@@ -1,0 +1,229 @@
+// Batched N-dimensional tensor-product evaluation and integration
+// (SplineEvaluatorND, spline_evaluator_nd.hpp; SplineEvaluator3D,
+// spline_evaluator_3d.hpp, is its N = 3 case).
+//
+// One thread per point: the per-dimension basis values (degree and
+// derivative order are kernel arguments, dispatched to the compile-time
+// basis routines by a wave-uniform switch), then the sum over the
+// (d_1+1) x ... x (d_N+1) coefficient patch in row-major order with the
+// basis product folded from the right, as eval_no_bc does
+// (spline_evaluator_nd.hpp:782-826).  Integration is one workgroup per batch
+// entry reducing coef(i) * (I_1(i_1) * (...)) over the whole coefficient
+// block (spline_evaluator_nd.hpp:597-632).
+#include "internal.hpp"
+
+namespace ddcx {
+
+namespace {
+
+struct NdArgs {
+    BSplinesView b[kMaxNd];
+    const double* integrals[kMaxNd];
+    int64_t st[kMaxNd];
+    int order[kMaxNd];
+    int lower[kMaxNd];
+    int upper[kMaxNd];
+};
+
+template <int D>
+__device__ int basis_of_order(BSplinesView const& b, double x, int order, double* v)
+{
+    if (order == 0) return eval_basis<D, true>(b, b.knots, x, v);
+    int j = 0;
+    auto take = [&]<int O>() {
+        if constexpr (O <= D) {
+            double der[(D + 1) * (O + 1)];
+            j = eval_basis_and_n_derivs<D, O, true>(b, b.knots, x, der);
+#pragma unroll
+            for (int i = 0; i < D + 1; ++i) v[i] = der[i * (O + 1) + O];
+        }
+    };
+    switch (order) {
+    case 1: take.template operator()<1>(); break;
+    case 2: take.template operator()<2>(); break;
+    case 3: take.template operator()<3>(); break;
+    default: break;
+    }
+    return j;
+}
+
+__device__ int basis_any(BSplinesView const& b, double x, int order, double* v)
+{
+    switch (b.degree) {
+    case 1: return basis_of_order<1>(b, x, order, v);
+    case 2: return basis_of_order<2>(b, x, order, v);
+    case 3: return basis_of_order<3>(b, x, order, v);
+    case 4: return basis_of_order<4>(b, x, order, v);
+    case 5: return basis_of_order<5>(b, x, order, v);
+    default: return 0;
+    }
+}
+
+// SplineEvaluatorND::eval (spline_evaluator_nd.hpp:684-720) when every order
+// is zero: periodic dimensions wrap first, then the first non-periodic
+// dimension outside [rmin, rmax] applies its rule (only Null reaches here:
+// 0).  Nonzero orders go straight to eval_no_bc, as deriv() does (:445-456).
+template <int N>
+__global__ void __launch_bounds__(256) evalnd_kernel(NdArgs a, double* __restrict__ out,
+                                                     const double* __restrict__ coords, int64_t npts,
+                                                     const double* __restrict__ coef, int64_t cbs, int64_t nbatch,
+                                                     int any_order)
+{
+    int64_t const t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= npts * nbatch) return;
+    int64_t const l = t / npts;
+    double x[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) x[k] = coords[N * t + k];
+    if (!any_order) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            BSplinesView const& b = a.b[k];
+            if (b.periodic && (x[k] < b.rmin || x[k] > b.rmax)) x[k] -= floor((x[k] - b.rmin) / b.length) * b.length;
+        }
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            BSplinesView const& b = a.b[k];
+            if (!b.periodic && ((x[k] < b.rmin && a.lower[k] == DDCX_EXTRAP_NULL) ||
+                                (x[k] > b.rmax && a.upper[k] == DDCX_EXTRAP_NULL))) {
+                out[t] = 0.0;
+                return;
+            }
+        }
+    }
+    double v[N][kMaxDegree + 1];
+    int64_t base = 0;
+    int n[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        int const j = basis_any(a.b[k], x[k], a.order[k], v[k]);
+        base += int64_t(j) * a.st[k];
+        n[k] = a.b[k].degree + 1;
+    }
+    const double* c = coef + l * cbs + base;
+    int idx[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) idx[k] = 0;
+    double y = 0.0;
+    while (true) {
+        double p = v[N - 1][idx[N - 1]];
+        int64_t off = idx[N - 1] * a.st[N - 1];
+#pragma unroll
+        for (int k = N - 2; k >= 0; --k) {
+            p = v[k][idx[k]] * p;
+            off += idx[k] * a.st[k];
+        }
+        y += c[off] * p;
+        int k = N - 1;
+        while (k >= 0 && ++idx[k] == n[k]) idx[k--] = 0;
+        if (k < 0) break;
+    }
+    out[t] = y;
+}
+
+constexpr int kIntThreads = 256;
+
+template <int N>
+__global__ void __launch_bounds__(kIntThreads) integratend_kernel(NdArgs a, double* __restrict__ out,
+                                                                  int64_t ostride, const double* __restrict__ coef,
+                                                                  int64_t cbs, int64_t nbatch)
+{
+    __shared__ double part[kIntThreads];
+    int64_t const l = blockIdx.x;
+    int n[N];
+    int64_t total = 1;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        n[k] = a.b[k].ncells + a.b[k].degree;
+        total *= n[k];
+    }
+    const double* c = coef + l * cbs;
+    double acc = 0.0;
+    for (int64_t f = threadIdx.x; f < total; f += kIntThreads) {
+        int64_t r = f;
+        int idx[N];
+#pragma unroll
+        for (int k = N - 1; k >= 0; --k) {
+            idx[k] = int(r % n[k]);
+            r /= n[k];
+        }
+        double p = a.integrals[N - 1][idx[N - 1]];
+        int64_t off = idx[N - 1] * a.st[N - 1];
+#pragma unroll
+        for (int k = N - 2; k >= 0; --k) {
+            p = a.integrals[k][idx[k]] * p;
+            off += idx[k] * a.st[k];
+        }
+        acc += c[off] * p;
+    }
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = kIntThreads / 2; s > 0; s /= 2) {
+        if (int(threadIdx.x) < s) part[threadIdx.x] += part[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[l * ostride] = part[0];
+}
+
+NdArgs make_args(BSplinesPlan const* const* bs, int nd, const int* orders, const int* lower, const int* upper,
+                 const int64_t* st)
+{
+    NdArgs a{};
+    for (int k = 0; k < nd; ++k) {
+        bs[k]->ensure_device();
+        a.b[k] = bs[k]->device_view();
+        a.integrals[k] = bs[k]->integrals_dev.as<double>();
+        a.st[k] = st[k];
+        a.order[k] = orders ? orders[k] : 0;
+        a.lower[k] = lower ? lower[k] : DDCX_EXTRAP_NULL;
+        a.upper[k] = upper ? upper[k] : DDCX_EXTRAP_NULL;
+    }
+    return a;
+}
+
+template <class F>
+void dispatch_nd(int nd, F&& f)
+{
+    switch (nd) {
+    case 1: f.template operator()<1>(); break;
+    case 2: f.template operator()<2>(); break;
+    case 3: f.template operator()<3>(); break;
+    case 4: f.template operator()<4>(); break;
+    default: throw Error(DDCX_ERR_UNSUPPORTED, "N-dimensional evaluation supports 1 to 4 dimensions");
+    }
+}
+
+} // namespace
+
+void launch_evalnd(BSplinesPlan const* const* bs, int nd, const int* orders, const int* lower, const int* upper,
+                   double* out, const double* coords, int64_t npts, const double* coef, int64_t cbs,
+                   const int64_t* st, int64_t nbatch, hipStream_t stream)
+{
+    if (npts * nbatch == 0) return;
+    NdArgs const a = make_args(bs, nd, orders, lower, upper, st);
+    int any = 0;
+    for (int k = 0; k < nd; ++k) any |= a.order[k];
+    int64_t const total = npts * nbatch;
+    int64_t const blocks = (total + 255) / 256;
+    if (blocks > int64_t(0x7fffffff)) throw Error(DDCX_ERR_INVALID, "batch too large for one launch");
+    dispatch_nd(nd, [&]<int N>() {
+        hipLaunchKernelGGL((evalnd_kernel<N>), dim3(unsigned(blocks)), dim3(256), 0, stream, a, out, coords, npts,
+                           coef, cbs, nbatch, any);
+    });
+    hip_check(hipGetLastError(), "evalnd_kernel launch");
+}
+
+void launch_integratend(BSplinesPlan const* const* bs, int nd, double* out, int64_t ostride, const double* coef,
+                        int64_t cbs, const int64_t* st, int64_t nbatch, hipStream_t stream)
+{
+    if (nbatch == 0) return;
+    NdArgs const a = make_args(bs, nd, nullptr, nullptr, nullptr, st);
+    if (nbatch > int64_t(0x7fffffff)) throw Error(DDCX_ERR_INVALID, "batch too large for one launch");
+    dispatch_nd(nd, [&]<int N>() {
+        hipLaunchKernelGGL((integratend_kernel<N>), dim3(unsigned(nbatch)), dim3(kIntThreads), 0, stream, a, out,
+                           ostride, coef, cbs, nbatch);
+    });
+    hip_check(hipGetLastError(), "integratend_kernel launch");
+}
+
+} // namespace ddcx

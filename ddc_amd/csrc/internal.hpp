@@ -338,6 +338,13 @@ void launch_eval_vjp(BSplinesPlan const& bs, int lower, int upper, const double*
                      int64_t nbatch, hipStream_t stream);
 void launch_integrate(BSplinesPlan const& bs, double* out, int64_t ostride, const double* coef, Layout cl,
                       int64_t nbatch, hipStream_t stream);
+// N-dimensional tensor-product evaluation and integration (eval_nd.hip)
+constexpr int kMaxNd = 4;
+void launch_evalnd(BSplinesPlan const* const* bs, int nd, const int* orders, const int* lower, const int* upper,
+                   double* out, const double* coords, int64_t npts, const double* coef, int64_t cbs,
+                   const int64_t* st, int64_t nbatch, hipStream_t stream);
+void launch_integratend(BSplinesPlan const* const* bs, int nd, double* out, int64_t ostride, const double* coef,
+                        int64_t cbs, const int64_t* st, int64_t nbatch, hipStream_t stream);
 void launch_eval2d(BSplinesPlan const& b1, BSplinesPlan const& b2, const int* rules, int d1, int d2, double* out,
                    const double* coords, int64_t npts, const double* coef, int64_t coef_batch_stride, int64_t st1,
                    int64_t st2, int64_t nbatch, hipStream_t stream);

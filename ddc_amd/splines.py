@@ -88,20 +88,35 @@ def _check_f64_cuda(t, name):
 
 def batched_layout(t, axis):
     """ddcx_layout of a tensor whose dimension `axis` is the row dimension and
-    whose other (at most two) dimensions are batch dimensions, enumerated in
-    row-major order.  Returns (layout, nbatch)."""
+    whose other dimensions are batch dimensions, enumerated in row-major order.
+    Neighbouring batch dimensions on the same side of `axis` that are
+    contiguous with each other (stride[d] == shape[d+1] * stride[d+1]) merge
+    into one; at most two batch dimensions may remain.  Returns
+    (layout, nbatch)."""
     axis = axis % t.dim()
-    bdims = [d for d in range(t.dim()) if d != axis]
-    if len(bdims) > 2:
-        raise ValueError("at most two batch dimensions are supported")
+    groups = []  # [count, stride] of merged batch dimensions, outermost first
+    for side in (range(axis), range(axis + 1, t.dim())):
+        cur = None
+        for d in side:
+            n, st = t.shape[d], t.stride(d)
+            if cur is not None and cur[1] == n * st:
+                cur = [cur[0] * n, st]
+            else:
+                if cur is not None:
+                    groups.append(cur)
+                cur = [n, st]
+        if cur is not None:
+            groups.append(cur)
+    if len(groups) > 2:
+        raise ValueError("at most two batch dimensions are supported (after merging contiguous ones)")
     rs = t.stride(axis)
-    if len(bdims) == 0:
+    if len(groups) == 0:
         return ddcx_layout(rs, 1, 0, 0), 1
-    if len(bdims) == 1:
-        d = bdims[0]
-        return ddcx_layout(rs, t.shape[d], t.stride(d), 0), t.shape[d]
-    d0, d1 = bdims
-    return ddcx_layout(rs, t.shape[d1], t.stride(d1), t.stride(d0)), t.shape[d0] * t.shape[d1]
+    if len(groups) == 1:
+        n, st = groups[0]
+        return ddcx_layout(rs, n, st, 0), n
+    (n0, s0), (n1, s1) = groups
+    return ddcx_layout(rs, n1, s1, s0), n0 * n1
 
 
 class _BSplinesBase:
@@ -443,3 +458,136 @@ class SplineEvaluator2D:
 
     def deriv_1_and_2(self, spline_eval, coords_eval, spline_coef):
         return self(spline_eval, coords_eval, spline_coef, 1, 1)
+
+
+class SplineBuilder3D:
+    """ddc::SplineBuilder3D (spline_builder_3d.hpp:45): one batched 1D build per
+    dimension, dimension 1 first, through two temporaries owned by this
+    object.  Arrays are (batch, n1, n2, n3) (or (n1, n2, n3)).  Hermite
+    closures are not part of this build."""
+
+    def __init__(self, builder1: SplineBuilder, builder2: SplineBuilder, builder3: SplineBuilder):
+        self.builders = (builder1, builder2, builder3)
+        for b in self.builders:
+            if b.s_nbe_xmin or b.s_nbe_xmax:
+                raise NotImplementedError("3D Hermite closures are not part of this build")
+        self._tmp = [None, None]
+
+    def _scratch(self, i, shape, device):
+        t = self._tmp[i]
+        if t is None or tuple(t.shape) != shape or t.device != device:
+            t = self._tmp[i] = torch.empty(shape, dtype=torch.float64, device=device)
+        return t
+
+    def __call__(self, spline, vals):
+        squeeze = vals.dim() == 3
+        v = vals.unsqueeze(0) if squeeze else vals
+        s = spline.unsqueeze(0) if squeeze else spline
+        nb, n1, n2, n3 = v.shape
+        b1, b2, b3 = self.builders
+        t1 = self._scratch(0, (nb, b1.bsplines.size(), n2, n3), v.device)
+        t2 = self._scratch(1, (nb, b1.bsplines.size(), b2.bsplines.size(), n3), v.device)
+        b1(t1, v, axis=1)
+        b2(t2, t1, axis=2)
+        b3(s, t2, axis=3)
+        return spline
+
+    def build_vjp(self, gspline, gvals):
+        """Adjoint of __call__: the transposed 1D solves in reverse order
+        (dimension 3, 2, then 1).  gspline: (batch, size1, size2, size3);
+        gvals: (batch, n1, n2, n3)."""
+        squeeze = gspline.dim() == 3
+        g = gspline.unsqueeze(0) if squeeze else gspline
+        gv = gvals.unsqueeze(0) if squeeze else gvals
+        nb, n1, n2, n3 = gv.shape
+        b1, b2, b3 = self.builders
+        t2 = torch.empty((nb, b1.bsplines.size(), b2.bsplines.size(), n3), dtype=torch.float64, device=g.device)
+        t1 = torch.empty((nb, b1.bsplines.size(), n2, n3), dtype=torch.float64, device=g.device)
+        b3.build_vjp(g, t2, axis=3)
+        b2.build_vjp(t2, t1, axis=2)
+        b1.build_vjp(t1, gv, axis=1)
+        return gvals
+
+
+class SplineEvaluatorND:
+    """ddc::SplineEvaluatorND (spline_evaluator_nd.hpp:49-827) over one 1D
+    evaluator per dimension (1 to 4 dimensions; Null or Periodic rules).
+    Coefficients are (batch, size_1, ..., size_N) with any strides, points
+    (batch, npts, N) and values (batch, npts)."""
+
+    def __init__(self, *evaluators: SplineEvaluator):
+        if not 1 <= len(evaluators) <= 4:
+            raise ValueError("SplineEvaluatorND takes 1 to 4 evaluators")
+        self.evaluators = tuple(evaluators)
+        self._hs = (C.c_void_p * len(evaluators))(*[e._h for e in evaluators])
+
+    def _coef(self, spline_coef):
+        nd = len(self.evaluators)
+        coef = spline_coef.unsqueeze(0) if spline_coef.dim() == nd else spline_coef
+        if coef.dim() != nd + 1:
+            raise ValueError(f"spline_coef must have {nd} (or {nd} + 1 batched) dimensions")
+        st = (C.c_int64 * nd)(*[coef.stride(k + 1) for k in range(nd)])
+        return coef, st
+
+    def deriv(self, orders, spline_eval, coords_eval, spline_coef):
+        """SplineEvaluatorND::deriv (spline_evaluator_nd.hpp:445-590): orders
+        is one derivative order per dimension."""
+        nd = len(self.evaluators)
+        coef, st = self._coef(spline_coef)
+        c = coords_eval.contiguous()
+        if c.shape[-1] != nd:
+            raise ValueError(f"coords_eval must end with a dimension of size {nd}")
+        if not spline_eval.is_contiguous():
+            raise ValueError("spline_eval must be contiguous")
+        nb = coef.shape[0]
+        npts = c.numel() // (nd * nb)
+        ords = None if orders is None else (C.c_int * nd)(*[int(o) for o in orders])
+        call("ddcx_evaluatornd_eval", self._hs, nd, ords, _ptr(spline_eval), _ptr(c), npts, _ptr(coef),
+             coef.stride(0), st, nb, _stream())
+        return spline_eval
+
+    def __call__(self, spline_eval, coords_eval, spline_coef):
+        """SplineEvaluatorND::operator() (spline_evaluator_nd.hpp:300-440)."""
+        return self.deriv(None, spline_eval, coords_eval, spline_coef)
+
+    def integrate(self, integrals_out, spline_coef):
+        """SplineEvaluatorND::integrate (spline_evaluator_nd.hpp:597-632):
+        integrals_out (batch,) receives the integral over the whole domain."""
+        nd = len(self.evaluators)
+        coef, st = self._coef(spline_coef)
+        nb = coef.shape[0]
+        out = integrals_out.reshape(-1)
+        if out.numel() != nb:
+            raise ValueError("integrals_out must hold one value per batch entry")
+        call("ddcx_evaluatornd_integrate", self._hs, nd, _ptr(out), out.stride(0), _ptr(coef), coef.stride(0), st,
+             nb, _stream())
+        return integrals_out
+
+
+class SplineEvaluator3D(SplineEvaluatorND):
+    """ddc::SplineEvaluator3D (spline_evaluator_3d.hpp:56-1019): the
+    three-dimensional SplineEvaluatorND."""
+
+    def __init__(self, ev1: SplineEvaluator, ev2: SplineEvaluator, ev3: SplineEvaluator):
+        super().__init__(ev1, ev2, ev3)
+
+    def deriv_dim_1(self, spline_eval, coords_eval, spline_coef):
+        return self.deriv((1, 0, 0), spline_eval, coords_eval, spline_coef)
+
+    def deriv_dim_2(self, spline_eval, coords_eval, spline_coef):
+        return self.deriv((0, 1, 0), spline_eval, coords_eval, spline_coef)
+
+    def deriv_dim_3(self, spline_eval, coords_eval, spline_coef):
+        return self.deriv((0, 0, 1), spline_eval, coords_eval, spline_coef)
+
+    def deriv_1_and_2(self, spline_eval, coords_eval, spline_coef):
+        return self.deriv((1, 1, 0), spline_eval, coords_eval, spline_coef)
+
+    def deriv_1_and_3(self, spline_eval, coords_eval, spline_coef):
+        return self.deriv((1, 0, 1), spline_eval, coords_eval, spline_coef)
+
+    def deriv_2_and_3(self, spline_eval, coords_eval, spline_coef):
+        return self.deriv((0, 1, 1), spline_eval, coords_eval, spline_coef)
+
+    def deriv_1_2_and_3(self, spline_eval, coords_eval, spline_coef):
+        return self.deriv((1, 1, 1), spline_eval, coords_eval, spline_coef)

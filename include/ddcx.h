@@ -208,6 +208,27 @@ ddcx_status ddcx_evaluator2d_eval(const ddcx_evaluator* e1, const ddcx_evaluator
                                   int64_t coef_batch_stride, int64_t st1, int64_t st2, int64_t nbatch,
                                   void* stream);
 
+/* SplineEvaluatorND::operator() / deriv(), spline_evaluator_nd.hpp:300-590
+ * (eval :684-720, eval_no_bc :782-826); SplineEvaluator3D
+ * (spline_evaluator_3d.hpp:443-1019) is its nd = 3 case.  evs: one evaluator
+ * per dimension (1 <= nd <= 4); their extrapolation rules must be Null or
+ * Periodic (Constant is refused: its reference form exists for 1D and 2D
+ * only).  orders: nd derivative orders (NULL: all 0), each at most
+ * min(degree, 3); with any nonzero order no wrap or rule is applied, as in
+ * deriv().  coords: the nd coordinates of point p of batch l at
+ * coords[nd*(l*npts + p)], out[l*npts + p]; coef of batch l at
+ * coef + l*coef_batch_stride, element (i_1..i_nd) at sum_k i_k*coef_strides[k]. */
+ddcx_status ddcx_evaluatornd_eval(const ddcx_evaluator* const* evs, int nd, const int* orders, double* out,
+                                  const double* coords, int64_t npts, const double* coef, int64_t coef_batch_stride,
+                                  const int64_t* coef_strides, int64_t nbatch, void* stream);
+
+/* SplineEvaluatorND::integrate(integrals, spline_coef),
+ * spline_evaluator_nd.hpp:597-632: out[l*out_stride] = integral of batch l's
+ * spline over the whole domain. */
+ddcx_status ddcx_evaluatornd_integrate(const ddcx_evaluator* const* evs, int nd, double* out, int64_t out_stride,
+                                       const double* coef, int64_t coef_batch_stride, const int64_t* coef_strides,
+                                       int64_t nbatch, void* stream);
+
 /* Vector-Jacobian product of ddcx_evaluator2d_eval (value form; no reference
  * counterpart): gcoef (same strides as coef, overwritten) = sum_p w_p
  * B1(x1_p) (x) B2(x2_p) (tensor scatter-add); gx (interleaved pairs like

@@ -81,6 +81,9 @@ def lib():
         L.or_spline_quadrature.argtypes = [_vp, _dp]
         L.or_spline2d_eval.argtypes = [_vp, _vp, _dp, _dp, _L, _dp, _L, _L, _L, _L, C.c_int, C.c_int,
                                        np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")]
+        L.or_splinend_eval.argtypes = [C.POINTER(_vp), C.c_int, _dp, _dp, _L, _dp, _L, C.POINTER(_L), _L,
+                                       C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.or_splinend_integrate.argtypes = [C.POINTER(_vp), C.c_int, _dp, _dp, _L, C.POINTER(_L), _L]
         L.or_spline_destroy.argtypes = [_vp]
         _LIB = L
     return _LIB
@@ -321,4 +324,58 @@ def spline2d_eval(s1: Spline, s2: Spline, coords, coef, d1=0, d2=0, rules=(0, 0,
     out = np.empty((nb, P))
     lib().or_spline2d_eval(s1.h, s2.h, coords, out, P, coef, s1.size * s2.size, s2.size, 1, nb, d1, d2,
                            np.asarray(rules, dtype=np.int32))
+    return out
+
+
+def splinend_build(splines, vals):
+    """SplineBuilder3D::operator() (spline_builder_3d.hpp; SplineBuilder2D for
+    two dimensions) without Hermite closures: one batched 1D build per
+    dimension, dimension 1 first.  vals (nbatch, n_1, ..., n_N) ->
+    coef (nbatch, size_1, ..., size_N)."""
+    cur = np.ascontiguousarray(vals, dtype=np.float64)
+    for k, s in enumerate(splines):
+        ax = k + 1
+        shape = list(cur.shape)
+        outer = int(np.prod(shape[:ax]))
+        inner = int(np.prod(shape[ax + 1:]))
+        shape[ax] = s.size
+        nxt = np.empty(shape)
+        # rows along axis ax; lines enumerate (outer, inner) in row-major order
+        lib().or_spline_build(s.h, cur, inner, inner, 1, cur.shape[ax] * inner, nxt, inner, inner, 1,
+                              s.size * inner, None, None, 1, 1, 0, 0, outer * inner)
+        cur = nxt
+    return cur
+
+
+def _nd_args(splines, coef):
+    nd = len(splines)
+    hs = (C.c_void_p * nd)(*[s.h for s in splines])
+    st = (C.c_long * nd)(*[coef.strides[k + 1] // 8 for k in range(nd)])
+    return nd, hs, st
+
+
+def splinend_eval(splines, coords, coef, orders=None, rules=None):
+    """SplineEvaluatorND (spline_evaluator_nd.hpp:684-826): coords
+    (nbatch, npts, N), coef (nbatch, size_1, ..., size_N) -> (nbatch, npts).
+    rules: (lower, upper) per dimension, 0 = Null (periodic dimensions wrap)."""
+    coords = np.ascontiguousarray(coords, dtype=np.float64)
+    coef = np.ascontiguousarray(coef, dtype=np.float64)
+    nd, hs, st = _nd_args(splines, coef)
+    nb, P = coords.shape[0], coords.shape[1]
+    orders = (C.c_int * nd)(*(orders or [0] * nd))
+    rules = (C.c_int * (2 * nd))(*(rules or [0] * (2 * nd)))
+    out = np.empty((nb, P))
+    if lib().or_splinend_eval(hs, nd, coords, out, P, coef, int(np.prod(coef.shape[1:])), st, nb, orders, rules):
+        raise RuntimeError(_err())
+    return out
+
+
+def splinend_integrate(splines, coef):
+    """SplineEvaluatorND::integrate (spline_evaluator_nd.hpp:597-632):
+    coef (nbatch, size_1, ..., size_N) -> (nbatch,)."""
+    coef = np.ascontiguousarray(coef, dtype=np.float64)
+    nd, hs, st = _nd_args(splines, coef)
+    out = np.empty(coef.shape[0])
+    if lib().or_splinend_integrate(hs, nd, out, coef, int(np.prod(coef.shape[1:])), st, coef.shape[0]):
+        raise RuntimeError(_err())
     return out

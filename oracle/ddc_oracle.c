@@ -1510,3 +1510,122 @@ int or_spline2d_eval(void* h1, void* h2, const double* coords, double* out, long
     }
     return 0;
 }
+
+/* ------------------------------------------------------------------------- */
+/* N-dimensional tensor-product evaluation (SplineEvaluatorND,                */
+/* spline_evaluator_nd.hpp; SplineEvaluator3D is its N = 3 case)              */
+/* ------------------------------------------------------------------------- */
+
+#define OR_MAXND 6
+
+/* SplineEvaluatorND::eval (spline_evaluator_nd.hpp:684-720) for all orders
+ * zero: every periodic dimension is wrapped first (update_coord_eval,
+ * :636-652), then the first non-periodic dimension outside [rmin, rmax]
+ * decides (check_needs_extrapolation, :655-680): rule 0 (Null) gives 0.
+ * With any nonzero order the point goes straight to eval_no_bc, as deriv()
+ * does (:445-456).  eval_no_bc (:782-826): per-dimension basis values
+ * (get_jmin, :722-751), then y += coef(jmin + idx) * (v_1 * (v_2 * (...)))
+ * over the multi-index in row-major order (for_each, :753-768; the basis
+ * product is a right fold). */
+static double evalnd(or_spline* const* s, int nd, const double* xin, const double* coef, const long* st,
+                     const int* orders, const int* rules)
+{
+    double x[OR_MAXND];
+    int any = 0;
+    for (int k = 0; k < nd; ++k) {
+        x[k] = xin[k];
+        any |= orders[k];
+    }
+    if (!any) {
+        for (int k = 0; k < nd; ++k) {
+            const or_bspl* b = &s[k]->bs;
+            if (b->periodic && (x[k] < bs_rmin(b) || x[k] > bs_rmax(b)))
+                x[k] -= floor((x[k] - bs_rmin(b)) / bs_length(b)) * bs_length(b);
+        }
+        for (int k = 0; k < nd; ++k) {
+            const or_bspl* b = &s[k]->bs;
+            if (b->periodic) continue;
+            if (x[k] < bs_rmin(b) && rules[2 * k] == 0) return 0.0;
+            if (x[k] > bs_rmax(b) && rules[2 * k + 1] == 0) return 0.0;
+        }
+    }
+    double v[OR_MAXND][OR_MAXDEG + 1];
+    int j[OR_MAXND], n[OR_MAXND];
+    double der[(OR_MAXDEG + 1) * (OR_MAXDEG + 1)];
+    for (int k = 0; k < nd; ++k) {
+        const or_bspl* b = &s[k]->bs;
+        n[k] = b->degree + 1;
+        if (orders[k] == 0) j[k] = bs_eval_basis(b, x[k], v[k]);
+        else {
+            j[k] = bs_eval_basis_and_n_derivs(b, x[k], orders[k], der);
+            for (int i = 0; i < n[k]; ++i) v[k][i] = der[i * (orders[k] + 1) + orders[k]];
+        }
+    }
+    int idx[OR_MAXND] = {0};
+    double y = 0.0;
+    for (;;) {
+        double p = v[nd - 1][idx[nd - 1]];
+        for (int k = nd - 2; k >= 0; --k) p = v[k][idx[k]] * p;
+        long off = 0;
+        for (int k = 0; k < nd; ++k) off += (long)(j[k] + idx[k]) * st[k];
+        y += coef[off] * p;
+        int k = nd - 1;
+        while (k >= 0 && ++idx[k] == n[k]) idx[k--] = 0;
+        if (k < 0) break;
+    }
+    return y;
+}
+
+/* batched ND evaluation: coords interleave the nd coordinates of each point;
+ * point p of batch l at coords[nd*(l*npts + p)], out[l*npts + p]; coef of
+ * batch l at coef + l*cbs, element (i_1..i_nd) at sum i_k st[k].
+ * rules: lower and upper rule of each dimension (0 Null). */
+int or_splinend_eval(void* const* hs, int nd, const double* coords, double* out, long npts, const double* coef,
+                     long cbs, const long* st, long nbatch, const int* orders, const int* rules)
+{
+    if (nd < 1 || nd > OR_MAXND) { set_err("dimension out of range"); return -1; }
+    or_spline* const* s = (or_spline* const*)hs;
+    long const total = npts * nbatch;
+#pragma omp parallel for schedule(static)
+    for (long t = 0; t < total; ++t) {
+        long const l = t / npts;
+        out[t] = evalnd(s, nd, coords + nd * t, coef + l * cbs, st, orders, rules);
+    }
+    return 0;
+}
+
+/* SplineEvaluatorND::integrate (spline_evaluator_nd.hpp:597-632): for each
+ * batch, sum over the coefficient multi-index (row-major) of
+ * coef(i) * (I_1(i_1) * (I_2(i_2) * (...))), I_k the integrals of dimension
+ * k's B-splines (integrals.hpp:30-177). */
+int or_splinend_integrate(void* const* hs, int nd, double* out, const double* coef, long cbs, const long* st,
+                          long nbatch)
+{
+    if (nd < 1 || nd > OR_MAXND) { set_err("dimension out of range"); return -1; }
+    or_spline* const* s = (or_spline* const*)hs;
+    double* q[OR_MAXND];
+    int n[OR_MAXND];
+    for (int k = 0; k < nd; ++k) {
+        n[k] = bs_size(&s[k]->bs);
+        q[k] = (double*)malloc(sizeof(double) * n[k]);
+        bs_integrals(&s[k]->bs, q[k]);
+    }
+#pragma omp parallel for schedule(static)
+    for (long l = 0; l < nbatch; ++l) {
+        int idx[OR_MAXND] = {0};
+        double acc = 0.0;
+        for (;;) {
+            double p = q[nd - 1][idx[nd - 1]];
+            for (int k = nd - 2; k >= 0; --k) p = q[k][idx[k]] * p;
+            long off = l * cbs;
+            for (int k = 0; k < nd; ++k) off += (long)idx[k] * st[k];
+            acc += coef[off] * p;
+            int k = nd - 1;
+            while (k >= 0 && ++idx[k] == n[k]) idx[k--] = 0;
+            if (k < 0) break;
+        }
+        out[l] = acc;
+    }
+    for (int k = 0; k < nd; ++k) free(q[k]);
+    return 0;
+}
