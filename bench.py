@@ -271,6 +271,46 @@ def measure_2d(D, timed, gen):
             "points": nb * P}
 
 
+def measure_3d(D, timed, gen):
+    """Tensor-product 3D (SplineBuilder3D / SplineEvaluator3D): x non-uniform
+    cubic (128 cells, Greville closures), y uniform periodic cubic (128
+    cells), z uniform cubic (128 cells, Greville closures), batch 4; build,
+    eval and deriv (1, 1, 1) at 2^22 scattered points per batch, integrate."""
+    import numpy as np
+    import torch
+
+    dev = torch.device("cuda:0")
+    nb, P, n = 4, 1 << 22, 128
+    rng = np.random.default_rng(0xDDC0006)
+    br = np.concatenate([[0.0], np.sort(rng.uniform(0.0, 1.0, n - 1)), [1.0]])
+    G = D.SplineBuilderClosure.GREVILLE
+    bs = [D.NonUniformBSplines(3, br), D.UniformBSplines(3, 0.0, 1.0, n, periodic=True),
+          D.UniformBSplines(3, 0.0, 1.0, n)]
+    b3 = D.SplineBuilder3D(D.SplineBuilder(bs[0], G, G), D.SplineBuilder(bs[1]), D.SplineBuilder(bs[2], G, G))
+    null, per = D.NullExtrapolationRule(), D.PeriodicExtrapolationRule()
+    ev = D.SplineEvaluator3D(D.SplineEvaluator(bs[0], null, null), D.SplineEvaluator(bs[1], per, per),
+                             D.SplineEvaluator(bs[2], null, null))
+    npts = [b.npoints() for b in b3.builders]
+    vals = torch.randn([nb] + npts, dtype=torch.float64, device=dev, generator=gen)
+    coef = torch.empty([nb] + [b.size() for b in bs], dtype=torch.float64, device=dev)
+    b3(coef, vals)
+    t_b = timed(lambda: b3(coef, vals), 5)
+    coords = torch.rand((nb, P, 3), dtype=torch.float64, device=dev, generator=gen)
+    out = torch.empty((nb, P), dtype=torch.float64, device=dev)
+    ev(out, coords, coef)
+    t_e = timed(lambda: ev(out, coords, coef), 3)
+    t_d = timed(lambda: ev.deriv((1, 1, 1), out, coords, coef), 3)
+    integ = torch.empty(nb, dtype=torch.float64, device=dev)
+    t_i = timed(lambda: ev.integrate(integ, coef), 5)
+    by_b = 8 * (vals.numel() + coef.numel())
+    by_e = 32 * nb * P + 8 * coef.numel()
+    return {"build_us": round(t_b * 1e6, 1), "build_GBps": round(by_b / t_b / 1e9, 1),
+            "eval_us": round(t_e * 1e6, 1), "eval_GBps": round(by_e / t_e / 1e9, 1),
+            "eval_Mpts_s": round(nb * P / t_e / 1e6, 1), "deriv111_us": round(t_d * 1e6, 1),
+            "integrate_us": round(t_i * 1e6, 1), "integrate_GBps": round(8 * coef.numel() / t_i / 1e9, 1),
+            "points": nb * P}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -383,6 +423,8 @@ def main():
         torch.cuda.empty_cache()
         other["4_advection_512x512"] = measure_advection(D, timed, n_k)
         other["5_tensor_2d_1024x1024x16"] = measure_2d(D, timed, gen)
+        torch.cuda.empty_cache()
+        other["6_tensor_3d_128x128x128x4"] = measure_3d(D, timed, gen)
         res["workloads"] = other
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(nc, 4096, P_line, 0xDDC0002)

@@ -12,6 +12,8 @@
 // block (spline_evaluator_nd.hpp:597-632).
 #include "internal.hpp"
 
+#include <utility>
+
 namespace ddcx {
 
 namespace {
@@ -24,6 +26,33 @@ struct NdArgs {
     int lower[kMaxNd];
     int upper[kMaxNd];
 };
+
+constexpr int kM = 6; // basis values per dimension (degree <= 5)
+
+// y += coef(jmin + idx) * (v_1[idx_1] * (v_2[idx_2] * (...))) over the
+// multi-index in row-major order, every index a compile-time constant (so
+// the basis tables stay in registers) behind a wave-uniform bound check.
+template <int N, int... I>
+__device__ __forceinline__ void tensor_acc(double const (&v)[N][kM], int const (&n)[N], int64_t const (&st)[N],
+                                           const double* c, double& y)
+{
+    constexpr int K = sizeof...(I);
+    if constexpr (K == N) {
+        constexpr int ix[N] = {I...};
+        double p = v[N - 1][ix[N - 1]];
+        int64_t off = ix[N - 1] * st[N - 1];
+#pragma unroll
+        for (int k = N - 2; k >= 0; --k) {
+            p = v[k][ix[k]] * p;
+            off += ix[k] * st[k];
+        }
+        y += c[off] * p;
+    } else {
+        [&]<int... J>(std::integer_sequence<int, J...>) {
+            ((J < n[K] ? tensor_acc<N, I..., J>(v, n, st, c, y) : void()), ...);
+        }(std::make_integer_sequence<int, kM>{});
+    }
+}
 
 template <int D>
 __device__ int basis_of_order(BSplinesView const& b, double x, int order, double* v)
@@ -91,72 +120,87 @@ __global__ void __launch_bounds__(256) evalnd_kernel(NdArgs a, double* __restric
             }
         }
     }
-    double v[N][kMaxDegree + 1];
+    double v[N][kM];
+    int64_t st[N];
     int64_t base = 0;
     int n[N];
 #pragma unroll
     for (int k = 0; k < N; ++k) {
         int const j = basis_any(a.b[k], x[k], a.order[k], v[k]);
-        base += int64_t(j) * a.st[k];
+        st[k] = a.st[k];
+        base += int64_t(j) * st[k];
         n[k] = a.b[k].degree + 1;
     }
-    const double* c = coef + l * cbs + base;
-    int idx[N];
-#pragma unroll
-    for (int k = 0; k < N; ++k) idx[k] = 0;
     double y = 0.0;
-    while (true) {
-        double p = v[N - 1][idx[N - 1]];
-        int64_t off = idx[N - 1] * a.st[N - 1];
-#pragma unroll
-        for (int k = N - 2; k >= 0; --k) {
-            p = v[k][idx[k]] * p;
-            off += idx[k] * a.st[k];
-        }
-        y += c[off] * p;
-        int k = N - 1;
-        while (k >= 0 && ++idx[k] == n[k]) idx[k--] = 0;
-        if (k < 0) break;
-    }
+    tensor_acc<N>(v, n, st, coef + l * cbs + base, y);
     out[t] = y;
 }
 
-constexpr int kIntThreads = 256;
+constexpr int kIntThreads = 1024;
 
+// One workgroup per batch entry.  Wave w takes the rows (all indices but the
+// last) in groups of RW (w*RW .. w*RW+RW-1, then the next group of the next
+// round), and its lanes sweep the last dimension, so the loads of a row are
+// contiguous when the last dimension is.  Each term is
+// coef(i) * (I_1(i_1) * (... * I_N(i_N))) as the reference forms it; the
+// per-lane sums are combined by a fixed LDS tree, so results do not depend on
+// scheduling.
 template <int N>
 __global__ void __launch_bounds__(kIntThreads) integratend_kernel(NdArgs a, double* __restrict__ out,
                                                                   int64_t ostride, const double* __restrict__ coef,
                                                                   int64_t cbs, int64_t nbatch)
 {
+    constexpr int W = 64, NW = kIntThreads / W;
     __shared__ double part[kIntThreads];
     int64_t const l = blockIdx.x;
+    int const lane = threadIdx.x % W;
+    int const wave = threadIdx.x / W;
     int n[N];
-    int64_t total = 1;
+    int64_t rows = 1;
 #pragma unroll
     for (int k = 0; k < N; ++k) {
         n[k] = a.b[k].ncells + a.b[k].degree;
-        total *= n[k];
+        if (k < N - 1) rows *= n[k];
     }
     const double* c = coef + l * cbs;
-    double acc = 0.0;
-    for (int64_t f = threadIdx.x; f < total; f += kIntThreads) {
-        int64_t r = f;
-        int idx[N];
+    const double* qn = a.integrals[N - 1];
+    int64_t const sn = a.st[N - 1];
+    // RW rows per wave at a time, with independent loads and sums
+    constexpr int RW = 4;
+    double acc[RW];
 #pragma unroll
-        for (int k = N - 1; k >= 0; --k) {
-            idx[k] = int(r % n[k]);
-            r /= n[k];
-        }
-        double p = a.integrals[N - 1][idx[N - 1]];
-        int64_t off = idx[N - 1] * a.st[N - 1];
+    for (int u = 0; u < RW; ++u) acc[u] = 0.0;
+    for (int64_t r0 = int64_t(wave) * RW; r0 < rows; r0 += int64_t(NW) * RW) {
+        int64_t off[RW];
+        double outer[RW][N > 1 ? N - 1 : 1];
 #pragma unroll
-        for (int k = N - 2; k >= 0; --k) {
-            p = a.integrals[k][idx[k]] * p;
-            off += idx[k] * a.st[k];
+        for (int u = 0; u < RW; ++u) {
+            uint32_t rr = uint32_t(min(r0 + u, rows - 1)); // rows < 2^31, checked at launch
+            off[u] = 0;
+#pragma unroll
+            for (int k = N - 2; k >= 0; --k) {
+                int const ik = int(rr % uint32_t(n[k]));
+                rr /= uint32_t(n[k]);
+                off[u] += ik * a.st[k];
+                outer[u][k] = a.integrals[k][ik];
+            }
         }
-        acc += c[off] * p;
+        for (int i = lane; i < n[N - 1]; i += W) {
+            double const q = qn[i];
+#pragma unroll
+            for (int u = 0; u < RW; ++u) {
+                double p = q;
+#pragma unroll
+                for (int k = N - 2; k >= 0; --k) p = outer[u][k] * p;
+                double const t = c[off[u] + i * sn] * p;
+                if (r0 + u < rows) acc[u] += t;
+            }
+        }
     }
-    part[threadIdx.x] = acc;
+    double sum = 0.0;
+#pragma unroll
+    for (int u = 0; u < RW; ++u) sum += acc[u];
+    part[threadIdx.x] = sum;
     __syncthreads();
     for (int s = kIntThreads / 2; s > 0; s /= 2) {
         if (int(threadIdx.x) < s) part[threadIdx.x] += part[threadIdx.x + s];
@@ -219,6 +263,9 @@ void launch_integratend(BSplinesPlan const* const* bs, int nd, double* out, int6
     if (nbatch == 0) return;
     NdArgs const a = make_args(bs, nd, nullptr, nullptr, nullptr, st);
     if (nbatch > int64_t(0x7fffffff)) throw Error(DDCX_ERR_INVALID, "batch too large for one launch");
+    int64_t rows = 1;
+    for (int k = 0; k + 1 < nd; ++k) rows *= bs[k]->size();
+    if (rows >= (int64_t(1) << 31)) throw Error(DDCX_ERR_UNSUPPORTED, "too many coefficient rows to integrate");
     dispatch_nd(nd, [&]<int N>() {
         hipLaunchKernelGGL((integratend_kernel<N>), dim3(unsigned(nbatch)), dim3(kIntThreads), 0, stream, a, out,
                            ostride, coef, cbs, nbatch);

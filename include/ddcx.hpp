@@ -15,6 +15,7 @@
 #include <optional>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "ddcx.h"
@@ -42,27 +43,34 @@ struct Span {
 };
 
 // Layout of a layout_right array whose extents are `ext` (outermost first)
-// and whose interpolation dimension is `axis` (at most two other dimensions).
+// and whose interpolation dimension is `axis`.  The other dimensions are batch
+// dimensions; neighbouring ones on the same side of `axis` merge (they are
+// contiguous in a layout_right array), so at most two remain.
 inline ddcx_layout batched_layout(std::vector<int64_t> const& ext, int axis, int64_t* nbatch = nullptr)
 {
     int const nd = int(ext.size());
     std::vector<int64_t> st(nd, 1);
     for (int d = nd - 2; d >= 0; --d) st[d] = st[d + 1] * ext[d + 1];
-    std::vector<int> others;
-    for (int d = 0; d < nd; ++d)
-        if (d != axis) others.push_back(d);
-    if (others.size() > 2) throw std::runtime_error("at most two batch dimensions");
+    std::vector<std::pair<int64_t, int64_t>> groups; // (count, stride), outermost first
+    auto side = [&](int d0, int d1) {
+        if (d0 >= d1) return;
+        int64_t n = 1;
+        for (int d = d0; d < d1; ++d) n *= ext[d];
+        groups.emplace_back(n, st[d1 - 1]);
+    };
+    side(0, axis);
+    side(axis + 1, nd);
     ddcx_layout l{st[axis], 1, 0, 0};
     int64_t nb = 1;
-    if (others.size() == 1) {
-        l.inner_count = ext[others[0]];
-        l.inner_stride = st[others[0]];
-        nb = ext[others[0]];
-    } else if (others.size() == 2) {
-        l.inner_count = ext[others[1]];
-        l.inner_stride = st[others[1]];
-        l.outer_stride = st[others[0]];
-        nb = ext[others[0]] * ext[others[1]];
+    if (groups.size() == 1) {
+        l.inner_count = groups[0].first;
+        l.inner_stride = groups[0].second;
+        nb = groups[0].first;
+    } else if (groups.size() == 2) {
+        l.inner_count = groups[1].first;
+        l.inner_stride = groups[1].second;
+        l.outer_stride = groups[0].second;
+        nb = groups[0].first * groups[1].first;
     }
     if (nbatch) *nbatch = nb;
     return l;
@@ -228,6 +236,96 @@ private:
         void operator()(ddcx_evaluator* h) const { ddcx_evaluator_destroy(h); }
     };
     std::unique_ptr<ddcx_evaluator, Del> h_;
+};
+
+// SplineEvaluatorND<Exec, Mem, TypeSeq<BSplines...>, TypeSeq<EvalDDim...>,
+// TypeSeq<Rules...>> (spline_evaluator_nd.hpp:49): one 1D evaluator per
+// dimension (1 to 4, Null or Periodic rules; not owned).  Points interleave
+// their nd coordinates (point p of batch l at coords[nd*(l*npts + p)]);
+// coefficients of batch l start at coef + l*coef_batch_stride, element
+// (i_1..i_nd) at sum_k i_k*strides[k].
+class SplineEvaluatorND {
+public:
+    explicit SplineEvaluatorND(std::vector<SplineEvaluator const*> evs)
+    {
+        for (auto const* e : evs) hs_.push_back(e->handle());
+    }
+    int dimension() const { return int(hs_.size()); }
+    // operator()(spline_eval, coords_eval, spline_coef), spline_evaluator_nd.hpp:300-440
+    void operator()(double* out, const double* coords, int64_t npts, const double* coef, int64_t coef_batch_stride,
+                    std::vector<int64_t> const& strides, int64_t nbatch, void* stream = nullptr) const
+    {
+        deriv({}, out, coords, npts, coef, coef_batch_stride, strides, nbatch, stream);
+    }
+    // deriv(deriv_order, spline_eval, coords_eval, spline_coef), spline_evaluator_nd.hpp:445-590
+    void deriv(std::vector<int> const& orders, double* out, const double* coords, int64_t npts, const double* coef,
+               int64_t coef_batch_stride, std::vector<int64_t> const& strides, int64_t nbatch,
+               void* stream = nullptr) const
+    {
+        if (int(strides.size()) != dimension() || (!orders.empty() && int(orders.size()) != dimension()))
+            throw std::runtime_error("ddcx: one stride and one order per dimension");
+        check(ddcx_evaluatornd_eval(hs_.data(), dimension(), orders.empty() ? nullptr : orders.data(), out, coords,
+                                    npts, coef, coef_batch_stride, strides.data(), nbatch, stream));
+    }
+    // integrate(integrals, spline_coef), spline_evaluator_nd.hpp:597-632
+    void integrate(double* out, int64_t out_stride, const double* coef, int64_t coef_batch_stride,
+                   std::vector<int64_t> const& strides, int64_t nbatch, void* stream = nullptr) const
+    {
+        if (int(strides.size()) != dimension()) throw std::runtime_error("ddcx: one stride per dimension");
+        check(ddcx_evaluatornd_integrate(hs_.data(), dimension(), out, out_stride, coef, coef_batch_stride,
+                                         strides.data(), nbatch, stream));
+    }
+
+private:
+    std::vector<const ddcx_evaluator*> hs_;
+};
+
+// SplineEvaluator3D<...> (spline_evaluator_3d.hpp:56): the three-dimensional case.
+class SplineEvaluator3D : public SplineEvaluatorND {
+public:
+    SplineEvaluator3D(SplineEvaluator const& e1, SplineEvaluator const& e2, SplineEvaluator const& e3)
+        : SplineEvaluatorND({&e1, &e2, &e3})
+    {
+    }
+};
+
+// SplineBuilder3D<...> (spline_builder_3d.hpp:45): one batched 1D build per
+// dimension, dimension 1 first, on layout_right arrays vals (nbatch, n1, n2,
+// n3) and spline (nbatch, s1, s2, s3), through two caller-provided device
+// workspaces of work1_size() and work2_size() doubles (nothing is allocated).
+// Hermite closures are not part of this build.
+class SplineBuilder3D {
+public:
+    SplineBuilder3D(SplineBuilder const& b1, SplineBuilder const& b2, SplineBuilder const& b3, int64_t size1,
+                    int64_t size2, int64_t size3)
+        : b_{&b1, &b2, &b3}, s_{size1, size2, size3}
+    {
+        for (auto const* b : b_)
+            if (b->info().nbv_lower || b->info().nbv_upper)
+                throw std::runtime_error("ddcx: 3D Hermite closures are not part of this build");
+    }
+    int64_t work1_size(int64_t nbatch) const { return nbatch * s_[0] * n(1) * n(2); }
+    int64_t work2_size(int64_t nbatch) const { return nbatch * s_[0] * s_[1] * n(2); }
+    void operator()(double* spline, const double* vals, int64_t nbatch, double* work1, double* work2,
+                    void* stream = nullptr) const
+    {
+        pass(0, work1, {nbatch, s_[0], n(1), n(2)}, vals, {nbatch, n(0), n(1), n(2)}, stream);
+        pass(1, work2, {nbatch, s_[0], s_[1], n(2)}, work1, {nbatch, s_[0], n(1), n(2)}, stream);
+        pass(2, spline, {nbatch, s_[0], s_[1], s_[2]}, work2, {nbatch, s_[0], s_[1], n(2)}, stream);
+    }
+
+private:
+    int64_t n(int k) const { return b_[k]->npoints(); }
+    void pass(int k, double* out, std::vector<int64_t> const& eo, const double* in, std::vector<int64_t> const& ei,
+              void* stream) const
+    {
+        int64_t nb = 0;
+        ddcx_layout const lo = batched_layout(eo, k + 1, &nb);
+        ddcx_layout const li = batched_layout(ei, k + 1);
+        (*b_[k])(Span<double>{out, lo, nb}, Span<double const>{in, li, nb}, std::nullopt, std::nullopt, stream);
+    }
+    SplineBuilder const* b_[3];
+    int64_t s_[3];
 };
 
 } // namespace ddcx

@@ -136,6 +136,22 @@ int main()
     ddcx_layout l = ddcx::batched_layout({5, 7, 11}, 1, &nbatch);
     std::printf("%lld %lld %lld %lld %lld\n", (long long)l.row_stride, (long long)l.inner_count,
                 (long long)l.inner_stride, (long long)l.outer_stride, (long long)nbatch);
+    for (int ax = 1; ax <= 3; ++ax) {
+        ddcx_layout m = ddcx::batched_layout({2, 5, 6, 7}, ax, &nbatch);
+        std::printf("%lld %lld %lld %lld %lld\n", (long long)m.row_stride, (long long)m.inner_count,
+                    (long long)m.inner_stride, (long long)m.outer_stride, (long long)nbatch);
+    }
+    ddcx::SplineEvaluator e1(bs, ddcx::Extrapolation::Null, ddcx::Extrapolation::Null);
+    ddcx::SplineEvaluator e2(ub, ddcx::Extrapolation::Periodic, ddcx::Extrapolation::Periodic);
+    ddcx::SplineEvaluator3D e3(e1, e2, e1);
+    try {
+        e3.deriv({4, 0, 0}, nullptr, nullptr, 0, nullptr, 0, {1, 1, 1}, 0);
+        std::printf("no error\n");
+    } catch (std::runtime_error const& e) {
+        std::printf("error: %s\n", e.what());
+    }
+    ddcx::SplineBuilder3D b3(b, pb, b, bs.size(), ub.size(), bs.size());
+    std::printf("%lld %lld %d\n", (long long)b3.work1_size(2), (long long)b3.work2_size(2), e3.dimension());
     return 0;
 }
 """
@@ -175,3 +191,12 @@ def test_cpp_facade(tmp_path):
     assert out[1].split() == [str(pb.npoints()), str(pb.offset()), str(pb.info.kernel)]
     assert out[2].startswith("error:")
     assert out[3].split() == ["11", "11", "1", "77", "55"]  # same as splines.batched_layout
+    import torch
+    t = torch.empty((2, 5, 6, 7), dtype=torch.float64)
+    for ax in (1, 2, 3):
+        lay, nb = D.batched_layout(t, ax)
+        assert out[3 + ax].split() == [str(v) for v in (lay.row_stride, lay.inner_count, lay.inner_stride,
+                                                       lay.outer_stride, nb)]
+    assert out[7].startswith("error:") and "order" in out[7]
+    n1, s1 = b.npoints(), bs.size()
+    assert out[8].split() == [str(2 * s1 * pb.npoints() * n1), str(2 * s1 * ub.size() * n1), "3"]
