@@ -24,6 +24,7 @@ from .splines import (  # noqa: F401
     batched_layout,
     build_eval,
     integrals,
+    knots_as_interpolation_points,
     n_boundary_equations,
 )
 

@@ -190,6 +190,24 @@ class NonUniformBSplines(_BSplinesBase):
         return False
 
 
+def knots_as_interpolation_points(bsplines, lower, upper):
+    """ddc::KnotsAsInterpolationPoints<BSplines, SBCLower, SBCUpper>
+    (knots_as_interpolation_points.hpp:36-92): the break points as
+    interpolation points, ncells + 1 of them (ncells for a periodic basis,
+    get_domain :99-106).  Uniform bases sample them as
+    UniformPointSampling::init(rmin, rmax, ncells + 1) does
+    (uniform_point_sampling.hpp:141-149): rmin + i * ((rmax - rmin) / ncells).
+    Not for Greville closures.  Pass the result as SplineBuilder(points=...)."""
+    if SplineBuilderClosure.GREVILLE in (SplineBuilderClosure(lower), SplineBuilderClosure(upper)):
+        raise ValueError("KnotsAsInterpolationPoints does not apply to Greville closures")
+    n = bsplines.ncells() + (0 if bsplines.is_periodic() else 1)
+    if bsplines.is_uniform():
+        step = (bsplines.rmax() - bsplines.rmin()) / bsplines.ncells()
+        return bsplines.rmin() + np.arange(n) * step
+    d = bsplines.degree()
+    return bsplines.knots()[d: d + n].copy()
+
+
 def integrals(bsplines):
     """ddc::integrals(), integrals.hpp:165-177 (host array of size() values)."""
     out = np.empty(bsplines.size())

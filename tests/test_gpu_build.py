@@ -119,3 +119,37 @@ def test_tiled_kernel_selected():
         bs, builder = case.product()
         assert builder.info.pivoted == 0
         assert builder.info.kernel == 1, case.id
+
+
+@pytest.mark.parametrize("degree,uniform,bc", [(3, True, H), (3, False, H), (5, False, HH), (3, True, P),
+                                               (4, False, P)], ids=["d3-U-H", "d3-NU-H", "d5-NU-HH", "d3-U-P",
+                                                                    "d4-NU-P"])
+def test_build_knots_as_interpolation_points(degree, uniform, bc, cuda):
+    """KnotsAsInterpolationPoints (knots_as_interpolation_points.hpp:36-92):
+    the break points as interpolation points; the oracle builds with the same
+    points."""
+    import torch
+    import ddc_amd as D
+    import oracle as O
+    from cases import breaks_for
+
+    n, nbatch = 40, 23
+    br = None if uniform else breaks_for("random", n, seed=4)
+    periodic = bc == P
+    if uniform:
+        bs = D.UniformBSplines(degree, 0.0, 1.0, n, periodic=periodic)
+    else:
+        bs = D.NonUniformBSplines(degree, br, periodic=periodic)
+    pts = D.knots_as_interpolation_points(bs, bc, bc)
+    builder = D.SplineBuilder(bs, D.SplineBuilderClosure(bc), D.SplineBuilderClosure(bc), points=pts)
+    assert builder.npoints() == pts.size
+    orc = (O.Spline(degree, True, bc, bc, ncells=n, rmin=0.0, rmax=1.0, points=pts) if uniform
+           else O.Spline(degree, False, bc, bc, breaks=br, points=pts))
+    f = values(orc.npts, nbatch, 3, x=orc.points())
+    dl = derivs(orc.nbv_l, nbatch, 5) if bc == H else None
+    dr = derivs(orc.nbv_r, nbatch, 6) if bc == H else None
+    ref = orc.build(f, dl, dr)
+    coef = torch.full((orc.size, nbatch), np.nan, dtype=torch.float64, device=cuda)
+    t = lambda a: None if a is None else torch.from_numpy(a).to(cuda)  # noqa: E731
+    builder(coef, t(f), t(dl), t(dr))
+    assert rel_err(coef.cpu().numpy(), ref) <= TOL

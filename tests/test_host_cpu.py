@@ -200,3 +200,23 @@ def test_cpp_facade(tmp_path):
     assert out[7].startswith("error:") and "order" in out[7]
     n1, s1 = b.npoints(), bs.size()
     assert out[8].split() == [str(2 * s1 * pb.npoints() * n1), str(2 * s1 * ub.size() * n1), "3"]
+
+
+def test_knots_as_interpolation_points():
+    """KnotsAsInterpolationPoints: break points, ncells + 1 of them (ncells
+    when periodic); uniform ones as UniformPointSampling::init samples them;
+    Greville closures refused."""
+    import ddc_amd as D
+
+    C = D.SplineBuilderClosure
+    u = D.UniformBSplines(3, -1.0, 2.0, 10)
+    p = D.knots_as_interpolation_points(u, C.HERMITE, C.HERMITE)
+    assert p.size == 11 and p[0] == -1.0
+    assert np.array_equal(p, -1.0 + np.arange(11) * (3.0 / 10))
+    up = D.UniformBSplines(3, 0.0, 1.0, 10, periodic=True)
+    assert D.knots_as_interpolation_points(up, C.PERIODIC, C.PERIODIC).size == 10
+    br = np.array([0.0, 0.1, 0.35, 0.5, 0.9, 1.0])
+    nu = D.NonUniformBSplines(2, br)
+    assert np.array_equal(D.knots_as_interpolation_points(nu, C.HOMOGENEOUS_HERMITE, C.HERMITE), br)
+    with pytest.raises(ValueError):
+        D.knots_as_interpolation_points(nu, C.GREVILLE, C.HERMITE)
