@@ -30,12 +30,6 @@ namespace tiled_build {
 
 using namespace tiled;
 
-// Interpolation-contiguous layouts (IC) stage the tile through LDS in chunks
-// of kStageRows rows of all LB lines: each line's rows are read and written
-// as contiguous runs, and every thread takes its L rows from LDS.
-constexpr int kStageRows = 256;
-constexpr int kStagePitch = kStageRows + 1; // doubles; odd pitch spreads the LDS banks
-
 template <int LB, int K, int L, int KF, int KB, bool FDIV, bool BDIVF, bool TLDS, bool IC>
 __global__ void __launch_bounds__(LB* K) build_tiled_kernel(TiledView tv, BuildView bv,
                                                            const double* __restrict__ vals, Layout vl,

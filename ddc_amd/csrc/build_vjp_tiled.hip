@@ -58,7 +58,7 @@ __device__ __forceinline__ void getrs_trans_small(int nb, const double* a, const
     }
 }
 
-template <int LB, int K, int L, int KF, int KB, bool FDIV, bool BDIVF, bool TLDS>
+template <int LB, int K, int L, int KF, int KB, bool FDIV, bool BDIVF, bool TLDS, bool IC>
 __global__ void __launch_bounds__(LB* K) build_vjp_tiled_kernel(TiledView tv, BuildView bv,
                                                                const double* __restrict__ gcoef, Layout cl,
                                                                double* __restrict__ gvals, Layout vl,
@@ -91,12 +91,49 @@ __global__ void __launch_bounds__(LB* K) build_vjp_tiled_kernel(TiledView tv, Bu
     uint32_t const vlo = uint32_t(vb - lay(vl, 0, line0)) + uint32_t(tv.q0 + r0) * uint32_t(vl.rs);
     uint32_t const crs = uint32_t(cl.rs), vrs = uint32_t(vl.rs);
 
+    // staging area after the row table (IC): LB x kStagePitch doubles, then
+    // the per-line offsets of gcoef and gvals
+    double* stg = rt_lds + (TLDS ? K * L * RS : 0);
+    uint32_t* loff = reinterpret_cast<uint32_t*>(stg + LB * kStagePitch);
+    int const nl = int(min<int64_t>(LB, nbatch - line0));
+    if constexpr (IC) {
+        if (tid < LB) {
+            int64_t const l = line0 + min(tid, nl - 1);
+            loff[tid] = uint32_t(lay(cl, 0, l) - lay(cl, 0, line0));
+            loff[LB + tid] = uint32_t(lay(vl, 0, l) - lay(vl, 0, line0));
+        }
+        __syncthreads();
+    }
+
     // 1. folded gradient: unknown i collects the coefficient rows j with
     //    (j - offset) mod nbasis == i (a single row except at the periodic ends)
     double y[L];
     {
         uint32_t const c0 = clo + uint32_t(top + r0 + off) * crs;
-        if (r0 + L <= nq) {
+        if constexpr (IC) {
+            // each line's main-block rows as contiguous runs through LDS
+            constexpr int PER = LB * kStageRows / NT;
+            for (int rbase = 0; rbase < K * L; rbase += kStageRows) {
+                double tmp[PER];
+#pragma unroll
+                for (int k = 0; k < PER; ++k) {
+                    int const e = tid + k * NT;
+                    int const l = e / kStageRows, u = rbase + e % kStageRows;
+                    tmp[k] = (l < nl && u < nq) ? cwg[loff[l] + uint32_t(top + u + off)] : 0.0;
+                }
+#pragma unroll
+                for (int k = 0; k < PER; ++k) {
+                    int const e = tid + k * NT;
+                    stg[(e / kStageRows) * kStagePitch + e % kStageRows] = tmp[k];
+                }
+                __syncthreads();
+                if (r0 >= rbase && r0 < rbase + kStageRows) {
+#pragma unroll
+                    for (int i = 0; i < L; ++i) y[i] = stg[ln * kStagePitch + (r0 - rbase) + i];
+                }
+                __syncthreads();
+            }
+        } else if (r0 + L <= nq) {
 #pragma unroll
             for (int i = 0; i < L; ++i) y[i] = cwg[c0 + uint32_t(i) * crs];
         } else {
@@ -209,7 +246,24 @@ __global__ void __launch_bounds__(LB* K) build_vjp_tiled_kernel(TiledView tv, Bu
     tiled_sweeps<LB, K, L, KF, KB, FDIV, BDIVF, KE>(y, T, RS, tv, ex, seg, ln);
 
     // 6. main-block rows to the value gradients
-    if (active) {
+    if constexpr (IC) {
+        constexpr int PER = LB * kStageRows / NT;
+        for (int rbase = 0; rbase < K * L; rbase += kStageRows) {
+            if (r0 >= rbase && r0 < rbase + kStageRows) {
+#pragma unroll
+                for (int i = 0; i < L; ++i) stg[ln * kStagePitch + (r0 - rbase) + i] = y[i];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < PER; ++k) {
+                int const e = tid + k * NT;
+                int const l = e / kStageRows, u = rbase + e % kStageRows;
+                if (l < nl && u < nq)
+                    vwg[loff[LB + l] + uint32_t(tv.q0 + u)] = stg[l * kStagePitch + e % kStageRows];
+            }
+            __syncthreads();
+        }
+    } else if (active) {
         if (r0 + L <= nq) {
 #pragma unroll
             for (int i = 0; i < L; ++i) vwg[vlo + uint32_t(i) * vrs] = y[i];
@@ -230,17 +284,26 @@ void launch_cfg(BuilderPlan const& p, const double* gcoef, Layout cl, double* gv
     constexpr size_t static_lds =
         size_t(KE * K * LB + kMaxGBlocksVjp * kNbMax * LB + kNbMax * LB) * sizeof(double);
     size_t const table = size_t(K) * L * p.tiled_t.rs * sizeof(double);
+    bool const ic = vl.rs == 1 && cl.rs == 1;
+    size_t const stage = ic ? size_t(LB) * kStagePitch * sizeof(double) + 2 * LB * sizeof(uint32_t) : 0;
+    bool const tlds = static_lds + table + stage <= size_t(160 * 1024) && p.tiled_t.rs <= 10;
+    size_t const lds = (tlds ? table : 0) + stage;
     int64_t const blocks = (nbatch + LB - 1) / LB;
-    if (static_lds + table <= size_t(160 * 1024) && p.tiled_t.rs <= 10) {
-        auto kern = build_vjp_tiled_kernel<LB, K, L, KF, KB, FDIV, BDIVF, true>;
-        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      int(table)),
-                  "hipFuncSetAttribute");
-        hipLaunchKernelGGL(kern, dim3(unsigned(blocks)), dim3(NT), table, stream, p.tview_t, p.view, gcoef, cl, gvals,
+    auto go = [&]<bool TL, bool I>() {
+        auto kern = build_vjp_tiled_kernel<LB, K, L, KF, KB, FDIV, BDIVF, TL, I>;
+        if (lds > 0)
+            hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)),
+                      "hipFuncSetAttribute");
+        hipLaunchKernelGGL(kern, dim3(unsigned(blocks)), dim3(NT), lds, stream, p.tview_t, p.view, gcoef, cl, gvals,
                            vl, gdl, gdr, dlay, nbatch);
+    };
+    if (ic) {
+        if (tlds) go.template operator()<true, true>();
+        else go.template operator()<false, true>();
     } else {
-        hipLaunchKernelGGL((build_vjp_tiled_kernel<LB, K, L, KF, KB, FDIV, BDIVF, false>), dim3(unsigned(blocks)),
-                           dim3(NT), 0, stream, p.tview_t, p.view, gcoef, cl, gvals, vl, gdl, gdr, dlay, nbatch);
+        if (tlds) go.template operator()<true, false>();
+        else go.template operator()<false, false>();
     }
     hip_check(hipGetLastError(), "build_vjp_tiled_kernel launch");
 }

@@ -71,6 +71,12 @@ __device__ __forceinline__ void getrs_small(int nb, const double* a, const int* 
     }
 }
 
+// Interpolation-contiguous layouts (IC) stage the tile through LDS in chunks
+// of kStageRows rows of all LB lines: each line's rows are read and written
+// as contiguous runs, and every thread takes its L rows from LDS.
+constexpr int kStageRows = 256;
+constexpr int kStagePitch = kStageRows + 1; // doubles; odd pitch spreads the LDS banks
+
 // The tiled kernels address a tile as a workgroup-uniform base plus 32-bit
 // per-lane offsets; true when every offset of the batch fits.
 inline bool offsets_fit(Layout const& l, int64_t rows, int64_t nbatch)
