@@ -170,8 +170,14 @@ def test_eval_stream(case, npts, cuda):
     bs, _ = case.product()
     x = _points(orc, npts, nbatch, 7, outside=0.1)
     ev = D.SplineEvaluator(bs, D.ConstantExtrapolationRule(), D.NullExtrapolationRule())
-    out = torch.full((npts, nbatch), np.nan, dtype=torch.float64, device=cuda)
+    # the output is the first npts rows of a larger array: point slots past
+    # the end of a line must leave the guard rows after it untouched
+    guard = 600
+    full = torch.full((npts + guard, nbatch), np.nan, dtype=torch.float64, device=cuda)
+    full[npts:] = -7.0
+    out = full[:npts]
     ev(out, torch.from_numpy(x).to(cuda), torch.from_numpy(coef).to(cuda))
+    assert bool((full[npts:] == -7.0).all())
     got = out.cpu().numpy()
     rng = np.random.default_rng(8)
     lines = np.unique(np.concatenate([np.arange(64), np.arange(nbatch - 64, nbatch),
