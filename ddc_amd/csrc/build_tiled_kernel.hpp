@@ -86,7 +86,7 @@ __global__ void __launch_bounds__(LB* K) build_tiled_kernel(TiledView tv, BuildV
             for (int k = 0; k < PER; ++k) {
                 int const e = tid + k * NT;
                 int const l = e / kStageRows, u = rbase + e % kStageRows;
-                tmp[k] = (l < nl && u < nq) ? vwg[loff[l] + uint32_t(tv.q0 + u)] : 0.0;
+                tmp[k] = (l < nl && u < nq) ? __builtin_nontemporal_load(&vwg[loff[l] + uint32_t(tv.q0 + u)]) : 0.0;
             }
 #pragma unroll
             for (int k = 0; k < PER; ++k) {
@@ -101,14 +101,16 @@ __global__ void __launch_bounds__(LB* K) build_tiled_kernel(TiledView tv, BuildV
             __syncthreads();
         }
     } else if (r0 + L <= nq) {
+        // values and coefficients are touched once: non-temporal loads and
+        // stores keep them from displacing what the next kernel reads
 #pragma unroll
-        for (int i = 0; i < L; ++i) y[i] = vwg[vlo + uint32_t(i) * vrs];
+        for (int i = 0; i < L; ++i) y[i] = __builtin_nontemporal_load(&vwg[vlo + uint32_t(i) * vrs]);
     } else {
         // last segments: rows at or past nq read row nq - 1 (in bounds) and are zeroed
         uint32_t const vl0 = vlo - uint32_t(r0) * vrs; // row q0 of this line
 #pragma unroll
         for (int i = 0; i < L; ++i) {
-            double const v = vwg[vl0 + uint32_t(min(r0 + i, nq - 1)) * vrs];
+            double const v = __builtin_nontemporal_load(&vwg[vl0 + uint32_t(min(r0 + i, nq - 1)) * vrs]);
             y[i] = r0 + i < nq ? v : 0.0;
         }
     }
@@ -210,7 +212,8 @@ __global__ void __launch_bounds__(LB* K) build_tiled_kernel(TiledView tv, BuildV
                 int const e = tid + k * NT;
                 int const l = e / kStageRows, u = rbase + e % kStageRows;
                 if (l < nl && u < nq)
-                    cwg[loff[LB + l] + uint32_t(tv.top + u + off)] = stg[l * kStagePitch + e % kStageRows];
+                    __builtin_nontemporal_store(stg[l * kStagePitch + e % kStageRows],
+                                                &cwg[loff[LB + l] + uint32_t(tv.top + u + off)]);
             }
             __syncthreads();
         }
@@ -221,11 +224,11 @@ __global__ void __launch_bounds__(LB* K) build_tiled_kernel(TiledView tv, BuildV
             // rows stored above
         } else if (r0 + L <= nq) {
 #pragma unroll
-            for (int i = 0; i < L; ++i) cwg[c0 + uint32_t(i) * crs] = y[i];
+            for (int i = 0; i < L; ++i) __builtin_nontemporal_store(y[i], &cwg[c0 + uint32_t(i) * crs]);
         } else {
 #pragma unroll
             for (int i = 0; i < L; ++i)
-                if (r0 + i < nq) cwg[c0 + uint32_t(i) * crs] = y[i];
+                if (r0 + i < nq) __builtin_nontemporal_store(y[i], &cwg[c0 + uint32_t(i) * crs]);
         }
         if (bv.periodic && (r0 < bv.degree || r0 + L > nbas - off)) {
 #pragma unroll

@@ -119,7 +119,8 @@ __global__ void __launch_bounds__(LB* K) build_vjp_tiled_kernel(TiledView tv, Bu
                 for (int k = 0; k < PER; ++k) {
                     int const e = tid + k * NT;
                     int const l = e / kStageRows, u = rbase + e % kStageRows;
-                    tmp[k] = (l < nl && u < nq) ? cwg[loff[l] + uint32_t(top + u + off)] : 0.0;
+                    tmp[k] = (l < nl && u < nq) ? __builtin_nontemporal_load(&cwg[loff[l] + uint32_t(top + u + off)])
+                                                : 0.0;
                 }
 #pragma unroll
                 for (int k = 0; k < PER; ++k) {
@@ -135,12 +136,12 @@ __global__ void __launch_bounds__(LB* K) build_vjp_tiled_kernel(TiledView tv, Bu
             }
         } else if (r0 + L <= nq) {
 #pragma unroll
-            for (int i = 0; i < L; ++i) y[i] = cwg[c0 + uint32_t(i) * crs];
+            for (int i = 0; i < L; ++i) y[i] = __builtin_nontemporal_load(&cwg[c0 + uint32_t(i) * crs]);
         } else {
             uint32_t const cl0 = clo + uint32_t(top + off) * crs;
 #pragma unroll
             for (int i = 0; i < L; ++i) {
-                double const v = cwg[cl0 + uint32_t(min(r0 + i, nq - 1)) * crs];
+                double const v = __builtin_nontemporal_load(&cwg[cl0 + uint32_t(min(r0 + i, nq - 1)) * crs]);
                 y[i] = r0 + i < nq ? v : 0.0;
             }
         }
@@ -259,18 +260,19 @@ __global__ void __launch_bounds__(LB* K) build_vjp_tiled_kernel(TiledView tv, Bu
                 int const e = tid + k * NT;
                 int const l = e / kStageRows, u = rbase + e % kStageRows;
                 if (l < nl && u < nq)
-                    vwg[loff[LB + l] + uint32_t(tv.q0 + u)] = stg[l * kStagePitch + e % kStageRows];
+                    __builtin_nontemporal_store(stg[l * kStagePitch + e % kStageRows],
+                                                &vwg[loff[LB + l] + uint32_t(tv.q0 + u)]);
             }
             __syncthreads();
         }
     } else if (active) {
         if (r0 + L <= nq) {
 #pragma unroll
-            for (int i = 0; i < L; ++i) vwg[vlo + uint32_t(i) * vrs] = y[i];
+            for (int i = 0; i < L; ++i) __builtin_nontemporal_store(y[i], &vwg[vlo + uint32_t(i) * vrs]);
         } else {
 #pragma unroll
             for (int i = 0; i < L; ++i)
-                if (r0 + i < nq) vwg[vlo + uint32_t(i) * vrs] = y[i];
+                if (r0 + i < nq) __builtin_nontemporal_store(y[i], &vwg[vlo + uint32_t(i) * vrs]);
         }
     }
 }

@@ -95,7 +95,7 @@ __global__ void __launch_bounds__(NT) eval_tiled_kernel(BSplinesView b, int lowe
     // of the tile is in flight before the first LDS write
     double xa[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) xa[u] = cp[at(prow + u * ROWS)];
+    for (int u = 0; u < U; ++u) xa[u] = __builtin_nontemporal_load(&cp[at(prow + u * ROWS)]);
     // the tile is loaded in chunks of TC elements per thread, every load of a
     // chunk in flight before its LDS writes
     if constexpr (LINE_MAJOR) {
@@ -146,12 +146,12 @@ __global__ void __launch_bounds__(NT) eval_tiled_kernel(BSplinesView b, int lowe
     for (int i = prow; i <= last; i += U * ROWS) {
         double xb[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) xb[u] = cp[at(i + (U + u) * ROWS)];
+        for (int u = 0; u < U; ++u) xb[u] = __builtin_nontemporal_load(&cp[at(i + (U + u) * ROWS)]);
         double y[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) y[u] = point_value<D, ORDER, KIND>(bl, knots, lower, upper, xa[u], tl, pitch);
 #pragma unroll
-        for (int u = 0; u < U; ++u) op[at(i + u * ROWS)] = y[u];
+        for (int u = 0; u < U; ++u) __builtin_nontemporal_store(y[u], &op[at(i + u * ROWS)]);
 #pragma unroll
         for (int u = 0; u < U; ++u) xa[u] = xb[u];
     }
@@ -168,14 +168,21 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_of(const double* p, uin
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(p), 0, int(bytes), kBufferWord3);
 }
 
+// Cache-policy bit NT (non-temporal) of the buffer instructions: every
+// coordinate, coefficient and output of the streaming kernel is touched once,
+// so its lines need not displace what the next kernel reads.
+constexpr int kNonTemporal = 2;
+
+template <int AUX = 0>
 __device__ __forceinline__ double buffer_load(__amdgpu_buffer_rsrc_t r, uint32_t off)
 {
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, int(off), 0, 0));
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, int(off), 0, AUX));
 }
 
+template <int AUX = 0>
 __device__ __forceinline__ void buffer_store(double v, __amdgpu_buffer_rsrc_t r, uint32_t off)
 {
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, int(off), 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, int(off), 0, AUX);
 }
 
 // Persistent form of eval_tiled_kernel for values with plain batch-contiguous
@@ -241,11 +248,11 @@ __global__ void __launch_bounds__(NT) eval_stream_kernel(BSplinesView b, int low
     rot = rot_of(t);
     double xa[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) xa[u] = buffer_load(rc, pt(prow + u * ROWS, lt));
+    for (int u = 0; u < U; ++u) xa[u] = buffer_load<kNonTemporal>(rc, pt(prow + u * ROWS, lt));
     {
         double tv[PR];
 #pragma unroll
-        for (int j = 0; j < PR; ++j) tv[j] = buffer_load(rk, cr(j, lt));
+        for (int j = 0; j < PR; ++j) tv[j] = buffer_load<kNonTemporal>(rk, cr(j, lt));
         int const col = int(int64_t(lt) - t * LB);
 #pragma unroll
         for (int j = 0; j < PR; ++j) {
@@ -269,21 +276,21 @@ __global__ void __launch_bounds__(NT) eval_stream_kernel(BSplinesView b, int low
             double xb[U];
             if (s + 1 < NST) {
 #pragma unroll
-                for (int u = 0; u < U; ++u) xb[u] = buffer_load(rc, pt((s + 1) * U * ROWS + prow + u * ROWS, lt));
+                for (int u = 0; u < U; ++u) xb[u] = buffer_load<kNonTemporal>(rc, pt((s + 1) * U * ROWS + prow + u * ROWS, lt));
             } else {
                 int const r0 = rot;
                 rot = rot_of(more ? tn : t);
 #pragma unroll
-                for (int u = 0; u < U; ++u) xb[u] = buffer_load(rcn, pt(prow + u * ROWS, ltn));
+                for (int u = 0; u < U; ++u) xb[u] = buffer_load<kNonTemporal>(rcn, pt(prow + u * ROWS, ltn));
                 rot = r0;
             }
 #pragma unroll
-            for (int j = s * PS; j < (s + 1) * PS && j < PR; ++j) pf[j] = buffer_load(rkn, cr(j, ltn));
+            for (int j = s * PS; j < (s + 1) * PS && j < PR; ++j) pf[j] = buffer_load<kNonTemporal>(rkn, cr(j, ltn));
             double y[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) y[u] = point_value<D, 0, KIND>(bl, knots, lower, upper, xa[u], tl, pitch);
 #pragma unroll
-            for (int u = 0; u < U; ++u) buffer_store(y[u], ro, pt(s * U * ROWS + prow + u * ROWS, lt));
+            for (int u = 0; u < U; ++u) buffer_store<kNonTemporal>(y[u], ro, pt(s * U * ROWS + prow + u * ROWS, lt));
 #pragma unroll
             for (int u = 0; u < U; ++u) xa[u] = xb[u];
         }

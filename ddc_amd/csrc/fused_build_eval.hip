@@ -55,12 +55,12 @@ __global__ void __launch_bounds__(LB* K) fused_build_eval_kernel(TiledView tv, B
     double y[L];
     if (r0 + L <= nq) {
 #pragma unroll
-        for (int i = 0; i < L; ++i) y[i] = vwg[vlo + uint32_t(i) * vrs];
+        for (int i = 0; i < L; ++i) y[i] = __builtin_nontemporal_load(&vwg[vlo + uint32_t(i) * vrs]);
     } else {
         uint32_t const vl0 = vlo - uint32_t(r0) * vrs; // row q0 of this line
 #pragma unroll
         for (int i = 0; i < L; ++i) {
-            double const v = vwg[vl0 + uint32_t(min(r0 + i, nq - 1)) * vrs];
+            double const v = __builtin_nontemporal_load(&vwg[vl0 + uint32_t(min(r0 + i, nq - 1)) * vrs]);
             y[i] = r0 + i < nq ? v : 0.0;
         }
     }
@@ -130,7 +130,7 @@ __global__ void __launch_bounds__(LB* K) fused_build_eval_kernel(TiledView tv, B
     double* tile = dyn;
     auto put = [&](int row, double v) {
         tile[row * LB + ln] = v;
-        if (store && active) coef[cb + int64_t(row) * cl.rs] = v;
+        if (store && active) __builtin_nontemporal_store(v, &coef[cb + int64_t(row) * cl.rs]);
     };
 #pragma unroll
     for (int i = 0; i < L; ++i) {
@@ -168,14 +168,14 @@ __global__ void __launch_bounds__(LB* K) fused_build_eval_kernel(TiledView tv, B
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         int64_t const ii = prow0 + int64_t(u) * ROWS;
-        xa[u] = ii < npts ? coords[pb + ii * pl.rs] : 0.0;
+        xa[u] = ii < npts ? __builtin_nontemporal_load(&coords[pb + ii * pl.rs]) : 0.0;
     }
     for (int64_t i = prow0; i < npts; i += U * ROWS) {
         double xb[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             int64_t const ii = i + int64_t(U + u) * ROWS;
-            xb[u] = ii < npts ? coords[pb + ii * pl.rs] : 0.0;
+            xb[u] = ii < npts ? __builtin_nontemporal_load(&coords[pb + ii * pl.rs]) : 0.0;
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -190,7 +190,7 @@ __global__ void __launch_bounds__(LB* K) fused_build_eval_kernel(TiledView tv, B
 #pragma unroll
                     for (int k = 0; k < D + 1; ++k) yv += tl[(jmin + k) * LB] * v[k];
                 }
-                out[pb + ii * pl.rs] = yv;
+                __builtin_nontemporal_store(yv, &out[pb + ii * pl.rs]);
             }
         }
 #pragma unroll
