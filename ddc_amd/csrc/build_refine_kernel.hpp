@@ -77,13 +77,13 @@ __global__ void __launch_bounds__(LB* K) build_refine_kernel(RefineView rv, Buil
     double bq[L];
     if (r0 + L <= nq) {
 #pragma unroll
-        for (int i = 0; i < L; ++i) bq[i] = vwg[vlo + uint32_t(i) * vrs];
+        for (int i = 0; i < L; ++i) bq[i] = __builtin_nontemporal_load(&vwg[vlo + uint32_t(i) * vrs]);
     } else {
         // last segments: rows at or past nq read row nq - 1 (in bounds) and are zeroed
         uint32_t const vl0 = vlo - uint32_t(r0) * vrs; // row q0 of this line
 #pragma unroll
         for (int i = 0; i < L; ++i) {
-            double const v = vwg[vl0 + uint32_t(min(r0 + i, nq - 1)) * vrs];
+            double const v = __builtin_nontemporal_load(&vwg[vl0 + uint32_t(min(r0 + i, nq - 1)) * vrs]);
             bq[i] = r0 + i < nq ? v : 0.0;
         }
     }
@@ -377,11 +377,11 @@ __global__ void __launch_bounds__(LB* K) build_refine_kernel(RefineView rv, Buil
     uint32_t const c0 = clo + uint32_t(rv.top + r0 + off) * crs;
     if (r0 + L <= nq) {
 #pragma unroll
-        for (int i = 0; i < L; ++i) cwg[c0 + uint32_t(i) * crs] = x[i];
+        for (int i = 0; i < L; ++i) __builtin_nontemporal_store(x[i], &cwg[c0 + uint32_t(i) * crs]);
     } else {
 #pragma unroll
         for (int i = 0; i < L; ++i)
-            if (r0 + i < nq) cwg[c0 + uint32_t(i) * crs] = x[i];
+            if (r0 + i < nq) __builtin_nontemporal_store(x[i], &cwg[c0 + uint32_t(i) * crs]);
     }
     if (bv.periodic && (r0 < bv.degree || r0 + L > nbas - off)) {
 #pragma unroll
