@@ -401,25 +401,61 @@ class SplineBuilder2D:
 
     def __init__(self, builder1: SplineBuilder, builder2: SplineBuilder):
         self.builder1, self.builder2 = builder1, builder2
-        if builder1.s_nbe_xmin or builder1.s_nbe_xmax or builder2.s_nbe_xmin or builder2.s_nbe_xmax:
-            raise NotImplementedError("2D Hermite closures are not part of this build")
-        self._tmp = None
+        self._tmp = {}
 
-    def __call__(self, spline, vals):
+    def _scratch(self, key, shape, device):
+        t = self._tmp.get(key)
+        if t is None or tuple(t.shape) != tuple(shape) or t.device != device:
+            t = self._tmp[key] = torch.empty(shape, dtype=torch.float64, device=device)
+        return t
+
+    def __call__(self, spline, vals, derivs_min1=None, derivs_max1=None, derivs_min2=None, derivs_max2=None,
+                 mixed_derivs_min1_min2=None, mixed_derivs_max1_min2=None, mixed_derivs_min1_max2=None,
+                 mixed_derivs_max1_max2=None):
+        """SplineBuilder2D::operator() (spline_builder_2d.hpp:514-651).  With
+        Hermite closures: derivs_*1 (batch, nbe1, n2) are the dimension-1
+        derivatives at each dimension-2 point, derivs_*2 (batch, n1, nbe2) the
+        dimension-2 derivatives at each dimension-1 point, mixed_* (batch,
+        nbe1, nbe2) the cross derivatives at the corners.  The dimension-2
+        derivatives are built along dimension 1 first (the corner derivatives
+        are their Hermite data), then dimension 2 is built over the
+        dimension-1 spline with them as its Hermite data."""
         squeeze = vals.dim() == 2
-        v = vals.unsqueeze(0) if squeeze else vals
-        s = spline.unsqueeze(0) if squeeze else spline
+
+        def b(t):
+            return None if t is None else (t.unsqueeze(0) if squeeze else t)
+
+        v, s = b(vals), b(spline)
         nb, n1, n2 = v.shape
-        shape1 = (nb, self.builder1.bsplines.size(), n2)
-        if self._tmp is None or tuple(self._tmp.shape) != shape1 or self._tmp.device != v.device:
-            self._tmp = torch.empty(shape1, dtype=torch.float64, device=v.device)
-        self.builder1(self._tmp, v, axis=1)
-        self.builder2(s, self._tmp, axis=2)
+        b1, b2 = self.builder1, self.builder2
+        s1 = b1.bsplines.size()
+        H = SplineBuilderClosure.HERMITE
+        dev = v.device
+        d1min = d1max = None
+        if b2.lower == H:
+            if derivs_min2 is None:
+                raise ValueError("derivs_min2 is required by the Hermite closure of dimension 2")
+            dm = b(derivs_min2)
+            d1min = self._scratch("dmin", (nb, s1, dm.shape[2]), dev)
+            b1(d1min, dm, b(mixed_derivs_min1_min2), b(mixed_derivs_max1_min2), axis=1)
+        spline1 = self._scratch("s1", (nb, s1, n2), dev)
+        b1(spline1, v, b(derivs_min1), b(derivs_max1), axis=1)
+        if b2.upper == H:
+            if derivs_max2 is None:
+                raise ValueError("derivs_max2 is required by the Hermite closure of dimension 2")
+            dM = b(derivs_max2)
+            d1max = self._scratch("dmax", (nb, s1, dM.shape[2]), dev)
+            b1(d1max, dM, b(mixed_derivs_min1_max2), b(mixed_derivs_max1_max2), axis=1)
+        b2(s, spline1, d1min, d1max, axis=2)
         return spline
 
     def build_vjp(self, gspline, gvals):
-        """Adjoint of __call__: the two transposed 1D solves in reverse order
-        (dim 2, then dim 1).  gspline: (batch, size1, size2); gvals: (batch, n1, n2)."""
+        """Adjoint of __call__ with respect to the values: the two transposed 1D
+        solves in reverse order (dim 2, then dim 1).  gspline: (batch, size1,
+        size2); gvals: (batch, n1, n2).  Hermite closures are not covered."""
+        H = SplineBuilderClosure.HERMITE
+        if any(H in (bb.lower, bb.upper) for bb in (self.builder1, self.builder2)):
+            raise NotImplementedError("the 2D adjoint covers closures without boundary derivative inputs")
         squeeze = gspline.dim() == 2
         g = gspline.unsqueeze(0) if squeeze else gspline
         gv = gvals.unsqueeze(0) if squeeze else gvals

@@ -299,20 +299,46 @@ class Spline:
         return (out[: self.nbv_l], out[self.nbv_l: out.size - self.nbv_r], out[out.size - self.nbv_r:])
 
 
-def spline2d_build(s1: Spline, s2: Spline, vals):
-    """SplineBuilder2D::operator() (spline_builder_2d.hpp:514-651) without Hermite
-    closures: vals (nbatch, n1, n2) -> coef (nbatch, size1, size2)."""
+def spline2d_build(s1: Spline, s2: Spline, vals, derivs_min1=None, derivs_max1=None, derivs_min2=None,
+                   derivs_max2=None, mixed_min1_min2=None, mixed_max1_min2=None, mixed_min1_max2=None,
+                   mixed_max1_max2=None):
+    """SplineBuilder2D::operator() (spline_builder_2d.hpp:514-651):
+    vals (nbatch, n1, n2) -> coef (nbatch, size1, size2).  Hermite data, as
+    in the reference: derivs_*1 (nbatch, nbe1, n2) along dimension 1 at each
+    dimension-2 point, derivs_*2 (nbatch, n1, nbe2), mixed_* (nbatch, nbe1,
+    nbe2) at the corners.  Dimension-2 derivatives are first built along
+    dimension 1 (with the corner derivatives as their dimension-1 Hermite
+    data, :583-640), then dimension 2 is built over the dimension-1 spline
+    with those as its Hermite data (:641-650)."""
     vals = np.ascontiguousarray(vals, dtype=np.float64)
     nb, n1, n2 = vals.shape
     L = lib()
-    spline1 = np.empty((nb, s1.size, n2))
-    # builder1 along dim 1 batched over (batch, dim2): row stride n2, inner = n2 (stride 1), outer stride n1*n2
-    L.or_spline_build(s1.h, vals, n2, n2, 1, n1 * n2, spline1, n2, n2, 1, s1.size * n2, None, None, 1, 1, 0, 0,
-                      nb * n2)
+
+    def c(a):
+        return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+    def build1(src, n_cols, dmin, dmax):
+        # builder1 along dim 1 of (nb, n1, n_cols), batched over (batch, column)
+        out = np.empty((nb, s1.size, n_cols))
+        dmin, dmax = c(dmin), c(dmax)
+        nbe = (dmin if dmin is not None else dmax).shape[1] if (dmin is not None or dmax is not None) else 1
+        L.or_spline_build(s1.h, src, n_cols, n_cols, 1, n1 * n_cols, out, n_cols, n_cols, 1, s1.size * n_cols,
+                          None if dmin is None else dmin.ctypes.data, None if dmax is None else dmax.ctypes.data,
+                          n_cols, n_cols, 1, nbe * n_cols, nb * n_cols)
+        return out
+
+    d1min = d1max = None
+    if derivs_min2 is not None:
+        d1min = build1(c(derivs_min2), c(derivs_min2).shape[2], mixed_min1_min2, mixed_max1_min2)
+    spline1 = build1(vals, n2, derivs_min1, derivs_max1)
+    if derivs_max2 is not None:
+        d1max = build1(c(derivs_max2), c(derivs_max2).shape[2], mixed_min1_max2, mixed_max1_max2)
     coef = np.empty((nb, s1.size, s2.size))
+    nbe2 = (d1min if d1min is not None else d1max).shape[2] if (d1min is not None or d1max is not None) else 1
     # builder2 along dim 2 batched over (batch, bsplines1)
     L.or_spline_build(s2.h, spline1, 1, s1.size, n2, s1.size * n2, coef, 1, s1.size, s2.size,
-                      s1.size * s2.size, None, None, 1, 1, 0, 0, nb * s1.size)
+                      s1.size * s2.size, None if d1min is None else d1min.ctypes.data,
+                      None if d1max is None else d1max.ctypes.data, 1, s1.size, nbe2, s1.size * nbe2, nb * s1.size)
     return coef
 
 

@@ -158,3 +158,58 @@ def test_build2d_vjp_is_jacobian_transpose(cuda):
     gv = torch.full((3, n1, n2), np.nan, dtype=torch.float64, device=cuda)
     D.SplineBuilder2D(b1, b2).build_vjp(torch.from_numpy(g).to(cuda), gv)
     assert rel_err(gv.cpu().numpy(), ref) <= 1e-13
+
+
+HERMITE_2D = [
+    # (degree, uniform, lower, upper, ncells) per dimension
+    ((3, False, "H", "H", 24), (5, True, "H", "H", 20)),
+    ((3, True, "P", "P", 16), (3, False, "H", "G", 22)),
+    ((4, True, "H", "HH", 18), (3, True, "G", "H", 15)),
+    ((5, False, "HH", "H", 21), (2, True, "P", "P", 12)),
+]
+
+
+@pytest.mark.parametrize("dims", HERMITE_2D, ids=["HHxHH", "PxHG", "HHHxGH", "HHHxP"])
+def test_build2d_hermite_matches_oracle(dims, cuda):
+    """SplineBuilder2D with Hermite closures (spline_builder_2d.hpp:514-651):
+    boundary derivatives of each dimension and the corner cross derivatives,
+    against the oracle's restatement."""
+    import torch
+    import ddc_amd as D
+    import oracle as O
+
+    bc = {"H": O.HERMITE, "HH": O.HOMOGENEOUS_HERMITE, "G": G, "P": P}
+    orcs, builders = [], []
+    for i, (d, uni, lo, up, n) in enumerate(dims):
+        br = None if uni else breaks_for("random", n, seed=20 + i)
+        periodic = lo == "P"
+        if uni:
+            o = O.Spline(d, True, bc[lo], bc[up], ncells=n, rmin=0.0, rmax=1.0)
+            bs = D.UniformBSplines(d, 0.0, 1.0, n, periodic=periodic)
+        else:
+            o = O.Spline(d, False, bc[lo], bc[up], breaks=br)
+            bs = D.NonUniformBSplines(d, br, periodic=periodic)
+        orcs.append(o)
+        builders.append(D.SplineBuilder(bs, D.SplineBuilderClosure(bc[lo]), D.SplineBuilderClosure(bc[up])))
+    o1, o2 = orcs
+    nb = 3
+    rng = np.random.default_rng(9)
+    r = lambda *s: rng.standard_normal(s)  # noqa: E731
+    vals = r(nb, o1.npts, o2.npts)
+    H1l, H1u = dims[0][2] == "H", dims[0][3] == "H"
+    H2l, H2u = dims[1][2] == "H", dims[1][3] == "H"
+    dmin1 = r(nb, o1.nbv_l, o2.npts) if H1l else None
+    dmax1 = r(nb, o1.nbv_r, o2.npts) if H1u else None
+    dmin2 = r(nb, o1.npts, o2.nbv_l) if H2l else None
+    dmax2 = r(nb, o1.npts, o2.nbv_r) if H2u else None
+    mm = r(nb, o1.nbv_l, o2.nbv_l) if H1l and H2l else None
+    Mm = r(nb, o1.nbv_r, o2.nbv_l) if H1u and H2l else None
+    mM = r(nb, o1.nbv_l, o2.nbv_r) if H1l and H2u else None
+    MM = r(nb, o1.nbv_r, o2.nbv_r) if H1u and H2u else None
+    ref = O.spline2d_build(o1, o2, vals, dmin1, dmax1, dmin2, dmax2, mm, Mm, mM, MM)
+    t = lambda a: None if a is None else torch.from_numpy(a).to(cuda)  # noqa: E731
+    coef = torch.full((nb, o1.size, o2.size), np.nan, dtype=torch.float64, device=cuda)
+    D.SplineBuilder2D(*builders)(coef, t(vals), t(dmin1), t(dmax1), t(dmin2), t(dmax2), t(mm), t(Mm), t(mM), t(MM))
+    got = coef.cpu().numpy()
+    assert np.all(np.isfinite(got))
+    assert rel_err(got, ref) <= TOL

@@ -125,3 +125,56 @@ def test_2d_separable_and_partials():
         cm[..., k] -= h
         fd = (O.spline2d_eval(o1, o2, cp, coef) - O.spline2d_eval(o1, o2, cm, coef))[0] / (2 * h)
         assert np.abs(fd - ref)[inner].max() <= 1e-6 * max(1.0, np.abs(ref).max())
+
+
+def _hermite_2d_data(s1, s2, P, Q, nb=1):
+    """Interpolation data of P(x) Q(y) for Hermite closures in both
+    dimensions: values, per-dimension boundary derivatives (row k is order
+    k + degree % 2) and the corner cross derivatives."""
+    x, y = s1.points(), s2.points()
+    o1, o2 = s1.degree % 2, s2.degree % 2
+
+    def d(p, k, t):
+        return p.deriv(k)(t) if k else p(t)
+
+    k1 = [k + o1 for k in range(s1.nbv_l)]
+    k2 = [k + o2 for k in range(s2.nbv_l)]
+    vals = np.outer(P(x), Q(y))
+    dmin1 = np.array([[d(P, k, s1.rmin) * Q(yy) for yy in y] for k in k1])
+    dmax1 = np.array([[d(P, k, s1.rmax) * Q(yy) for yy in y] for k in k1])
+    dmin2 = np.array([[P(xx) * d(Q, k, s2.rmin) for k in k2] for xx in x])
+    dmax2 = np.array([[P(xx) * d(Q, k, s2.rmax) for k in k2] for xx in x])
+    mixed = {(a, b): np.array([[d(P, i, a1) * d(Q, j, b1) for j in k2] for i in k1])
+             for a, a1 in (("min", s1.rmin), ("max", s1.rmax)) for b, b1 in (("min", s2.rmin), ("max", s2.rmax))}
+    rep = lambda a: np.repeat(a[None], nb, axis=0)  # noqa: E731
+    return (rep(vals), rep(dmin1), rep(dmax1), rep(dmin2), rep(dmax2),
+            {k: rep(v) for k, v in mixed.items()})
+
+
+def test_2d_hermite_build_reproduces_tensor_polynomials():
+    """SplineBuilder2D with Hermite closures in both dimensions (boundary
+    derivatives, and the corner cross derivatives) reproduces P(x) Q(y) of
+    degree <= the spline degrees exactly: values, partial derivatives."""
+    from cases import breaks_for
+
+    s1 = O.Spline(3, False, O.HERMITE, O.HERMITE, breaks=breaks_for("random", 7, seed=2))
+    s2 = O.Spline(5, True, O.HERMITE, O.HERMITE, ncells=6, rmin=-0.5, rmax=1.5)
+    P = np.polynomial.Polynomial([0.4, -1.1, 0.6, 0.9])
+    Q = np.polynomial.Polynomial([1.0, 0.3, -0.7, 0.2, 0.5, -0.4])
+    vals, dmin1, dmax1, dmin2, dmax2, mixed = _hermite_2d_data(s1, s2, P, Q)
+    coef = O.spline2d_build(s1, s2, vals, dmin1, dmax1, dmin2, dmax2, mixed["min", "min"], mixed["max", "min"],
+                            mixed["min", "max"], mixed["max", "max"])
+    rng = np.random.default_rng(3)
+    xy = np.stack([rng.uniform(s1.rmin, s1.rmax, 300), rng.uniform(s2.rmin, s2.rmax, 300)], axis=-1)[None]
+    ref = P(xy[0, :, 0]) * Q(xy[0, :, 1])
+    got = O.spline2d_eval(s1, s2, xy, coef)[0]
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+    dref = P.deriv()(xy[0, :, 0]) * Q.deriv()(xy[0, :, 1])
+    dgot = O.spline2d_eval(s1, s2, xy, coef, 1, 1)[0]
+    assert np.abs(dgot - dref).max() <= 1e-10 * np.abs(dref).max()
+    # a wrong corner derivative changes the interpolant: the data is used
+    bad = mixed["min", "min"].copy()
+    bad[0, 0, 0] += 1.0
+    coef2 = O.spline2d_build(s1, s2, vals, dmin1, dmax1, dmin2, dmax2, bad, mixed["max", "min"],
+                             mixed["min", "max"], mixed["max", "max"])
+    assert np.abs(coef2 - coef).max() > 1e-6
