@@ -65,6 +65,8 @@ SIGNATURES = {
     "ddcx_evaluator2d_eval_vjp": [_P, _P, _P, _P, _P, _I64, _P, _P, _I64, _I64, _I64, _I64, _P],
     "ddcx_build_eval": [_P, _P, _P, _L, _P, _L, _P, _P, _L, _P, _P, _L, _I64, _I64, _P],
     "ddcx_evaluatornd_eval": [_P, C.c_int, _P, _P, _P, _I64, _P, _I64, _P, _I64, _P],
+    "ddcx_evaluator2d_eval_vjp_workspace": [_P, _P, _I64, _I64, _P],
+    "ddcx_evaluator2d_eval_vjp_binned": [_P, _P, _P, _P, _P, _I64, _P, _P, _I64, _I64, _I64, _I64, _P, _I64, _P],
     "ddcx_evaluatornd_integrate": [_P, C.c_int, _P, _I64, _P, _I64, _P, _I64, _P],
     "ddcx_last_error": [],
     "ddcx_version": [],

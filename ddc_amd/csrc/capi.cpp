@@ -330,6 +330,34 @@ ddcx_status ddcx_evaluator2d_eval_vjp(const ddcx_evaluator* e1, const ddcx_evalu
     });
 }
 
+ddcx_status ddcx_evaluator2d_eval_vjp_workspace(const ddcx_evaluator* e1, const ddcx_evaluator* e2, int64_t npts,
+                                                int64_t nbatch, int64_t* bytes)
+{
+    return guarded([&] {
+        require(e1 && e2 && bytes, "null argument");
+        require(npts >= 0 && nbatch >= 0, "negative size");
+        *bytes = ddcx::eval2d_vjp_workspace(*e1->bs, *e2->bs, npts, nbatch);
+    });
+}
+
+ddcx_status ddcx_evaluator2d_eval_vjp_binned(const ddcx_evaluator* e1, const ddcx_evaluator* e2,
+                                             const double* coords, const double* w, double* gx, int64_t npts,
+                                             const double* coef, double* gcoef, int64_t coef_batch_stride,
+                                             int64_t st1, int64_t st2, int64_t nbatch, void* work,
+                                             int64_t work_bytes, void* stream)
+{
+    return guarded([&] {
+        require(e1 && e2, "null evaluator");
+        require(npts >= 0 && nbatch >= 0, "negative size");
+        if (nbatch == 0) return;
+        require(coef && gcoef && work, "null data pointer");
+        require(npts == 0 || (coords && w && gx), "null data pointer");
+        int const rules[4] = {rule_of(e1->lower), rule_of(e1->upper), rule_of(e2->lower), rule_of(e2->upper)};
+        ddcx::launch_eval2d_vjp_binned(*e1->bs, *e2->bs, rules, coords, w, gx, npts, coef, gcoef, coef_batch_stride,
+                                       st1, st2, nbatch, work, work_bytes, static_cast<hipStream_t>(stream));
+    });
+}
+
 ddcx_status ddcx_evaluatornd_eval(const ddcx_evaluator* const* evs, int nd, const int* orders, double* out,
                                   const double* coords, int64_t npts, const double* coef, int64_t coef_batch_stride,
                                   const int64_t* coef_strides, int64_t nbatch, void* stream)

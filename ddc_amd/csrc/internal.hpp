@@ -348,6 +348,12 @@ void launch_integratend(BSplinesPlan const* const* bs, int nd, double* out, int6
 void launch_eval2d(BSplinesPlan const& b1, BSplinesPlan const& b2, const int* rules, int d1, int d2, double* out,
                    const double* coords, int64_t npts, const double* coef, int64_t coef_batch_stride, int64_t st1,
                    int64_t st2, int64_t nbatch, hipStream_t stream);
+// binned 2D adjoint (eval2d_binned.hip): workspace bytes, 0 if unsupported
+int64_t eval2d_vjp_workspace(BSplinesPlan const& bp1, BSplinesPlan const& bp2, int64_t npts, int64_t nbatch);
+void launch_eval2d_vjp_binned(BSplinesPlan const& bp1, BSplinesPlan const& bp2, const int* rules,
+                              const double* coords, const double* w, double* gx, int64_t npts, const double* coef,
+                              double* gcoef, int64_t cbs, int64_t st1, int64_t st2, int64_t nbatch, void* work,
+                              int64_t work_bytes, hipStream_t stream);
 void launch_eval2d_vjp(BSplinesPlan const& b1, BSplinesPlan const& b2, const int* rules, const double* coords,
                        const double* w, double* gx, int64_t npts, const double* coef, double* gcoef,
                        int64_t coef_batch_stride, int64_t st1, int64_t st2, int64_t nbatch, hipStream_t stream);

@@ -500,8 +500,20 @@ class SplineEvaluator2D:
             raise ValueError("gx must be contiguous")
         nb = coef.shape[0]
         npts = c.numel() // (2 * nb)
-        call("ddcx_evaluator2d_eval_vjp", self.ev1._h, self.ev2._h, _ptr(c), _ptr(ww), _ptr(gx), npts, _ptr(coef),
-             _ptr(gc), coef.stride(0), coef.stride(1), coef.stride(2), nb, _stream())
+        need = C.c_int64(0)
+        call("ddcx_evaluator2d_eval_vjp_workspace", self.ev1._h, self.ev2._h, npts, nb, C.byref(need))
+        if need.value > 0:
+            # points binned by cell tile (eval2d_binned.hip); the workspace is
+            # kept for the next call of the same or a smaller size
+            ws = getattr(self, "_work", None)
+            if ws is None or ws.numel() < need.value or ws.device != c.device:
+                ws = self._work = torch.empty(need.value, dtype=torch.uint8, device=c.device)
+            call("ddcx_evaluator2d_eval_vjp_binned", self.ev1._h, self.ev2._h, _ptr(c), _ptr(ww), _ptr(gx), npts,
+                 _ptr(coef), _ptr(gc), coef.stride(0), coef.stride(1), coef.stride(2), nb, _ptr(ws), ws.numel(),
+                 _stream())
+        else:
+            call("ddcx_evaluator2d_eval_vjp", self.ev1._h, self.ev2._h, _ptr(c), _ptr(ww), _ptr(gx), npts,
+                 _ptr(coef), _ptr(gc), coef.stride(0), coef.stride(1), coef.stride(2), nb, _stream())
         return gx, gcoef
 
     def deriv_dim_1(self, spline_eval, coords_eval, spline_coef):

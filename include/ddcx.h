@@ -208,6 +208,20 @@ ddcx_status ddcx_evaluator2d_eval(const ddcx_evaluator* e1, const ddcx_evaluator
                                   int64_t coef_batch_stride, int64_t st1, int64_t st2, int64_t nbatch,
                                   void* stream);
 
+/* ddcx_evaluator2d_eval_vjp with the points binned by cell tile (no
+ * reference counterpart): the same results, with per-tile accumulation in
+ * LDS instead of one global atomic per point and coefficient.  work: device
+ * memory of at least *bytes as given by ddcx_evaluator2d_eval_vjp_workspace
+ * (0 when the grid or degrees are out of the binned kernels' range: then use
+ * ddcx_evaluator2d_eval_vjp).  Calls that share a workspace must not overlap. */
+ddcx_status ddcx_evaluator2d_eval_vjp_workspace(const ddcx_evaluator* e1, const ddcx_evaluator* e2, int64_t npts,
+                                                int64_t nbatch, int64_t* bytes);
+ddcx_status ddcx_evaluator2d_eval_vjp_binned(const ddcx_evaluator* e1, const ddcx_evaluator* e2,
+                                             const double* coords, const double* w, double* gx, int64_t npts,
+                                             const double* coef, double* gcoef, int64_t coef_batch_stride,
+                                             int64_t st1, int64_t st2, int64_t nbatch, void* work,
+                                             int64_t work_bytes, void* stream);
+
 /* SplineEvaluatorND::operator() / deriv(), spline_evaluator_nd.hpp:300-590
  * (eval :684-720, eval_no_bc :782-826); SplineEvaluator3D
  * (spline_evaluator_3d.hpp:443-1019) is its nd = 3 case.  evs: one evaluator
