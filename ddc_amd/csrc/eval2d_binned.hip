@@ -25,7 +25,7 @@ namespace ddcx {
 namespace {
 
 constexpr int kBinT = 256;       // threads of the count / scatter workgroups
-constexpr int kBinPts = 4;       // points per thread there
+constexpr int kBinPts = 16;      // points per thread there (4096 per workgroup)
 constexpr int kMaxBins = 4096;   // tiles per batch entry
 constexpr int kTileThreads = 256;
 
@@ -248,18 +248,25 @@ __global__ void __launch_bounds__(kTileThreads) bin_tile_kernel(BSplinesView b1,
     }
 }
 
-Bin2D bins_for(BSplinesPlan const& bp1, BSplinesPlan const& bp2)
+// Tiles of 64 x 64 cells (few tiles per batch entry, so the count and scatter
+// workgroups make few global additions), or 32 x 32 when 64 would leave fewer
+// than 2048 tile workgroups in the launch (or more tiles than the LDS
+// histograms hold).  Two (T + 5)^2 patches of doubles fit in LDS for T <= 64.
+Bin2D bins_for(BSplinesPlan const& bp1, BSplinesPlan const& bp2, int64_t nbatch)
 {
-    Bin2D g{};
-    int const n1 = bp1.view.ncells, n2 = bp2.view.ncells;
-    for (g.T = 32; g.T <= 64; g.T *= 2) { // two (T + 5)^2 patches of doubles fit in LDS
-        g.nt1 = (n1 + g.T - 1) / g.T;
-        g.nt2 = (n2 + g.T - 1) / g.T;
+    auto make = [&](int T) {
+        Bin2D g{};
+        g.T = T;
+        g.nt1 = (bp1.view.ncells + T - 1) / T;
+        g.nt2 = (bp2.view.ncells + T - 1) / T;
         g.nbins = g.nt1 * g.nt2;
-        if (g.nbins <= kMaxBins) return g;
-    }
-    g.nbins = 0;
-    return g;
+        return g;
+    };
+    Bin2D const g64 = make(64), g32 = make(32);
+    if (g64.nbins <= kMaxBins && (int64_t(g64.nbins) * nbatch >= 2048 || g32.nbins > kMaxBins)) return g64;
+    if (g32.nbins <= kMaxBins) return g32;
+    Bin2D none{};
+    return none;
 }
 
 struct Work2D {
@@ -284,7 +291,7 @@ Work2D carve(Bin2D const& g, int64_t npts, int64_t nbatch, void* base)
 
 int64_t eval2d_vjp_workspace(BSplinesPlan const& bp1, BSplinesPlan const& bp2, int64_t npts, int64_t nbatch)
 {
-    Bin2D const g = bins_for(bp1, bp2);
+    Bin2D const g = bins_for(bp1, bp2, nbatch);
     if (g.nbins == 0 || npts >= (int64_t(1) << 31) || bp1.view.degree > 5 || bp2.view.degree > 5) return 0;
     return carve(g, npts, nbatch, nullptr).bytes;
 }
@@ -297,7 +304,7 @@ void launch_eval2d_vjp_binned(BSplinesPlan const& bp1, BSplinesPlan const& bp2, 
     bp1.ensure_device();
     bp2.ensure_device();
     BSplinesView const b1 = bp1.device_view(), b2 = bp2.device_view();
-    Bin2D const g = bins_for(bp1, bp2);
+    Bin2D const g = bins_for(bp1, bp2, nbatch);
     int64_t const need = eval2d_vjp_workspace(bp1, bp2, npts, nbatch);
     if (need == 0) throw Error(DDCX_ERR_UNSUPPORTED, "binned 2D adjoint: grid or degree out of range");
     if (work_bytes < need) throw Error(DDCX_ERR_INVALID, "binned 2D adjoint: workspace too small");
