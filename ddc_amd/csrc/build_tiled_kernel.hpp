@@ -115,6 +115,18 @@ __global__ void __launch_bounds__(LB* K) build_tiled_kernel(TiledView tv, BuildV
         }
     }
 
+    // the border rows' right-hand side does not depend on the sweeps: segment
+    // 0 loads it now, with the tile, and parks it in x2s (which it overwrites
+    // with the border solution later), so the border step does not wait on
+    // memory while the other waves wait at its barrier
+    if (tv.nb > 0 && seg == 0) {
+        int64_t const db = lay(dlay, 0, lc);
+#pragma unroll
+        for (int k = 0; k < kNbMax; ++k)
+            if (k < tv.nb)
+                x2s[k][ln] = active ? rhs_src(bv, vals, vl, vb, dl, dr, dlay, db, k < tv.top ? k : tv.nq + k) : 0.0;
+    }
+
     // per-row table: staged in LDS when it fits (one copy per workgroup)
     const double* T;
     if constexpr (TLDS) {
@@ -149,11 +161,8 @@ __global__ void __launch_bounds__(LB* K) build_tiled_kernel(TiledView tv, BuildV
         __syncthreads();
         if (seg == 0) {
             double b2[kNbMax];
-            int64_t const db = lay(dlay, 0, lc);
 #pragma unroll
-            for (int k = 0; k < kNbMax; ++k)
-                b2[k] = (active && k < nb) ? rhs_src(bv, vals, vl, vb, dl, dr, dlay, db, k < tv.top ? k : nq + k)
-                                           : 0.0;
+            for (int k = 0; k < kNbMax; ++k) b2[k] = k < nb ? x2s[k][ln] : 0.0;
             for (int z = 0; z < tv.bl_nnz; ++z) {
                 int const r = tv.bl_r[z];
                 double const v = tv.bl_v[z] * lamx[z][ln];
