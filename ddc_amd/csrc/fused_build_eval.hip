@@ -64,6 +64,15 @@ __global__ void __launch_bounds__(LB* K) fused_build_eval_kernel(TiledView tv, B
             y[i] = r0 + i < nq ? v : 0.0;
         }
     }
+    // the border rows' right-hand side, loaded with the tile and parked in
+    // x2s (segment 0 overwrites it with the border solution later)
+    if (tv.nb > 0 && seg == 0) {
+        int64_t const db = lay(dlay, 0, lc);
+#pragma unroll
+        for (int k = 0; k < kNbMax; ++k)
+            if (k < tv.nb)
+                x2s[k][ln] = active ? rhs_src(bv, vals, vl, vb, dl, dr, dlay, db, k < tv.top ? k : nq + k) : 0.0;
+    }
     {
         int const n2 = (K * L * RS) / 2;
         const double2* src = reinterpret_cast<const double2*>(tv.rt);
@@ -90,11 +99,8 @@ __global__ void __launch_bounds__(LB* K) fused_build_eval_kernel(TiledView tv, B
         __syncthreads();
         if (seg == 0) {
             double b2[kNbMax];
-            int64_t const db = lay(dlay, 0, lc);
 #pragma unroll
-            for (int k = 0; k < kNbMax; ++k)
-                b2[k] = (active && k < nb) ? rhs_src(bv, vals, vl, vb, dl, dr, dlay, db, k < tv.top ? k : nq + k)
-                                           : 0.0;
+            for (int k = 0; k < kNbMax; ++k) b2[k] = k < nb ? x2s[k][ln] : 0.0;
             for (int z = 0; z < tv.bl_nnz; ++z) {
                 int const r = tv.bl_r[z];
                 double const v = tv.bl_v[z] * lamx[z][ln];
