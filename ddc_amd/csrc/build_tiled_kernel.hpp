@@ -42,6 +42,7 @@ __global__ void __launch_bounds__(LB* K) build_tiled_kernel(TiledView tv, BuildV
     __shared__ double ex[KE][K][LB];
     __shared__ double lamx[kLamMax][LB];
     __shared__ double x2s[kNbMax][LB];
+    __shared__ BorderLds bl;
     extern __shared__ double rt_lds[];
 
     int const tid = threadIdx.x;
@@ -65,9 +66,11 @@ __global__ void __launch_bounds__(LB* K) build_tiled_kernel(TiledView tv, BuildV
     uint32_t const clo = uint32_t(cb - lay(cl, 0, line0));
     uint32_t const vrs = uint32_t(vl.rs), crs = uint32_t(cl.rs);
 
-    // staging area after the row table: LB x kStagePitch doubles, then the
-    // per-line offsets of vals and coef
-    double* stg = rt_lds + (TLDS ? K * L * RS : 0);
+    // dynamic LDS: the row table and the padded Q^-1 gamma blocks (TLDS),
+    // then the staging area, LB x kStagePitch doubles, and the per-line
+    // offsets of vals and coef (IC)
+    double* gpad = rt_lds + K * L * RS;
+    double* stg = rt_lds + (TLDS ? K * L * RS + tv.ngb * L * kNbMax : 0);
     uint32_t* loff = reinterpret_cast<uint32_t*>(stg + LB * kStagePitch);
     int const nl = int(min<int64_t>(LB, nbatch - line0));
 
@@ -127,7 +130,11 @@ __global__ void __launch_bounds__(LB* K) build_tiled_kernel(TiledView tv, BuildV
                 x2s[k][ln] = active ? rhs_src(bv, vals, vl, vb, dl, dr, dlay, db, k < tv.top ? k : tv.nq + k) : 0.0;
     }
 
-    // per-row table: staged in LDS when it fits (one copy per workgroup)
+    // the segment's Q^-1 gamma block (-1: none), read before it is needed
+    int const gblk = tv.nb > 0 ? tv.gseg[seg] : -1;
+
+    // per-row table and border data: staged in LDS when they fit (one copy
+    // per workgroup)
     const double* T;
     if constexpr (TLDS) {
         // a few double2 per thread (RS <= 10 doubles per row, K * L <= 2 * NT rows)
@@ -135,9 +142,12 @@ __global__ void __launch_bounds__(LB* K) build_tiled_kernel(TiledView tv, BuildV
         const double2* src = reinterpret_cast<const double2*>(tv.rt);
         double2* dst = reinterpret_cast<double2*>(rt_lds);
         for (int idx = tid; idx < n2; idx += NT) dst[idx] = src[idx];
+        stage_border<NT, true>(tv, bl, gpad, tid);
         __syncthreads();
         T = rt_lds + r0 * RS;
     } else {
+        stage_border<NT, false>(tv, bl, gpad, tid); // couplings only
+        __syncthreads();
         T = tv.rt + size_t(r0) * RS;
     }
 
@@ -149,7 +159,7 @@ __global__ void __launch_bounds__(LB* K) build_tiled_kernel(TiledView tv, BuildV
     int const nbas = bv.nbasis;
     if (nb > 0) {
         for (int z = 0; z < tv.bl_nnz; ++z) {
-            int const c = tv.bl_c[z];
+            int const c = bl.c[z];
             if (c >= r0 && c < r0 + L) {
                 double v = 0.0;
 #pragma unroll
@@ -164,8 +174,8 @@ __global__ void __launch_bounds__(LB* K) build_tiled_kernel(TiledView tv, BuildV
 #pragma unroll
             for (int k = 0; k < kNbMax; ++k) b2[k] = k < nb ? x2s[k][ln] : 0.0;
             for (int z = 0; z < tv.bl_nnz; ++z) {
-                int const r = tv.bl_r[z];
-                double const v = tv.bl_v[z] * lamx[z][ln];
+                int const r = bl.r[z];
+                double const v = bl.v[z] * lamx[z][ln];
 #pragma unroll
                 for (int k = 0; k < kNbMax; ++k)
                     if (k == r) b2[k] -= v;
@@ -189,20 +199,23 @@ __global__ void __launch_bounds__(LB* K) build_tiled_kernel(TiledView tv, BuildV
             }
         }
         __syncthreads();
-        int const gblk = tv.gseg[seg];
         if (gblk >= 0) { // Q^-1 gamma is nonzero only in the segments near the block ends
             double x2[kNbMax];
 #pragma unroll
             for (int k = 0; k < kNbMax; ++k) x2[k] = k < nb ? x2s[k][ln] : 0.0;
-            const double* Gb = tv.Gblk + size_t(gblk) * L * nb;
+            if constexpr (TLDS) {
+                apply_gamma<L>(y, gpad + gblk * L * kNbMax, x2);
+            } else {
+                const double* Gb = tv.Gblk + size_t(gblk) * L * nb;
 #pragma unroll
-            for (int i = 0; i < L; ++i) {
-                double t = y[i];
+                for (int i = 0; i < L; ++i) {
+                    double t = y[i];
 #pragma unroll
-                for (int k = 0; k < kNbMax; ++k)
-                    if (k < nb) t = fma(-Gb[i * nb + k], x2[k], t);
-                y[i] = t;
-                chunk_fence(i);
+                    for (int k = 0; k < kNbMax; ++k)
+                        if (k < nb) t = fma(-Gb[i * nb + k], x2[k], t);
+                    y[i] = t;
+                    chunk_fence(i);
+                }
             }
         }
     }
@@ -276,10 +289,11 @@ void launch_cfg(BuilderPlan const& p, double* coef, Layout cl, const double* val
 {
     constexpr int NT = 1024, LB = NT / K;
     constexpr int KE = KF > KB ? KF : KB;
-    constexpr size_t static_lds = size_t(KE * K * LB + kLamMax * LB + kNbMax * LB) * sizeof(double);
+    constexpr size_t static_lds =
+        size_t(KE * K * LB + kLamMax * LB + kNbMax * LB) * sizeof(double) + sizeof(BorderLds);
     bool const ic = vl.rs == 1 && cl.rs == 1;
     size_t const stage = ic ? size_t(LB) * kStagePitch * sizeof(double) + 2 * LB * sizeof(uint32_t) : 0;
-    size_t const table = size_t(K) * L * p.tiled.rs * sizeof(double);
+    size_t const table = (size_t(K) * L * p.tiled.rs + gamma_lds_doubles(p.tview)) * sizeof(double);
     bool const tlds = static_lds + table + stage <= size_t(160 * 1024) && p.tiled.rs <= 10;
     size_t const lds = (tlds ? table : 0) + stage;
     auto go = [&]<bool TL, bool I>() {

@@ -32,7 +32,10 @@ __global__ void __launch_bounds__(LB* K) fused_build_eval_kernel(TiledView tv, B
     __shared__ double ex[KE][K][LB];
     __shared__ double lamx[kLamMax][LB];
     __shared__ double x2s[kNbMax][LB];
-    extern __shared__ double dyn[]; // row table during the build, coefficient tile (size x LB) after
+    __shared__ BorderLds bl;
+    // row table and padded Q^-1 gamma blocks during the build, coefficient
+    // tile (size x LB) after
+    extern __shared__ double dyn[];
 
     int const tid = threadIdx.x;
     int const ln = tid % LB;
@@ -73,11 +76,14 @@ __global__ void __launch_bounds__(LB* K) fused_build_eval_kernel(TiledView tv, B
             if (k < tv.nb)
                 x2s[k][ln] = active ? rhs_src(bv, vals, vl, vb, dl, dr, dlay, db, k < tv.top ? k : nq + k) : 0.0;
     }
+    int const gblk = tv.nb > 0 ? tv.gseg[seg] : -1; // Q^-1 gamma block (-1: none)
+    double* const gpad = dyn + K * L * RS;
     {
         int const n2 = (K * L * RS) / 2;
         const double2* src = reinterpret_cast<const double2*>(tv.rt);
         double2* dst = reinterpret_cast<double2*>(dyn);
         for (int idx = tid; idx < n2; idx += NT) dst[idx] = src[idx];
+        stage_border<NT, true>(tv, bl, gpad, tid);
         __syncthreads();
     }
     tiled_sweeps<LB, K, L, KF, KB, FDIV, BDIVF, KE>(y, dyn + r0 * RS, RS, tv, ex, seg, ln);
@@ -87,7 +93,7 @@ __global__ void __launch_bounds__(LB* K) fused_build_eval_kernel(TiledView tv, B
     int const nbas = bv.nbasis;
     if (nb > 0) {
         for (int z = 0; z < tv.bl_nnz; ++z) {
-            int const c = tv.bl_c[z];
+            int const c = bl.c[z];
             if (c >= r0 && c < r0 + L) {
                 double v = 0.0;
 #pragma unroll
@@ -102,8 +108,8 @@ __global__ void __launch_bounds__(LB* K) fused_build_eval_kernel(TiledView tv, B
 #pragma unroll
             for (int k = 0; k < kNbMax; ++k) b2[k] = k < nb ? x2s[k][ln] : 0.0;
             for (int z = 0; z < tv.bl_nnz; ++z) {
-                int const r = tv.bl_r[z];
-                double const v = tv.bl_v[z] * lamx[z][ln];
+                int const r = bl.r[z];
+                double const v = bl.v[z] * lamx[z][ln];
 #pragma unroll
                 for (int k = 0; k < kNbMax; ++k)
                     if (k == r) b2[k] -= v;
@@ -114,20 +120,11 @@ __global__ void __launch_bounds__(LB* K) fused_build_eval_kernel(TiledView tv, B
                 if (k < nb) x2s[k][ln] = b2[k];
         }
         __syncthreads();
-        int const gblk = tv.gseg[seg];
         if (gblk >= 0) {
             double x2[kNbMax];
 #pragma unroll
             for (int k = 0; k < kNbMax; ++k) x2[k] = k < nb ? x2s[k][ln] : 0.0;
-            const double* Gb = tv.Gblk + size_t(gblk) * L * nb;
-#pragma unroll
-            for (int i = 0; i < L; ++i) {
-                double t = y[i];
-#pragma unroll
-                for (int k = 0; k < kNbMax; ++k)
-                    if (k < nb) t = fma(-Gb[i * nb + k], x2[k], t);
-                y[i] = t;
-            }
+            apply_gamma<L>(y, gpad + gblk * L * kNbMax, x2);
         }
     }
     __syncthreads(); // the row table is dead from here on
@@ -271,12 +268,13 @@ bool launch_build_eval_fused(BuilderPlan const& p, BSplinesPlan const& bs, int l
     int const K = t.K, L = t.L, LB = 1024 / K;
     if (L != 16 || (K != 64 && K != 128)) return false;
     if (!offsets_fit(vl, int64_t(K) * L + p.tview_f.q0, nbatch)) return false;
-    // dynamic LDS: the row table during the build, then the coefficient tile
-    size_t const table = size_t(K) * L * t.rs * sizeof(double);
+    // dynamic LDS: the row table and Q^-1 gamma blocks during the build, then
+    // the coefficient tile
+    size_t const table = (size_t(K) * L * t.rs + gamma_lds_doubles(p.tview_f)) * sizeof(double);
     size_t const tile = size_t(bs.size()) * LB * sizeof(double);
     size_t const lds = std::max(table, tile);
     int const KE = std::max(t.kf, t.kb);
-    size_t const stat = size_t(KE * K * LB + kLamMax * LB + kNbMax * LB) * sizeof(double);
+    size_t const stat = size_t(KE * K * LB + kLamMax * LB + kNbMax * LB) * sizeof(double) + sizeof(BorderLds);
     if (lds + stat > size_t(160 * 1024)) return false;
     BSplinesView const b = bs.device_view();
     if (K == 64)

@@ -581,6 +581,7 @@ void BuilderPlan::ensure_device() const
             tv.q0 = bs->view.periodic ? 0 : prob.top - nbe_l;
             tv.bl_nnz = int(prob.bl_v.size());
             tv.rs = tiled.rs;
+            tv.ngb = tv.nb > 0 ? int(tiled.Gblk.size() / (size_t(tiled.L) * tv.nb)) : 0;
             tv.rt = t_rt.as<double>();
             tv.Mf = t_Mf.as<double>();
             tv.Mb = t_Mb.as<double>();
@@ -611,6 +612,7 @@ void BuilderPlan::ensure_device() const
             tv.q0 = bs->view.periodic ? 0 : prob.top - nbe_l;
             tv.bl_nnz = int(prob.bl_v.size());
             tv.rs = th.rs;
+            tv.ngb = tv.nb > 0 ? int(th.Gblk.size() / (size_t(th.L) * tv.nb)) : 0;
             tv.rt = rt.as<double>();
             tv.Mf = mf.as<double>();
             tv.Mb = mb.as<double>();
@@ -643,6 +645,7 @@ void BuilderPlan::ensure_device() const
             tv.q0 = bs->view.periodic ? 0 : prob.top - nbe_l;
             tv.bl_nnz = int(prob.bl_v.size());
             tv.rs = tiled_t.rs;
+            tv.ngb = tv.nb > 0 ? int(tiled_t.Gblk.size() / (size_t(tiled_t.L) * tv.nb)) : 0;
             tv.rt = tt_rt.as<double>();
             tv.Mf = tt_Mf.as<double>();
             tv.Mb = tt_Mb.as<double>();

@@ -190,6 +190,7 @@ struct TiledHost {
 
 struct TiledView {
     int K, L, kf, kb, wf, wb, nq, nb, top, q0, bl_nnz, rs;
+    int ngb; // number of L x nb blocks in Gblk
     const double* rt;
     const double* Mf;
     const double* Mb;

@@ -22,6 +22,54 @@ __device__ __forceinline__ void chunk_fence(int i)
     if (i % M == M - 1) __builtin_amdgcn_sched_barrier(0);
 }
 
+// The border data a workgroup reads after its sweeps, copied to LDS in its
+// prologue so that the border step waits on no global load:
+// - the lambda couplings (row, column, value);
+// - the Q^-1 gamma blocks with each row padded to kNbMax columns (zeros past
+//   nb), so that the correction is kNbMax unconditional FMAs per row (an
+//   FMA with a zero entry and a zero x2 leaves the row exactly as it was).
+struct BorderLds {
+    int c[kLamMax];
+    int r[kLamMax];
+    double v[kLamMax];
+};
+
+// NT threads (NT >= kLamMax); GAMMA: also the padded Q^-1 gamma blocks
+template <int NT, bool GAMMA>
+__device__ __forceinline__ void stage_border(TiledView const& tv, BorderLds& bl, double* gpad, int tid)
+{
+    static_assert(NT >= kLamMax, "one coupling per thread");
+    if (tid < tv.bl_nnz) {
+        bl.c[tid] = tv.bl_c[tid];
+        bl.r[tid] = tv.bl_r[tid];
+        bl.v[tid] = tv.bl_v[tid];
+    }
+    if constexpr (GAMMA) {
+        int const n = tv.ngb * tv.L * kNbMax;
+        for (int idx = tid; idx < n; idx += NT) {
+            int const k = idx % kNbMax;
+            gpad[idx] = k < tv.nb ? tv.Gblk[(idx / kNbMax) * tv.nb + k] : 0.0;
+        }
+    }
+}
+
+// y -= G x2 for one segment's L x kNbMax padded block G
+template <int L>
+__device__ __forceinline__ void apply_gamma(double (&y)[L], const double* G, double const (&x2)[kNbMax])
+{
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+        double t = y[i];
+#pragma unroll
+        for (int k = 0; k < kNbMax; ++k) t = fma(-G[i * kNbMax + k], x2[k], t);
+        y[i] = t;
+        chunk_fence(i);
+    }
+}
+
+// doubles of LDS that stage_border's padded Q^-1 gamma blocks take
+inline size_t gamma_lds_doubles(TiledView const& tv) { return size_t(tv.ngb) * tv.L * kNbMax; }
+
 __device__ __forceinline__ int64_t lay(Layout const& L, int64_t r, int64_t l)
 {
     return r * L.rs + (l % L.inner) * L.is + (l / L.inner) * L.os;
