@@ -70,6 +70,7 @@ __global__ void __launch_bounds__(LB* K) build_vjp_tiled_kernel(TiledView tv, Bu
     __shared__ double ex[KE][K][LB];
     __shared__ double part[kMaxGBlocksVjp][kNbMax][LB];
     __shared__ double x2s[kNbMax][LB];
+    __shared__ BorderLds bl;
     extern __shared__ double rt_lds[];
 
     int const tid = threadIdx.x;
@@ -91,9 +92,11 @@ __global__ void __launch_bounds__(LB* K) build_vjp_tiled_kernel(TiledView tv, Bu
     uint32_t const vlo = uint32_t(vb - lay(vl, 0, line0)) + uint32_t(tv.q0 + r0) * uint32_t(vl.rs);
     uint32_t const crs = uint32_t(cl.rs), vrs = uint32_t(vl.rs);
 
-    // staging area after the row table (IC): LB x kStagePitch doubles, then
-    // the per-line offsets of gcoef and gvals
-    double* stg = rt_lds + (TLDS ? K * L * RS : 0);
+    // dynamic LDS: the row table and the padded Q^-1 gamma blocks (TLDS),
+    // then the staging area (IC), LB x kStagePitch doubles, and the per-line
+    // offsets of gcoef and gvals
+    double* gpad = rt_lds + K * L * RS;
+    double* stg = rt_lds + (TLDS ? K * L * RS + tv.ngb * L * kNbMax : 0);
     uint32_t* loff = reinterpret_cast<uint32_t*>(stg + LB * kStagePitch);
     int const nl = int(min<int64_t>(LB, nbatch - line0));
     if constexpr (IC) {
@@ -157,28 +160,61 @@ __global__ void __launch_bounds__(LB* K) build_vjp_tiled_kernel(TiledView tv, Bu
         }
     }
 
+    // the border rows' incoming gradient (folded like the main rows), read
+    // now and parked in x2s, which segment 0 overwrites with x2 later
+    if (nb > 0 && seg == 0) {
+#pragma unroll
+        for (int k = 0; k < kNbMax; ++k) {
+            if (k < nb) {
+                int const j1 = (k < top ? k : nq + k) + off;
+                double v = active ? gcoef[cb + int64_t(j1) * cl.rs] : 0.0;
+                if (active && bv.periodic) {
+                    if (j1 - nbas >= 0) v += gcoef[cb + int64_t(j1 - nbas) * cl.rs];
+                    if (j1 + nbas < nsz) v += gcoef[cb + int64_t(j1 + nbas) * cl.rs];
+                }
+                x2s[k][ln] = v;
+            }
+        }
+    }
+
+    // per-row table and border data in LDS when they fit
+    int const gblk = nb > 0 ? tv.gseg[seg] : -1;
     const double* T;
     if constexpr (TLDS) {
         int const n2 = (K * L * RS) / 2;
         const double2* src = reinterpret_cast<const double2*>(tv.rt);
         double2* dst = reinterpret_cast<double2*>(rt_lds);
         for (int idx = tid; idx < n2; idx += NT) dst[idx] = src[idx];
+        stage_border<NT, true>(tv, bl, gpad, tid);
+        __syncthreads();
         T = rt_lds + r0 * RS;
     } else {
+        stage_border<NT, false>(tv, bl, gpad, tid); // couplings only
         T = tv.rt + size_t(r0) * RS;
     }
 
     // 2. partial sums of (Q^-1 gamma)^T y over this segment's rows
-    int const gblk = tv.gseg[seg];
     if (gblk >= 0) {
-        const double* Gb = tv.Gblk + size_t(gblk) * L * nb;
+        if constexpr (TLDS) {
+            // padded columns (k >= nb) sum to zero and are never read
+            const double* Gb = gpad + gblk * L * kNbMax;
 #pragma unroll
-        for (int k = 0; k < kNbMax; ++k) {
-            if (k < nb) {
+            for (int k = 0; k < kNbMax; ++k) {
                 double acc = 0.0;
 #pragma unroll
-                for (int i = 0; i < L; ++i) acc = fma(Gb[i * nb + k], y[i], acc);
+                for (int i = 0; i < L; ++i) acc = fma(Gb[i * kNbMax + k], y[i], acc);
                 part[gblk][k][ln] = acc;
+            }
+        } else {
+            const double* Gb = tv.Gblk + size_t(gblk) * L * nb;
+#pragma unroll
+            for (int k = 0; k < kNbMax; ++k) {
+                if (k < nb) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int i = 0; i < L; ++i) acc = fma(Gb[i * nb + k], y[i], acc);
+                    part[gblk][k][ln] = acc;
+                }
             }
         }
     }
@@ -187,18 +223,11 @@ __global__ void __launch_bounds__(LB* K) build_vjp_tiled_kernel(TiledView tv, Bu
     // 3. the border: b2 -= sum of the partials (segment order), b2 = S^-T b2
     if (seg == 0) {
         double b2[kNbMax];
-        int nblk = 0;
-        for (int s = 0; s < K; ++s) nblk += tv.gseg[s] >= 0;
+        int const nblk = tv.ngb;
 #pragma unroll
         for (int k = 0; k < kNbMax; ++k) {
             if (k < nb) {
-                int const i = k < top ? k : nq + k;
-                int const j1 = i + off;
-                double v = active ? gcoef[cb + int64_t(j1) * cl.rs] : 0.0;
-                if (active && bv.periodic) {
-                    if (j1 - nbas >= 0) v += gcoef[cb + int64_t(j1 - nbas) * cl.rs];
-                    if (j1 + nbas < nsz) v += gcoef[cb + int64_t(j1 + nbas) * cl.rs];
-                }
+                double v = x2s[k][ln];
                 for (int g = 0; g < nblk; ++g) v -= part[g][k][ln];
                 b2[k] = v;
             } else {
@@ -234,9 +263,9 @@ __global__ void __launch_bounds__(LB* K) build_vjp_tiled_kernel(TiledView tv, Bu
 
     // 4. y -= lambda^T x2
     for (int z = 0; z < tv.bl_nnz; ++z) {
-        int const c = tv.bl_c[z];
+        int const c = bl.c[z];
         if (c >= r0 && c < r0 + L) {
-            double const v = tv.bl_v[z] * x2s[tv.bl_r[z]][ln];
+            double const v = bl.v[z] * x2s[bl.r[z]][ln];
 #pragma unroll
             for (int i = 0; i < L; ++i)
                 if (r0 + i == c) y[i] -= v;
@@ -284,8 +313,8 @@ void launch_cfg(BuilderPlan const& p, const double* gcoef, Layout cl, double* gv
     constexpr int NT = 1024, LB = NT / K;
     constexpr int KE = KF > KB ? KF : KB;
     constexpr size_t static_lds =
-        size_t(KE * K * LB + kMaxGBlocksVjp * kNbMax * LB + kNbMax * LB) * sizeof(double);
-    size_t const table = size_t(K) * L * p.tiled_t.rs * sizeof(double);
+        size_t(KE * K * LB + kMaxGBlocksVjp * kNbMax * LB + kNbMax * LB) * sizeof(double) + sizeof(BorderLds);
+    size_t const table = (size_t(K) * L * p.tiled_t.rs + gamma_lds_doubles(p.tview_t)) * sizeof(double);
     bool const ic = vl.rs == 1 && cl.rs == 1;
     size_t const stage = ic ? size_t(LB) * kStagePitch * sizeof(double) + 2 * LB * sizeof(uint32_t) : 0;
     bool const tlds = static_lds + table + stage <= size_t(160 * 1024) && p.tiled_t.rs <= 10;
