@@ -136,6 +136,102 @@ __global__ void __launch_bounds__(256) evalnd_kernel(NdArgs a, double* __restric
     out[t] = y;
 }
 
+// Same degree D in every dimension: the basis routines inline with the
+// degree fixed, the tables stay in registers and the patch loop has exactly
+// (D+1)^N terms, in the same row-major order and right-folded products as
+// tensor_acc, so the values are bit-identical to evalnd_kernel's.
+template <int N, int D, int... I>
+__device__ __forceinline__ void tensor_acc_fixed(double const (&v)[N][kM], int64_t const (&st)[N], const double* c,
+                                                 double& y)
+{
+    constexpr int K = sizeof...(I);
+    if constexpr (K == N) {
+        constexpr int ix[N] = {I...};
+        double p = v[N - 1][ix[N - 1]];
+        int64_t off = ix[N - 1] * st[N - 1];
+#pragma unroll
+        for (int k = N - 2; k >= 0; --k) {
+            p = v[k][ix[k]] * p;
+            off += ix[k] * st[k];
+        }
+        y += c[off] * p;
+    } else {
+        [&]<int... J>(std::integer_sequence<int, J...>) {
+            (tensor_acc_fixed<N, D, I..., J>(v, st, c, y), ...);
+        }(std::make_integer_sequence<int, D + 1>{});
+    }
+}
+
+template <int D, bool ANY>
+__device__ __forceinline__ int basis_fixed(BSplinesView const& b, double x, int order, double* v)
+{
+    if constexpr (ANY) {
+        if (order != 0) {
+            int j = 0;
+            auto take = [&]<int O>() {
+                if constexpr (O <= D) {
+                    double der[(D + 1) * (O + 1)];
+                    j = eval_basis_and_n_derivs<D, O, true>(b, b.knots, x, der);
+#pragma unroll
+                    for (int i = 0; i < D + 1; ++i) v[i] = der[i * (O + 1) + O];
+                }
+            };
+            switch (order) {
+            case 1: take.template operator()<1>(); break;
+            case 2: take.template operator()<2>(); break;
+            case 3: take.template operator()<3>(); break;
+            default: break;
+            }
+            return j;
+        }
+    }
+    return eval_basis<D, true>(b, b.knots, x, v);
+}
+
+// evalnd_kernel with every dimension of degree D; ANY: some order is nonzero
+// (no extrapolation rules then, as in evalnd_kernel)
+template <int N, int D, bool ANY>
+__global__ void __launch_bounds__(256) evalnd_fixed_kernel(NdArgs a, double* __restrict__ out,
+                                                           const double* __restrict__ coords, int64_t npts,
+                                                           const double* __restrict__ coef, int64_t cbs,
+                                                           int64_t nbatch)
+{
+    int64_t const t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= npts * nbatch) return;
+    int64_t const l = t / npts;
+    double x[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) x[k] = coords[N * t + k];
+    if constexpr (!ANY) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            BSplinesView const& b = a.b[k];
+            if (b.periodic && (x[k] < b.rmin || x[k] > b.rmax)) x[k] -= floor((x[k] - b.rmin) / b.length) * b.length;
+        }
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            BSplinesView const& b = a.b[k];
+            if (!b.periodic && ((x[k] < b.rmin && a.lower[k] == DDCX_EXTRAP_NULL) ||
+                                (x[k] > b.rmax && a.upper[k] == DDCX_EXTRAP_NULL))) {
+                out[t] = 0.0;
+                return;
+            }
+        }
+    }
+    double v[N][kM];
+    int64_t st[N];
+    int64_t base = 0;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        int const j = basis_fixed<D, ANY>(a.b[k], x[k], ANY ? a.order[k] : 0, v[k]);
+        st[k] = a.st[k];
+        base += int64_t(j) * st[k];
+    }
+    double y = 0.0;
+    tensor_acc_fixed<N, D>(v, st, coef + l * cbs + base, y);
+    out[t] = y;
+}
+
 constexpr int kIntThreads = 1024;
 
 // One workgroup per batch entry.  Wave w takes the rows (all indices but the
@@ -250,6 +346,37 @@ void launch_evalnd(BSplinesPlan const* const* bs, int nd, const int* orders, con
     int64_t const total = npts * nbatch;
     int64_t const blocks = (total + 255) / 256;
     if (blocks > int64_t(0x7fffffff)) throw Error(DDCX_ERR_INVALID, "batch too large for one launch");
+    // one degree in every dimension (N = 2, 3, and 4 up to cubic): the fixed-degree kernel
+    int deg = a.b[0].degree;
+    for (int k = 1; k < nd; ++k)
+        if (a.b[k].degree != deg) deg = -1;
+    bool launched = false;
+    auto fixed = [&]<int N>() {
+        auto go = [&]<int D>() {
+            if (deg != D) return;
+            if (any)
+                hipLaunchKernelGGL((evalnd_fixed_kernel<N, D, true>), dim3(unsigned(blocks)), dim3(256), 0, stream, a,
+                                   out, coords, npts, coef, cbs, nbatch);
+            else
+                hipLaunchKernelGGL((evalnd_fixed_kernel<N, D, false>), dim3(unsigned(blocks)), dim3(256), 0, stream,
+                                   a, out, coords, npts, coef, cbs, nbatch);
+            launched = true;
+        };
+        go.template operator()<1>();
+        go.template operator()<2>();
+        go.template operator()<3>();
+        if constexpr (N < 4) {
+            go.template operator()<4>();
+            go.template operator()<5>();
+        }
+    };
+    if (nd == 2) fixed.template operator()<2>();
+    else if (nd == 3) fixed.template operator()<3>();
+    else if (nd == 4) fixed.template operator()<4>();
+    if (launched) {
+        hip_check(hipGetLastError(), "evalnd_fixed_kernel launch");
+        return;
+    }
     dispatch_nd(nd, [&]<int N>() {
         hipLaunchKernelGGL((evalnd_kernel<N>), dim3(unsigned(blocks)), dim3(256), 0, stream, a, out, coords, npts,
                            coef, cbs, nbatch, any);

@@ -124,8 +124,11 @@ def test_eval3d_matches_oracle(spec, orders, cuda):
 
 
 @pytest.mark.parametrize("spec", [[(3, "nug", 20)], [(2, "up", 9), (5, "nug", 14)],
-                                  [(1, "ug", 5), (3, "up", 8), (2, "nug", 6), (4, "up", 7)]],
-                         ids=["1d", "2d", "4d"])
+                                  [(1, "ug", 5), (3, "up", 8), (2, "nug", 6), (4, "up", 7)],
+                                  [(4, "nug", 11), (4, "up", 9)], [(5, "up", 7), (5, "nug", 12)],
+                                  [(2, "ug", 5), (2, "up", 6), (2, "nug", 7), (2, "up", 4)],
+                                  [(3, "up", 4), (3, "nug", 5), (3, "ug", 6), (3, "up", 5)]],
+                         ids=["1d", "2d", "4d", "2d_same_d4", "2d_same_d5", "4d_same_d2", "4d_same_d3"])
 def test_evalnd_other_dimensions(spec, cuda):
     import torch
     import ddc_amd as D
