@@ -1,11 +1,13 @@
-// FP64 tiled build on the 32 x 32 tiling (main blocks of up to 1024 rows) and
-// the launcher that picks the tiling.
+// FP64 tiled build on the 32 x 32 tiling (main blocks of 256 to 1024 rows)
+// and the launcher that picks the tiling.
 #include "build_tiled_kernel.hpp"
 
 namespace ddcx {
 
 bool launch_build_tiled_k64(BuilderPlan const& p, double* coef, Layout cl, const double* vals, Layout vl,
                             const double* dl, const double* dr, Layout dlay, int64_t nbatch, hipStream_t stream);
+bool launch_build_tiled_short(BuilderPlan const& p, double* coef, Layout cl, const double* vals, Layout vl,
+                              const double* dl, const double* dr, Layout dlay, int64_t nbatch, hipStream_t stream);
 
 bool launch_build_tiled(BuilderPlan const& p, double* coef, Layout cl, const double* vals, Layout vl,
                         const double* dl, const double* dr, Layout dlay, int64_t nbatch, hipStream_t stream)
@@ -18,6 +20,8 @@ bool launch_build_tiled(BuilderPlan const& p, double* coef, Layout cl, const dou
         return tiled_build::launch_K<32, 32>(p, coef, cl, vals, vl, dl, dr, dlay, nbatch, stream);
     if (p.tiled.K == 64 && p.tiled.L == 32)
         return launch_build_tiled_k64(p, coef, cl, vals, vl, dl, dr, dlay, nbatch, stream);
+    if (p.tiled.K == 32 && p.tiled.L == 8)
+        return launch_build_tiled_short(p, coef, cl, vals, vl, dl, dr, dlay, nbatch, stream);
     return false;
 }
 

@@ -373,6 +373,8 @@ bool launch_build_vjp_tiled(BuilderPlan const& p, const double* gcoef, Layout cl
         return launch_K<32, 32>(p, gcoef, cl, gvals, vl, gdl, gdr, dlay, nbatch, stream);
     if (p.tiled_t.K == 64 && p.tiled_t.L == 32)
         return launch_K<64, 32>(p, gcoef, cl, gvals, vl, gdl, gdr, dlay, nbatch, stream);
+    if (p.tiled_t.K == 32 && p.tiled_t.L == 8) // short lines
+        return launch_K<32, 8>(p, gcoef, cl, gvals, vl, gdl, gdr, dlay, nbatch, stream);
     return false;
 }
 

@@ -12,7 +12,7 @@ namespace ddcx {
 
 namespace {
 
-constexpr int kMaxWindow = 4;
+constexpr int kMaxWindow = 8; // segments a carry may cross (about 7 for degree 5 on 8-row segments)
 constexpr int kMaxLambda = 64;
 constexpr int kMaxBorderTiled = 8;
 constexpr double kCarryTol = 0x1p-60;
@@ -62,11 +62,11 @@ Mat identity(int k)
 }
 
 // Reason the tiled kernels cannot take this matrix, or nullptr.
-const char* tiled_unsupported(HostProblem const& pr)
+const char* tiled_unsupported(HostProblem const& pr, int min_rows)
 {
     int const nq = pr.q.n;
     if (!pr.blocked) return "plain dense matrix";
-    if (nq < 256) return "main block too small";
+    if (nq < min_rows) return "main block too small";
     if (nq > 2048) return "main block larger than 2048 rows";
     if (pr.nb() > kMaxBorderTiled) return "border larger than 8";
     if (int(pr.bl_v.size()) > kMaxLambda) return "too many border couplings";
@@ -78,10 +78,11 @@ const char* tiled_unsupported(HostProblem const& pr)
 // tables of the solve with Q^T (the PDS blocks are symmetric; for the
 // unpivoted band Q^T = U^T L^T is a forward sweep with U^T, scaled by the
 // reciprocal pivots, and a unit backward sweep with L^T).
-void tile_tables(HostProblem const& pr, int K, int L, double tol, TiledHost& t, bool transpose = false)
+void tile_tables(HostProblem const& pr, int K, int L, double tol, TiledHost& t, bool transpose = false,
+                 int min_rows = 256)
 {
     t = TiledHost{};
-    if (char const* why = tiled_unsupported(pr)) { t.why = why; return; }
+    if (char const* why = tiled_unsupported(pr, min_rows)) { t.why = why; return; }
     HostQ const& q = pr.q;
     int const nq = q.n;
     t.K = K;
@@ -259,9 +260,16 @@ void make_tiled(BuilderPlan& p)
 {
     // K segments of L rows (K*L >= nq) on 1024 threads, LB = 1024/K lines
     // (32 lines give 256-byte rows in batch-contiguous layouts)
-    int const K = p.prob.q.n <= 1024 ? 32 : 64;
-    tile_tables(p.prob, K, 32, kCarryTol, p.tiled);
-    tile_tables(p.prob, K, 32, kCarryTol, p.tiled_t, true);
+    int const nq = p.prob.q.n;
+    if (nq < 256) {
+        // short lines (64 to 255 rows): 32 segments of 8 rows
+        tile_tables(p.prob, 32, 8, kCarryTol, p.tiled, false, 64);
+        tile_tables(p.prob, 32, 8, kCarryTol, p.tiled_t, true, 64);
+    } else {
+        int const K = nq <= 1024 ? 32 : 64;
+        tile_tables(p.prob, K, 32, kCarryTol, p.tiled);
+        tile_tables(p.prob, K, 32, kCarryTol, p.tiled_t, true);
+    }
     // the fused build + eval kernel keeps 16 (nq <= 1024) or 8 lines per
     // workgroup so that their coefficient tile fits in LDS: 16-row segments
     tile_tables(p.prob, p.prob.q.n <= 1024 ? 64 : 128, 16, kCarryTol, p.tiled_f);

@@ -112,6 +112,19 @@ TILED_CASES = [
     Case(3, True, P, ncells=1800),
 ]
 
+# Short lines: main blocks of 64 to 255 rows on the 32 x 8 tiling (FP64 build
+# only; the mixed-precision and fused kernels start at 256 rows).
+SHORT_TILED_CASES = [
+    Case(3, True, P, ncells=128),
+    Case(3, False, G, ncells=128, mesh="random"),
+    Case(3, True, G, ncells=128),
+    Case(5, True, P, ncells=200),
+    Case(2, False, G, ncells=100, mesh="random"),
+    Case(4, True, G, ncells=90),
+    Case(3, False, H, ncells=180, mesh="random"),
+    Case(5, False, HH, ncells=250, mesh="uniform"),
+]
+
 # The precision-form workload: Hermite closures on a geometrically stretched
 # mesh (ratio 1.05 over 2048 cells, smallest cell ~2e-45 of the domain).
 STRETCHED_CASES = [

@@ -103,7 +103,8 @@ def test_build_config1_cpu_case(cuda):
     assert rel_err(got, ref) <= TOL
 
 
-@pytest.mark.parametrize("case", __import__("cases").TILED_CASES, ids=lambda c: c.id)
+@pytest.mark.parametrize("case", __import__("cases").TILED_CASES + __import__("cases").SHORT_TILED_CASES,
+                         ids=lambda c: c.id)
 @pytest.mark.parametrize("layout", ["batch_contiguous", "interp_contiguous"])
 def test_build_tiled_matches_oracle(case, layout, cuda):
     bs, builder = case.product()
@@ -113,9 +114,9 @@ def test_build_tiled_matches_oracle(case, layout, cuda):
 
 def test_tiled_kernel_selected():
     """The hot configurations run the register-resident kernel."""
-    from cases import TILED_CASES
+    from cases import SHORT_TILED_CASES, TILED_CASES
 
-    for case in TILED_CASES:
+    for case in TILED_CASES + SHORT_TILED_CASES:
         bs, builder = case.product()
         assert builder.info.pivoted == 0
         assert builder.info.kernel == 1, case.id

@@ -7,7 +7,7 @@ atomics, in an order that differs from the oracle's sequential sum).
 import numpy as np
 import pytest
 
-from cases import ALL_CASES, TILED_CASES, Case, G, H, P
+from cases import ALL_CASES, SHORT_TILED_CASES, TILED_CASES, Case, G, H, P
 from conftest import rel_err
 
 pytestmark = pytest.mark.gpu
@@ -48,10 +48,10 @@ def test_build_vjp_matches_oracle(case, layout, cuda):
     assert rel_err(got, ref) <= 1e-13
 
 
-@pytest.mark.parametrize("case", TILED_CASES, ids=lambda c: c.id)
+@pytest.mark.parametrize("case", TILED_CASES + SHORT_TILED_CASES, ids=lambda c: c.id)
 @pytest.mark.parametrize("layout", ["batch_contiguous", "interp_contiguous"])
 def test_build_vjp_tiled(case, layout, cuda):
-    """Main blocks of 256..2048 rows: the tiled transposed solve."""
+    """Main blocks of 64..2048 rows: the tiled transposed solve."""
     got, ref = _vjp(case, 40, layout, cuda, seed=2)
     assert np.all(np.isfinite(got))
     assert rel_err(got, ref) <= 1e-13
