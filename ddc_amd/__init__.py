@@ -21,6 +21,7 @@ from .splines import (  # noqa: F401
     SplineEvaluatorND,
     SplineSolver,
     UniformBSplines,
+    advect,
     batched_layout,
     build_eval,
     integrals,

@@ -186,6 +186,35 @@ ddcx_status ddcx_build_eval(const ddcx_builder* h, const ddcx_evaluator* e, doub
     });
 }
 
+ddcx_status ddcx_advect(const ddcx_builder* h, const ddcx_evaluator* e, double* coef, ddcx_layout coef_layout,
+                        const double* vals, ddcx_layout vals_layout, const double* derivs_xmin,
+                        const double* derivs_xmax, ddcx_layout derivs_layout, const double* shift, double* out,
+                        ddcx_layout out_layout, int64_t nbatch, void* stream)
+{
+    return guarded([&] {
+        require(h && e, "null handle");
+        require(nbatch >= 0, "negative size");
+        if (nbatch == 0) return;
+        require(vals && shift && out, "null data pointer");
+        ddcx::BuilderPlan const& p = *h->p;
+        if (p.bcl == DDCX_HERMITE && p.nbv_l > 0) require(derivs_xmin != nullptr, "HERMITE lower closure needs derivs_xmin");
+        if (p.bcu == DDCX_HERMITE && p.nbv_r > 0) require(derivs_xmax != nullptr, "HERMITE upper closure needs derivs_xmax");
+        require(p.bs.get() == e->bs.get(), "builder and evaluator must share one B-spline basis");
+        auto const cl = ddcx::to_layout(coef_layout), vl = ddcx::to_layout(vals_layout),
+                   dl = ddcx::to_layout(derivs_layout), ol = ddcx::to_layout(out_layout);
+        p.ensure_device();
+        e->bs->ensure_device();
+        bool const done =
+            p.precision == DDCX_FP64 &&
+            ddcx::launch_build_eval_fused(p, *e->bs, rule_of(e->lower), rule_of(e->upper), coef, cl, vals, vl,
+                                          derivs_xmin, derivs_xmax, dl, out, nullptr, ol, int64_t(p.points.size()),
+                                          nbatch, static_cast<hipStream_t>(stream), shift);
+        if (!done)
+            throw ddcx::Error(DDCX_ERR_UNSUPPORTED,
+                              "advection step: the fused kernel does not cover this case (precision, layout or basis)");
+    });
+}
+
 ddcx_status ddcx_builder_build_vjp(const ddcx_builder* h, const double* gcoef, ddcx_layout coef_layout, double* gvals,
                                    ddcx_layout vals_layout, double* gderivs_xmin, double* gderivs_xmax,
                                    ddcx_layout derivs_layout, int64_t nbatch, void* stream)

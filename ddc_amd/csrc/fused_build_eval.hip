@@ -23,7 +23,9 @@ __global__ void __launch_bounds__(LB* K) fused_build_eval_kernel(TiledView tv, B
                                                                 double* __restrict__ coef, Layout cl,
                                                                 double* __restrict__ out,
                                                                 const double* __restrict__ coords, Layout pl,
-                                                                int64_t npts, int64_t nbatch)
+                                                                int64_t npts, int64_t nbatch,
+                                                                const double* __restrict__ fpts,
+                                                                const double* __restrict__ fshift)
 {
     constexpr int KE = KF > KB ? KF : KB;
     constexpr int NT = LB * K;
@@ -167,18 +169,24 @@ __global__ void __launch_bounds__(LB* K) fused_build_eval_kernel(TiledView tv, B
     int64_t const pb = lay(pl, 0, lc);
     const double* tl = tile + ln;
     int const prow0 = seg; // point row of this thread within a pass
+    // the point: read from coords, or the foot fpts[i] - fshift[line] of an
+    // advection step (npts interpolation points)
+    double const sh = fshift ? fshift[lc] : 0.0;
+    auto point = [&](int64_t ii) {
+        return fshift ? fpts[ii] - sh : __builtin_nontemporal_load(&coords[pb + ii * pl.rs]);
+    };
     double xa[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         int64_t const ii = prow0 + int64_t(u) * ROWS;
-        xa[u] = ii < npts ? __builtin_nontemporal_load(&coords[pb + ii * pl.rs]) : 0.0;
+        xa[u] = ii < npts ? point(ii) : 0.0;
     }
     for (int64_t i = prow0; i < npts; i += U * ROWS) {
         double xb[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             int64_t const ii = i + int64_t(U + u) * ROWS;
-            xb[u] = ii < npts ? __builtin_nontemporal_load(&coords[pb + ii * pl.rs]) : 0.0;
+            xb[u] = ii < npts ? point(ii) : 0.0;
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -204,7 +212,8 @@ __global__ void __launch_bounds__(LB* K) fused_build_eval_kernel(TiledView tv, B
 template <int K, int KF, int KB, bool FDIV, bool BDIVF, int D>
 void launch_one(BuilderPlan const& p, BSplinesView const& b, int lower, int upper, double* coef, Layout cl,
                 const double* vals, Layout vl, const double* dl, const double* dr, Layout dlay, double* out,
-                const double* coords, Layout pl, int64_t npts, int64_t nbatch, hipStream_t stream, size_t lds)
+                const double* coords, Layout pl, int64_t npts, int64_t nbatch, hipStream_t stream, size_t lds,
+                const double* fpts, const double* fshift)
 {
     constexpr int NT = 1024, LB = NT / K, L = 16;
     auto kern = fused_build_eval_kernel<LB, K, L, KF, KB, FDIV, BDIVF, D>;
@@ -213,14 +222,15 @@ void launch_one(BuilderPlan const& p, BSplinesView const& b, int lower, int uppe
               "hipFuncSetAttribute");
     int64_t const blocks = (nbatch + LB - 1) / LB;
     hipLaunchKernelGGL(kern, dim3(unsigned(blocks)), dim3(NT), lds, stream, p.tview_f, p.view, b, lower, upper, vals,
-                       vl, dl, dr, dlay, coef, cl, out, coords, pl, npts, nbatch);
+                       vl, dl, dr, dlay, coef, cl, out, coords, pl, npts, nbatch, fpts, fshift);
     hip_check(hipGetLastError(), "fused_build_eval_kernel launch");
 }
 
 template <int K>
 bool launch_K(BuilderPlan const& p, BSplinesView const& b, int lower, int upper, double* coef, Layout cl,
               const double* vals, Layout vl, const double* dl, const double* dr, Layout dlay, double* out,
-              const double* coords, Layout pl, int64_t npts, int64_t nbatch, hipStream_t stream, size_t lds)
+              const double* coords, Layout pl, int64_t npts, int64_t nbatch, hipStream_t stream, size_t lds,
+              const double* fpts, const double* fshift)
 {
     TiledHost const& t = p.tiled_f;
     int const deg = b.degree;
@@ -228,7 +238,7 @@ bool launch_K(BuilderPlan const& p, BSplinesView const& b, int lower, int upper,
     auto go = [&]<int KF, int KB, bool FD, bool BD, int D>() {
         if (!done && deg == D) {
             launch_one<K, KF, KB, FD, BD, D>(p, b, lower, upper, coef, cl, vals, vl, dl, dr, dlay, out, coords, pl,
-                                            npts, nbatch, stream, lds);
+                                            npts, nbatch, stream, lds, fpts, fshift);
             done = true;
         }
     };
@@ -260,10 +270,12 @@ bool launch_K(BuilderPlan const& p, BSplinesView const& b, int lower, int upper,
 bool launch_build_eval_fused(BuilderPlan const& p, BSplinesPlan const& bs, int lower, int upper, double* coef,
                              Layout cl, const double* vals, Layout vl, const double* dl, const double* dr,
                              Layout dlay, double* out, const double* coords, Layout pl, int64_t npts, int64_t nbatch,
-                             hipStream_t stream)
+                             hipStream_t stream, const double* feet_shift)
 {
     TiledHost const& t = p.tiled_f;
     if (!t.ok || p.bs.get() != &bs) return false;
+    if (feet_shift && npts != int64_t(p.points.size())) return false;
+    const double* const fpts = feet_shift ? p.d_points.as<double>() : nullptr;
     if (vl.rs == 1 || pl.rs == 1 || (coef && cl.rs == 1)) return false; // batch-contiguous layouts only
     int const K = t.K, L = t.L, LB = 1024 / K;
     if (L != 16 || (K != 64 && K != 128)) return false;
@@ -279,9 +291,9 @@ bool launch_build_eval_fused(BuilderPlan const& p, BSplinesPlan const& bs, int l
     BSplinesView const b = bs.device_view();
     if (K == 64)
         return launch_K<64>(p, b, lower, upper, coef, cl, vals, vl, dl, dr, dlay, out, coords, pl, npts, nbatch,
-                            stream, lds);
+                            stream, lds, fpts, feet_shift);
     return launch_K<128>(p, b, lower, upper, coef, cl, vals, vl, dl, dr, dlay, out, coords, pl, npts, nbatch, stream,
-                         lds);
+                         lds, fpts, feet_shift);
 }
 
 } // namespace ddcx

@@ -551,6 +551,7 @@ void BuilderPlan::ensure_device() const
         d_blr.upload(prob.bl_r);
         d_blc.upload(prob.bl_c);
         d_blv.upload(prob.bl_v);
+        d_points.upload(points);
         SolveView& sv = view.sv;
         sv.q = d_q.as<double>();
         sv.ipiv = d_ipiv.as<int>();

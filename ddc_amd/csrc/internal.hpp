@@ -264,6 +264,7 @@ struct BuilderPlan {
     std::vector<double> points;
     HostProblem prob;
     mutable DeviceBuffer d_q, d_ipiv, d_slu, d_sipiv, d_trr, d_trc, d_trv, d_blr, d_blc, d_blv;
+    mutable DeviceBuffer d_points; // the interpolation points, for the in-kernel feet of ddcx_advect
     mutable BuildView view{}; // device pointers filled by ensure_device()
     TiledHost tiled;
     mutable DeviceBuffer t_Mf, t_Mb, t_rt, t_gseg, t_Gblk;
@@ -319,10 +320,13 @@ bool launch_build_vjp_tiled(BuilderPlan const& p, const double* gcoef, Layout cl
                             double* gdl, double* gdr, Layout dlay, int64_t nbatch, hipStream_t stream);
 // Fused build + value-form evaluation; coef may be null (coefficients kept in
 // LDS only).  Returns false when the fused kernel does not cover the case.
+// With feet_shift (one value per line), coords is not read: point i of line l
+// is the interpolation point x_i minus feet_shift[l], and npts is the
+// builder's number of interpolation points.
 bool launch_build_eval_fused(BuilderPlan const& p, BSplinesPlan const& bs, int lower, int upper, double* coef,
                              Layout cl, const double* vals, Layout vl, const double* dl, const double* dr,
                              Layout dlay, double* out, const double* coords, Layout pl, int64_t npts, int64_t nbatch,
-                             hipStream_t stream);
+                             hipStream_t stream, const double* feet_shift = nullptr);
 void launch_build(BuilderPlan const& p, double* coef, Layout cl, const double* vals, Layout vl, const double* dl,
                   const double* dr, Layout dlay, int64_t nbatch, hipStream_t stream);
 void launch_build_vjp(BuilderPlan const& p, const double* gcoef, Layout cl, double* gvals, Layout vl, double* gdl,

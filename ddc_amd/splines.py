@@ -394,6 +394,33 @@ def build_eval(builder, evaluator, spline_eval, coords_eval, vals, spline=None, 
     return spline_eval
 
 
+def advect(builder, evaluator, out, vals, shift, spline=None, derivs_xmin=None, derivs_xmax=None, axis=0):
+    """One semi-Lagrangian advection step (examples/characteristics_advection.cpp:
+    237-274) in one fused kernel (ddcx_advect): out[i, l] is the spline of
+    vals along `axis` evaluated at the foot x_i - shift[l] of interpolation
+    point x_i, for each batch line l (shift has one value per line, in the
+    batch order of vals).  The feet are computed in the kernel and the
+    coefficients stay on chip unless `spline` is given.  Raises where the
+    fused kernel does not apply (FP64 builder, degree 2..5, batch-contiguous
+    arrays, main blocks of 256 to 2048 rows)."""
+    _check_f64_cuda(vals, "vals")
+    _check_f64_cuda(shift, "shift")
+    vl, nbatch = batched_layout(vals, axis)
+    ol, _ = batched_layout(out, axis)
+    if out.shape[axis] != builder.npoints():
+        raise ValueError("out must have npoints rows along axis")
+    if shift.numel() != nbatch or not shift.is_contiguous():
+        raise ValueError("shift must be a contiguous tensor with one value per batch line")
+    cl = batched_layout(spline, axis)[0] if spline is not None else ddcx_layout(1, 1, 0, 0)
+    dl = ddcx_layout(1, 1, 0, 0)
+    for d in (derivs_xmin, derivs_xmax):
+        if d is not None:
+            dl, _ = batched_layout(d, axis)
+    call("ddcx_advect", builder._h, evaluator._h, _ptr(spline), cl, _ptr(vals), vl, _ptr(derivs_xmin),
+         _ptr(derivs_xmax), dl, _ptr(shift), _ptr(out), ol, nbatch, _stream())
+    return out
+
+
 class SplineBuilder2D:
     """ddc::SplineBuilder2D (spline_builder_2d.hpp:27-651): builds along dim 1
     batched over (batch, dim 2) into a temporary, then along dim 2 batched over

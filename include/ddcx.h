@@ -155,6 +155,20 @@ ddcx_status ddcx_build_eval(const ddcx_builder* h, const ddcx_evaluator* e, doub
                             const double* derivs_xmax, ddcx_layout derivs_layout, double* out, const double* coords,
                             ddcx_layout points_layout, int64_t npts, int64_t nbatch, void* stream);
 
+/* One step of the semi-Lagrangian advection of the reference's example
+ * (examples/characteristics_advection.cpp:237-274, the "feet" of
+ * benchmarks/splines.cpp:188-206): the spline of vals is built along the
+ * interpolation dimension and evaluated at the feet x_i - shift[l] of every
+ * interpolation point x_i of line l, all in one fused kernel.  The feet are
+ * computed in the kernel and the coefficients stay on chip; coef may be NULL,
+ * and when it is given the coefficients are stored too.  out has npoints rows
+ * per line, described by out_layout.  Covers what ddcx_build_eval's fused
+ * kernel covers; other cases return DDCX_ERR_UNSUPPORTED. */
+ddcx_status ddcx_advect(const ddcx_builder* h, const ddcx_evaluator* e, double* coef, ddcx_layout coef_layout,
+                        const double* vals, ddcx_layout vals_layout, const double* derivs_xmin,
+                        const double* derivs_xmax, ddcx_layout derivs_layout, const double* shift, double* out,
+                        ddcx_layout out_layout, int64_t nbatch, void* stream);
+
 /* Vector-Jacobian product of ddcx_builder_build (no reference counterpart;
  * built on the transposed solve SplinesLinearProblem::solve(b, true),
  * splines_linear_problem.hpp:77): given gcoef, writes gvals and the Hermite
