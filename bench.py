@@ -203,8 +203,10 @@ def measure_precision_form(D, degree, B, timed, n, gen):
 def measure_advection(D, timed, n):
     """One periodic advection step on 512 x 512 (uniform periodic cubic along
     x, batched over y): build, eval at the feet x - sin(2 pi y) dt, and the
-    adjoint (eval_vjp then build_vjp).  The data (~6 MB) stays in cache, so
-    the figures are microseconds, not a fraction of HBM bandwidth."""
+    adjoint (eval_vjp then build_vjp); for FP64 also the step as one fused
+    kernel with the feet computed in it (ddcx_advect).  The data (~6 MB)
+    stays in cache, so the figures are microseconds, not a fraction of HBM
+    bandwidth."""
     import numpy as np
     import torch
 
@@ -215,7 +217,8 @@ def measure_advection(D, timed, n):
     x = torch.from_numpy(D.SplineBuilder(bs).interpolation_points()).to(dev)
     y = torch.arange(N, device=dev, dtype=torch.float64) / N
     vals = torch.sin(2 * np.pi * x)[:, None] * torch.cos(2 * np.pi * y)[None, :]
-    feet = x[:, None] - torch.sin(2 * np.pi * y)[None, :] * 0.01
+    shift = (torch.sin(2 * np.pi * y) * 0.01).contiguous()
+    feet = x[:, None] - shift[None, :]
     w = torch.ones_like(feet)
     coef = torch.empty((bs.size(), N), dtype=torch.float64, device=dev)
     out, gx, gc, gv = torch.empty_like(feet), torch.empty_like(feet), torch.empty_like(coef), torch.empty_like(vals)
@@ -231,6 +234,9 @@ def measure_advection(D, timed, n):
         t_a = timed(lambda: (ev.eval_vjp(feet, w, coef, gx, gc), b.build_vjp(gc, gv)), n)
         res[name] = {"build_us": round(t_b * 1e6, 1), "eval_us": round(t_e * 1e6, 1),
                      "adjoint_us": round(t_a * 1e6, 1), "step_us": round((t_b + t_e) * 1e6, 1)}
+        if prec == D.Precision.FP64:
+            D.advect(b, ev, out, vals, shift)
+            res[name]["fused_step_us"] = round(timed(lambda: D.advect(b, ev, out, vals, shift), n) * 1e6, 1)
     return res
 
 
