@@ -185,6 +185,7 @@ public:
         check(ddcx_builder_quadrature(h_.get(), ql.data(), qv.data(), qr.data()));
         return {ql, qv, qr};
     }
+    ddcx_builder* handle() const { return h_.get(); }
 
 private:
     struct Del {
@@ -237,6 +238,39 @@ private:
     };
     std::unique_ptr<ddcx_evaluator, Del> h_;
 };
+
+// builder(spline, vals, derivs...) then evaluator(out, coords, spline) on the
+// same lines in one pass (ddcx_build_eval).  spline.data may be null: the
+// coefficients then stay on chip (fused kernel only).
+inline void build_eval(SplineBuilder const& b, SplineEvaluator const& e, Span<double> spline,
+                       Span<double const> vals, Span<double> out, Span<double const> coords, int64_t npts,
+                       std::optional<Span<double const>> derivs_xmin = std::nullopt,
+                       std::optional<Span<double const>> derivs_xmax = std::nullopt, void* stream = nullptr)
+{
+    ddcx_layout dl{1, 1, 0, 0};
+    if (derivs_xmin) dl = derivs_xmin->layout;
+    if (derivs_xmax) dl = derivs_xmax->layout;
+    check(ddcx_build_eval(b.handle(), e.handle(), spline.data, spline.layout, vals.data, vals.layout,
+                          derivs_xmin ? derivs_xmin->data : nullptr, derivs_xmax ? derivs_xmax->data : nullptr, dl,
+                          out.data, coords.data, coords.layout, npts, vals.nbatch, stream));
+}
+
+// One semi-Lagrangian advection step (examples/characteristics_advection.cpp:
+// 237-274) in one fused kernel (ddcx_advect): out = the spline of vals at the
+// feet x_i - shift[l]; shift holds one value per batch line (device memory).
+// spline.data may be null (coefficients kept on chip).
+inline void advect(SplineBuilder const& b, SplineEvaluator const& e, Span<double> out, Span<double const> vals,
+                   const double* shift, Span<double> spline = {nullptr, {1, 1, 0, 0}, 0},
+                   std::optional<Span<double const>> derivs_xmin = std::nullopt,
+                   std::optional<Span<double const>> derivs_xmax = std::nullopt, void* stream = nullptr)
+{
+    ddcx_layout dl{1, 1, 0, 0};
+    if (derivs_xmin) dl = derivs_xmin->layout;
+    if (derivs_xmax) dl = derivs_xmax->layout;
+    check(ddcx_advect(b.handle(), e.handle(), spline.data, spline.layout, vals.data, vals.layout,
+                      derivs_xmin ? derivs_xmin->data : nullptr, derivs_xmax ? derivs_xmax->data : nullptr, dl, shift,
+                      out.data, out.layout, vals.nbatch, stream));
+}
 
 // SplineEvaluatorND<Exec, Mem, TypeSeq<BSplines...>, TypeSeq<EvalDDim...>,
 // TypeSeq<Rules...>> (spline_evaluator_nd.hpp:49): one 1D evaluator per
