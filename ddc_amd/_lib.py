@@ -38,6 +38,16 @@ class ddcx_builder_info(C.Structure):
                 ("pivoted", C.c_int), ("refine_steps", C.c_int), ("refine_contraction", C.c_double)]
 
 
+class ddcx_problem_desc(C.Structure):
+    _fields_ = [("kind", C.c_int), ("n", C.c_int), ("kl", C.c_int), ("ku", C.c_int), ("pds", C.c_int),
+                ("bottom_size", C.c_int), ("top_size", C.c_int)]
+
+
+class ddcx_problem_info(C.Structure):
+    _fields_ = [("size", C.c_int), ("rhs_rows", C.c_int), ("main_kind", C.c_int), ("blocked", C.c_int),
+                ("main_size", C.c_int), ("pivoted", C.c_int)]
+
+
 # name -> (argtypes) ; all return ddcx_status (int) unless listed in _RESTYPES
 _P = C.c_void_p
 _PP = C.POINTER(C.c_void_p)
@@ -69,6 +79,13 @@ SIGNATURES = {
     "ddcx_evaluator2d_eval_vjp_workspace": [_P, _P, _I64, _I64, _P],
     "ddcx_evaluator2d_eval_vjp_binned": [_P, _P, _P, _P, _P, _I64, _P, _P, _I64, _I64, _I64, _I64, _P, _I64, _P],
     "ddcx_evaluatornd_integrate": [_P, C.c_int, _P, _I64, _P, _I64, _P, _I64, _P],
+    "ddcx_problem_create": [C.POINTER(ddcx_problem_desc), _PP],
+    "ddcx_problem_destroy": [_P],
+    "ddcx_problem_set_element": [_P, C.c_int, C.c_int, C.c_double],
+    "ddcx_problem_get_element": [_P, C.c_int, C.c_int, C.POINTER(C.c_double)],
+    "ddcx_problem_setup_solver": [_P],
+    "ddcx_problem_get_info": [_P, C.POINTER(ddcx_problem_info)],
+    "ddcx_problem_solve": [_P, _P, _I64, _I64, C.c_int, _P],
     "ddcx_last_error": [],
     "ddcx_version": [],
 }

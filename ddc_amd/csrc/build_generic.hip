@@ -50,7 +50,8 @@ struct LineIO {
 
 // ---- forward + backward sweeps over the main block Q (rows top .. top+nq-1) ----
 
-__device__ void sweep_tridiag(SolveView const& sv, LineIO const& io)
+template <class IO>
+__device__ void sweep_tridiag(SolveView const& sv, IO const& io)
 {
     int const n = sv.nq, t = sv.top;
     const double* d = sv.q;
@@ -73,8 +74,8 @@ __device__ void sweep_tridiag(SolveView const& sv, LineIO const& io)
     }
 }
 
-template <int K>
-__device__ void sweep_pdsband(SolveView const& sv, LineIO const& io)
+template <int K, class IO>
+__device__ void sweep_pdsband(SolveView const& sv, IO const& io)
 {
     int const n = sv.nq, t = sv.top;
     const double* q = sv.q;
@@ -107,8 +108,8 @@ __device__ void sweep_pdsband(SolveView const& sv, LineIO const& io)
     }
 }
 
-template <int KL, int KU>
-__device__ void sweep_band(SolveView const& sv, LineIO const& io)
+template <int KL, int KU, class IO>
+__device__ void sweep_band(SolveView const& sv, IO const& io)
 {
     constexpr int KD = KL + KU;
     int const n = sv.nq, t = sv.top;
@@ -172,7 +173,8 @@ __device__ void getrs_local(int n, const double* a, const int* ipiv, double* b)
     }
 }
 
-__device__ void sweep_dense(SolveView const& sv, LineIO const& io)
+template <class IO>
+__device__ void sweep_dense(SolveView const& sv, IO const& io)
 {
     int const n = sv.nq, t = sv.top;
     double b[kMaxDenseQ];
@@ -183,11 +185,10 @@ __device__ void sweep_dense(SolveView const& sv, LineIO const& io)
 
 constexpr int kMaxBorder = 16;
 
-// 2x2-block corrections (2x2_blocks.cpp:290-291) on the border rows, then the
-// periodic duplication (spline_builder.hpp:1074-1100).
-__device__ void border_and_dup(BuildView const& bv, LineIO const& io)
+// 2x2-block corrections (2x2_blocks.cpp:290-291) on the border rows.
+template <class IO>
+__device__ void solve_border(SolveView const& sv, IO const& io)
 {
-    SolveView const& sv = bv.sv;
     int const nq = sv.nq, nb = sv.nb, t = sv.top;
     if (nb > 0) {
         double b2[kMaxBorder];
@@ -203,6 +204,12 @@ __device__ void border_and_dup(BuildView const& bv, LineIO const& io)
         }
         for (int k = 0; k < nb; ++k) io.c(k < t ? k : nq + k) = b2[k];
     }
+}
+
+// The border solve, then the periodic duplication (spline_builder.hpp:1074-1100).
+__device__ void border_and_dup(BuildView const& bv, LineIO const& io)
+{
+    solve_border(bv.sv, io);
     if (bv.periodic) {
         int const off = bv.offset, nbas = bv.nbasis;
         double* base = io.coef;
@@ -229,18 +236,22 @@ __global__ void __launch_bounds__(256) build_generic_kernel(BuildView bv, const 
 }
 
 struct TridiagSweep {
-    static __device__ void run(SolveView const& sv, LineIO const& io) { sweep_tridiag(sv, io); }
+    template <class IO>
+    static __device__ void run(SolveView const& sv, IO const& io) { sweep_tridiag(sv, io); }
 };
 template <int K>
 struct PDSBandSweep {
-    static __device__ void run(SolveView const& sv, LineIO const& io) { sweep_pdsband<K>(sv, io); }
+    template <class IO>
+    static __device__ void run(SolveView const& sv, IO const& io) { sweep_pdsband<K>(sv, io); }
 };
 template <int KL, int KU>
 struct BandSweep {
-    static __device__ void run(SolveView const& sv, LineIO const& io) { sweep_band<KL, KU>(sv, io); }
+    template <class IO>
+    static __device__ void run(SolveView const& sv, IO const& io) { sweep_band<KL, KU>(sv, io); }
 };
 struct DenseSweep {
-    static __device__ void run(SolveView const& sv, LineIO const& io) { sweep_dense(sv, io); }
+    template <class IO>
+    static __device__ void run(SolveView const& sv, IO const& io) { sweep_dense(sv, io); }
 };
 
 template <class Sweep>
@@ -362,8 +373,9 @@ __device__ void getrs_trans_local(int n, const double* a, const int* ipiv, doubl
 }
 
 // Q^T solve in place on the main-block rows (PDS types are symmetric: the
-// reference ignores `transpose` for them).
-__device__ void q_solve_trans(SolveView const& sv, VjpIO const& io)
+// reference ignores `transpose` for them).  io.w(r) is main-block row r.
+template <class IO>
+__device__ void q_solve_trans(SolveView const& sv, IO const& io)
 {
     int const n = sv.nq;
     switch (QType(sv.qtype)) {
@@ -480,6 +492,136 @@ void launch_build_vjp(BuilderPlan const& p, const double* gcoef, Layout cl, doub
     hipLaunchKernelGGL(build_vjp_generic_kernel, dim3(unsigned(blocks)), dim3(threads), 0, stream, p.view, gcoef, cl,
                        gvals, vl, gdl, gdr, dlay, nbatch);
     hip_check(hipGetLastError(), "build_vjp_generic_kernel launch");
+}
+
+} // namespace ddcx
+
+// ---------------------------------------------------------------------------
+// SplinesLinearProblem::solve(b, transpose) on its own
+// (splines_linear_problem.hpp:77): one thread per right-hand side, in place,
+// through the same sweeps as the builds.  Row i of right-hand side j is
+// b[i * ldb + j], the reference's LayoutRight MultiRHS view (rows, nrhs).
+// ---------------------------------------------------------------------------
+
+namespace ddcx {
+
+namespace {
+
+struct ColumnIO {
+    double* b;
+    int64_t ldb;
+    int top;
+    __device__ double src(int i) const { return b[int64_t(i) * ldb]; }
+    __device__ double& c(int i) const { return b[int64_t(i) * ldb]; }
+    __device__ double& w(int r) const { return b[int64_t(top + r) * ldb]; } // main-block row r
+};
+
+// dgbtrs('N') with run-time bandwidths, in place: the band shapes the spline
+// builder never produces (kl != ku, or kl > 6).
+struct BandSweepRT {
+    static __device__ void run(SolveView const& sv, ColumnIO const& io)
+    {
+        int const n = sv.nq, t = sv.top, kl = sv.kl, kd = sv.kl + sv.ku;
+        const double* q = sv.q;
+        auto AB = [&](int r, int cidx) { return q[size_t(r) * n + cidx]; };
+        for (int j = 0; j < n - 1; ++j) {
+            int const lm = min(kl, n - 1 - j);
+            int const p = sv.ipiv[j];
+            if (p != j) {
+                double const tmp = io.c(t + p);
+                io.c(t + p) = io.c(t + j);
+                io.c(t + j) = tmp;
+            }
+            double const temp = -1.0 * io.c(t + j);
+            for (int i = 0; i < lm; ++i) io.c(t + j + 1 + i) = io.c(t + j + 1 + i) + AB(kd + 1 + i, j) * temp;
+        }
+        for (int j = n - 1; j >= 0; --j) { // dtbsv('U','N','N')
+            double const xj = io.c(t + j) / AB(kd, j);
+            io.c(t + j) = xj;
+            for (int i = 1; i <= kd && j - i >= 0; ++i) io.c(t + j - i) = io.c(t + j - i) - xj * AB(kd - i, j);
+        }
+    }
+};
+
+template <class Sweep>
+__global__ void __launch_bounds__(256) problem_solve_kernel(SolveView sv, double* __restrict__ b, int64_t ldb,
+                                                            int64_t nrhs)
+{
+    int64_t const j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= nrhs) return;
+    ColumnIO const io{b + j, ldb, sv.top};
+    Sweep::run(sv, io);
+    solve_border(sv, io);
+}
+
+// the transposed order of 2x2_blocks.cpp:292-297 (and HostProblem::solve)
+__global__ void __launch_bounds__(256) problem_solve_trans_kernel(SolveView sv, double* __restrict__ b, int64_t ldb,
+                                                                  int64_t nrhs)
+{
+    int64_t const j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= nrhs) return;
+    ColumnIO const io{b + j, ldb, sv.top};
+    int const nq = sv.nq, nb = sv.nb, t = sv.top;
+    double b2[kMaxBorder];
+    for (int k = 0; k < nb; ++k) b2[k] = io.src(k < t ? k : nq + k);
+    if (nb > 0) {
+        for (int z = 0; z < sv.tr_nnz; ++z) b2[sv.tr_c[z]] -= sv.tr_v[z] * io.w(sv.tr_r[z]);
+        getrs_trans_local(nb, sv.s_lu, sv.s_ipiv, b2);
+        for (int z = 0; z < sv.bl_nnz; ++z) io.w(sv.bl_c[z]) -= sv.bl_v[z] * b2[sv.bl_r[z]];
+    }
+    q_solve_trans(sv, io);
+    for (int k = 0; k < nb; ++k) io.c(k < t ? k : nq + k) = b2[k];
+}
+
+} // namespace
+
+void launch_problem_solve(SolveView const& sv, double* b, int64_t ldb, int64_t nrhs, bool transpose,
+                          hipStream_t stream)
+{
+    if (sv.nb > kMaxBorder) throw Error(DDCX_ERR_UNSUPPORTED, "border block larger than 16 rows");
+    auto const qt = QType(sv.qtype);
+    if (qt == QType::Dense && sv.nq > kMaxDenseQ) throw Error(DDCX_ERR_UNSUPPORTED, "dense block larger than 32 rows");
+    if (nrhs == 0) return;
+    int const threads = 256;
+    dim3 const grid(unsigned((nrhs + threads - 1) / threads));
+    if (transpose) {
+        hipLaunchKernelGGL(problem_solve_trans_kernel, grid, dim3(threads), 0, stream, sv, b, ldb, nrhs);
+        hip_check(hipGetLastError(), "problem_solve_trans_kernel launch");
+        return;
+    }
+    auto go = [&]<class S>() {
+        hipLaunchKernelGGL(problem_solve_kernel<S>, grid, dim3(threads), 0, stream, sv, b, ldb, nrhs);
+        hip_check(hipGetLastError(), "problem_solve_kernel launch");
+    };
+    switch (qt) {
+    case QType::PDSTridiag: go.template operator()<TridiagSweep>(); return;
+    case QType::Dense: go.template operator()<DenseSweep>(); return;
+    case QType::PDSBand:
+        switch (sv.kl) {
+        case 1: go.template operator()<PDSBandSweep<1>>(); return;
+        case 2: go.template operator()<PDSBandSweep<2>>(); return;
+        case 3: go.template operator()<PDSBandSweep<3>>(); return;
+        case 4: go.template operator()<PDSBandSweep<4>>(); return;
+        case 5: go.template operator()<PDSBandSweep<5>>(); return;
+        case 6: go.template operator()<PDSBandSweep<6>>(); return;
+        default: throw Error(DDCX_ERR_UNSUPPORTED, "positive-definite band solves support 1 to 6 off-diagonals");
+        }
+    case QType::Band:
+        if (sv.kl == sv.ku) {
+            switch (sv.kl) {
+            case 1: go.template operator()<BandSweep<1, 1>>(); return;
+            case 2: go.template operator()<BandSweep<2, 2>>(); return;
+            case 3: go.template operator()<BandSweep<3, 3>>(); return;
+            case 4: go.template operator()<BandSweep<4, 4>>(); return;
+            case 5: go.template operator()<BandSweep<5, 5>>(); return;
+            case 6: go.template operator()<BandSweep<6, 6>>(); return;
+            default: break;
+            }
+        }
+        go.template operator()<BandSweepRT>();
+        return;
+    }
+    throw Error(DDCX_ERR_UNSUPPORTED, "no solve kernel for this matrix shape");
 }
 
 } // namespace ddcx

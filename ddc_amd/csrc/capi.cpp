@@ -17,6 +17,9 @@ struct ddcx_evaluator {
     std::shared_ptr<ddcx::BSplinesPlan> bs;
     ddcx_extrap lower, upper;
 };
+struct ddcx_problem {
+    ddcx::ProblemPlan p;
+};
 
 namespace {
 
@@ -437,6 +440,140 @@ ddcx_status ddcx_evaluatornd_integrate(const ddcx_evaluator* const* evs, int nd,
         require(out && coef, "null data pointer");
         ddcx::launch_integratend(bs, nd, out, out_stride, coef, coef_batch_stride, coef_strides, nbatch,
                                  static_cast<hipStream_t>(stream));
+    });
+}
+
+ddcx_status ddcx_problem_create(const ddcx_problem_desc* desc, ddcx_problem** out)
+{
+    return guarded([&] {
+        require(desc && out, "null argument");
+        *out = nullptr;
+        require(desc->n >= 1, "matrix size must be positive");
+        auto h = std::make_unique<ddcx_problem>();
+        ddcx::HostProblem& pr = h->p.prob;
+        bool const pds = desc->pds != 0;
+        switch (desc->kind) {
+        case DDCX_PROBLEM_DENSE:
+            pr.n = desc->n;
+            pr.q.init(ddcx::QType::Dense, desc->n, 0, 0);
+            break;
+        case DDCX_PROBLEM_BAND:
+            require(desc->kl >= 0 && desc->ku >= 0 && desc->kl < desc->n && desc->ku < desc->n, "bad bandwidths");
+            pr.n = desc->n;
+            ddcx::make_band_q(pr.q, desc->n, desc->kl, desc->ku, pds);
+            break;
+        case DDCX_PROBLEM_BLOCK_BAND:
+            require(desc->kl >= 0 && desc->ku >= 0, "bad bandwidths");
+            ddcx::make_block_problem(pr, desc->n, desc->kl, desc->ku, pds, desc->bottom_size, desc->top_size);
+            break;
+        case DDCX_PROBLEM_PERIODIC_BAND:
+            require(desc->kl >= 0 && desc->ku >= 0, "bad bandwidths");
+            ddcx::make_periodic_band_problem(pr, desc->n, desc->kl, desc->ku, pds);
+            break;
+        case DDCX_PROBLEM_BAND_STORAGE:
+            require(desc->kl >= 0 && desc->ku >= 0 && desc->kl < desc->n && desc->ku < desc->n, "bad bandwidths");
+            pr.n = desc->n;
+            pr.q.init(ddcx::QType::Band, desc->n, desc->kl, desc->ku);
+            break;
+        case DDCX_PROBLEM_BLOCK_DENSE:
+            ddcx::make_dense_block_problem(pr, desc->n, desc->bottom_size, desc->top_size);
+            break;
+        default: throw ddcx::Error(DDCX_ERR_INVALID, "unknown problem kind");
+        }
+        *out = h.release();
+    });
+}
+
+ddcx_status ddcx_problem_destroy(ddcx_problem* h)
+{
+    return guarded([&] { delete h; });
+}
+
+ddcx_status ddcx_problem_set_element(ddcx_problem* h, int i, int j, double aij)
+{
+    return guarded([&] {
+        require(h, "null problem");
+        require(!h->p.factored, "set_element after setup_solver");
+        require(i >= 0 && j >= 0 && i < h->p.prob.n && j < h->p.prob.n, "element index out of range");
+        h->p.prob.set(i, j, aij);
+    });
+}
+
+ddcx_status ddcx_problem_get_element(const ddcx_problem* h, int i, int j, double* aij)
+{
+    return guarded([&] {
+        require(h && aij, "null argument");
+        ddcx::HostProblem const& pr = h->p.prob;
+        require(i >= 0 && j >= 0 && i < pr.n && j < pr.n, "element index out of range");
+        if (!h->p.factored) {
+            *aij = pr.get(i, j);
+            return;
+        }
+        // the assembled matrix kept by setup(): main block, gamma, delta and
+        // lambda (lambda is not modified by setup) in the 3x3 index space
+        ddcx::HostProblem orig;
+        orig.n = pr.n;
+        orig.top = pr.top;
+        orig.blocked = pr.blocked;
+        orig.q = pr.q_orig;
+        if (pr.blocked) {
+            orig.schur = pr.schur;
+            orig.schur.q = pr.delta_orig;
+            orig.gamma = pr.gamma_orig;
+            orig.lambda = pr.lambda;
+        }
+        *aij = orig.get(i, j);
+    });
+}
+
+ddcx_status ddcx_problem_setup_solver(ddcx_problem* h)
+{
+    return guarded([&] {
+        require(h, "null problem");
+        require(!h->p.factored, "setup_solver called twice");
+        h->p.prob.setup();
+        h->p.factored = true;
+        if (ddcx::device_present()) {
+            h->p.buf.upload(h->p.prob, h->p.sv);
+            h->p.on_device = true;
+        }
+    });
+}
+
+ddcx_status ddcx_problem_get_info(const ddcx_problem* h, ddcx_problem_info* info)
+{
+    return guarded([&] {
+        require(h && info, "null argument");
+        ddcx::HostProblem const& pr = h->p.prob;
+        info->size = pr.n;
+        info->rhs_rows = pr.rhs_rows();
+        info->main_kind = int(pr.q.type);
+        info->blocked = pr.blocked ? (pr.top > 0 ? 2 : 1) : 0;
+        info->main_size = pr.q.n;
+        info->pivoted = 0;
+        if (h->p.factored) {
+            auto piv = [](std::vector<int> const& ip) {
+                for (size_t j = 0; j < ip.size(); ++j)
+                    if (ip[j] != int(j)) return 1;
+                return 0;
+            };
+            if (pr.q.type == ddcx::QType::Band || pr.q.type == ddcx::QType::Dense) info->pivoted |= piv(pr.q.ipiv);
+            if (pr.blocked) info->pivoted |= piv(pr.schur.ipiv);
+        }
+    });
+}
+
+ddcx_status ddcx_problem_solve(const ddcx_problem* h, double* b, int64_t ldb, int64_t nrhs, int transpose,
+                               void* stream)
+{
+    return guarded([&] {
+        require(h, "null problem");
+        require(h->p.factored, "solve before setup_solver");
+        require(nrhs >= 0 && ldb >= nrhs, "ldb must be at least nrhs");
+        if (nrhs == 0) return;
+        require(b, "null data pointer");
+        if (!h->p.on_device) throw ddcx::Error(DDCX_ERR_HIP, "no HIP device was present at setup_solver");
+        ddcx::launch_problem_solve(h->p.sv, b, ldb, nrhs, transpose != 0, static_cast<hipStream_t>(stream));
     });
 }
 

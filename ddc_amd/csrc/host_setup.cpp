@@ -421,6 +421,63 @@ void HostProblem::solve(double* b, long ld, bool transpose) const
     for (int k = 0; k < nb_; ++k) b[long(phys(k)) * ld] = b2[k];
 }
 
+// SplinesLinearProblemMaker, splines_linear_problem_maker.hpp:56-146
+void make_band_q(HostQ& q, int n, int kl, int ku, bool pds) // make_new_band
+{
+    if (kl == ku && kl == 1 && pds) q.init(QType::PDSTridiag, n, 1, 1);
+    else if (kl == ku && pds) q.init(QType::PDSBand, n, kl, kl);
+    else if (2 * kl + ku + 1 >= n) q.init(QType::Dense, n, 0, 0);
+    else q.init(QType::Band, n, kl, ku);
+}
+
+static void init_border(HostProblem& pr)
+{
+    int const nq = pr.q.n, nbd = pr.n - nq;
+    pr.schur.init(QType::Dense, nbd, 0, 0);
+    pr.gamma.assign(size_t(nq) * nbd, 0.0);
+    pr.lambda.assign(size_t(nbd) * nq, 0.0);
+}
+
+// make_new_block_matrix_with_band_main_block
+void make_block_problem(HostProblem& pr, int n, int kl, int ku, bool pds, int bottom, int top)
+{
+    if (bottom < 0 || top < 0 || n - top - bottom < 1)
+        throw Error(DDCX_ERR_INVALID, "block sizes leave no main block");
+    pr.n = n;
+    pr.blocked = true;
+    pr.top = top;
+    make_band_q(pr.q, n - top - bottom, kl, ku, pds);
+    init_border(pr);
+}
+
+// make_new_periodic_band_matrix
+void make_periodic_band_problem(HostProblem& pr, int n, int kl, int ku, bool pds)
+{
+    if (kl >= n || ku >= n) throw Error(DDCX_ERR_INVALID, "band wider than the matrix");
+    int const bottom = std::max(kl, ku), top = n - bottom;
+    if (long(bottom) * (n + top) + long(2 * kl + ku + 1) * top >= long(n) * n) {
+        pr.n = n;
+        pr.blocked = false;
+        pr.top = 0;
+        pr.q.init(QType::Dense, n, 0, 0);
+    } else {
+        make_block_problem(pr, n, kl, ku, pds, bottom, 0);
+    }
+}
+
+// SplinesLinearProblem2x2Blocks(n, Dense(n - bottom)) when top == 0, else
+// SplinesLinearProblem3x3Blocks(n, top, Dense(n - top - bottom))
+void make_dense_block_problem(HostProblem& pr, int n, int bottom, int top)
+{
+    if (bottom < 0 || top < 0 || n - top - bottom < 1)
+        throw Error(DDCX_ERR_INVALID, "block sizes leave no main block");
+    pr.n = n;
+    pr.blocked = true;
+    pr.top = top;
+    pr.q.init(QType::Dense, n - top - bottom, 0, 0);
+    init_border(pr);
+}
+
 // ---------------------------------------------------------------------------
 // B-splines
 // ---------------------------------------------------------------------------
@@ -525,44 +582,76 @@ std::shared_ptr<BSplinesPlan> make_bsplines(ddcx_bsplines_desc const& d)
         double const inv_deg = 1.0 / (D + 1);
         for (int i = 0; i < nb; ++i) p->integrals[i] = (p->knots_host[i + D + 1] - p->knots_host[i]) * inv_deg;
     }
+    if (device_present()) p->upload();
     return p;
+}
+
+bool device_present()
+{
+    static bool const present = [] {
+        int n = 0;
+        return hipGetDeviceCount(&n) == hipSuccess && n > 0;
+    }();
+    return present;
+}
+
+void BSplinesPlan::upload()
+{
+    if (!view.uniform) knots_dev.upload(knots_host);
+    if (!cells_host.empty()) cells_dev.upload(cells_host);
+    integrals_dev.upload(integrals);
+    on_device = true;
 }
 
 void BSplinesPlan::ensure_device() const
 {
-    std::call_once(uploaded, [&] {
-        if (!view.uniform) knots_dev.upload(knots_host);
-        if (!cells_host.empty()) cells_dev.upload(cells_host);
-        integrals_dev.upload(integrals);
-    });
+    if (!on_device) throw Error(DDCX_ERR_HIP, "no HIP device was present when the B-splines were created");
+}
+
+void SolveBuffers::upload(HostProblem const& pr, SolveView& sv)
+{
+    sv.qtype = int(pr.q.type);
+    sv.nq = pr.q.n;
+    sv.kl = pr.q.kl;
+    sv.ku = pr.q.ku;
+    sv.nb = pr.nb();
+    sv.top = pr.top;
+    sv.tr_nnz = int(pr.tr_v.size());
+    sv.bl_nnz = int(pr.bl_v.size());
+    q.upload(pr.q.q);
+    ipiv.upload(pr.q.ipiv);
+    slu.upload(pr.schur.q);
+    sipiv.upload(pr.schur.ipiv);
+    trr.upload(pr.tr_r);
+    trc.upload(pr.tr_c);
+    trv.upload(pr.tr_v);
+    blr.upload(pr.bl_r);
+    blc.upload(pr.bl_c);
+    blv.upload(pr.bl_v);
+    sv.q = q.as<double>();
+    sv.ipiv = ipiv.as<int>();
+    sv.s_lu = slu.as<double>();
+    sv.s_ipiv = sipiv.as<int>();
+    sv.tr_r = trr.as<int>();
+    sv.tr_c = trc.as<int>();
+    sv.tr_v = trv.as<double>();
+    sv.bl_r = blr.as<int>();
+    sv.bl_c = blc.as<int>();
+    sv.bl_v = blv.as<double>();
 }
 
 void BuilderPlan::ensure_device() const
 {
     bs->ensure_device();
-    std::call_once(uploaded, [&] {
-        d_q.upload(prob.q.q);
-        d_ipiv.upload(prob.q.ipiv);
-        d_slu.upload(prob.schur.q);
-        d_sipiv.upload(prob.schur.ipiv);
-        d_trr.upload(prob.tr_r);
-        d_trc.upload(prob.tr_c);
-        d_trv.upload(prob.tr_v);
-        d_blr.upload(prob.bl_r);
-        d_blc.upload(prob.bl_c);
-        d_blv.upload(prob.bl_v);
+    if (!on_device) throw Error(DDCX_ERR_HIP, "no HIP device was present when the builder was created");
+}
+
+void BuilderPlan::upload()
+{
+    {
+        d_solve.upload(prob, view.sv);
         d_points.upload(points);
         SolveView& sv = view.sv;
-        sv.q = d_q.as<double>();
-        sv.ipiv = d_ipiv.as<int>();
-        sv.s_lu = d_slu.as<double>();
-        sv.s_ipiv = d_sipiv.as<int>();
-        sv.tr_r = d_trr.as<int>();
-        sv.tr_c = d_trc.as<int>();
-        sv.tr_v = d_trv.as<double>();
-        sv.bl_r = d_blr.as<int>();
-        sv.bl_c = d_blc.as<int>();
-        sv.bl_v = d_blv.as<double>();
         if (tiled.ok) {
             t_Mf.upload(tiled.Mf);
             t_Mb.upload(tiled.Mb);
@@ -704,7 +793,8 @@ void BuilderPlan::ensure_device() const
             rv.s_lu = sv.s_lu;
             rv.s_ipiv = sv.s_ipiv;
         }
-    });
+    }
+    on_device = true;
 }
 
 // ---------------------------------------------------------------------------
@@ -882,32 +972,8 @@ std::unique_ptr<BuilderPlan> make_builder(std::shared_ptr<BSplinesPlan> bs, ddcx
     bool const pds = b.uniform;
     HostProblem& pr = p->prob;
     pr.n = nb;
-    auto make_band = [&](HostQ& q, int n, int kl, int ku) {
-        if (kl == ku && kl == 1 && pds) q.init(QType::PDSTridiag, n, 1, 1);
-        else if (kl == ku && pds) q.init(QType::PDSBand, n, kl, kl);
-        else if (2 * kl + ku + 1 >= n) q.init(QType::Dense, n, 0, 0);
-        else q.init(QType::Band, n, kl, ku);
-    };
-    auto make_blocks = [&](int bottom, int top) {
-        pr.blocked = true;
-        pr.top = top;
-        make_band(pr.q, nb - top - bottom, ubw, ubw);
-        int const nq = pr.q.n, nbd = nb - nq;
-        pr.schur.init(QType::Dense, nbd, 0, 0);
-        pr.gamma.assign(size_t(nq) * nbd, 0.0);
-        pr.lambda.assign(size_t(nbd) * nq, 0.0);
-    };
-    if (b.periodic) {
-        int const bottom = ubw, top = nb - bottom;
-        if (long(bottom) * (nb + top) + long(2 * ubw + ubw + 1) * top >= long(nb) * nb) {
-            pr.blocked = false;
-            pr.q.init(QType::Dense, nb, 0, 0);
-        } else {
-            make_blocks(bottom, 0);
-        }
-    } else {
-        make_blocks(lower_bs, upper_bs);
-    }
+    if (b.periodic) make_periodic_band_problem(pr, nb, ubw, ubw, pds);
+    else make_block_problem(pr, nb, ubw, ubw, pds, lower_bs, upper_bs);
 
     // build_matrix_system, spline_builder.hpp:784-886
     dispatch_degree(D, [&]<int DD>() {
@@ -960,17 +1026,8 @@ std::unique_ptr<BuilderPlan> make_builder(std::shared_ptr<BSplinesPlan> bs, ddcx
     });
     pr.setup();
 
-    // device plan view (pointers are set by ensure_device)
+    // device plan view (the solve view is filled by upload)
     BuildView& bv = p->view;
-    SolveView& sv = bv.sv;
-    sv.qtype = int(pr.q.type);
-    sv.nq = pr.q.n;
-    sv.kl = pr.q.kl;
-    sv.ku = pr.q.ku;
-    sv.nb = pr.nb();
-    sv.top = pr.top;
-    sv.tr_nnz = int(pr.tr_v.size());
-    sv.bl_nnz = int(pr.bl_v.size());
     bv.degree = D;
     bv.periodic = b.periodic;
     bv.nbasis = nb;
@@ -989,6 +1046,7 @@ std::unique_ptr<BuilderPlan> make_builder(std::shared_ptr<BSplinesPlan> bs, ddcx
     }
     make_tiled(*p);
     if (p->precision == DDCX_FP32_REFINE) make_refine(*p);
+    if (device_present()) p->upload();
     return p;
 }
 

@@ -100,22 +100,25 @@ struct HostProblem {
 // Plans
 // ---------------------------------------------------------------------------
 
-// Device copies are made on first device use, so that host-only queries
-// (knots, points, quadrature weights, validation errors) work without a GPU.
+// Device copies are made at creation, on the device current then, when a HIP
+// device is present; without one the plans serve host-only queries (knots,
+// points, quadrature weights, validation errors) and every device entry point
+// reports DDCX_ERR_HIP.
 struct BSplinesPlan {
     BSplinesView view{}; // host view (knots -> host vector)
     std::vector<double> knots_host;
-    mutable DeviceBuffer knots_dev;
+    DeviceBuffer knots_dev;
     // cell-search brackets of the device view (non-uniform bases with fewer
     // than 65536 cells; see BSplinesView::cells); the host view searches the
     // whole range
     std::vector<uint32_t> cells_host;
     double inv_bucket = 0.0;
-    mutable DeviceBuffer cells_dev;
+    DeviceBuffer cells_dev;
     std::vector<double> integrals; // integrals() over the full spline domain (size() entries)
-    mutable DeviceBuffer integrals_dev;
-    mutable std::once_flag uploaded;
-    void ensure_device() const;
+    DeviceBuffer integrals_dev;
+    bool on_device = false; // uploaded at creation when a HIP device is present
+    void upload();
+    void ensure_device() const; // throws unless the device copies exist
     int nbasis() const { return view.ncells + (view.periodic ? 0 : view.degree); }
     int size() const { return view.ncells + view.degree; }
     BSplinesView device_view() const
@@ -145,6 +148,32 @@ struct SolveView {
     const int* bl_r;
     const int* bl_c;
     const double* bl_v;
+};
+
+// SplinesLinearProblemMaker (splines_linear_problem_maker.hpp:56-146) on host
+// problems (host_setup.cpp)
+void make_band_q(HostQ& q, int n, int kl, int ku, bool pds);
+void make_block_problem(HostProblem& pr, int n, int kl, int ku, bool pds, int bottom, int top);
+void make_periodic_band_problem(HostProblem& pr, int n, int kl, int ku, bool pds);
+void make_dense_block_problem(HostProblem& pr, int n, int bottom, int top);
+
+// Device copy of a factored HostProblem.  upload() fills every field of the
+// SolveView: sizes, and pointers into these buffers.
+struct SolveBuffers {
+    DeviceBuffer q, ipiv, slu, sipiv, trr, trc, trv, blr, blc, blv;
+    void upload(HostProblem const& pr, SolveView& sv);
+};
+
+// SplinesLinearProblem<ExecSpace> (splines_linear_problem.hpp:26-127) built by
+// SplinesLinearProblemMaker (splines_linear_problem_maker.hpp:25-170), for the
+// solver entry points of the C ABI: set_element before setup, factor on the
+// host in FP64 at setup, then device solves of many right-hand sides.
+struct ProblemPlan {
+    HostProblem prob;
+    bool factored = false;
+    bool on_device = false;
+    SolveBuffers buf;
+    SolveView sv{};
 };
 
 struct BuildView {
@@ -263,25 +292,26 @@ struct BuilderPlan {
     double dx = 0;
     std::vector<double> points;
     HostProblem prob;
-    mutable DeviceBuffer d_q, d_ipiv, d_slu, d_sipiv, d_trr, d_trc, d_trv, d_blr, d_blc, d_blv;
-    mutable DeviceBuffer d_points; // the interpolation points, for the in-kernel feet of ddcx_advect
-    mutable BuildView view{}; // device pointers filled by ensure_device()
+    SolveBuffers d_solve;
+    DeviceBuffer d_points; // the interpolation points, for the in-kernel feet of ddcx_advect
+    BuildView view{}; // device pointers filled by upload()
     TiledHost tiled;
-    mutable DeviceBuffer t_Mf, t_Mb, t_rt, t_gseg, t_Gblk;
-    mutable TiledView tview{};
+    DeviceBuffer t_Mf, t_Mb, t_rt, t_gseg, t_Gblk;
+    TiledView tview{};
     // tables of the fused build + eval kernel (fused_build_eval.hip)
     TiledHost tiled_f;
-    mutable DeviceBuffer tf_Mf, tf_Mb, tf_rt, tf_gseg, tf_Gblk;
-    mutable TiledView tview_f{};
+    DeviceBuffer tf_Mf, tf_Mb, tf_rt, tf_gseg, tf_Gblk;
+    TiledView tview_f{};
     // tables of the transposed solve (build_vjp_tiled.hip)
     TiledHost tiled_t;
-    mutable DeviceBuffer tt_Mf, tt_Mb, tt_rt, tt_gseg, tt_Gblk;
-    mutable TiledView tview_t{};
+    DeviceBuffer tt_Mf, tt_Mb, tt_rt, tt_gseg, tt_Gblk;
+    TiledView tview_t{};
     RefineHost refine;
-    mutable DeviceBuffer r_rt, r_Mf, r_Mb, r_gseg_blk, r_Gblk, r_rq, r_gr, r_gc, r_gv, r_gamseg, r_delta;
-    mutable RefineView rview{};
-    mutable std::once_flag uploaded;
-    void ensure_device() const;
+    DeviceBuffer r_rt, r_Mf, r_Mb, r_gseg_blk, r_Gblk, r_rq, r_gr, r_gc, r_gv, r_gamseg, r_delta;
+    RefineView rview{};
+    bool on_device = false;
+    void upload();
+    void ensure_device() const; // throws unless the device copies exist
 };
 
 // segments holding Q^-1 gamma blocks that the tiled adjoint can reduce over
@@ -304,6 +334,9 @@ inline Layout to_layout(ddcx_layout const& l)
 {
     return Layout{l.row_stride, l.inner_count > 0 ? l.inner_count : 1, l.inner_stride, l.outer_stride};
 }
+
+// true when the HIP runtime reports at least one device (checked once)
+bool device_present();
 
 // host setup entry points (host_setup.cpp)
 std::shared_ptr<BSplinesPlan> make_bsplines(ddcx_bsplines_desc const& d);
@@ -329,6 +362,11 @@ bool launch_build_eval_fused(BuilderPlan const& p, BSplinesPlan const& bs, int l
                              hipStream_t stream, const double* feet_shift = nullptr);
 void launch_build(BuilderPlan const& p, double* coef, Layout cl, const double* vals, Layout vl, const double* dl,
                   const double* dr, Layout dlay, int64_t nbatch, hipStream_t stream);
+// SplinesLinearProblem::solve(b, transpose) on the device: b holds nrhs
+// right-hand sides, row i of right-hand side j at b[i * ldb + j]
+// (build_generic.hip).
+void launch_problem_solve(SolveView const& sv, double* b, int64_t ldb, int64_t nrhs, bool transpose,
+                          hipStream_t stream);
 void launch_build_vjp(BuilderPlan const& p, const double* gcoef, Layout cl, double* gvals, Layout vl, double* gdl,
                       double* gdr, Layout dlay, int64_t nbatch, hipStream_t stream);
 bool launch_eval_tiled(BSplinesPlan const& bs, int lower, int upper, int order, double* out, const double* coords,

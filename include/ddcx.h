@@ -12,7 +12,10 @@
  *    calls are DEVICE pointers (hipMalloc'd or equivalent); the `stream`
  *    argument is a hipStream_t (NULL = default stream).  Calls are
  *    stream-ordered and asynchronous; nothing allocates device memory after
- *    create (plans own all device data).
+ *    create (plans own all device data).  Plans upload their device data at
+ *    create (ddcx_problem: at setup), synchronously, to the device current
+ *    then; call them with streams of that device.  Without a HIP device the
+ *    handles still answer host queries, and device calls return DDCX_ERR_HIP.
  *  - A batched array is described by a ddcx_layout: element (row r, batch
  *    line l) lives at  r*row_stride + (l % inner_count)*inner_stride
  *    + (l / inner_count)*outer_stride  (units: elements).  This covers every
@@ -266,6 +269,61 @@ ddcx_status ddcx_evaluator2d_eval_vjp(const ddcx_evaluator* e1, const ddcx_evalu
                                       const double* w, double* gx, int64_t npts, const double* coef, double* gcoef,
                                       int64_t coef_batch_stride, int64_t st1, int64_t st2, int64_t nbatch,
                                       void* stream);
+
+/* ---- The linear problem on its own ------------------------------------------
+ * SplinesLinearProblem<ExecSpace> (splines_linear_problem.hpp:26-127): the
+ * matrix object the builder owns, as made by SplinesLinearProblemMaker
+ * (splines_linear_problem_maker.hpp:25-170).  Elements are set on the host
+ * before setup; setup factors on the host in FP64 (the LAPACK algorithms of
+ * the reference) and uploads the factors; solves then run on the device with
+ * the sweep code of the builds. */
+typedef enum ddcx_problem_kind {
+    DDCX_PROBLEM_DENSE = 0,         /* SplinesLinearProblemDense(n) */
+    DDCX_PROBLEM_BAND = 1,          /* make_new_band(n, kl, ku, pds) */
+    DDCX_PROBLEM_BLOCK_BAND = 2,    /* make_new_block_matrix_with_band_main_block(n, kl, ku, pds, bottom, top) */
+    DDCX_PROBLEM_PERIODIC_BAND = 3, /* make_new_periodic_band_matrix(n, kl, ku, pds) */
+    DDCX_PROBLEM_BLOCK_DENSE = 4,   /* SplinesLinearProblem2x2Blocks(n, Dense(n - bottom)) for top == 0, else
+                                       SplinesLinearProblem3x3Blocks(n, top, Dense(n - top - bottom)) */
+    DDCX_PROBLEM_BAND_STORAGE = 5   /* SplinesLinearProblemBand(n, kl, ku) built directly (the maker picks
+                                       dense storage for wide bands) */
+} ddcx_problem_kind;
+
+typedef struct ddcx_problem_desc {
+    ddcx_problem_kind kind;
+    int n;           /* matrix size */
+    int kl, ku;      /* band kinds: sub- and super-diagonals */
+    int pds;         /* band kinds: positive-definite symmetric band */
+    int bottom_size; /* block kinds: bottom-right block size */
+    int top_size;    /* block kinds: top-left block size (3x3 blocks when > 0) */
+} ddcx_problem_desc;
+
+typedef struct ddcx_problem_info {
+    int size;        /* SplinesLinearProblem::size() */
+    int rhs_rows;    /* required_number_of_rhs_rows() */
+    int main_kind;   /* 0 dense, 1 band, 2 pds band, 3 pds tridiag (the main block, or the whole matrix) */
+    int blocked;     /* 0 plain, 1 2x2 blocks, 2 3x3 blocks */
+    int main_size;
+    int pivoted;     /* after setup: 1 if a band or dense factor interchanged rows */
+} ddcx_problem_info;
+
+typedef struct ddcx_problem ddcx_problem;
+
+ddcx_status ddcx_problem_create(const ddcx_problem_desc* desc, ddcx_problem** out);
+ddcx_status ddcx_problem_destroy(ddcx_problem* h);
+/* set_element / get_element (splines_linear_problem.hpp:63-70); set only
+ * before setup.  After setup, get returns the assembled (unfactored) entry. */
+ddcx_status ddcx_problem_set_element(ddcx_problem* h, int i, int j, double aij);
+ddcx_status ddcx_problem_get_element(const ddcx_problem* h, int i, int j, double* aij);
+/* setup_solver() (splines_linear_problem.hpp:72): factor and upload.
+ * DDCX_ERR_SINGULAR with the LAPACK info as the reference throws it. */
+ddcx_status ddcx_problem_setup_solver(ddcx_problem* h);
+ddcx_status ddcx_problem_get_info(const ddcx_problem* h, ddcx_problem_info* info);
+/* solve(b, transpose) (splines_linear_problem.hpp:77), in place on the device:
+ * b holds nrhs right-hand sides, row i of right-hand side j at b[i * ldb + j]
+ * (the reference's LayoutRight MultiRHS view (rhs_rows, nrhs)); rows past
+ * size() are workspace, as in the reference.  Stream-ordered. */
+ddcx_status ddcx_problem_solve(const ddcx_problem* h, double* b, int64_t ldb, int64_t nrhs, int transpose,
+                               void* stream);
 
 const char* ddcx_last_error(void);
 const char* ddcx_version(void);

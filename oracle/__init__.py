@@ -96,7 +96,7 @@ def _err():
 class Problem:
     """A SplinesLinearProblem restated on the CPU (for solver known-answer tests)."""
 
-    KINDS = {"dense": 0, "band": 1, "block": 2, "periodic": 3, "dense2x2": 4, "dense3x3": 5}
+    KINDS = {"dense": 0, "band": 1, "block": 2, "periodic": 3, "dense2x2": 4, "dense3x3": 5, "band_storage": 6}
     TYPES = ["dense", "band", "pds_band", "pds_tridiag", "2x2", "3x3"]
 
     def __init__(self, kind, n, kl=0, ku=0, pds=False, bottom=0, top=0):

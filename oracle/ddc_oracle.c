@@ -905,8 +905,10 @@ static int prob_setup(or_prob* p)
 void* or_problem_create(int kind, int n, int kl, int ku, int pds, int bottom, int top)
 {
     /* kind: 0 dense, 1 maker_band, 2 block (2x2/3x3 with band main), 3 periodic band,
-     *       4 2x2 with a Dense main block, 5 3x3 with a Dense center block */
+     *       4 2x2 with a Dense main block, 5 3x3 with a Dense center block,
+     *       6 SplinesLinearProblemBand(n, kl, ku) built directly */
     switch (kind) {
+    case 6: return make_band(n, kl, ku);
     case 0: return make_dense(n);
     case 1: return maker_band(n, kl, ku, pds);
     case 2: return maker_block_band(n, kl, ku, pds, bottom, top);

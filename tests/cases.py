@@ -125,6 +125,15 @@ SHORT_TILED_CASES = [
     Case(5, False, HH, ncells=250, mesh="uniform"),
 ]
 
+# Degree-6 non-uniform bases on random meshes, whose band factorisation
+# (dgbtrf) interchanges rows: the pivoted sweeps of the generic kernels.
+PIVOTED_CASES = [
+    Case(6, False, G, ncells=128, mesh="random", seed=3),
+    Case(6, False, H, ncells=128, mesh="random", seed=3),
+    Case(6, False, H, ncells=300, mesh="random", seed=0),
+    Case(6, False, G, ncells=1024, mesh="random", seed=1),
+]
+
 # The precision-form workload: Hermite closures on a geometrically stretched
 # mesh (ratio 1.05 over 2048 cells, smallest cell ~2e-45 of the domain).
 STRETCHED_CASES = [

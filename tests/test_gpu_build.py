@@ -5,7 +5,7 @@ Bar: relative error (max norm) <= 1e-13 for FP64 builds.
 import numpy as np
 import pytest
 
-from cases import ALL_CASES, Case, G, H, HH, P, derivs, values
+from cases import ALL_CASES, PIVOTED_CASES, Case, G, H, HH, P, derivs, values
 from conftest import rel_err
 
 TOL = 1e-13
@@ -48,6 +48,16 @@ def _run(case, nbatch, layout, cuda, seed=0):
 @pytest.mark.parametrize("layout", ["batch_contiguous", "interp_contiguous"])
 def test_build_matches_oracle(case, layout, cuda):
     got, ref = _run(case, 37, layout, cuda)
+    assert rel_err(got, ref) <= TOL
+
+
+@pytest.mark.parametrize("case", PIVOTED_CASES, ids=lambda c: c.id)
+@pytest.mark.parametrize("layout", ["batch_contiguous", "interp_contiguous"])
+def test_build_pivoted_band(case, layout, cuda):
+    """Band factors with row interchanges (dgbtrf pivots, splines_linear_problem_band.cpp:109-170)."""
+    _, builder = case.product()
+    assert builder.info.pivoted == 1 and builder.info.matrix_kind == 1
+    got, ref = _run(case, 45, layout, cuda, seed=4)
     assert rel_err(got, ref) <= TOL
 
 

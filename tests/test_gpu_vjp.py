@@ -7,7 +7,7 @@ atomics, in an order that differs from the oracle's sequential sum).
 import numpy as np
 import pytest
 
-from cases import ALL_CASES, SHORT_TILED_CASES, TILED_CASES, Case, G, H, P
+from cases import ALL_CASES, PIVOTED_CASES, SHORT_TILED_CASES, TILED_CASES, Case, G, H, P
 from conftest import rel_err
 
 pytestmark = pytest.mark.gpu
@@ -44,6 +44,17 @@ def _vjp(case, nbatch, layout, cuda, seed=0):
 @pytest.mark.parametrize("layout", ["batch_contiguous", "interp_contiguous"])
 def test_build_vjp_matches_oracle(case, layout, cuda):
     got, ref = _vjp(case, 13, layout, cuda)
+    assert np.all(np.isfinite(got))
+    assert rel_err(got, ref) <= 1e-13
+
+
+@pytest.mark.parametrize("case", PIVOTED_CASES, ids=lambda c: c.id)
+@pytest.mark.parametrize("layout", ["batch_contiguous", "interp_contiguous"])
+def test_build_vjp_pivoted_band(case, layout, cuda):
+    """The transposed solve with the interchanges applied in reverse (dgbtrs trans='T')."""
+    _, builder = case.product()
+    assert builder.info.pivoted == 1
+    got, ref = _vjp(case, 29, layout, cuda, seed=3)
     assert np.all(np.isfinite(got))
     assert rel_err(got, ref) <= 1e-13
 

@@ -220,3 +220,40 @@ def test_knots_as_interpolation_points():
     assert np.array_equal(D.knots_as_interpolation_points(nu, C.HOMOGENEOUS_HERMITE, C.HERMITE), br)
     with pytest.raises(ValueError):
         D.knots_as_interpolation_points(nu, C.GREVILLE, C.HERMITE)
+
+
+def test_linear_problem_host_side():
+    """ddcx_problem_*: the maker's storage choices, set/get before and after
+    setup, setup's factorisation, and the argument errors (no device work)."""
+    import ddc_amd.linear_problem as LP
+    from ddc_amd import DDCXError
+    from matrices import nonsym
+
+    assert LP.make_new_band(10, 3, 3, False).main_kind() == "dense"  # 2 kl + ku + 1 >= n
+    assert LP.make_new_band(20, 3, 3, False).main_kind() == "band"
+    assert LP.make_new_band(20, 1, 1, True).main_kind() == "pds_tridiag"
+    assert LP.make_new_band(20, 2, 2, True).main_kind() == "pds_band"
+    assert LP.make_new_band(20, 0, 4, True).main_kind() == "band"  # kl != ku: not symmetric storage
+    p = LP.make_new_block_matrix_with_band_main_block(20, 2, 2, False, 3, 2)
+    assert p.blocked() == 2 and p.required_number_of_rhs_rows() == 22
+    assert LP.make_new_periodic_band_matrix(10, 4, 4, False).blocked() == 0  # dense is smaller
+    A = nonsym(20, 2)
+    p = LP.make_new_block_matrix_with_band_main_block(20, 2, 2, False, 3)
+    for i in range(20):
+        for j in range(20):
+            if A[i, j]:
+                p.set_element(i, j, A[i, j])
+    got = np.array([[p.get_element(i, j) for j in range(20)] for i in range(20)])
+    assert np.array_equal(got, A)
+    p.setup_solver()
+    got = np.array([[p.get_element(i, j) for j in range(20)] for i in range(20)])
+    assert np.array_equal(got, A)
+    with pytest.raises(DDCXError):
+        p.set_element(0, 0, 1.0)
+    with pytest.raises(DDCXError):
+        p.get_element(20, 0)
+    with pytest.raises(DDCXError):
+        LP.make_new_block_matrix_with_band_main_block(5, 1, 1, False, 3, 2)  # no main block left
+    q = LP.make_new_dense(3)
+    with pytest.raises(DDCXError, match="SINGULAR"):
+        q.setup_solver()  # all zeros: dgetrf info > 0
