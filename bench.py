@@ -44,32 +44,92 @@ def read_traffic(kernel):
     return None
 
 
-def cpu_baseline(ncells, B_sample, P_line, seed):
-    """Oracle (CPU restatement, OpenMP over lines) on a bounded sample: full
-    build + eval passes over B_sample lines, repeated for at least 10 s."""
+def physical_cores():
+    """Physical cores among the CPUs this process may run on (one per
+    (package, core) pair of the allowed logical CPUs)."""
+    cpus = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else list(range(os.cpu_count() or 1))
+    seen = set()
+    for c in cpus:
+        base = f"/sys/devices/system/cpu/cpu{c}/topology/"
+        try:
+            key = (open(base + "physical_package_id").read().strip(), open(base + "core_id").read().strip())
+        except OSError:
+            key = ("?", str(c))
+        seen.add(key)
+    return max(1, len(seen))
+
+
+def _median_runs(fn, min_runs=5, min_seconds=4.0):
+    """1 warm-up call, then at least min_runs timed calls (more until
+    min_seconds have passed); returns (median seconds, runs)."""
+    fn()
+    times = []
+    t_start = time.perf_counter()
+    while len(times) < min_runs or time.perf_counter() - t_start < min_seconds:
+        t0 = time.perf_counter()
+        fn()
+        times.append(time.perf_counter() - t0)
+    times.sort()
+    return times[len(times) // 2], len(times)
+
+
+def cpu_baseline_run(seed):
+    """The oracle (CPU restatement, OpenMP over lines; its threads pinned by
+    the environment this process was started with) on
+      - config 1 at its own shape: uniform periodic cubic, 1000 cells x 1000
+        lines, interpolation-contiguous values [y][x], 1000 uniform points per
+        line, f = sin(2 pi x k_y) + 0.1 N(0, 1);
+      - a 4096-line sample of the headline workload (1024 cells,
+        batch-contiguous, 1024 points per line).
+    Build and eval are timed separately: 1 warm-up, median of >= 5 runs."""
     import numpy as np
 
     import oracle as O
 
-    orc = O.Spline(3, True, O.PERIODIC, O.PERIODIC, ncells=ncells)
-    rng = np.random.default_rng(seed)
-    x = orc.points()
-    f = np.sin(2 * np.pi * x[:, None] * rng.integers(1, 9, B_sample)[None, :])
-    xe = rng.uniform(0, 1, (P_line, B_sample))
-    orc.build(f)  # warm-up
-    reps, t0 = 0, time.perf_counter()
-    while True:
-        coef = orc.build(f)
-        orc.eval(xe, coef)
-        reps += 1
-        if time.perf_counter() - t0 > 10.0 and reps >= 2:
-            break
-    dt = (time.perf_counter() - t0) / reps
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    mpts = (x.size * B_sample + P_line * B_sample) / dt / 1e6
-    return {"value": round(mpts, 2), "unit": "Mpts/s", "cores": cores, "kind": "port",
-            "sample": f"oracle build+eval, uniform periodic cubic, {ncells} cells x {B_sample} lines, "
-                      f"{P_line} points/line, {reps} reps over {dt * reps:.1f} s"}
+    res = {}
+    for name, ncells, B, layout in (("config1", 1000, 1000, "interp_contiguous"),
+                                    ("headline_sample", 1024, 4096, "batch_contiguous")):
+        orc = O.Spline(3, True, O.PERIODIC, O.PERIODIC, ncells=ncells)
+        rng = np.random.default_rng(seed)
+        x = orc.points()
+        k = rng.integers(1, 9, B)
+        f = np.sin(2 * np.pi * x[:, None] * k[None, :]) + 0.1 * rng.standard_normal((x.size, B))
+        xe = rng.uniform(0, 1, (ncells, B))
+        if layout == "interp_contiguous":
+            f, xe = np.ascontiguousarray(f.T), np.ascontiguousarray(xe.T)
+        coef = orc.build(f, layout=layout)
+        t_b, n_b = _median_runs(lambda: orc.build(f, layout=layout))
+        t_e, n_e = _median_runs(lambda: orc.eval(xe, coef, layout=layout))
+        m, P = x.size, xe.size
+        res[name] = {"ncells": ncells, "lines": B, "layout": layout, "eval_points": P,
+                     "build_ms": round(t_b * 1e3, 3), "eval_ms": round(t_e * 1e3, 3), "runs": [n_b, n_e],
+                     "build_eval_Mpts_s": round((m * B + P) / (t_b + t_e) / 1e6, 2)}
+    return res
+
+
+def cpu_baseline(seed):
+    """Times cpu_baseline_run in a child process with OMP_NUM_THREADS = the
+    physical cores (or the box's OMP_NUM_THREADS share, if smaller),
+    OMP_PROC_BIND=close and OMP_PLACES=cores, so that the OpenMP runtime is
+    started with those settings (this process has already loaded torch's)."""
+    import subprocess
+
+    phys = physical_cores()
+    share = int(os.environ.get("OMP_NUM_THREADS", phys) or phys)
+    cores = max(1, min(phys, share))
+    env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-only", "--seed", str(seed)],
+                       env=env, capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        raise RuntimeError(f"cpu baseline failed: {r.stderr[-2000:]}")
+    detail = json.loads(r.stdout.strip().splitlines()[-1])
+    c1 = detail["config1"]
+    return {"value": c1["build_eval_Mpts_s"], "unit": "Mpts/s", "cores": cores, "physical_cores": phys,
+            "kind": "port",
+            "sample": "oracle (OpenMP, OMP_PROC_BIND=close, OMP_PLACES=cores), config 1 at its shape: uniform "
+                      "periodic cubic, 1000 cells x 1000 lines interp-contiguous, 1e6 eval points; build and eval "
+                      "medians of >= 5 runs after 1 warm-up; value = (m*B + P) / (t_build + t_eval)",
+            "detail": detail}
 
 
 def _timer(stream):
@@ -327,7 +387,12 @@ def main():
     ap.add_argument("--batch", type=int, default=65536)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--headline-only", action="store_true", help="skip the other workloads")
+    ap.add_argument("--cpu-baseline-only", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--seed", type=int, default=0xDDC0001, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_baseline_only:  # child process of cpu_baseline(): no GPU use
+        print(json.dumps(cpu_baseline_run(args.seed)))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: the spline path shards by batch line with no communication; "
               "run with --gpus 1", file=sys.stderr)
@@ -434,7 +499,7 @@ def main():
         other["6_tensor_3d_128x128x128x4"] = measure_3d(D, timed, gen)
         res["workloads"] = other
     if not args.no_cpu:
-        res["cpu_baseline"] = cpu_baseline(nc, 4096, P_line, 0xDDC0002)
+        res["cpu_baseline"] = cpu_baseline(0xDDC0001)
     print(json.dumps(res))
 
 

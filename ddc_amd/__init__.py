@@ -5,6 +5,7 @@ kernels, C ABI in ``include/ddcx.h``); this package is the host-side mirror of
 the reference's ``ddc::SplineBuilder`` / ``ddc::SplineEvaluator`` interface.
 """
 from ._lib import LIB_PATH, DDCXError, lib  # noqa: F401
+from . import autograd  # noqa: F401
 from .splines import (  # noqa: F401
     ConstantExtrapolationRule,
     NonUniformBSplines,

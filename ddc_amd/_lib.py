@@ -38,6 +38,11 @@ class ddcx_builder_info(C.Structure):
                 ("pivoted", C.c_int), ("refine_steps", C.c_int), ("refine_contraction", C.c_double)]
 
 
+class ddcx_extrap_rule(C.Structure):
+    _fields_ = [("kind", C.c_int), ("eval_pos", C.c_double), ("eval_pos_not_interest_min", C.c_double),
+                ("eval_pos_not_interest_max", C.c_double)]
+
+
 class ddcx_problem_desc(C.Structure):
     _fields_ = [("kind", C.c_int), ("n", C.c_int), ("kl", C.c_int), ("ku", C.c_int), ("pds", C.c_int),
                 ("bottom_size", C.c_int), ("top_size", C.c_int)]
@@ -66,6 +71,7 @@ SIGNATURES = {
     "ddcx_builder_build_vjp": [_P, _P, _L, _P, _L, _P, _P, _L, _I64, _P],
     "ddcx_builder_quadrature": [_P, _P, _P, _P],
     "ddcx_evaluator_create": [_P, C.c_int, C.c_int, _PP],
+    "ddcx_evaluator_create_with_rules": [_P, C.POINTER(ddcx_extrap_rule), C.POINTER(ddcx_extrap_rule), _PP],
     "ddcx_evaluator_destroy": [_P],
     "ddcx_evaluator_eval": [_P, _P, _P, _L, _I64, _P, _L, _I64, _P],
     "ddcx_evaluator_deriv": [_P, C.c_int, _P, _P, _L, _I64, _P, _L, _I64, _P],

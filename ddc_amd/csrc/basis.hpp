@@ -98,11 +98,31 @@ __host__ __device__ inline int nonuniform_cell(BSplinesView const& b, const doub
     return icell;
 }
 
+// The extrapolation rules of a 1D evaluator (SplineEvaluator's lower and
+// upper rules, spline_evaluator.hpp:140-153): kind 0 Null, 1 Constant.  A
+// Constant rule evaluates the spline at its eval_pos instead of the point
+// (ConstantExtrapolationRule::m_eval_pos, constant_extrapolation_rule.hpp:39-72).
+struct Rules1D {
+    int lower, upper;
+    double pos_lower, pos_upper;
+};
+
+// The four rules of a 2D evaluator, lower and upper of dimension 1 then of
+// dimension 2 (SplineEvaluator2D, spline_evaluator_2d.hpp:679-720).  A
+// Constant rule of dimension k evaluates at pos in dimension k and, when the
+// other dimension is not periodic, clamps the other coordinate to
+// [ni_min, ni_max] (constant_extrapolation_rule.hpp:99-124, :160-172).
+struct Rules2D {
+    int kind[4];
+    double pos[4];
+    double ni_min[4], ni_max[4];
+};
+
 // SplineEvaluator::eval, spline_evaluator.hpp:529-554: periodic wrap or
-// extrapolation (rules 0 = Null, 1 = Constant).  Returns false when the Null
-// rule gives 0; sets x to the position actually evaluated (the bound, for the
-// Constant rule) and `outside` when a rule applied.
-__host__ __device__ inline bool apply_bc(BSplinesView const& b, int lower, int upper, double& x, bool& outside)
+// extrapolation.  Returns false when the Null rule gives 0; sets x to the
+// position actually evaluated (the rule's eval_pos, for the Constant rule)
+// and `outside` when a rule applied.
+__host__ __device__ inline bool apply_bc(BSplinesView const& b, Rules1D const& r, double& x, bool& outside)
 {
     outside = false;
     if (b.periodic) {
@@ -111,12 +131,52 @@ __host__ __device__ inline bool apply_bc(BSplinesView const& b, int lower, int u
     }
     if (x < b.rmin) {
         outside = true;
-        if (lower == 0) return false;
-        x = b.rmin;
+        if (r.lower == 0) return false;
+        x = r.pos_lower;
     } else if (x > b.rmax) {
         outside = true;
-        if (upper == 0) return false;
-        x = b.rmax;
+        if (r.upper == 0) return false;
+        x = r.pos_upper;
+    }
+    return true;
+}
+
+// SplineEvaluator2D::eval, spline_evaluator_2d.hpp:679-720: periodic wraps,
+// then the first rule that applies (dimension 1 before dimension 2).  Returns
+// false when a Null rule gives 0; otherwise (x1, x2) is the position
+// evaluated, and in1 / in2 say whether that coordinate still depends on the
+// point's (false where a rule replaced or clamped it).
+__host__ __device__ inline bool place2d(BSplinesView const& b1, BSplinesView const& b2, Rules2D const& r, double& x1,
+                                        double& x2, bool& in1, bool& in2)
+{
+    in1 = in2 = true;
+    if (b1.periodic && (x1 < b1.rmin || x1 > b1.rmax)) x1 -= floor((x1 - b1.rmin) / b1.length) * b1.length;
+    if (b2.periodic && (x2 < b2.rmin || x2 > b2.rmax)) x2 -= floor((x2 - b2.rmin) / b2.length) * b2.length;
+    auto clamp = [](double v, double lo, double hi, bool& in) {
+        if (v < lo) {
+            in = false;
+            return lo;
+        }
+        if (v > hi) {
+            in = false;
+            return hi;
+        }
+        return v;
+    };
+    if (!b1.periodic && (x1 < b1.rmin || x1 > b1.rmax)) {
+        int const k = x1 < b1.rmin ? 0 : 1;
+        if (r.kind[k] == 0) return false;
+        x1 = r.pos[k];
+        in1 = false;
+        if (!b2.periodic) x2 = clamp(x2, r.ni_min[k], r.ni_max[k], in2);
+        return true;
+    }
+    if (!b2.periodic && (x2 < b2.rmin || x2 > b2.rmax)) {
+        int const k = x2 < b2.rmin ? 2 : 3;
+        if (r.kind[k] == 0) return false;
+        x2 = r.pos[k];
+        in2 = false;
+        if (!b1.periodic) x1 = clamp(x1, r.ni_min[k], r.ni_max[k], in1);
     }
     return true;
 }

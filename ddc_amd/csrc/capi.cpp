@@ -2,6 +2,8 @@
 // handles and layouts to the internal plans, catches every exception and
 // returns a status with a thread-local message.
 #include <algorithm>
+#include <cmath>
+#include <memory>
 #include <cstring>
 #include <string>
 
@@ -16,6 +18,11 @@ struct ddcx_builder {
 struct ddcx_evaluator {
     std::shared_ptr<ddcx::BSplinesPlan> bs;
     ddcx_extrap lower, upper;
+    // Constant rules: ConstantExtrapolationRule's eval_pos (resolved) and,
+    // for 2D use, the clamp range of the other coordinate (NaN: that
+    // dimension's [rmin, rmax])
+    double pos[2];
+    double ni_min[2], ni_max[2];
 };
 struct ddcx_problem {
     ddcx::ProblemPlan p;
@@ -52,6 +59,52 @@ void require(bool cond, const char* msg)
 int rule_of(ddcx_extrap e)
 {
     return e == DDCX_EXTRAP_CONSTANT ? DDCX_EXTRAP_CONSTANT : DDCX_EXTRAP_NULL;
+}
+
+ddcx::Rules1D rules1d(const ddcx_evaluator* h)
+{
+    return ddcx::Rules1D{rule_of(h->lower), rule_of(h->upper), h->pos[0], h->pos[1]};
+}
+
+// The four rules of SplineEvaluator2D(e1's lower, upper, e2's lower, upper);
+// a Constant rule's default clamp range is the other dimension's domain.
+ddcx::Rules2D rules2d(const ddcx_evaluator* e1, const ddcx_evaluator* e2)
+{
+    ddcx::Rules2D r{};
+    const ddcx_evaluator* ev[2] = {e1, e2};
+    for (int d = 0; d < 2; ++d) {
+        ddcx::BSplinesView const& other = ev[1 - d]->bs->view;
+        for (int s = 0; s < 2; ++s) {
+            int const k = 2 * d + s;
+            r.kind[k] = rule_of(s == 0 ? ev[d]->lower : ev[d]->upper);
+            r.pos[k] = ev[d]->pos[s];
+            r.ni_min[k] = std::isnan(ev[d]->ni_min[s]) ? other.rmin : ev[d]->ni_min[s];
+            r.ni_max[k] = std::isnan(ev[d]->ni_max[s]) ? other.rmax : ev[d]->ni_max[s];
+            require(r.ni_min[k] <= r.ni_max[k],
+                    "ConstantExtrapolationRule: eval_pos_not_interest_min exceeds eval_pos_not_interest_max");
+        }
+    }
+    return r;
+}
+
+// ddcx_evaluator_create_with_rules: one rule's positions, checked
+void set_rule(ddcx_evaluator* h, int side, ddcx_extrap_rule const& r)
+{
+    ddcx::BSplinesView const& v = h->bs->view;
+    (side == 0 ? h->lower : h->upper) = r.kind;
+    double const bound = side == 0 ? v.rmin : v.rmax;
+    h->pos[side] = bound;
+    h->ni_min[side] = h->ni_max[side] = std::nan("");
+    if (r.kind != DDCX_EXTRAP_CONSTANT) return;
+    if (!std::isnan(r.eval_pos)) {
+        require(r.eval_pos >= v.rmin && r.eval_pos <= v.rmax,
+                "ConstantExtrapolationRule: eval_pos must lie inside the domain [rmin, rmax]");
+        h->pos[side] = r.eval_pos;
+    }
+    require(std::isnan(r.eval_pos_not_interest_min) == std::isnan(r.eval_pos_not_interest_max),
+            "ConstantExtrapolationRule: give both eval_pos_not_interest_min and _max, or neither");
+    h->ni_min[side] = r.eval_pos_not_interest_min;
+    h->ni_max[side] = r.eval_pos_not_interest_max;
 }
 
 } // namespace
@@ -177,14 +230,14 @@ ddcx_status ddcx_build_eval(const ddcx_builder* h, const ddcx_evaluator* e, doub
         p.ensure_device();
         e->bs->ensure_device();
         if (p.precision == DDCX_FP64 && npts > 0 &&
-            ddcx::launch_build_eval_fused(p, *e->bs, rule_of(e->lower), rule_of(e->upper), coef, cl, vals, vl,
+            ddcx::launch_build_eval_fused(p, *e->bs, rules1d(e), coef, cl, vals, vl,
                                           derivs_xmin, derivs_xmax, dl, out, coords, pl, npts, nbatch, st))
             return;
         require(coef != nullptr,
                 "the fused kernel does not cover this case (precision, layout or basis); pass a coefficient array");
         ddcx::launch_build(p, coef, cl, vals, vl, derivs_xmin, derivs_xmax, dl, nbatch, st);
         if (npts > 0)
-            ddcx::launch_eval(*e->bs, rule_of(e->lower), rule_of(e->upper), 0, out, coords, pl, npts, coef, cl, nbatch,
+            ddcx::launch_eval(*e->bs, rules1d(e), 0, out, coords, pl, npts, coef, cl, nbatch,
                               st);
     });
 }
@@ -209,7 +262,7 @@ ddcx_status ddcx_advect(const ddcx_builder* h, const ddcx_evaluator* e, double* 
         e->bs->ensure_device();
         bool const done =
             p.precision == DDCX_FP64 &&
-            ddcx::launch_build_eval_fused(p, *e->bs, rule_of(e->lower), rule_of(e->upper), coef, cl, vals, vl,
+            ddcx::launch_build_eval_fused(p, *e->bs, rules1d(e), coef, cl, vals, vl,
                                           derivs_xmin, derivs_xmax, dl, out, nullptr, ol, int64_t(p.points.size()),
                                           nbatch, static_cast<hipStream_t>(stream), shift);
         if (!done)
@@ -262,7 +315,30 @@ ddcx_status ddcx_evaluator_create(const ddcx_bsplines* b, ddcx_extrap lower, ddc
 {
     return guarded([&] {
         require(b && out, "null argument");
-        *out = new ddcx_evaluator{b->p, lower, upper};
+        *out = nullptr;
+        ddcx_extrap_rule const nan_rule{DDCX_EXTRAP_NULL, std::nan(""), std::nan(""), std::nan("")};
+        ddcx_extrap_rule lo = nan_rule, up = nan_rule;
+        lo.kind = lower;
+        up.kind = upper;
+        auto h = std::make_unique<ddcx_evaluator>();
+        h->bs = b->p;
+        set_rule(h.get(), 0, lo);
+        set_rule(h.get(), 1, up);
+        *out = h.release();
+    });
+}
+
+ddcx_status ddcx_evaluator_create_with_rules(const ddcx_bsplines* b, const ddcx_extrap_rule* lower,
+                                             const ddcx_extrap_rule* upper, ddcx_evaluator** out)
+{
+    return guarded([&] {
+        require(b && lower && upper && out, "null argument");
+        *out = nullptr;
+        auto h = std::make_unique<ddcx_evaluator>();
+        h->bs = b->p;
+        set_rule(h.get(), 0, *lower);
+        set_rule(h.get(), 1, *upper);
+        *out = h.release();
     });
 }
 
@@ -279,7 +355,7 @@ ddcx_status ddcx_evaluator_eval(const ddcx_evaluator* h, double* out, const doub
         require(npts >= 0 && nbatch >= 0, "negative size");
         if (npts * nbatch == 0) return;
         require(out && coords && coef, "null data pointer");
-        ddcx::launch_eval(*h->bs, rule_of(h->lower), rule_of(h->upper), 0, out, coords, ddcx::to_layout(points_layout),
+        ddcx::launch_eval(*h->bs, rules1d(h), 0, out, coords, ddcx::to_layout(points_layout),
                           npts, coef, ddcx::to_layout(coef_layout), nbatch, static_cast<hipStream_t>(stream));
     });
 }
@@ -294,7 +370,7 @@ ddcx_status ddcx_evaluator_deriv(const ddcx_evaluator* h, int order, double* out
         require(npts >= 0 && nbatch >= 0, "negative size");
         if (npts * nbatch == 0) return;
         require(out && coords && coef, "null data pointer");
-        ddcx::launch_eval(*h->bs, rule_of(h->lower), rule_of(h->upper), order, out, coords,
+        ddcx::launch_eval(*h->bs, rules1d(h), order, out, coords,
                           ddcx::to_layout(points_layout), npts, coef, ddcx::to_layout(coef_layout), nbatch,
                           static_cast<hipStream_t>(stream));
     });
@@ -323,7 +399,7 @@ ddcx_status ddcx_evaluator_eval_vjp(const ddcx_evaluator* h, const double* coord
         if (nbatch == 0) return;
         require(gcoef && coef, "null data pointer");
         if (npts > 0) require(coords && w && gx, "null data pointer");
-        ddcx::launch_eval_vjp(*h->bs, rule_of(h->lower), rule_of(h->upper), coords, w, gx,
+        ddcx::launch_eval_vjp(*h->bs, rules1d(h), coords, w, gx,
                               ddcx::to_layout(points_layout), npts, coef, gcoef, ddcx::to_layout(coef_layout), nbatch,
                               static_cast<hipStream_t>(stream));
     });
@@ -339,7 +415,7 @@ ddcx_status ddcx_evaluator2d_eval(const ddcx_evaluator* e1, const ddcx_evaluator
         require(npts >= 0 && nbatch >= 0, "negative size");
         if (npts * nbatch == 0) return;
         require(out && coords && coef, "null data pointer");
-        int const rules[4] = {rule_of(e1->lower), rule_of(e1->upper), rule_of(e2->lower), rule_of(e2->upper)};
+        ddcx::Rules2D const rules = rules2d(e1, e2);
         ddcx::launch_eval2d(*e1->bs, *e2->bs, rules, order1, order2, out, coords, npts, coef, coef_batch_stride, st1,
                             st2, nbatch, static_cast<hipStream_t>(stream));
     });
@@ -356,7 +432,7 @@ ddcx_status ddcx_evaluator2d_eval_vjp(const ddcx_evaluator* e1, const ddcx_evalu
         if (nbatch == 0) return;
         require(coef && gcoef, "null data pointer");
         require(npts == 0 || (coords && w && gx), "null data pointer");
-        int const rules[4] = {rule_of(e1->lower), rule_of(e1->upper), rule_of(e2->lower), rule_of(e2->upper)};
+        ddcx::Rules2D const rules = rules2d(e1, e2);
         ddcx::launch_eval2d_vjp(*e1->bs, *e2->bs, rules, coords, w, gx, npts, coef, gcoef, coef_batch_stride, st1,
                                 st2, nbatch, static_cast<hipStream_t>(stream));
     });
@@ -384,7 +460,7 @@ ddcx_status ddcx_evaluator2d_eval_vjp_binned(const ddcx_evaluator* e1, const ddc
         if (nbatch == 0) return;
         require(coef && gcoef && work, "null data pointer");
         require(npts == 0 || (coords && w && gx), "null data pointer");
-        int const rules[4] = {rule_of(e1->lower), rule_of(e1->upper), rule_of(e2->lower), rule_of(e2->upper)};
+        ddcx::Rules2D const rules = rules2d(e1, e2);
         ddcx::launch_eval2d_vjp_binned(*e1->bs, *e2->bs, rules, coords, w, gx, npts, coef, gcoef, coef_batch_stride,
                                        st1, st2, nbatch, work, work_bytes, static_cast<hipStream_t>(stream));
     });

@@ -35,31 +35,6 @@ struct Bin2D {
     int nbins;
 };
 
-// Wrap / rule of one point, as eval2d_vjp_kernel applies them.  Returns false
-// when the Null rule gives 0; sets in1 / in2 to false for a clamped dimension.
-__device__ __forceinline__ bool place2d(BSplinesView const& b1, BSplinesView const& b2, int4 rules, double& x1,
-                                        double& x2, bool& in1, bool& in2)
-{
-    in1 = in2 = true;
-    if (b1.periodic && (x1 < b1.rmin || x1 > b1.rmax)) x1 -= floor((x1 - b1.rmin) / b1.length) * b1.length;
-    if (b2.periodic && (x2 < b2.rmin || x2 > b2.rmax)) x2 -= floor((x2 - b2.rmin) / b2.length) * b2.length;
-    auto clamp2 = [](double v, BSplinesView const& bb, bool& in) {
-        if (v < bb.rmin || v > bb.rmax) in = false;
-        return v < bb.rmin ? bb.rmin : (v > bb.rmax ? bb.rmax : v);
-    };
-    if (!b1.periodic && (x1 < b1.rmin || x1 > b1.rmax)) {
-        if ((x1 < b1.rmin ? rules.x : rules.y) == DDCX_EXTRAP_NULL) return false;
-        x1 = clamp2(x1, b1, in1);
-        if (!b2.periodic) x2 = clamp2(x2, b2, in2);
-    }
-    if (!b2.periodic && (x2 < b2.rmin || x2 > b2.rmax)) {
-        if ((x2 < b2.rmin ? rules.z : rules.w) == DDCX_EXTRAP_NULL) return false;
-        x2 = clamp2(x2, b2, in2);
-        if (!b1.periodic) x1 = clamp2(x1, b1, in1);
-    }
-    return true;
-}
-
 // first B-spline index of the cell holding x (the jmin of eval_basis)
 __device__ __forceinline__ int cell_of(BSplinesView const& b, double x)
 {
@@ -71,7 +46,7 @@ __device__ __forceinline__ int cell_of(BSplinesView const& b, double x)
 }
 
 // tile of point t, or -1 (Null rule: gx = 0 written here when `write_gx`)
-__device__ __forceinline__ int bin_of(BSplinesView const& b1, BSplinesView const& b2, int4 rules, Bin2D const& g,
+__device__ __forceinline__ int bin_of(BSplinesView const& b1, BSplinesView const& b2, Rules2D rules, Bin2D const& g,
                                       const double* coords, double* gx, int64_t t, bool write_gx)
 {
     double x1 = coords[2 * t], x2 = coords[2 * t + 1];
@@ -102,7 +77,7 @@ __global__ void __launch_bounds__(256) zero_patch_kernel(double* __restrict__ g,
     g[l * cbs + (r / s2) * st1 + (r % s2) * st2] = 0.0;
 }
 
-__global__ void __launch_bounds__(kBinT) bin_count_kernel(BSplinesView b1, BSplinesView b2, int4 rules, Bin2D g,
+__global__ void __launch_bounds__(kBinT) bin_count_kernel(BSplinesView b1, BSplinesView b2, Rules2D rules, Bin2D g,
                                                           const double* __restrict__ coords, double* __restrict__ gx,
                                                           int64_t npts, int64_t bpb, int* __restrict__ counts)
 {
@@ -161,7 +136,7 @@ __global__ void __launch_bounds__(1024) bin_scan_kernel(Bin2D g, const int* __re
     if (t == 1023) starts[l * (g.nbins + 1) + g.nbins] = part[1023];
 }
 
-__global__ void __launch_bounds__(kBinT) bin_scatter_kernel(BSplinesView b1, BSplinesView b2, int4 rules, Bin2D g,
+__global__ void __launch_bounds__(kBinT) bin_scatter_kernel(BSplinesView b1, BSplinesView b2, Rules2D rules, Bin2D g,
                                                             const double* __restrict__ coords, int64_t npts,
                                                             int64_t bpb, int* __restrict__ cursor,
                                                             int* __restrict__ perm)
@@ -191,7 +166,7 @@ __global__ void __launch_bounds__(kBinT) bin_scatter_kernel(BSplinesView b1, BSp
 }
 
 template <int D1, int D2>
-__global__ void __launch_bounds__(kTileThreads) bin_tile_kernel(BSplinesView b1, BSplinesView b2, int4 rules, Bin2D g,
+__global__ void __launch_bounds__(kTileThreads) bin_tile_kernel(BSplinesView b1, BSplinesView b2, Rules2D rules, Bin2D g,
                                                                 const double* __restrict__ coords,
                                                                 const double* __restrict__ w, double* __restrict__ gx,
                                                                 int64_t npts, const double* __restrict__ coef,
@@ -296,7 +271,7 @@ int64_t eval2d_vjp_workspace(BSplinesPlan const& bp1, BSplinesPlan const& bp2, i
     return carve(g, npts, nbatch, nullptr).bytes;
 }
 
-void launch_eval2d_vjp_binned(BSplinesPlan const& bp1, BSplinesPlan const& bp2, const int* rules,
+void launch_eval2d_vjp_binned(BSplinesPlan const& bp1, BSplinesPlan const& bp2, Rules2D const& rules,
                               const double* coords, const double* w, double* gx, int64_t npts, const double* coef,
                               double* gcoef, int64_t cbs, int64_t st1, int64_t st2, int64_t nbatch, void* work,
                               int64_t work_bytes, hipStream_t stream)
@@ -309,7 +284,6 @@ void launch_eval2d_vjp_binned(BSplinesPlan const& bp1, BSplinesPlan const& bp2, 
     if (need == 0) throw Error(DDCX_ERR_UNSUPPORTED, "binned 2D adjoint: grid or degree out of range");
     if (work_bytes < need) throw Error(DDCX_ERR_INVALID, "binned 2D adjoint: workspace too small");
     Work2D const ws = carve(g, npts, nbatch, work);
-    int4 const r{rules[0], rules[1], rules[2], rules[3]};
     auto blocks = [](int64_t n, int t) {
         int64_t const b = (n + t - 1) / t;
         if (b > int64_t(0x7fffffff)) throw Error(DDCX_ERR_INVALID, "batch too large for one launch");
@@ -322,11 +296,11 @@ void launch_eval2d_vjp_binned(BSplinesPlan const& bp1, BSplinesPlan const& bp2, 
                        nbatch * g.nbins);
     if (npts > 0) {
         int64_t const bpb = (npts + kBinT * kBinPts - 1) / (kBinT * kBinPts);
-        hipLaunchKernelGGL(bin_count_kernel, dim3(blocks(bpb * nbatch, 1)), dim3(kBinT), 0, stream, b1, b2, r, g,
+        hipLaunchKernelGGL(bin_count_kernel, dim3(blocks(bpb * nbatch, 1)), dim3(kBinT), 0, stream, b1, b2, rules, g,
                            coords, gx, npts, bpb, ws.counts);
         hipLaunchKernelGGL(bin_scan_kernel, dim3(unsigned(nbatch)), dim3(1024), 0, stream, g, ws.counts, ws.starts,
                            ws.cursor);
-        hipLaunchKernelGGL(bin_scatter_kernel, dim3(blocks(bpb * nbatch, 1)), dim3(kBinT), 0, stream, b1, b2, r, g,
+        hipLaunchKernelGGL(bin_scatter_kernel, dim3(blocks(bpb * nbatch, 1)), dim3(kBinT), 0, stream, b1, b2, rules, g,
                            coords, npts, bpb, ws.cursor, ws.perm);
         bool launched = false;
         dispatch_degree(b1.degree, [&]<int D1>() {
@@ -338,7 +312,7 @@ void launch_eval2d_vjp_binned(BSplinesPlan const& bp1, BSplinesPlan const& bp2, 
                                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)),
                               "hipFuncSetAttribute");
                     hipLaunchKernelGGL(kern, dim3(blocks(nbatch * g.nbins, 1)), dim3(kTileThreads), lds, stream, b1,
-                                       b2, r, g, coords, w, gx, npts, coef, gcoef, cbs, st1, st2, ws.starts, ws.perm);
+                                       b2, rules, g, coords, w, gx, npts, coef, gcoef, cbs, st1, st2, ws.starts, ws.perm);
                     launched = true;
                 }
             });

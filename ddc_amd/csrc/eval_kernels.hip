@@ -36,7 +36,7 @@ __device__ __forceinline__ double eval_no_bc(BSplinesView const& b, double x, co
 }
 
 template <int D, int ORDER>
-__global__ void __launch_bounds__(256) eval_kernel(BSplinesView b, int lower, int upper, double* __restrict__ out,
+__global__ void __launch_bounds__(256) eval_kernel(BSplinesView b, Rules1D rules, double* __restrict__ out,
                                                    const double* __restrict__ coords, Layout pl, int64_t npts,
                                                    const double* __restrict__ coef, Layout cl, int64_t nbatch,
                                                    int point_fastest)
@@ -51,7 +51,7 @@ __global__ void __launch_bounds__(256) eval_kernel(BSplinesView b, int lower, in
     double y;
     if constexpr (ORDER == 0) {
         bool outside;
-        y = apply_bc(b, lower, upper, x, outside) ? eval_no_bc<D, 0>(b, x, c, cl.rs) : 0.0;
+        y = apply_bc(b, rules, x, outside) ? eval_no_bc<D, 0>(b, x, c, cl.rs) : 0.0;
     } else {
         y = eval_no_bc<D, ORDER>(b, x, c, cl.rs);
     }
@@ -63,7 +63,7 @@ __global__ void __launch_bounds__(256) eval_kernel(BSplinesView b, int lower, in
 // gradient is scattered with atomics (each line's coefficients are a small,
 // cache-resident array).
 template <int D>
-__global__ void __launch_bounds__(256) eval_vjp_kernel(BSplinesView b, int lower, int upper,
+__global__ void __launch_bounds__(256) eval_vjp_kernel(BSplinesView b, Rules1D rules,
                                                        const double* __restrict__ coords, const double* __restrict__ w,
                                                        double* __restrict__ gx, Layout pl, int64_t npts,
                                                        const double* __restrict__ coef, double* __restrict__ gcoef,
@@ -77,7 +77,7 @@ __global__ void __launch_bounds__(256) eval_vjp_kernel(BSplinesView b, int lower
     double x = coords[o];
     double const wi = w[o];
     bool outside;
-    if (!apply_bc(b, lower, upper, x, outside)) {
+    if (!apply_bc(b, rules, x, outside)) {
         gx[o] = 0.0;
         return;
     }
@@ -117,7 +117,7 @@ __global__ void __launch_bounds__(256) integrate_kernel(const double* __restrict
 
 // SplineEvaluator2D::eval / eval_no_bc, spline_evaluator_2d.hpp:679-825
 template <int D1, int D2, int O1, int O2>
-__global__ void __launch_bounds__(256) eval2d_kernel(BSplinesView b1, BSplinesView b2, int4 rules,
+__global__ void __launch_bounds__(256) eval2d_kernel(BSplinesView b1, BSplinesView b2, Rules2D rules,
                                                      double* __restrict__ out, const double* __restrict__ coords,
                                                      int64_t npts, const double* __restrict__ coef, int64_t cbs,
                                                      int64_t st1, int64_t st2, int64_t nbatch)
@@ -128,30 +128,10 @@ __global__ void __launch_bounds__(256) eval2d_kernel(BSplinesView b1, BSplinesVi
     double x1 = coords[2 * t];
     double x2 = coords[2 * t + 1];
     if constexpr (O1 == 0 && O2 == 0) {
-        if (b1.periodic && (x1 < b1.rmin || x1 > b1.rmax)) x1 -= floor((x1 - b1.rmin) / b1.length) * b1.length;
-        if (b2.periodic && (x2 < b2.rmin || x2 > b2.rmax)) x2 -= floor((x2 - b2.rmin) / b2.length) * b2.length;
-        auto clamp2 = [&](double v, BSplinesView const& bb) { return v < bb.rmin ? bb.rmin : (v > bb.rmax ? bb.rmax : v); };
-        if (!b1.periodic) {
-            if (x1 < b1.rmin || x1 > b1.rmax) {
-                int const rule = x1 < b1.rmin ? rules.x : rules.y;
-                if (rule == DDCX_EXTRAP_NULL) {
-                    out[t] = 0.0;
-                    return;
-                }
-                x1 = x1 < b1.rmin ? b1.rmin : b1.rmax;
-                if (!b2.periodic) x2 = clamp2(x2, b2);
-            }
-        }
-        if (!b2.periodic) {
-            if (x2 < b2.rmin || x2 > b2.rmax) {
-                int const rule = x2 < b2.rmin ? rules.z : rules.w;
-                if (rule == DDCX_EXTRAP_NULL) {
-                    out[t] = 0.0;
-                    return;
-                }
-                x2 = x2 < b2.rmin ? b2.rmin : b2.rmax;
-                if (!b1.periodic) x1 = clamp2(x1, b1);
-            }
+        bool in1, in2;
+        if (!place2d(b1, b2, rules, x1, x2, in1, in2)) {
+            out[t] = 0.0;
+            return;
         }
     }
     double v1[D1 + 1], v2[D2 + 1];
@@ -184,7 +164,7 @@ __global__ void __launch_bounds__(256) eval2d_kernel(BSplinesView b1, BSplinesVi
 // has zero gradient, the Null rule zeroes both), and the coefficient
 // gradient w * B1_i(x1) B2_j(x2) scattered with atomics.
 template <int D1, int D2>
-__global__ void __launch_bounds__(256) eval2d_vjp_kernel(BSplinesView b1, BSplinesView b2, int4 rules,
+__global__ void __launch_bounds__(256) eval2d_vjp_kernel(BSplinesView b1, BSplinesView b2, Rules2D rules,
                                                          const double* __restrict__ coords,
                                                          const double* __restrict__ w, double* __restrict__ gx,
                                                          int64_t npts, const double* __restrict__ coef,
@@ -197,32 +177,11 @@ __global__ void __launch_bounds__(256) eval2d_vjp_kernel(BSplinesView b1, BSplin
     double x1 = coords[2 * t];
     double x2 = coords[2 * t + 1];
     double const wt = w[t];
-    bool in1 = true, in2 = true;
-    if (b1.periodic && (x1 < b1.rmin || x1 > b1.rmax)) x1 -= floor((x1 - b1.rmin) / b1.length) * b1.length;
-    if (b2.periodic && (x2 < b2.rmin || x2 > b2.rmax)) x2 -= floor((x2 - b2.rmin) / b2.length) * b2.length;
-    auto clamp2 = [](double v, BSplinesView const& bb, bool& in) {
-        if (v < bb.rmin || v > bb.rmax) in = false;
-        return v < bb.rmin ? bb.rmin : (v > bb.rmax ? bb.rmax : v);
-    };
-    if (!b1.periodic && (x1 < b1.rmin || x1 > b1.rmax)) {
-        int const rule = x1 < b1.rmin ? rules.x : rules.y;
-        if (rule == DDCX_EXTRAP_NULL) {
-            gx[2 * t] = 0.0;
-            gx[2 * t + 1] = 0.0;
-            return;
-        }
-        x1 = clamp2(x1, b1, in1);
-        if (!b2.periodic) x2 = clamp2(x2, b2, in2);
-    }
-    if (!b2.periodic && (x2 < b2.rmin || x2 > b2.rmax)) {
-        int const rule = x2 < b2.rmin ? rules.z : rules.w;
-        if (rule == DDCX_EXTRAP_NULL) {
-            gx[2 * t] = 0.0;
-            gx[2 * t + 1] = 0.0;
-            return;
-        }
-        x2 = clamp2(x2, b2, in2);
-        if (!b1.periodic) x1 = clamp2(x1, b1, in1);
+    bool in1, in2;
+    if (!place2d(b1, b2, rules, x1, x2, in1, in2)) {
+        gx[2 * t] = 0.0;
+        gx[2 * t + 1] = 0.0;
+        return;
     }
     double e1[(D1 + 1) * 2], e2[(D2 + 1) * 2];
     int const j1 = eval_basis_and_n_derivs<D1, 1, true>(b1, b1.knots, x1, e1);
@@ -262,27 +221,27 @@ unsigned grid_for(int64_t n, int threads)
 }
 
 template <int D, int O>
-void eval_launch(BSplinesView const& b, int lower, int upper, double* out, const double* coords, Layout pl,
+void eval_launch(BSplinesView const& b, Rules1D rules, double* out, const double* coords, Layout pl,
                  int64_t npts, const double* coef, Layout cl, int64_t nbatch, hipStream_t stream)
 {
     int const point_fastest = pl.rs == 1 ? 1 : 0;
-    hipLaunchKernelGGL((eval_kernel<D, O>), dim3(grid_for(npts * nbatch, 256)), dim3(256), 0, stream, b, lower, upper,
+    hipLaunchKernelGGL((eval_kernel<D, O>), dim3(grid_for(npts * nbatch, 256)), dim3(256), 0, stream, b, rules,
                        out, coords, pl, npts, coef, cl, nbatch, point_fastest);
 }
 
 template <int D>
-void eval_order(int order, BSplinesView const& b, int lower, int upper, double* out, const double* coords, Layout pl,
+void eval_order(int order, BSplinesView const& b, Rules1D rules, double* out, const double* coords, Layout pl,
                 int64_t npts, const double* coef, Layout cl, int64_t nbatch, hipStream_t stream)
 {
     switch (order) {
-    case 0: eval_launch<D, 0>(b, lower, upper, out, coords, pl, npts, coef, cl, nbatch, stream); return;
-    case 1: eval_launch<D, 1>(b, lower, upper, out, coords, pl, npts, coef, cl, nbatch, stream); return;
-    case 2: if constexpr (D >= 2) { eval_launch<D, 2>(b, lower, upper, out, coords, pl, npts, coef, cl, nbatch, stream); return; } break;
-    case 3: if constexpr (D >= 3) { eval_launch<D, 3>(b, lower, upper, out, coords, pl, npts, coef, cl, nbatch, stream); return; } break;
-    case 4: if constexpr (D >= 4) { eval_launch<D, 4>(b, lower, upper, out, coords, pl, npts, coef, cl, nbatch, stream); return; } break;
-    case 5: if constexpr (D >= 5) { eval_launch<D, 5>(b, lower, upper, out, coords, pl, npts, coef, cl, nbatch, stream); return; } break;
-    case 6: if constexpr (D >= 6) { eval_launch<D, 6>(b, lower, upper, out, coords, pl, npts, coef, cl, nbatch, stream); return; } break;
-    case 7: if constexpr (D >= 7) { eval_launch<D, 7>(b, lower, upper, out, coords, pl, npts, coef, cl, nbatch, stream); return; } break;
+    case 0: eval_launch<D, 0>(b, rules, out, coords, pl, npts, coef, cl, nbatch, stream); return;
+    case 1: eval_launch<D, 1>(b, rules, out, coords, pl, npts, coef, cl, nbatch, stream); return;
+    case 2: if constexpr (D >= 2) { eval_launch<D, 2>(b, rules, out, coords, pl, npts, coef, cl, nbatch, stream); return; } break;
+    case 3: if constexpr (D >= 3) { eval_launch<D, 3>(b, rules, out, coords, pl, npts, coef, cl, nbatch, stream); return; } break;
+    case 4: if constexpr (D >= 4) { eval_launch<D, 4>(b, rules, out, coords, pl, npts, coef, cl, nbatch, stream); return; } break;
+    case 5: if constexpr (D >= 5) { eval_launch<D, 5>(b, rules, out, coords, pl, npts, coef, cl, nbatch, stream); return; } break;
+    case 6: if constexpr (D >= 6) { eval_launch<D, 6>(b, rules, out, coords, pl, npts, coef, cl, nbatch, stream); return; } break;
+    case 7: if constexpr (D >= 7) { eval_launch<D, 7>(b, rules, out, coords, pl, npts, coef, cl, nbatch, stream); return; } break;
     default: break;
     }
     throw Error(DDCX_ERR_INVALID, "derivative order must be in [1, degree]");
@@ -290,26 +249,26 @@ void eval_order(int order, BSplinesView const& b, int lower, int upper, double* 
 
 } // namespace
 
-void launch_eval(BSplinesPlan const& bs, int lower, int upper, int order, double* out, const double* coords,
+void launch_eval(BSplinesPlan const& bs, Rules1D rules, int order, double* out, const double* coords,
                  Layout pl, int64_t npts, const double* coef, Layout cl, int64_t nbatch, hipStream_t stream)
 {
     if (npts * nbatch == 0) return;
     bs.ensure_device();
-    if (launch_eval_tiled(bs, lower, upper, order, out, coords, pl, npts, coef, cl, nbatch, stream)) return;
+    if (launch_eval_tiled(bs, rules, order, out, coords, pl, npts, coef, cl, nbatch, stream)) return;
     BSplinesView const b = bs.device_view();
     bool const ok = dispatch_degree(b.degree, [&]<int D>() {
-        eval_order<D>(order, b, lower, upper, out, coords, pl, npts, coef, cl, nbatch, stream);
+        eval_order<D>(order, b, rules, out, coords, pl, npts, coef, cl, nbatch, stream);
     });
     if (!ok) throw Error(DDCX_ERR_UNSUPPORTED, "degree out of range");
     hip_check(hipGetLastError(), "eval_kernel launch");
 }
 
-void launch_eval_vjp(BSplinesPlan const& bs, int lower, int upper, const double* coords, const double* w, double* gx,
+void launch_eval_vjp(BSplinesPlan const& bs, Rules1D rules, const double* coords, const double* w, double* gx,
                      Layout pl, int64_t npts, const double* coef, double* gcoef, Layout cl, int64_t nbatch,
                      hipStream_t stream)
 {
     bs.ensure_device();
-    if (launch_eval_vjp_tiled(bs, lower, upper, coords, w, gx, pl, npts, coef, gcoef, cl, nbatch, stream)) return;
+    if (launch_eval_vjp_tiled(bs, rules, coords, w, gx, pl, npts, coef, gcoef, cl, nbatch, stream)) return;
     BSplinesView const b = bs.device_view();
     int const size = bs.size();
     if (int64_t(size) * nbatch > 0)
@@ -318,8 +277,8 @@ void launch_eval_vjp(BSplinesPlan const& bs, int lower, int upper, const double*
     if (npts * nbatch == 0) return;
     int const point_fastest = pl.rs == 1 ? 1 : 0;
     bool const ok = dispatch_degree(b.degree, [&]<int D>() {
-        hipLaunchKernelGGL(eval_vjp_kernel<D>, dim3(grid_for(npts * nbatch, 256)), dim3(256), 0, stream, b, lower,
-                           upper, coords, w, gx, pl, npts, coef, gcoef, cl, nbatch, point_fastest);
+        hipLaunchKernelGGL(eval_vjp_kernel<D>, dim3(grid_for(npts * nbatch, 256)), dim3(256), 0, stream, b, rules,
+                           coords, w, gx, pl, npts, coef, gcoef, cl, nbatch, point_fastest);
     });
     if (!ok) throw Error(DDCX_ERR_UNSUPPORTED, "degree out of range");
     hip_check(hipGetLastError(), "eval_vjp_kernel launch");
@@ -335,7 +294,7 @@ void launch_integrate(BSplinesPlan const& bs, double* out, int64_t ostride, cons
     hip_check(hipGetLastError(), "integrate_kernel launch");
 }
 
-void launch_eval2d(BSplinesPlan const& bp1, BSplinesPlan const& bp2, const int* rules, int d1, int d2, double* out,
+void launch_eval2d(BSplinesPlan const& bp1, BSplinesPlan const& bp2, Rules2D const& rules, int d1, int d2, double* out,
                    const double* coords, int64_t npts, const double* coef, int64_t cbs, int64_t st1, int64_t st2,
                    int64_t nbatch, hipStream_t stream)
 {
@@ -343,12 +302,11 @@ void launch_eval2d(BSplinesPlan const& bp1, BSplinesPlan const& bp2, const int* 
     bp1.ensure_device();
     bp2.ensure_device();
     BSplinesView const b1 = bp1.device_view(), b2 = bp2.device_view();
-    int4 const r{rules[0], rules[1], rules[2], rules[3]};
     unsigned const grid = grid_for(npts * nbatch, 256);
     bool launched = false;
     auto go = [&]<int D1, int D2>() {
         auto ord = [&]<int O1, int O2>() {
-            hipLaunchKernelGGL((eval2d_kernel<D1, D2, O1, O2>), dim3(grid), dim3(256), 0, stream, b1, b2, r, out,
+            hipLaunchKernelGGL((eval2d_kernel<D1, D2, O1, O2>), dim3(grid), dim3(256), 0, stream, b1, b2, rules, out,
                                coords, npts, coef, cbs, st1, st2, nbatch);
             launched = true;
         };
@@ -390,7 +348,7 @@ void launch_eval2d(BSplinesPlan const& bp1, BSplinesPlan const& bp2, const int* 
     hip_check(hipGetLastError(), "eval2d_kernel launch");
 }
 
-void launch_eval2d_vjp(BSplinesPlan const& bp1, BSplinesPlan const& bp2, const int* rules, const double* coords,
+void launch_eval2d_vjp(BSplinesPlan const& bp1, BSplinesPlan const& bp2, Rules2D const& rules, const double* coords,
                        const double* w, double* gx, int64_t npts, const double* coef, double* gcoef, int64_t cbs,
                        int64_t st1, int64_t st2, int64_t nbatch, hipStream_t stream)
 {
@@ -402,11 +360,10 @@ void launch_eval2d_vjp(BSplinesPlan const& bp1, BSplinesPlan const& bp2, const i
                        0, stream, gcoef, bp1.size(), bp2.size(), cbs, st1, st2, nbatch);
     hip_check(hipGetLastError(), "zero_coef2d_kernel launch");
     if (npts == 0) return;
-    int4 const r{rules[0], rules[1], rules[2], rules[3]};
     unsigned const grid = grid_for(npts * nbatch, 256);
     bool launched = false;
     auto go = [&]<int D1, int D2>() {
-        hipLaunchKernelGGL((eval2d_vjp_kernel<D1, D2>), dim3(grid), dim3(256), 0, stream, b1, b2, r, coords, w, gx,
+        hipLaunchKernelGGL((eval2d_vjp_kernel<D1, D2>), dim3(grid), dim3(256), 0, stream, b1, b2, rules, coords, w, gx,
                            npts, coef, gcoef, cbs, st1, st2, nbatch);
         launched = true;
     };

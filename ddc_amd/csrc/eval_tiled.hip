@@ -27,7 +27,7 @@ __device__ __forceinline__ int64_t lay(Layout const& L, int64_t r, int64_t l)
 }
 
 template <int D, int ORDER, int KIND>
-__device__ __forceinline__ double point_value(BSplinesView const& b, const double* kn, int lower, int upper, double x,
+__device__ __forceinline__ double point_value(BSplinesView const& b, const double* kn, Rules1D rules, double x,
                                               const double* tile, int pitch)
 {
     double v[D + 1];
@@ -36,11 +36,11 @@ __device__ __forceinline__ double point_value(BSplinesView const& b, const doubl
         if (b.periodic) {
             if (x < b.rmin || x > b.rmax) x -= floor((x - b.rmin) / b.length) * b.length;
         } else if (x < b.rmin) {
-            if (lower == DDCX_EXTRAP_NULL) return 0.0;
-            x = b.rmin;
+            if (rules.lower == DDCX_EXTRAP_NULL) return 0.0;
+            x = rules.pos_lower;
         } else if (x > b.rmax) {
-            if (upper == DDCX_EXTRAP_NULL) return 0.0;
-            x = b.rmax;
+            if (rules.upper == DDCX_EXTRAP_NULL) return 0.0;
+            x = rules.pos_upper;
         }
         jmin = eval_basis<D, true, KIND>(b, kn, x, v);
     } else {
@@ -56,7 +56,7 @@ __device__ __forceinline__ double point_value(BSplinesView const& b, const doubl
 }
 
 template <int D, int ORDER, int LB, int NT, bool LINE_MAJOR, int KIND>
-__global__ void __launch_bounds__(NT) eval_tiled_kernel(BSplinesView b, int lower, int upper, double* __restrict__ out,
+__global__ void __launch_bounds__(NT) eval_tiled_kernel(BSplinesView b, Rules1D rules, double* __restrict__ out,
                                                         const double* __restrict__ coords, Layout pl, int64_t npts,
                                                         const double* __restrict__ coef, Layout cl, int64_t nbatch,
                                                         int ncoef, int pitch, int nknots)
@@ -149,7 +149,7 @@ __global__ void __launch_bounds__(NT) eval_tiled_kernel(BSplinesView b, int lowe
         for (int u = 0; u < U; ++u) xb[u] = __builtin_nontemporal_load(&cp[at(i + (U + u) * ROWS)]);
         double y[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) y[u] = point_value<D, ORDER, KIND>(bl, knots, lower, upper, xa[u], tl, pitch);
+        for (int u = 0; u < U; ++u) y[u] = point_value<D, ORDER, KIND>(bl, knots, rules, xa[u], tl, pitch);
 #pragma unroll
         for (int u = 0; u < U; ++u) __builtin_nontemporal_store(y[u], &op[at(i + u * ROWS)]);
 #pragma unroll
@@ -199,7 +199,7 @@ __device__ __forceinline__ void buffer_store(double v, __amdgpu_buffer_rsrc_t r,
 // repeat the last row, and lines past the batch repeat the last line.
 // Addresses are 32-bit byte offsets into buffers of at most 4 GiB.
 template <int D, int KIND, int NT, int U, int NST, int PR>
-__global__ void __launch_bounds__(NT) eval_stream_kernel(BSplinesView b, int lower, int upper,
+__global__ void __launch_bounds__(NT) eval_stream_kernel(BSplinesView b, Rules1D rules,
                                                            double* __restrict__ out,
                                                            const double* __restrict__ coords, uint32_t prs,
                                                            uint32_t pbytes, const double* __restrict__ coef,
@@ -288,7 +288,7 @@ __global__ void __launch_bounds__(NT) eval_stream_kernel(BSplinesView b, int low
             for (int j = s * PS; j < (s + 1) * PS && j < PR; ++j) pf[j] = buffer_load<kNonTemporal>(rkn, cr(j, ltn));
             double y[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) y[u] = point_value<D, 0, KIND>(bl, knots, lower, upper, xa[u], tl, pitch);
+            for (int u = 0; u < U; ++u) y[u] = point_value<D, 0, KIND>(bl, knots, rules, xa[u], tl, pitch);
 #pragma unroll
             for (int u = 0; u < U; ++u) buffer_store<kNonTemporal>(y[u], ro, pt(s * U * ROWS + prow + u * ROWS, lt));
 #pragma unroll
@@ -313,7 +313,7 @@ __global__ void __launch_bounds__(NT) eval_stream_kernel(BSplinesView b, int low
 // coefficient gradient of the LB lines accumulated in an LDS tile with LDS
 // atomics, then written once (every row, so no separate zeroing).
 template <int D, int LB, int NT, bool LINE_MAJOR>
-__global__ void __launch_bounds__(NT) eval_vjp_tiled_kernel(BSplinesView b, int lower, int upper,
+__global__ void __launch_bounds__(NT) eval_vjp_tiled_kernel(BSplinesView b, Rules1D rules,
                                                             const double* __restrict__ coords,
                                                             const double* __restrict__ w, double* __restrict__ gx,
                                                             Layout pl, int64_t npts, const double* __restrict__ coef,
@@ -397,7 +397,7 @@ __global__ void __launch_bounds__(NT) eval_vjp_tiled_kernel(BSplinesView b, int 
                 if (ii >= npts) continue;
                 double x = xa[u];
                 bool outside;
-                if (!apply_bc(bl, lower, upper, x, outside)) {
+                if (!apply_bc(bl, rules, x, outside)) {
                     gx[pb + ii * pl.rs] = 0.0;
                     continue;
                 }
@@ -445,7 +445,7 @@ int cu_count()
 // line), a uniform basis, npts <= 1024, size() <= 1056, buffers below 4 GiB
 // including the offsets of the point slots past the end of a line.
 template <int D>
-bool launch_stream(BSplinesPlan const& bs, int lower, int upper, double* out, const double* coords, Layout pl,
+bool launch_stream(BSplinesPlan const& bs, Rules1D rules, double* out, const double* coords, Layout pl,
                    int64_t npts, const double* coef, Layout cl, int64_t nbatch, hipStream_t stream)
 {
     int const ncoef = bs.size();
@@ -478,7 +478,7 @@ bool launch_stream(BSplinesPlan const& bs, int lower, int upper, double* out, co
         hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)),
                   "hipFuncSetAttribute");
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), lds, stream, b, lower, upper, out, coords, uint32_t(pl.rs),
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), lds, stream, b, rules, out, coords, uint32_t(pl.rs),
                            pbytes, coef, uint32_t(cl.rs), cbytes, nbatch, ncoef, pitch, nknots, ntiles, int(npts));
         hip_check(hipGetLastError(), "eval_stream_kernel launch");
         return true;
@@ -487,7 +487,7 @@ bool launch_stream(BSplinesPlan const& bs, int lower, int upper, double* out, co
 }
 
 template <int D, int ORDER>
-bool launch_orders(BSplinesPlan const& bs, int lower, int upper, double* out, const double* coords, Layout pl,
+bool launch_orders(BSplinesPlan const& bs, Rules1D rules, double* out, const double* coords, Layout pl,
                    int64_t npts, const double* coef, Layout cl, int64_t nbatch, hipStream_t stream)
 {
     int const ncoef = bs.size();
@@ -506,14 +506,14 @@ bool launch_orders(BSplinesPlan const& bs, int lower, int upper, double* out, co
                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)),
                   "hipFuncSetAttribute");
         int64_t const blocks = (nbatch + LB - 1) / LB;
-        hipLaunchKernelGGL(kern, dim3(unsigned(blocks)), dim3(NT), lds, stream, b, lower, upper, out, coords, pl, npts,
+        hipLaunchKernelGGL(kern, dim3(unsigned(blocks)), dim3(NT), lds, stream, b, rules, out, coords, pl, npts,
                            coef, cl, nbatch, ncoef, pitch, nknots);
         hip_check(hipGetLastError(), "eval_tiled_kernel launch");
         return true;
     };
     if (ncoef > 2560) return false;
     if constexpr (ORDER == 0) {
-        if (launch_stream<D>(bs, lower, upper, out, coords, pl, npts, coef, cl, nbatch, stream)) return true;
+        if (launch_stream<D>(bs, rules, out, coords, pl, npts, coef, cl, nbatch, stream)) return true;
     }
     if (fits(16, 17)) return go.template operator()<16, 1024>(17);
     if (fits(16, 16)) return go.template operator()<16, 1024>(16);
@@ -524,7 +524,7 @@ bool launch_orders(BSplinesPlan const& bs, int lower, int upper, double* out, co
 
 } // namespace
 
-bool launch_eval_tiled(BSplinesPlan const& bs, int lower, int upper, int order, double* out, const double* coords,
+bool launch_eval_tiled(BSplinesPlan const& bs, Rules1D rules, int order, double* out, const double* coords,
                        Layout pl, int64_t npts, const double* coef, Layout cl, int64_t nbatch, hipStream_t stream)
 {
     // tiny batches, and mixed layouts (points and coefficients not both
@@ -538,7 +538,7 @@ bool launch_eval_tiled(BSplinesPlan const& bs, int lower, int upper, int order, 
         auto ord = [&]<int O>() {
             if constexpr (O <= D) {
                 if (!launched && order == O)
-                    launched = launch_orders<D, O>(bs, lower, upper, out, coords, pl, npts, coef, cl, nbatch, stream);
+                    launched = launch_orders<D, O>(bs, rules, out, coords, pl, npts, coef, cl, nbatch, stream);
             }
         };
         ord.template operator()<0>();
@@ -549,7 +549,7 @@ bool launch_eval_tiled(BSplinesPlan const& bs, int lower, int upper, int order, 
     return launched;
 }
 
-bool launch_eval_vjp_tiled(BSplinesPlan const& bs, int lower, int upper, const double* coords, const double* w,
+bool launch_eval_vjp_tiled(BSplinesPlan const& bs, Rules1D rules, const double* coords, const double* w,
                            double* gx, Layout pl, int64_t npts, const double* coef, double* gcoef, Layout cl,
                            int64_t nbatch, hipStream_t stream)
 {
@@ -575,7 +575,7 @@ bool launch_eval_vjp_tiled(BSplinesPlan const& bs, int lower, int upper, const d
                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)),
                   "hipFuncSetAttribute");
         int64_t const blocks = (nbatch + LB - 1) / LB;
-        hipLaunchKernelGGL(kern, dim3(unsigned(blocks)), dim3(NT), lds, stream, b, lower, upper, coords, w, gx, pl,
+        hipLaunchKernelGGL(kern, dim3(unsigned(blocks)), dim3(NT), lds, stream, b, rules, coords, w, gx, pl,
                            npts, coef, gcoef, cl, nbatch, ncoef, pitch, nknots);
         hip_check(hipGetLastError(), "eval_vjp_tiled_kernel launch");
         launched = true;

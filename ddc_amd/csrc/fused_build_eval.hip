@@ -17,7 +17,7 @@ using namespace tiled;
 
 template <int LB, int K, int L, int KF, int KB, bool FDIV, bool BDIVF, int D>
 __global__ void __launch_bounds__(LB* K) fused_build_eval_kernel(TiledView tv, BuildView bv, BSplinesView b,
-                                                                int lower, int upper, const double* __restrict__ vals,
+                                                                Rules1D rules, const double* __restrict__ vals,
                                                                 Layout vl, const double* __restrict__ dl,
                                                                 const double* __restrict__ dr, Layout dlay,
                                                                 double* __restrict__ coef, Layout cl,
@@ -195,7 +195,7 @@ __global__ void __launch_bounds__(LB* K) fused_build_eval_kernel(TiledView tv, B
                 double x = xa[u];
                 bool outside;
                 double yv = 0.0;
-                if (apply_bc(b, lower, upper, x, outside)) {
+                if (apply_bc(b, rules, x, outside)) {
                     double v[D + 1];
                     int const jmin = eval_basis<D, true>(b, b.knots, x, v);
 #pragma unroll
@@ -210,7 +210,7 @@ __global__ void __launch_bounds__(LB* K) fused_build_eval_kernel(TiledView tv, B
 }
 
 template <int K, int KF, int KB, bool FDIV, bool BDIVF, int D>
-void launch_one(BuilderPlan const& p, BSplinesView const& b, int lower, int upper, double* coef, Layout cl,
+void launch_one(BuilderPlan const& p, BSplinesView const& b, Rules1D rules, double* coef, Layout cl,
                 const double* vals, Layout vl, const double* dl, const double* dr, Layout dlay, double* out,
                 const double* coords, Layout pl, int64_t npts, int64_t nbatch, hipStream_t stream, size_t lds,
                 const double* fpts, const double* fshift)
@@ -221,13 +221,13 @@ void launch_one(BuilderPlan const& p, BSplinesView const& b, int lower, int uppe
                                   int(lds)),
               "hipFuncSetAttribute");
     int64_t const blocks = (nbatch + LB - 1) / LB;
-    hipLaunchKernelGGL(kern, dim3(unsigned(blocks)), dim3(NT), lds, stream, p.tview_f, p.view, b, lower, upper, vals,
+    hipLaunchKernelGGL(kern, dim3(unsigned(blocks)), dim3(NT), lds, stream, p.tview_f, p.view, b, rules, vals,
                        vl, dl, dr, dlay, coef, cl, out, coords, pl, npts, nbatch, fpts, fshift);
     hip_check(hipGetLastError(), "fused_build_eval_kernel launch");
 }
 
 template <int K>
-bool launch_K(BuilderPlan const& p, BSplinesView const& b, int lower, int upper, double* coef, Layout cl,
+bool launch_K(BuilderPlan const& p, BSplinesView const& b, Rules1D rules, double* coef, Layout cl,
               const double* vals, Layout vl, const double* dl, const double* dr, Layout dlay, double* out,
               const double* coords, Layout pl, int64_t npts, int64_t nbatch, hipStream_t stream, size_t lds,
               const double* fpts, const double* fshift)
@@ -237,7 +237,7 @@ bool launch_K(BuilderPlan const& p, BSplinesView const& b, int lower, int upper,
     bool done = false;
     auto go = [&]<int KF, int KB, bool FD, bool BD, int D>() {
         if (!done && deg == D) {
-            launch_one<K, KF, KB, FD, BD, D>(p, b, lower, upper, coef, cl, vals, vl, dl, dr, dlay, out, coords, pl,
+            launch_one<K, KF, KB, FD, BD, D>(p, b, rules, coef, cl, vals, vl, dl, dr, dlay, out, coords, pl,
                                             npts, nbatch, stream, lds, fpts, fshift);
             done = true;
         }
@@ -267,7 +267,7 @@ bool launch_K(BuilderPlan const& p, BSplinesView const& b, int lower, int upper,
 
 } // namespace
 
-bool launch_build_eval_fused(BuilderPlan const& p, BSplinesPlan const& bs, int lower, int upper, double* coef,
+bool launch_build_eval_fused(BuilderPlan const& p, BSplinesPlan const& bs, Rules1D rules, double* coef,
                              Layout cl, const double* vals, Layout vl, const double* dl, const double* dr,
                              Layout dlay, double* out, const double* coords, Layout pl, int64_t npts, int64_t nbatch,
                              hipStream_t stream, const double* feet_shift)
@@ -290,9 +290,9 @@ bool launch_build_eval_fused(BuilderPlan const& p, BSplinesPlan const& bs, int l
     if (lds + stat > size_t(160 * 1024)) return false;
     BSplinesView const b = bs.device_view();
     if (K == 64)
-        return launch_K<64>(p, b, lower, upper, coef, cl, vals, vl, dl, dr, dlay, out, coords, pl, npts, nbatch,
+        return launch_K<64>(p, b, rules, coef, cl, vals, vl, dl, dr, dlay, out, coords, pl, npts, nbatch,
                             stream, lds, fpts, feet_shift);
-    return launch_K<128>(p, b, lower, upper, coef, cl, vals, vl, dl, dr, dlay, out, coords, pl, npts, nbatch, stream,
+    return launch_K<128>(p, b, rules, coef, cl, vals, vl, dl, dr, dlay, out, coords, pl, npts, nbatch, stream,
                          lds, fpts, feet_shift);
 }
 

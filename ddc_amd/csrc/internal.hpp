@@ -320,11 +320,6 @@ constexpr int kMaxGBlocksVjp = 8;
 void make_tiled(BuilderPlan& p);
 void make_refine(BuilderPlan& p);
 
-struct EvaluatorPlan {
-    std::shared_ptr<BSplinesPlan> bs;
-    ddcx_extrap lower, upper;
-};
-
 // Batched layout: element (row r, line l) at r*rs + (l % inner)*is + (l / inner)*os.
 struct Layout {
     int64_t rs, inner, is, os;
@@ -356,7 +351,7 @@ bool launch_build_vjp_tiled(BuilderPlan const& p, const double* gcoef, Layout cl
 // With feet_shift (one value per line), coords is not read: point i of line l
 // is the interpolation point x_i minus feet_shift[l], and npts is the
 // builder's number of interpolation points.
-bool launch_build_eval_fused(BuilderPlan const& p, BSplinesPlan const& bs, int lower, int upper, double* coef,
+bool launch_build_eval_fused(BuilderPlan const& p, BSplinesPlan const& bs, Rules1D rules, double* coef,
                              Layout cl, const double* vals, Layout vl, const double* dl, const double* dr,
                              Layout dlay, double* out, const double* coords, Layout pl, int64_t npts, int64_t nbatch,
                              hipStream_t stream, const double* feet_shift = nullptr);
@@ -369,14 +364,14 @@ void launch_problem_solve(SolveView const& sv, double* b, int64_t ldb, int64_t n
                           hipStream_t stream);
 void launch_build_vjp(BuilderPlan const& p, const double* gcoef, Layout cl, double* gvals, Layout vl, double* gdl,
                       double* gdr, Layout dlay, int64_t nbatch, hipStream_t stream);
-bool launch_eval_tiled(BSplinesPlan const& bs, int lower, int upper, int order, double* out, const double* coords,
+bool launch_eval_tiled(BSplinesPlan const& bs, Rules1D rules, int order, double* out, const double* coords,
                        Layout pl, int64_t npts, const double* coef, Layout cl, int64_t nbatch, hipStream_t stream);
-bool launch_eval_vjp_tiled(BSplinesPlan const& bs, int lower, int upper, const double* coords, const double* w,
+bool launch_eval_vjp_tiled(BSplinesPlan const& bs, Rules1D rules, const double* coords, const double* w,
                            double* gx, Layout pl, int64_t npts, const double* coef, double* gcoef, Layout cl,
                            int64_t nbatch, hipStream_t stream);
-void launch_eval(BSplinesPlan const& bs, int lower, int upper, int order, double* out, const double* coords,
+void launch_eval(BSplinesPlan const& bs, Rules1D rules, int order, double* out, const double* coords,
                  Layout pl, int64_t npts, const double* coef, Layout cl, int64_t nbatch, hipStream_t stream);
-void launch_eval_vjp(BSplinesPlan const& bs, int lower, int upper, const double* coords, const double* w,
+void launch_eval_vjp(BSplinesPlan const& bs, Rules1D rules, const double* coords, const double* w,
                      double* gx, Layout pl, int64_t npts, const double* coef, double* gcoef, Layout cl,
                      int64_t nbatch, hipStream_t stream);
 void launch_integrate(BSplinesPlan const& bs, double* out, int64_t ostride, const double* coef, Layout cl,
@@ -388,16 +383,16 @@ void launch_evalnd(BSplinesPlan const* const* bs, int nd, const int* orders, con
                    const int64_t* st, int64_t nbatch, hipStream_t stream);
 void launch_integratend(BSplinesPlan const* const* bs, int nd, double* out, int64_t ostride, const double* coef,
                         int64_t cbs, const int64_t* st, int64_t nbatch, hipStream_t stream);
-void launch_eval2d(BSplinesPlan const& b1, BSplinesPlan const& b2, const int* rules, int d1, int d2, double* out,
+void launch_eval2d(BSplinesPlan const& b1, BSplinesPlan const& b2, Rules2D const& rules, int d1, int d2, double* out,
                    const double* coords, int64_t npts, const double* coef, int64_t coef_batch_stride, int64_t st1,
                    int64_t st2, int64_t nbatch, hipStream_t stream);
 // binned 2D adjoint (eval2d_binned.hip): workspace bytes, 0 if unsupported
 int64_t eval2d_vjp_workspace(BSplinesPlan const& bp1, BSplinesPlan const& bp2, int64_t npts, int64_t nbatch);
-void launch_eval2d_vjp_binned(BSplinesPlan const& bp1, BSplinesPlan const& bp2, const int* rules,
+void launch_eval2d_vjp_binned(BSplinesPlan const& bp1, BSplinesPlan const& bp2, Rules2D const& rules,
                               const double* coords, const double* w, double* gx, int64_t npts, const double* coef,
                               double* gcoef, int64_t cbs, int64_t st1, int64_t st2, int64_t nbatch, void* work,
                               int64_t work_bytes, hipStream_t stream);
-void launch_eval2d_vjp(BSplinesPlan const& b1, BSplinesPlan const& b2, const int* rules, const double* coords,
+void launch_eval2d_vjp(BSplinesPlan const& b1, BSplinesPlan const& b2, Rules2D const& rules, const double* coords,
                        const double* w, double* gx, int64_t npts, const double* coef, double* gcoef,
                        int64_t coef_batch_stride, int64_t st1, int64_t st2, int64_t nbatch, hipStream_t stream);
 

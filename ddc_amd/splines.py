@@ -21,7 +21,8 @@ import enum
 import numpy as np
 import torch
 
-from ._lib import (call, ddcx_builder_desc, ddcx_builder_info, ddcx_bsplines_desc, ddcx_layout, lib)
+from ._lib import (call, ddcx_builder_desc, ddcx_builder_info, ddcx_bsplines_desc, ddcx_extrap_rule, ddcx_layout,
+                   lib)
 
 
 class SplineBuilderClosure(enum.IntEnum):
@@ -45,25 +46,54 @@ class Precision(enum.IntEnum):
     FP32_REFINE = 1
 
 
+_NAN = float("nan")
+
+
 class NullExtrapolationRule:
     """null_extrapolation_rule.hpp:14-26: 0 outside the domain."""
 
     code = 0
 
+    def _desc(self):
+        return ddcx_extrap_rule(self.code, _NAN, _NAN, _NAN)
+
 
 class ConstantExtrapolationRule:
-    """constant_extrapolation_rule.hpp:27-73: value at the boundary outside the domain."""
+    """ddc::ConstantExtrapolationRule (constant_extrapolation_rule.hpp:27-195):
+    outside the domain the spline takes its value at ``eval_pos``, a
+    coordinate inside the domain (ConstantExtrapolationRule<DimI>(eval_pos),
+    :39-40).  In a SplineEvaluator2D, the two-dimensional form
+    ConstantExtrapolationRule<DimI, DimNI>(eval_pos, min, max) (:99-124) also
+    clamps the other coordinate to [eval_pos_not_interest_min,
+    eval_pos_not_interest_max] when the other dimension is not periodic.
+    ``eval_pos=None`` evaluates at the bound the rule guards (rmin for a lower
+    rule, rmax for an upper one); a None pair clamps to the other dimension's
+    domain."""
 
     code = 1
 
-    def __init__(self, eval_pos=None):
+    def __init__(self, eval_pos=None, eval_pos_not_interest_min=None, eval_pos_not_interest_max=None):
+        if (eval_pos_not_interest_min is None) != (eval_pos_not_interest_max is None):
+            raise ValueError("give both eval_pos_not_interest_min and eval_pos_not_interest_max, or neither")
         self.eval_pos = eval_pos
+        self.eval_pos_not_interest_min = eval_pos_not_interest_min
+        self.eval_pos_not_interest_max = eval_pos_not_interest_max
+
+    def _desc(self):
+        def f(v):
+            return _NAN if v is None else float(v)
+
+        return ddcx_extrap_rule(self.code, f(self.eval_pos), f(self.eval_pos_not_interest_min),
+                                f(self.eval_pos_not_interest_max))
 
 
 class PeriodicExtrapolationRule:
     """periodic_extrapolation_rule.hpp:12-24 (the periodic wrap applies instead)."""
 
     code = 2
+
+    def _desc(self):
+        return ddcx_extrap_rule(self.code, _NAN, _NAN, _NAN)
 
 
 def n_boundary_equations(bc, degree):
@@ -263,37 +293,55 @@ class SplineBuilder:
     def offset(self):
         return self.info.offset
 
+    def _layouts(self, spline, vals, derivs_xmin, derivs_xmax, axis, names=("spline", "vals", "derivs_xmin",
+                                                                             "derivs_xmax")):
+        """Checks the arrays of one build (or its adjoint) and returns
+        (coef layout, vals layout, derivs layout, nbatch).  Every check runs
+        before any launch: the kernels index every row of every line.
+        spline may be None (coefficients kept on chip by the fused kernels)."""
+        _check_f64_cuda(vals, names[1])
+        if vals.shape[axis] != self.info.npoints:
+            raise ValueError(f"{names[1]} has {vals.shape[axis]} interpolation points, expected {self.info.npoints}")
+        vl, nbatch = batched_layout(vals, axis)
+        cl = ddcx_layout(1, 1, 0, 0)
+        if spline is not None:
+            _check_f64_cuda(spline, names[0])
+            if vals.dim() != spline.dim():
+                raise ValueError(f"{names[0]} and {names[1]} must have the same number of dimensions")
+            if spline.shape[axis] != self.info.size:
+                raise ValueError(f"{names[0]} has {spline.shape[axis]} coefficients, expected {self.info.size}")
+            cl, nb2 = batched_layout(spline, axis)
+            if nb2 != nbatch:
+                raise ValueError(f"{names[0]} and {names[1]} batch domains differ")
+        dl = ddcx_layout(1, 1, 0, 0)
+        layouts = []
+        for d, name, nbv in ((derivs_xmin, names[2], self.s_nbv_xmin), (derivs_xmax, names[3], self.s_nbv_xmax)):
+            if d is None:
+                if nbv > 0 and (self.lower if name == names[2] else self.upper) == SplineBuilderClosure.HERMITE:
+                    raise ValueError(f"{name} is required by the Hermite closure")
+                continue
+            _check_f64_cuda(d, name)
+            if d.dim() != vals.dim() or d.shape[axis] < nbv:
+                raise ValueError(f"{name} must have the dimensions of {names[1]} with {nbv} rows along axis")
+            dl, nb3 = batched_layout(d, axis)
+            if nb3 != nbatch:
+                raise ValueError(f"{name} batch domain differs")
+            layouts.append((dl.row_stride, dl.inner_count, dl.inner_stride, dl.outer_stride))
+        if len(layouts) == 2 and layouts[0] != layouts[1]:
+            raise ValueError(f"{names[2]} and {names[3]} must have the same layout")
+        return cl, vl, dl, nbatch
+
     def __call__(self, spline, vals, derivs_xmin=None, derivs_xmax=None, axis=0):
         """SplineBuilder::operator(), spline_builder.hpp:538-557."""
-        _check_f64_cuda(spline, "spline")
-        _check_f64_cuda(vals, "vals")
-        if vals.shape[axis] != self.info.npoints:
-            raise ValueError(f"vals has {vals.shape[axis]} interpolation points, expected {self.info.npoints}")
-        if spline.shape[axis] != self.info.size:
-            raise ValueError(f"spline has {spline.shape[axis]} coefficients, expected {self.info.size}")
-        vl, nbatch = batched_layout(vals, axis)
-        cl, nb2 = batched_layout(spline, axis)
-        if nb2 != nbatch:
-            raise ValueError("spline and vals batch domains differ")
-        dl = ddcx_layout(1, 1, 0, 0)
-        for d, name in ((derivs_xmin, "derivs_xmin"), (derivs_xmax, "derivs_xmax")):
-            if d is not None:
-                _check_f64_cuda(d, name)
-                dl, nb3 = batched_layout(d, axis)
-                if nb3 != nbatch:
-                    raise ValueError(f"{name} batch domain differs")
+        cl, vl, dl, nbatch = self._layouts(spline, vals, derivs_xmin, derivs_xmax, axis)
         call("ddcx_builder_build", self._h, _ptr(spline), cl, _ptr(vals), vl, _ptr(derivs_xmin), _ptr(derivs_xmax),
              dl, nbatch, _stream())
         return spline
 
     def build_vjp(self, gcoef, gvals, gderivs_xmin=None, gderivs_xmax=None, axis=0):
         """Adjoint of __call__: gvals (and Hermite derivative gradients) from gcoef."""
-        cl, nbatch = batched_layout(gcoef, axis)
-        vl, _ = batched_layout(gvals, axis)
-        dl = ddcx_layout(1, 1, 0, 0)
-        for d in (gderivs_xmin, gderivs_xmax):
-            if d is not None:
-                dl, _ = batched_layout(d, axis)
+        cl, vl, dl, nbatch = self._layouts(gcoef, gvals, gderivs_xmin, gderivs_xmax, axis,
+                                           ("gcoef", "gvals", "gderivs_xmin", "gderivs_xmax"))
         call("ddcx_builder_build_vjp", self._h, _ptr(gcoef), cl, _ptr(gvals), vl, _ptr(gderivs_xmin),
              _ptr(gderivs_xmax), dl, nbatch, _stream())
         return gvals
@@ -318,7 +366,8 @@ class SplineEvaluator:
         self.bsplines = bsplines
         self.lower_extrap_rule, self.upper_extrap_rule = lo, up
         h = C.c_void_p()
-        call("ddcx_evaluator_create", bsplines._h, lo.code, up.code, C.byref(h))
+        dlo, dup = lo._desc(), up._desc()
+        call("ddcx_evaluator_create_with_rules", bsplines._h, C.byref(dlo), C.byref(dup), C.byref(h))
         self._h = h
 
     def __del__(self):
@@ -367,10 +416,16 @@ class SplineEvaluator:
         return integrals_out
 
     def eval_vjp(self, coords_eval, w, spline_coef, gx, gcoef, axis=0):
-        """Adjoint of __call__: gcoef = dL/dcoef, gx = dL/dcoords for L = sum(w * f(coords))."""
-        pl, nbatch = batched_layout(coords_eval, axis)
-        cl, _ = batched_layout(spline_coef, axis)
-        call("ddcx_evaluator_eval_vjp", self._h, _ptr(coords_eval), _ptr(w), _ptr(gx), pl, coords_eval.shape[axis],
+        """Adjoint of __call__: gcoef = dL/dcoef, gx = dL/dcoords for L = sum(w * f(coords)).
+        w and gx share the layout of coords_eval, gcoef that of spline_coef."""
+        pl, npts, cl, nbatch = self._args(gx, coords_eval, spline_coef, axis)
+        _check_f64_cuda(w, "w")
+        _check_f64_cuda(gcoef, "gcoef")
+        if w.shape != coords_eval.shape or w.stride() != coords_eval.stride():
+            raise ValueError("w must have the shape and layout of coords_eval")
+        if gcoef.shape != spline_coef.shape or gcoef.stride() != spline_coef.stride():
+            raise ValueError("gcoef must have the shape and layout of spline_coef")
+        call("ddcx_evaluator_eval_vjp", self._h, _ptr(coords_eval), _ptr(w), _ptr(gx), pl, npts,
              _ptr(spline_coef), _ptr(gcoef), cl, nbatch, _stream())
         return gcoef, gx
 
@@ -381,14 +436,16 @@ def build_eval(builder, evaluator, spline_eval, coords_eval, vals, spline=None, 
     in one pass (ddcx_build_eval).  With spline=None the coefficients stay on
     chip; that needs the fused kernel (FP64 builder, degree 2..5, batch-
     contiguous arrays)."""
-    _check_f64_cuda(vals, "vals")
-    vl, nbatch = batched_layout(vals, axis)
-    pl, _ = batched_layout(coords_eval, axis)
-    cl = batched_layout(spline, axis)[0] if spline is not None else ddcx_layout(1, 1, 0, 0)
-    dl = ddcx_layout(1, 1, 0, 0)
-    for d in (derivs_xmin, derivs_xmax):
-        if d is not None:
-            dl, _ = batched_layout(d, axis)
+    if evaluator.bsplines is not builder.bsplines:
+        raise ValueError("the evaluator must use the builder's basis")
+    cl, vl, dl, nbatch = builder._layouts(spline, vals, derivs_xmin, derivs_xmax, axis)
+    _check_f64_cuda(spline_eval, "spline_eval")
+    _check_f64_cuda(coords_eval, "coords_eval")
+    if spline_eval.shape != coords_eval.shape or spline_eval.stride() != coords_eval.stride():
+        raise ValueError("spline_eval and coords_eval must have the same shape and layout")
+    pl, nb2 = batched_layout(coords_eval, axis)
+    if nb2 != nbatch:
+        raise ValueError("coords_eval and vals batch domains differ")
     call("ddcx_build_eval", builder._h, evaluator._h, _ptr(spline), cl, _ptr(vals), vl, _ptr(derivs_xmin),
          _ptr(derivs_xmax), dl, _ptr(spline_eval), _ptr(coords_eval), pl, coords_eval.shape[axis], nbatch, _stream())
     return spline_eval
@@ -403,19 +460,16 @@ def advect(builder, evaluator, out, vals, shift, spline=None, derivs_xmin=None, 
     coefficients stay on chip unless `spline` is given.  Raises where the
     fused kernel does not apply (FP64 builder, degree 2..5, batch-contiguous
     arrays, main blocks of 256 to 2048 rows)."""
-    _check_f64_cuda(vals, "vals")
+    if evaluator.bsplines is not builder.bsplines:
+        raise ValueError("the evaluator must use the builder's basis")
     _check_f64_cuda(shift, "shift")
-    vl, nbatch = batched_layout(vals, axis)
+    _check_f64_cuda(out, "out")
+    cl, vl, dl, nbatch = builder._layouts(spline, vals, derivs_xmin, derivs_xmax, axis)
+    if out.shape != vals.shape:
+        raise ValueError("out must have the shape of vals")
     ol, _ = batched_layout(out, axis)
-    if out.shape[axis] != builder.npoints():
-        raise ValueError("out must have npoints rows along axis")
     if shift.numel() != nbatch or not shift.is_contiguous():
         raise ValueError("shift must be a contiguous tensor with one value per batch line")
-    cl = batched_layout(spline, axis)[0] if spline is not None else ddcx_layout(1, 1, 0, 0)
-    dl = ddcx_layout(1, 1, 0, 0)
-    for d in (derivs_xmin, derivs_xmax):
-        if d is not None:
-            dl, _ = batched_layout(d, axis)
     call("ddcx_advect", builder._h, evaluator._h, _ptr(spline), cl, _ptr(vals), vl, _ptr(derivs_xmin),
          _ptr(derivs_xmax), dl, _ptr(shift), _ptr(out), ol, nbatch, _stream())
     return out

@@ -188,9 +188,37 @@ ddcx_status ddcx_builder_build_vjp(const ddcx_builder* h, const double* gcoef, d
 ddcx_status ddcx_builder_quadrature(const ddcx_builder* h, double* q_xmin, double* q_vals, double* q_xmax);
 
 /* SplineEvaluator<Exec, Mem, BSplines, EvalDDim, LowerRule, UpperRule> ctor,
- * spline_evaluator.hpp:140-153. */
+ * spline_evaluator.hpp:140-153.  A Constant rule made here evaluates at the
+ * domain bound it guards (rmin for the lower rule, rmax for the upper), i.e.
+ * ConstantExtrapolationRule(rmin) / (rmax); in a 2D evaluator it clamps the
+ * other coordinate to that dimension's [rmin, rmax]. */
 ddcx_status ddcx_evaluator_create(const ddcx_bsplines* b, ddcx_extrap lower, ddcx_extrap upper,
                                   ddcx_evaluator** out);
+
+/* An extrapolation rule with its constructor arguments.  For kind
+ * DDCX_EXTRAP_CONSTANT:
+ *  - eval_pos: ConstantExtrapolationRule<DimI>(eval_pos)
+ *    (constant_extrapolation_rule.hpp:39-40): out-of-domain points take the
+ *    spline's value (and, in the adjoint, basis weights) at eval_pos, which
+ *    must lie in [rmin, rmax];
+ *  - eval_pos_not_interest_min/max: the 2D form ConstantExtrapolationRule<DimI,
+ *    DimNI>(eval_pos, min, max) (:99-124): in ddcx_evaluator2d_* calls, when the
+ *    other dimension is not periodic, its coordinate is clamped to [min, max].
+ * NaN selects the default: the guarded bound for eval_pos, the other
+ * dimension's [rmin, rmax] for the pair (give both or neither).  Other kinds
+ * ignore the positions. */
+typedef struct ddcx_extrap_rule {
+    ddcx_extrap kind;
+    double eval_pos;
+    double eval_pos_not_interest_min;
+    double eval_pos_not_interest_max;
+} ddcx_extrap_rule;
+
+/* SplineEvaluator ctor with rule objects (spline_evaluator.hpp:140-153,
+ * taking LowerExtrapolationRule / UpperExtrapolationRule by value).
+ * DDCX_ERR_INVALID for an eval_pos outside [rmin, rmax]. */
+ddcx_status ddcx_evaluator_create_with_rules(const ddcx_bsplines* b, const ddcx_extrap_rule* lower,
+                                             const ddcx_extrap_rule* upper, ddcx_evaluator** out);
 ddcx_status ddcx_evaluator_destroy(ddcx_evaluator* h);
 
 /* SplineEvaluator::operator()(spline_eval, coords_eval, spline_coef),

@@ -74,13 +74,13 @@ def lib():
         L.or_spline_build_vjp.argtypes = [_vp, _dp, _L, _L, _L, _L, _dp, _L, _L, _L, _L, _vp, _vp,
                                           _L, _L, _L, _L, _L]
         L.or_spline_eval.argtypes = [_vp, _dp, _dp, _L, _L, _L, _L, _L, _dp, _L, _L, _L, _L, _L, C.c_int,
-                                     C.c_int, C.c_int]
+                                     C.c_int, C.c_int, _vp]
         L.or_spline_eval_vjp.argtypes = [_vp, _dp, _dp, _dp, _L, _L, _L, _L, _L, _dp, _dp, _L, _L, _L, _L, _L,
-                                         C.c_int, C.c_int]
+                                         C.c_int, C.c_int, _vp]
         L.or_spline_integrate.argtypes = [_vp, _dp, _L, _L, _L, _L, _dp, _L, _L]
         L.or_spline_quadrature.argtypes = [_vp, _dp]
         L.or_spline2d_eval.argtypes = [_vp, _vp, _dp, _dp, _L, _dp, _L, _L, _L, _L, C.c_int, C.c_int,
-                                       np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")]
+                                       np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS"), _vp]
         L.or_splinend_eval.argtypes = [C.POINTER(_vp), C.c_int, _dp, _dp, _L, _dp, _L, C.POINTER(_L), _L,
                                        C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.or_splinend_integrate.argtypes = [C.POINTER(_vp), C.c_int, _dp, _dp, _L, C.POINTER(_L), _L]
@@ -259,8 +259,19 @@ class Spline:
             return gvals, gdl[: self.nbv_l], gdr[: self.nbv_r]
         return gvals, gdl[:, : self.nbv_l], gdr[:, : self.nbv_r]
 
-    def eval(self, coords, coef, deriv=0, layout="batch_contiguous", extrap=("null", "null")):
+    @staticmethod
+    def _rule_pos(eval_pos):
+        """(lower, upper) eval_pos of constant rules, or None for the bounds."""
+        if eval_pos is None:
+            return None, None
+        rp = np.ascontiguousarray(eval_pos, dtype=np.float64)
+        assert rp.shape == (2,)
+        return rp, rp.ctypes.data
+
+    def eval(self, coords, coef, deriv=0, layout="batch_contiguous", extrap=("null", "null"), eval_pos=None):
+        """eval_pos: (lower, upper) positions of constant rules (None: rmin, rmax)."""
         ax = 0 if layout == "batch_contiguous" else 1
+        keep, rp = self._rule_pos(eval_pos)
         coords = np.ascontiguousarray(coords, dtype=np.float64)
         coef = np.ascontiguousarray(coef, dtype=np.float64)
         B = coords.shape[1 - ax]
@@ -268,11 +279,13 @@ class Spline:
         out = np.empty_like(coords)
         rules = [0 if e == "null" else 1 for e in extrap]
         lib().or_spline_eval(self.h, coords, out, *self._lay(coords, ax), P, coef, *self._lay(coef, ax), B,
-                             deriv, rules[0], rules[1])
+                             deriv, rules[0], rules[1], rp)
+        del keep
         return out
 
-    def eval_vjp(self, coords, w, coef, layout="batch_contiguous", extrap=("null", "null")):
+    def eval_vjp(self, coords, w, coef, layout="batch_contiguous", extrap=("null", "null"), eval_pos=None):
         ax = 0 if layout == "batch_contiguous" else 1
+        keep, rp = self._rule_pos(eval_pos)
         coords = np.ascontiguousarray(coords, dtype=np.float64)
         w = np.ascontiguousarray(w, dtype=np.float64)
         coef = np.ascontiguousarray(coef, dtype=np.float64)
@@ -282,7 +295,8 @@ class Spline:
         gc = np.empty_like(coef)
         rules = [0 if e == "null" else 1 for e in extrap]
         lib().or_spline_eval_vjp(self.h, coords, w, gx, *self._lay(coords, ax), P, coef, gc,
-                                 *self._lay(coef, ax), B, rules[0], rules[1])
+                                 *self._lay(coef, ax), B, rules[0], rules[1], rp)
+        del keep
         return gc, gx
 
     def integrate(self, coef, layout="batch_contiguous"):
@@ -342,14 +356,18 @@ def spline2d_build(s1: Spline, s2: Spline, vals, derivs_min1=None, derivs_max1=N
     return coef
 
 
-def spline2d_eval(s1: Spline, s2: Spline, coords, coef, d1=0, d2=0, rules=(0, 0, 0, 0)):
-    """coords (nbatch, npts, 2), coef (nbatch, size1, size2) -> (nbatch, npts)."""
+def spline2d_eval(s1: Spline, s2: Spline, coords, coef, d1=0, d2=0, rules=(0, 0, 0, 0), rule_pos=None):
+    """coords (nbatch, npts, 2), coef (nbatch, size1, size2) -> (nbatch, npts).
+    rules: lower 1, upper 1, lower 2, upper 2 (0 null, 1 constant).  rule_pos:
+    None (constant rules at the guarded bound, clamping the other coordinate
+    to its domain) or (pos[4], min[4], max[4]) per rule."""
     coords = np.ascontiguousarray(coords, dtype=np.float64)
     coef = np.ascontiguousarray(coef, dtype=np.float64)
     nb, P, _ = coords.shape
     out = np.empty((nb, P))
+    rp = None if rule_pos is None else np.ascontiguousarray(np.asarray(rule_pos, dtype=np.float64).reshape(12))
     lib().or_spline2d_eval(s1.h, s2.h, coords, out, P, coef, s1.size * s2.size, s2.size, 1, nb, d1, d2,
-                           np.asarray(rules, dtype=np.int32))
+                           np.asarray(rules, dtype=np.int32), None if rp is None else rp.ctypes.data)
     return out
 
 

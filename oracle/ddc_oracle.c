@@ -1302,9 +1302,13 @@ int or_spline_build_vjp(void* h, const double* gcoef, long crs, long cinner, lon
 }
 
 /* SplineEvaluator::eval / eval_no_bc, spline_evaluator.hpp:529-605, for one
- * point of one line.  extrap rule: 0 null, 1 constant (evaluate at bound).
+ * point of one line.  extrap rule: 0 null, 1 constant; the constant rule
+ * evaluates at its eval_pos (ConstantExtrapolationRule<DimI>::operator(),
+ * constant_extrapolation_rule.hpp:52-72): rp[0] for the lower rule, rp[1] for
+ * the upper (rp NULL: rmin, rmax).
  * deriv: 0 = value, k > 0 = k-th derivative (no boundary handling, as deriv()). */
-static double eval1(const or_spline* s, double x, const double* coef, long stride, int deriv, int el, int eu)
+static double eval1(const or_spline* s, double x, const double* coef, long stride, int deriv, int el, int eu,
+                    const double* rp)
 {
     const or_bspl* b = &s->bs;
     int const D = b->degree;
@@ -1315,10 +1319,10 @@ static double eval1(const or_spline* s, double x, const double* coef, long strid
         } else {
             if (x < bs_rmin(b)) {
                 if (el == 0) return 0.0;
-                x = bs_rmin(b);
+                x = rp ? rp[0] : bs_rmin(b);
             } else if (x > bs_rmax(b)) {
                 if (eu == 0) return 0.0;
-                x = bs_rmax(b);
+                x = rp ? rp[1] : bs_rmax(b);
             }
         }
         int const jmin = bs_eval_basis(b, x, v);
@@ -1336,7 +1340,8 @@ static double eval1(const or_spline* s, double x, const double* coef, long strid
 
 /* batched evaluate/deriv, spline_evaluator.hpp:257-294 and :357-430 */
 int or_spline_eval(void* h, const double* coords, double* out, long prs, long pinner, long pis, long pos, long npts,
-                   const double* coef, long crs, long cinner, long cis, long cos_, long nbatch, int deriv, int el, int eu)
+                   const double* coef, long crs, long cinner, long cis, long cos_, long nbatch, int deriv, int el, int eu,
+                   const double* rp)
 {
     or_spline* s = (or_spline*)h;
     or_lay const PL = {prs, pinner, pis, pos}, CL = {crs, cinner, cis, cos_};
@@ -1345,7 +1350,7 @@ int or_spline_eval(void* h, const double* coords, double* out, long prs, long pi
         const double* c = coef + lay_off(&CL, 0, l);
         for (long i = 0; i < npts; ++i) {
             long const o = lay_off(&PL, i, l);
-            out[o] = eval1(s, coords[o], c, crs, deriv, el, eu);
+            out[o] = eval1(s, coords[o], c, crs, deriv, el, eu, rp);
         }
     }
     return 0;
@@ -1353,10 +1358,10 @@ int or_spline_eval(void* h, const double* coords, double* out, long prs, long pi
 
 /* Adjoint of or_spline_eval (value form): gcoef += w * B(x); gx = w * f'(x)
  * inside the domain (periodic wrap has slope 1), 0 outside (null rule: value
- * 0; constant rule: value independent of x). */
+ * 0; constant rule: value independent of x, basis weights taken at eval_pos). */
 int or_spline_eval_vjp(void* h, const double* coords, const double* w, double* gx, long prs, long pinner, long pis,
                        long pos, long npts, const double* coef, double* gcoef, long crs, long cinner, long cis,
-                       long cos_, long nbatch, int el, int eu)
+                       long cos_, long nbatch, int el, int eu, const double* rp)
 {
     or_spline* s = (or_spline*)h;
     const or_bspl* b = &s->bs;
@@ -1377,16 +1382,16 @@ int or_spline_eval_vjp(void* h, const double* coords, const double* w, double* g
             } else if (x < bs_rmin(b)) {
                 inside = 0;
                 if (el == 0) { gx[o] = 0.0; continue; }
-                x = bs_rmin(b);
+                x = rp ? rp[0] : bs_rmin(b);
             } else if (x > bs_rmax(b)) {
                 inside = 0;
                 if (eu == 0) { gx[o] = 0.0; continue; }
-                x = bs_rmax(b);
+                x = rp ? rp[1] : bs_rmax(b);
             }
             double v[OR_MAXDEG + 1];
             int const jmin = bs_eval_basis(b, x, v);
             for (int k = 0; k < D + 1; ++k) gc[(jmin + k) * crs] += wi * v[k];
-            gx[o] = inside ? wi * eval1(s, x, c, crs, 1, el, eu) : 0.0;
+            gx[o] = inside ? wi * eval1(s, x, c, crs, 1, el, eu, rp) : 0.0;
         }
     }
     return 0;
@@ -1439,13 +1444,20 @@ int or_spline_quadrature(void* h, double* out)
     return 0;
 }
 
+/* clamp of the not-of-interest coordinate, Kokkos::clamp in
+ * constant_extrapolation_rule.hpp:168-172 */
+static double clampd(double v, double lo, double hi) { return v < lo ? lo : (hi < v ? hi : v); }
+
 /* SplineEvaluator2D::eval / eval_no_bc, spline_evaluator_2d.hpp:679-825, with
- * periodic wrap per dimension and null (0) / constant (1) rules; the constant
- * rule evaluates at the boundary of the dimension of interest and clamps the
- * other coordinate to [rmin, rmax] of its dimension
- * (constant_extrapolation_rule.hpp:81-195). */
+ * periodic wrap per dimension, then the rules of dimension 1 and of dimension
+ * 2 in that order (rules[0..3]: lower 1, upper 1, lower 2, upper 2; 0 null, 1
+ * constant).  A constant rule returns the value at (eval_pos, the other
+ * coordinate clamped to [min, max] unless that dimension is periodic)
+ * (ConstantExtrapolationRule<DimI, DimNI>::operator(),
+ * constant_extrapolation_rule.hpp:157-195).  rp: pos[4], min[4], max[4] per
+ * rule (NULL: the guarded bound, and the other dimension's [rmin, rmax]). */
 static double eval2(const or_spline* s1, const or_spline* s2, double x1, double x2, const double* coef, long st1,
-                    long st2, int d1, int d2, const int* rules)
+                    long st2, int d1, int d2, const int* rules, const double* rp)
 {
     const or_bspl* b1 = &s1->bs;
     const or_bspl* b2 = &s2->bs;
@@ -1454,26 +1466,24 @@ static double eval2(const or_spline* s1, const or_spline* s2, double x1, double 
             x1 -= floor((x1 - bs_rmin(b1)) / bs_length(b1)) * bs_length(b1);
         if (b2->periodic && (x2 < bs_rmin(b2) || x2 > bs_rmax(b2)))
             x2 -= floor((x2 - bs_rmin(b2)) / bs_length(b2)) * bs_length(b2);
-        if (!b1->periodic) {
-            if (x1 < bs_rmin(b1)) {
-                if (rules[0] == 0) return 0.0;
-                x1 = bs_rmin(b1);
-                if (!b2->periodic) x2 = x2 < bs_rmin(b2) ? bs_rmin(b2) : (x2 > bs_rmax(b2) ? bs_rmax(b2) : x2);
-            } else if (x1 > bs_rmax(b1)) {
-                if (rules[1] == 0) return 0.0;
-                x1 = bs_rmax(b1);
-                if (!b2->periodic) x2 = x2 < bs_rmin(b2) ? bs_rmin(b2) : (x2 > bs_rmax(b2) ? bs_rmax(b2) : x2);
-            }
-        }
-        if (!b2->periodic) {
-            if (x2 < bs_rmin(b2)) {
-                if (rules[2] == 0) return 0.0;
-                x2 = bs_rmin(b2);
-                if (!b1->periodic) x1 = x1 < bs_rmin(b1) ? bs_rmin(b1) : (x1 > bs_rmax(b1) ? bs_rmax(b1) : x1);
-            } else if (x2 > bs_rmax(b2)) {
-                if (rules[3] == 0) return 0.0;
-                x2 = bs_rmax(b2);
-                if (!b1->periodic) x1 = x1 < bs_rmin(b1) ? bs_rmin(b1) : (x1 > bs_rmax(b1) ? bs_rmax(b1) : x1);
+        int k = -1;
+        if (!b1->periodic && x1 < bs_rmin(b1)) k = 0;
+        else if (!b1->periodic && x1 > bs_rmax(b1)) k = 1;
+        else if (!b2->periodic && x2 < bs_rmin(b2)) k = 2;
+        else if (!b2->periodic && x2 > bs_rmax(b2)) k = 3;
+        if (k >= 0) {
+            if (rules[k] == 0) return 0.0;
+            const or_bspl* bi = k < 2 ? b1 : b2; /* dimension of interest */
+            const or_bspl* bn = k < 2 ? b2 : b1; /* the other one */
+            double const pos = rp ? rp[k] : ((k & 1) ? bs_rmax(bi) : bs_rmin(bi));
+            double const lo = rp ? rp[4 + k] : bs_rmin(bn);
+            double const hi = rp ? rp[8 + k] : bs_rmax(bn);
+            if (k < 2) {
+                x1 = pos;
+                if (!b2->periodic) x2 = clampd(x2, lo, hi);
+            } else {
+                x2 = pos;
+                if (!b1->periodic) x1 = clampd(x1, lo, hi);
             }
         }
     }
@@ -1500,7 +1510,7 @@ static double eval2(const or_spline* s1, const or_spline* s2, double x1, double 
  * batch l at coords[2*(l*npts + p)], out[l*npts + p]; coef of batch l at
  * coef + l*cbs with strides st1, st2. */
 int or_spline2d_eval(void* h1, void* h2, const double* coords, double* out, long npts, const double* coef, long cbs,
-                     long st1, long st2, long nbatch, int d1, int d2, const int* rules)
+                     long st1, long st2, long nbatch, int d1, int d2, const int* rules, const double* rp)
 {
     or_spline* s1 = (or_spline*)h1;
     or_spline* s2 = (or_spline*)h2;
@@ -1508,7 +1518,7 @@ int or_spline2d_eval(void* h1, void* h2, const double* coords, double* out, long
 #pragma omp parallel for schedule(static)
     for (long t = 0; t < total; ++t) {
         long const l = t / npts;
-        out[t] = eval2(s1, s2, coords[2 * t], coords[2 * t + 1], coef + l * cbs, st1, st2, d1, d2, rules);
+        out[t] = eval2(s1, s2, coords[2 * t], coords[2 * t + 1], coef + l * cbs, st1, st2, d1, d2, rules, rp);
     }
     return 0;
 }

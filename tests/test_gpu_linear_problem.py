@@ -30,6 +30,7 @@ def solve_and_validate(prob, orc, A, cuda, nrhs_extra=300):
     import torch
 
     N = A.shape[0]
+    _load(prob, A)
     for i in range(N):
         for j in range(N):
             assert prob.get_element(i, j) == A[i, j]

@@ -178,3 +178,101 @@ def test_2d_hermite_build_reproduces_tensor_polynomials():
     coef2 = O.spline2d_build(s1, s2, vals, dmin1, dmax1, dmin2, dmax2, bad, mixed["max", "min"],
                              mixed["min", "max"], mixed["max", "max"])
     assert np.abs(coef2 - coef).max() > 1e-6
+
+
+def _scipy_f(orc, c):
+    """The spline of coefficients c (size() entries) as a scipy BSpline on the
+    oracle's knots (values inside [rmin, rmax])."""
+    return si.BSpline(orc.knots(), c, orc.degree, extrapolate=False)
+
+
+@pytest.mark.parametrize("uniform", [True, False])
+def test_constant_rule_eval_pos(uniform):
+    """ConstantExtrapolationRule<DimI>(eval_pos) (constant_extrapolation_rule.hpp:
+    39-72): points below rmin take f(eval_pos_lower), points above rmax
+    f(eval_pos_upper), at interior positions, checked against scipy at those
+    positions; the adjoint scatters the basis weights at eval_pos and gives
+    a zero coordinate gradient there."""
+    if uniform:
+        orc = O.Spline(3, True, O.GREVILLE, O.GREVILLE, ncells=10)
+    else:
+        orc = O.Spline(3, False, O.GREVILLE, O.GREVILLE, breaks=breaks_for("random", 10, seed=2))
+    rng = np.random.default_rng(4)
+    c = _coef(orc, rng, False)
+    L = orc.rmax - orc.rmin
+    plo, phi = orc.rmin + 0.31 * L, orc.rmin + 0.77 * L
+    x = np.concatenate([rng.uniform(orc.rmin - L, orc.rmin - 1e-9, (15, 1)),
+                        rng.uniform(orc.rmin, orc.rmax, (10, 1)),
+                        rng.uniform(orc.rmax + 1e-9, orc.rmax + L, (15, 1))])
+    spl = _scipy_f(orc, c[:, 0])
+    ref = spl(x[:, 0])
+    ref[:15] = spl(plo)
+    ref[25:] = spl(phi)
+    got = orc.eval(x, c, extrap=("constant", "constant"), eval_pos=(plo, phi))[:, 0]
+    assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()
+    # the bound form differs: eval_pos is used
+    bound = orc.eval(x, c, extrap=("constant", "constant"))[:, 0]
+    assert np.abs(bound[:15] - ref[:15]).max() > 1e-6
+    w = rng.standard_normal((40, 1))
+    gc, gx = orc.eval_vjp(x, w, c, extrap=("constant", "constant"), eval_pos=(plo, phi))
+    dm = si.BSpline.design_matrix(np.array([plo, phi]), orc.knots(), 3).toarray()
+    inside = si.BSpline.design_matrix(x[15:25, 0], orc.knots(), 3).toarray()
+    gref = w[:15, 0].sum() * dm[0] + w[25:, 0].sum() * dm[1] + inside.T @ w[15:25, 0]
+    assert np.abs(gc[:, 0] - gref).max() <= 1e-13 * np.abs(gref).max()
+    assert np.all(gx[:15] == 0.0) and np.all(gx[25:] == 0.0)
+    assert np.abs(gx[15:25, 0] - w[15:25, 0] * spl.derivative()(x[15:25, 0])).max() <= 1e-12
+
+
+@pytest.mark.parametrize("dim2_periodic", [False, True])
+def test_2d_constant_rule_eval_pos(dim2_periodic):
+    """ConstantExtrapolationRule<DimI, DimNI>(eval_pos, min, max)
+    (constant_extrapolation_rule.hpp:99-195) in SplineEvaluator2D
+    (spline_evaluator_2d.hpp:679-720), on separable coefficients c1 (x) c2 so
+    that f = f1(x) f2(y) with f1, f2 evaluated by scipy: a point outside in
+    dimension 1 takes f1(pos) f2(y') with y' = clamp(y, min, max) (or y
+    wrapped, for a periodic dimension 2); else a point outside in dimension
+    2 takes f1(clamp(x, min, max)) f2(pos).  Points on all four sides and at
+    the corners, where dimension 1 wins."""
+    o1 = O.Spline(3, False, O.GREVILLE, O.GREVILLE, breaks=breaks_for("random", 9, seed=7))
+    if dim2_periodic:
+        o2 = O.Spline(3, True, O.PERIODIC, O.PERIODIC, ncells=8)
+    else:
+        o2 = O.Spline(3, True, O.GREVILLE, O.GREVILLE, ncells=8, rmin=-1.0, rmax=2.0)
+    rng = np.random.default_rng(9)
+    c1 = _coef(o1, rng, False)[:, 0]
+    c2 = _coef(o2, rng, dim2_periodic)[:, 0]
+    coef = np.outer(c1, c2)[None]
+    f1, f2 = _scipy_f(o1, c1), _scipy_f(o2, c2)
+    L1, L2 = o1.rmax - o1.rmin, o2.rmax - o2.rmin
+    pos = [o1.rmin + 0.2 * L1, o1.rmin + 0.9 * L1, o2.rmin + 0.35 * L2, o2.rmin + 0.6 * L2]
+    lo = [o2.rmin + 0.1 * L2, o2.rmin + 0.15 * L2, o1.rmin + 0.25 * L1, o1.rmin + 0.05 * L1]
+    hi = [o2.rmin + 0.8 * L2, o2.rmin + 0.7 * L2, o1.rmin + 0.75 * L1, o1.rmin + 0.95 * L1]
+    P = 400
+    xs = rng.uniform(o1.rmin - 0.5 * L1, o1.rmax + 0.5 * L1, P)
+    ys = rng.uniform(o2.rmin - 0.5 * L2, o2.rmax + 0.5 * L2, P)
+    coords = np.stack([xs, ys], -1)[None]
+
+    def wrap2(y):
+        return y - np.floor((y - o2.rmin) / L2) * L2 if dim2_periodic else y
+
+    ref = np.empty(P)
+    for p in range(P):
+        x, y = xs[p], wrap2(ys[p])
+        k = 0 if x < o1.rmin else 1 if x > o1.rmax else -1
+        if k < 0 and not dim2_periodic:
+            k = 2 if y < o2.rmin else 3 if y > o2.rmax else -1
+        if k in (0, 1):
+            yy = y if dim2_periodic else min(max(y, lo[k]), hi[k])
+            ref[p] = f1(pos[k]) * f2(yy)
+        elif k in (2, 3):
+            ref[p] = f1(min(max(x, lo[k]), hi[k])) * f2(pos[k])
+        else:
+            ref[p] = f1(x) * f2(y)
+    got = O.spline2d_eval(o1, o2, coords, coef, rules=(1, 1, 1, 1), rule_pos=(pos, lo, hi))[0]
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+    # the default positions are the bounds and the other dimension's domain
+    dflt = O.spline2d_eval(o1, o2, coords, coef, rules=(1, 1, 1, 1))[0]
+    rp = ([o1.rmin, o1.rmax, o2.rmin, o2.rmax], [o2.rmin, o2.rmin, o1.rmin, o1.rmin],
+          [o2.rmax, o2.rmax, o1.rmax, o1.rmax])
+    assert np.array_equal(dflt, O.spline2d_eval(o1, o2, coords, coef, rules=(1, 1, 1, 1), rule_pos=rp)[0])
+    assert np.abs(dflt - got).max() > 1e-6
