@@ -322,6 +322,14 @@ def measure_2d(D, timed, gen):
     coef = torch.empty((nb, bs1.size(), bs2.size()), dtype=torch.float64, device=dev)
     b2d(coef, vals)
     t_b = timed(lambda: b2d(coef, vals), 5)
+    # the same build with FP32 sweeps + FP64 refinement in both passes
+    F = D.Precision.FP32_REFINE
+    b2f = D.SplineBuilder2D(D.SplineBuilder(bs1, G, G, precision=F), D.SplineBuilder(bs2, precision=F))
+    coef32 = torch.empty_like(coef)
+    b2f(coef32, vals)
+    t_bf = timed(lambda: b2f(coef32, vals), 5)
+    err32 = ((coef32 - coef).abs().amax(2) / coef.abs().amax(2)).max().item()
+    del coef32
     coords = torch.rand((nb, P, 2), dtype=torch.float64, device=dev, generator=gen)
     out = torch.empty((nb, P), dtype=torch.float64, device=dev)
     ev(out, coords, coef)
@@ -333,6 +341,8 @@ def measure_2d(D, timed, gen):
     by_b = 8 * nb * (n1 * n2 + bs1.size() * bs2.size())
     by_e = 24 * nb * P + 8 * coef.numel()
     return {"build_us": round(t_b * 1e6, 1), "build_GBps": round(by_b / t_b / 1e9, 1),
+            "fp32_refine_build_us": round(t_bf * 1e6, 1), "fp32_refine_build_GBps": round(by_b / t_bf / 1e9, 1),
+            "fp32_refine_max_line_rel_err_vs_fp64": float(f"{err32:.3e}"),
             "eval_us": round(t_e * 1e6, 1), "eval_GBps": round(by_e / t_e / 1e9, 1),
             "eval_Mpts_s": round(nb * P / t_e / 1e6, 1), "eval_vjp_us": round(t_a * 1e6, 1),
             "points": nb * P}

@@ -488,8 +488,7 @@ ddcx_status ddcx_evaluatornd_eval(const ddcx_evaluator* const* evs, int nd, cons
                 throw ddcx::Error(DDCX_ERR_UNSUPPORTED, "N-dimensional evaluation supports degrees 1..5");
             if (orders) {
                 require(orders[k] >= 0, "negative derivative order");
-                if (orders[k] > std::min(bs[k]->view.degree, 3))
-                    throw ddcx::Error(DDCX_ERR_UNSUPPORTED, "derivative order above min(degree, 3)");
+                require(orders[k] <= bs[k]->view.degree, "derivative order above the degree");
             }
         }
         if (npts * nbatch == 0) return;

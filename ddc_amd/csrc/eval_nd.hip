@@ -71,6 +71,8 @@ __device__ int basis_of_order(BSplinesView const& b, double x, int order, double
     case 1: take.template operator()<1>(); break;
     case 2: take.template operator()<2>(); break;
     case 3: take.template operator()<3>(); break;
+    case 4: take.template operator()<4>(); break;
+    case 5: take.template operator()<5>(); break;
     default: break;
     }
     return j;
@@ -180,6 +182,8 @@ __device__ __forceinline__ int basis_fixed(BSplinesView const& b, double x, int 
             case 1: take.template operator()<1>(); break;
             case 2: take.template operator()<2>(); break;
             case 3: take.template operator()<3>(); break;
+            case 4: take.template operator()<4>(); break;
+            case 5: take.template operator()<5>(); break;
             default: break;
             }
             return j;

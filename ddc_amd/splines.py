@@ -608,41 +608,89 @@ class SplineEvaluator2D:
 
 
 class SplineBuilder3D:
-    """ddc::SplineBuilder3D (spline_builder_3d.hpp:45): one batched 1D build per
-    dimension, dimension 1 first, through two temporaries owned by this
-    object.  Arrays are (batch, n1, n2, n3) (or (n1, n2, n3)).  Hermite
-    closures are not part of this build."""
+    """ddc::SplineBuilder3D (spline_builder_3d.hpp:45-1090): arrays are
+    (batch, n1, n2, n3) (or (n1, n2, n3)).  With Hermite closures the call
+    takes the reference's derivative arguments by name (derivs_min1 ...
+    mixed_derivs_max1_max2_max3, shaped as the reference's batched derivative
+    domains, e.g. derivs_min2 (batch, n1, nbe2, n3), mixed_derivs_min1_max3
+    (batch, nbe1, n2, nbe3)).  As the reference does: the dimension-2 and
+    dimension-3 derivatives and the 2-3 cross derivatives are built along
+    dimension 1 (with the mixed derivatives involving dimension 1 as their
+    Hermite data), the values along dimension 1, then SplineBuilder2D over
+    dimensions 2 and 3.  Intermediates are torch tensors owned by this
+    object."""
+
+    DERIV_NAMES = tuple(
+        [f"derivs_{s}{k}" for k in (1, 2, 3) for s in ("min", "max")]
+        + [f"mixed_derivs_{a}{p}_{b}{q}" for p, q in ((1, 2), (2, 3), (1, 3)) for b in ("min", "max")
+           for a in ("min", "max")]
+        + [f"mixed_derivs_{a}1_{b}2_{c}3" for c in ("min", "max") for b in ("min", "max") for a in ("min", "max")])
 
     def __init__(self, builder1: SplineBuilder, builder2: SplineBuilder, builder3: SplineBuilder):
         self.builders = (builder1, builder2, builder3)
-        for b in self.builders:
-            if b.s_nbe_xmin or b.s_nbe_xmax:
-                raise NotImplementedError("3D Hermite closures are not part of this build")
-        self._tmp = [None, None]
+        self._b23 = SplineBuilder2D(builder2, builder3)
+        self._tmp = {}
 
-    def _scratch(self, i, shape, device):
-        t = self._tmp[i]
-        if t is None or tuple(t.shape) != shape or t.device != device:
-            t = self._tmp[i] = torch.empty(shape, dtype=torch.float64, device=device)
+    def _scratch(self, key, shape, device):
+        t = self._tmp.get(key)
+        if t is None or tuple(t.shape) != tuple(shape) or t.device != device:
+            t = self._tmp[key] = torch.empty(shape, dtype=torch.float64, device=device)
         return t
 
-    def __call__(self, spline, vals):
+    def __call__(self, spline, vals, **derivs):
+        """SplineBuilder3D::operator() (spline_builder_3d.hpp:779-1090)."""
+        unknown = set(derivs) - set(self.DERIV_NAMES)
+        if unknown:
+            raise TypeError(f"unknown derivative arguments: {sorted(unknown)}")
         squeeze = vals.dim() == 3
-        v = vals.unsqueeze(0) if squeeze else vals
-        s = spline.unsqueeze(0) if squeeze else spline
-        nb, n1, n2, n3 = v.shape
+
+        def b(t):
+            return None if t is None else (t.unsqueeze(0) if squeeze else t)
+
+        v, s = b(vals), b(spline)
+        d = {k: b(t) for k, t in derivs.items() if t is not None}
+        nb = v.shape[0]
         b1, b2, b3 = self.builders
-        t1 = self._scratch(0, (nb, b1.bsplines.size(), n2, n3), v.device)
-        t2 = self._scratch(1, (nb, b1.bsplines.size(), b2.bsplines.size(), n3), v.device)
-        b1(t1, v, axis=1)
-        b2(t2, t1, axis=2)
-        b3(s, t2, axis=3)
+        H = SplineBuilderClosure.HERMITE
+        s1 = b1.bsplines.size()
+        dev = v.device
+
+        def along1(key, name, lo, hi):
+            if name not in d:
+                raise ValueError(f"{name} is required by the Hermite closures")
+            src = d[name]
+            out = self._scratch(key, (nb, s1) + tuple(src.shape[2:]), dev)
+            b1(out, src, d.get(lo), d.get(hi), axis=1)
+            return out
+
+        d2 = {sd: along1(("d2", sd), f"derivs_{sd}2", f"mixed_derivs_min1_{sd}2", f"mixed_derivs_max1_{sd}2")
+              for sd, bc in (("min", b2.lower), ("max", b2.upper)) if bc == H}
+        d3 = {sd: along1(("d3", sd), f"derivs_{sd}3", f"mixed_derivs_min1_{sd}3", f"mixed_derivs_max1_{sd}3")
+              for sd, bc in (("min", b3.lower), ("max", b3.upper)) if bc == H}
+        m23 = {}
+        for a2, bc2 in (("min", b2.lower), ("max", b2.upper)):
+            for a3, bc3 in (("min", b3.lower), ("max", b3.upper)):
+                if bc2 == H and bc3 == H:
+                    m23[a2, a3] = along1(("m23", a2, a3), f"mixed_derivs_{a2}2_{a3}3",
+                                         f"mixed_derivs_min1_{a2}2_{a3}3", f"mixed_derivs_max1_{a2}2_{a3}3")
+        spline1 = self._scratch("s1", (nb, s1) + tuple(v.shape[2:]), dev)
+        b1(spline1, v, d.get("derivs_min1"), d.get("derivs_max1"), axis=1)
+
+        def flat(t):
+            return None if t is None else t.reshape((nb * s1,) + tuple(t.shape[2:]))
+
+        self._b23(flat(s), flat(spline1), flat(d2.get("min")), flat(d2.get("max")), flat(d3.get("min")),
+                  flat(d3.get("max")), flat(m23.get(("min", "min"))), flat(m23.get(("max", "min"))),
+                  flat(m23.get(("min", "max"))), flat(m23.get(("max", "max"))))
         return spline
 
     def build_vjp(self, gspline, gvals):
         """Adjoint of __call__: the transposed 1D solves in reverse order
         (dimension 3, 2, then 1).  gspline: (batch, size1, size2, size3);
-        gvals: (batch, n1, n2, n3)."""
+        gvals: (batch, n1, n2, n3).  Closures with derivative inputs are not
+        covered."""
+        if any(SplineBuilderClosure.HERMITE in (bb.lower, bb.upper) for bb in self.builders):
+            raise NotImplementedError("the 3D adjoint covers closures without boundary derivative inputs")
         squeeze = gspline.dim() == 3
         g = gspline.unsqueeze(0) if squeeze else gspline
         gv = gvals.unsqueeze(0) if squeeze else gvals

@@ -272,8 +272,8 @@ ddcx_status ddcx_evaluator2d_eval_vjp_binned(const ddcx_evaluator* e1, const ddc
  * (spline_evaluator_3d.hpp:443-1019) is its nd = 3 case.  evs: one evaluator
  * per dimension (1 <= nd <= 4); their extrapolation rules must be Null or
  * Periodic (Constant is refused: its reference form exists for 1D and 2D
- * only).  orders: nd derivative orders (NULL: all 0), each at most
- * min(degree, 3); with any nonzero order no wrap or rule is applied, as in
+ * only).  orders: nd derivative orders (NULL: all 0), each at most the
+ * degree (spline_evaluator_nd.hpp:735); with any nonzero order no wrap or rule is applied, as in
  * deriv().  coords: the nd coordinates of point p of batch l at
  * coords[nd*(l*npts + p)], out[l*npts + p]; coef of batch l at
  * coef + l*coef_batch_stride, element (i_1..i_nd) at sum_k i_k*coef_strides[k]. */

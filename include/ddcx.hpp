@@ -2,15 +2,22 @@
 // call shapes of DDC's spline classes: ddcx::UniformBSplines /
 // NonUniformBSplines (bsplines_uniform.hpp:48-370,
 // bsplines_non_uniform.hpp:44-392), ddcx::SplineBuilder
-// (spline_builder.hpp:57-617) and ddcx::SplineEvaluator
-// (spline_evaluator.hpp:33-606).  Errors are rethrown as std::runtime_error,
+// (spline_builder.hpp:57-617), ddcx::SplineEvaluator
+// (spline_evaluator.hpp:33-606) with the extrapolation rules
+// (null_extrapolation_rule.hpp, constant_extrapolation_rule.hpp,
+// periodic_extrapolation_rule.hpp), ddcx::SplineBuilder2D /
+// SplineEvaluator2D (spline_builder_2d.hpp:27-651,
+// spline_evaluator_2d.hpp:37-826), SplineBuilder3D / SplineEvaluator3D /
+// SplineEvaluatorND.  Errors are rethrown as std::runtime_error,
 // as the reference throws.  Device arrays are described by ddcx::Span, a
 // pointer plus the strides of a layout_right array with one interpolation
 // dimension and up to two batch dimensions (what a ddc::ChunkSpan's mdspan
 // provides through stride()).
 #pragma once
 
+#include <cmath>
 #include <cstdint>
+#include <algorithm>
 #include <memory>
 #include <optional>
 #include <stdexcept>
@@ -61,6 +68,47 @@ inline ddcx_layout batched_layout(std::vector<int64_t> const& ext, int axis, int
     side(0, axis);
     side(axis + 1, nd);
     ddcx_layout l{st[axis], 1, 0, 0};
+    int64_t nb = 1;
+    if (groups.size() == 1) {
+        l.inner_count = groups[0].first;
+        l.inner_stride = groups[0].second;
+        nb = groups[0].first;
+    } else if (groups.size() == 2) {
+        l.inner_count = groups[1].first;
+        l.inner_stride = groups[1].second;
+        l.outer_stride = groups[0].second;
+        nb = groups[0].first * groups[1].first;
+    }
+    if (nbatch) *nbatch = nb;
+    return l;
+}
+
+// Layout of a strided array (what a ChunkSpan's mdspan gives through
+// extent(d) and stride(d), outermost first) whose interpolation dimension is
+// `axis`.  Neighbouring batch dimensions on the same side of `axis` merge
+// when they are contiguous with each other (stride[d] == ext[d+1] *
+// stride[d+1]); at most two batch dimensions may remain.
+inline ddcx_layout batched_layout(std::vector<int64_t> const& ext, std::vector<int64_t> const& stride, int axis,
+                                  int64_t* nbatch = nullptr)
+{
+    int const nd = int(ext.size());
+    if (int(stride.size()) != nd || axis < 0 || axis >= nd) throw std::runtime_error("ddcx: bad extents or axis");
+    std::vector<std::pair<int64_t, int64_t>> groups; // (count, stride), outermost first
+    auto side = [&](int d0, int d1) {
+        bool open = false;
+        for (int d = d0; d < d1; ++d) {
+            if (open && groups.back().second == ext[d] * stride[d]) {
+                groups.back() = {groups.back().first * ext[d], stride[d]};
+            } else {
+                groups.emplace_back(ext[d], stride[d]);
+                open = true;
+            }
+        }
+    };
+    side(0, axis);
+    side(axis + 1, nd);
+    if (groups.size() > 2) throw std::runtime_error("ddcx: at most two batch dimensions (after merging)");
+    ddcx_layout l{stride[axis], 1, 0, 0};
     int64_t nb = 1;
     if (groups.size() == 1) {
         l.inner_count = groups[0].first;
@@ -195,9 +243,37 @@ private:
     ddcx_builder_info info_{};
 };
 
+// Extrapolation rule objects, passed to the SplineEvaluator constructors as
+// the reference passes its rule objects (spline_evaluator.hpp:140-153).
+struct ExtrapolationRule {
+    ddcx_extrap_rule rule;
+};
+// NullExtrapolationRule (null_extrapolation_rule.hpp:14-26): 0 outside the domain
+inline ExtrapolationRule NullExtrapolationRule() { return {{DDCX_EXTRAP_NULL, NAN, NAN, NAN}}; }
+// PeriodicExtrapolationRule<DimI> (periodic_extrapolation_rule.hpp:12-24)
+inline ExtrapolationRule PeriodicExtrapolationRule() { return {{DDCX_EXTRAP_PERIODIC, NAN, NAN, NAN}}; }
+// ConstantExtrapolationRule<DimI>(eval_pos) (constant_extrapolation_rule.hpp:39-40)
+inline ExtrapolationRule ConstantExtrapolationRule(double eval_pos) { return {{DDCX_EXTRAP_CONSTANT, eval_pos, NAN, NAN}}; }
+// ConstantExtrapolationRule<DimI, DimNI>(eval_pos, eval_pos_not_interest_min,
+// eval_pos_not_interest_max) (constant_extrapolation_rule.hpp:99-110), for
+// the evaluators of a SplineEvaluator2D
+inline ExtrapolationRule ConstantExtrapolationRule(double eval_pos, double eval_pos_not_interest_min,
+                                                   double eval_pos_not_interest_max)
+{
+    return {{DDCX_EXTRAP_CONSTANT, eval_pos, eval_pos_not_interest_min, eval_pos_not_interest_max}};
+}
+
 // SplineEvaluator<Exec, Mem, BSplines, EvalDDim, LowerRule, UpperRule>
 class SplineEvaluator {
 public:
+    // SplineEvaluator(lower_extrap_rule, upper_extrap_rule), spline_evaluator.hpp:140-153
+    SplineEvaluator(BSplines const& bs, ExtrapolationRule const& lower, ExtrapolationRule const& upper)
+    {
+        ddcx_evaluator* h = nullptr;
+        check(ddcx_evaluator_create_with_rules(bs.handle(), &lower.rule, &upper.rule, &h));
+        h_.reset(h);
+    }
+    // rules by kind; a Constant rule evaluates at the bound it guards
     SplineEvaluator(BSplines const& bs, Extrapolation lower, Extrapolation upper)
     {
         ddcx_evaluator* h = nullptr;
@@ -223,16 +299,26 @@ public:
     {
         check(ddcx_evaluator_integrate(h_.get(), out, out_stride, coef.data, coef.layout, coef.nbatch, stream));
     }
+    // adjoint of operator(): gcoef = sum_p w_p dB(x_p)/dcoef, gx_p = w_p f'(x_p);
+    // w and gx must have the layout of coords, gcoef that of coef
     void eval_vjp(Span<double const> coords, Span<double const> w, Span<double> gx, Span<double const> coef,
                   Span<double> gcoef, int64_t npts, void* stream = nullptr) const
     {
-        (void)w.layout;
+        if (!same(w.layout, coords.layout) || !same(gx.layout, coords.layout))
+            throw std::runtime_error("ddcx: eval_vjp needs w and gx in the layout of coords");
+        if (!same(gcoef.layout, coef.layout))
+            throw std::runtime_error("ddcx: eval_vjp needs gcoef in the layout of coef");
         check(ddcx_evaluator_eval_vjp(h_.get(), coords.data, w.data, gx.data, coords.layout, npts, coef.data,
                                       gcoef.data, coef.layout, coef.nbatch, stream));
     }
     ddcx_evaluator* handle() const { return h_.get(); }
 
 private:
+    static bool same(ddcx_layout const& a, ddcx_layout const& b)
+    {
+        return a.row_stride == b.row_stride && a.inner_count == b.inner_count && a.inner_stride == b.inner_stride &&
+               a.outer_stride == b.outer_stride;
+    }
     struct Del {
         void operator()(ddcx_evaluator* h) const { ddcx_evaluator_destroy(h); }
     };
@@ -271,6 +357,145 @@ inline void advect(SplineBuilder const& b, SplineEvaluator const& e, Span<double
                       derivs_xmin ? derivs_xmin->data : nullptr, derivs_xmax ? derivs_xmax->data : nullptr, dl, shift,
                       out.data, out.layout, vals.nbatch, stream));
 }
+
+// SplineBuilder2D<Exec, Mem, BSplines1, BSplines2, DDimI1, DDimI2, BCs...,
+// Solver> (spline_builder_2d.hpp:27-651) over one 1D builder per dimension
+// (not owned), on layout_right device arrays: vals (nbatch, n1, n2) ->
+// spline (nbatch, size1, size2).  With Hermite closures: derivs_min1 /
+// derivs_max1 (nbatch, nbe1, n2) along dimension 1 at each dimension-2 point,
+// derivs_min2 / derivs_max2 (nbatch, n1, nbe2), and the corner cross
+// derivatives mixed_* (nbatch, nbe1, nbe2).  The dimension-2 derivatives are
+// built along dimension 1 first (with the corner derivatives as their
+// Hermite data), then dimension 2 over the dimension-1 spline (:514-651).
+// The intermediates live in a caller workspace of workspace_size(nbatch)
+// doubles; nothing is allocated.
+class SplineBuilder2D {
+public:
+    SplineBuilder2D(SplineBuilder const& b1, SplineBuilder const& b2, int64_t size1) : b1_(&b1), b2_(&b2), s1_(size1)
+    {
+    }
+    int64_t nbe1() const { return std::max(b1_->info().nbv_lower, b1_->info().nbv_upper); }
+    int64_t nbe2() const { return std::max(b2_->info().nbv_lower, b2_->info().nbv_upper); }
+    int64_t workspace_size(int64_t nbatch) const
+    {
+        return nbatch * s1_ * (b2_->npoints() + b2_->info().nbv_lower + b2_->info().nbv_upper);
+    }
+    void operator()(double* spline, const double* vals, int64_t nbatch, double* workspace,
+                    const double* derivs_min1 = nullptr, const double* derivs_max1 = nullptr,
+                    const double* derivs_min2 = nullptr, const double* derivs_max2 = nullptr,
+                    const double* mixed_min1_min2 = nullptr, const double* mixed_max1_min2 = nullptr,
+                    const double* mixed_min1_max2 = nullptr, const double* mixed_max1_max2 = nullptr,
+                    void* stream = nullptr) const
+    {
+        int64_t const n1 = b1_->npoints(), n2 = b2_->npoints(), s2 = b2_->info().size;
+        int64_t const l2 = b2_->info().nbv_lower, u2 = b2_->info().nbv_upper, e1 = nbe1();
+        double* spline1 = workspace;
+        double* d1min = spline1 + nbatch * s1_ * n2;
+        double* d1max = d1min + nbatch * s1_ * l2;
+        auto dim1 = [&](double* out, const double* in, int64_t cols, const double* dmin, const double* dmax) {
+            int64_t nb = 0;
+            ddcx_layout const lo = batched_layout({nbatch, s1_, cols}, 1, &nb);
+            ddcx_layout const li = batched_layout({nbatch, n1, cols}, 1);
+            ddcx_layout const ld = batched_layout({nbatch, std::max<int64_t>(e1, 1), cols}, 1);
+            check(ddcx_builder_build(b1_->handle(), out, lo, in, li, dmin, dmax, ld, nb, stream));
+        };
+        if (l2 > 0) dim1(d1min, derivs_min2, l2, mixed_min1_min2, mixed_max1_min2);
+        dim1(spline1, vals, n2, derivs_min1, derivs_max1);
+        if (u2 > 0) dim1(d1max, derivs_max2, u2, mixed_min1_max2, mixed_max1_max2);
+        int64_t nb = 0;
+        ddcx_layout const lo = batched_layout({nbatch, s1_, s2}, 2, &nb);
+        ddcx_layout const li = batched_layout({nbatch, s1_, n2}, 2);
+        ddcx_layout const ld = batched_layout({nbatch, s1_, std::max<int64_t>(nbe2(), 1)}, 2);
+        check(ddcx_builder_build(b2_->handle(), spline, lo, spline1, li, l2 > 0 ? d1min : nullptr,
+                                 u2 > 0 ? d1max : nullptr, ld, nb, stream));
+    }
+    // adjoint with respect to the values (closures without derivative inputs):
+    // the transposed 1D solves in reverse order, through the same workspace
+    void build_vjp(const double* gspline, double* gvals, int64_t nbatch, double* workspace,
+                   void* stream = nullptr) const
+    {
+        if (nbe1() || nbe2()) throw std::runtime_error("ddcx: the 2D adjoint covers closures without Hermite data");
+        int64_t const n1 = b1_->npoints(), n2 = b2_->npoints(), s2 = b2_->info().size;
+        int64_t nb = 0;
+        ddcx_layout const lg = batched_layout({nbatch, s1_, s2}, 2, &nb);
+        ddcx_layout const lt = batched_layout({nbatch, s1_, n2}, 2);
+        ddcx_layout const none{1, 1, 0, 0};
+        check(ddcx_builder_build_vjp(b2_->handle(), gspline, lg, workspace, lt, nullptr, nullptr, none, nb, stream));
+        ddcx_layout const lt1 = batched_layout({nbatch, s1_, n2}, 1, &nb);
+        ddcx_layout const lv = batched_layout({nbatch, n1, n2}, 1);
+        check(ddcx_builder_build_vjp(b1_->handle(), workspace, lt1, gvals, lv, nullptr, nullptr, none, nb, stream));
+    }
+
+private:
+    SplineBuilder const* b1_;
+    SplineBuilder const* b2_;
+    int64_t s1_;
+};
+
+// SplineEvaluator2D<Exec, Mem, BSplines1, BSplines2, DDim1, DDim2, rules...>
+// (spline_evaluator_2d.hpp:37-826) over one 1D evaluator per dimension (not
+// owned; their rules are the 2D rules: lower/upper of dimension 1, then of
+// dimension 2).  Points interleave (x1, x2): point p of batch l at
+// coords[2*(l*npts + p)], out[l*npts + p]; coefficients of batch l start at
+// coef + l*coef_batch_stride, element (i1, i2) at i1*st1 + i2*st2.
+class SplineEvaluator2D {
+public:
+    SplineEvaluator2D(SplineEvaluator const& e1, SplineEvaluator const& e2) : e1_(&e1), e2_(&e2) {}
+    // operator()(spline_eval, coords_eval, spline_coef), spline_evaluator_2d.hpp:370-420
+    void operator()(double* out, const double* coords, int64_t npts, const double* coef, int64_t coef_batch_stride,
+                    int64_t st1, int64_t st2, int64_t nbatch, void* stream = nullptr) const
+    {
+        deriv(0, 0, out, coords, npts, coef, coef_batch_stride, st1, st2, nbatch, stream);
+    }
+    // deriv_dim_1 / deriv_dim_2 / deriv_1_and_2 (:460-600) as derivative orders
+    void deriv(int order1, int order2, double* out, const double* coords, int64_t npts, const double* coef,
+               int64_t coef_batch_stride, int64_t st1, int64_t st2, int64_t nbatch, void* stream = nullptr) const
+    {
+        check(ddcx_evaluator2d_eval(e1_->handle(), e2_->handle(), order1, order2, out, coords, npts, coef,
+                                    coef_batch_stride, st1, st2, nbatch, stream));
+    }
+    void deriv_dim_1(double* out, const double* coords, int64_t npts, const double* coef, int64_t cbs, int64_t st1,
+                     int64_t st2, int64_t nbatch, void* stream = nullptr) const
+    {
+        deriv(1, 0, out, coords, npts, coef, cbs, st1, st2, nbatch, stream);
+    }
+    void deriv_dim_2(double* out, const double* coords, int64_t npts, const double* coef, int64_t cbs, int64_t st1,
+                     int64_t st2, int64_t nbatch, void* stream = nullptr) const
+    {
+        deriv(0, 1, out, coords, npts, coef, cbs, st1, st2, nbatch, stream);
+    }
+    void deriv_1_and_2(double* out, const double* coords, int64_t npts, const double* coef, int64_t cbs,
+                       int64_t st1, int64_t st2, int64_t nbatch, void* stream = nullptr) const
+    {
+        deriv(1, 1, out, coords, npts, coef, cbs, st1, st2, nbatch, stream);
+    }
+    // bytes of device workspace eval_vjp needs (0: no workspace)
+    int64_t eval_vjp_workspace(int64_t npts, int64_t nbatch) const
+    {
+        int64_t bytes = 0;
+        check(ddcx_evaluator2d_eval_vjp_workspace(e1_->handle(), e2_->handle(), npts, nbatch, &bytes));
+        return bytes;
+    }
+    // adjoint of operator(): gcoef (strides of coef, overwritten) and gx
+    // (interleaved like coords) from the weights w (nbatch*npts).  With a
+    // workspace of eval_vjp_workspace() bytes the points are binned by cell
+    // tile; with none, one atomic per point and coefficient.
+    void eval_vjp(const double* coords, const double* w, double* gx, int64_t npts, const double* coef, double* gcoef,
+                  int64_t coef_batch_stride, int64_t st1, int64_t st2, int64_t nbatch, void* work = nullptr,
+                  int64_t work_bytes = 0, void* stream = nullptr) const
+    {
+        if (work && work_bytes > 0)
+            check(ddcx_evaluator2d_eval_vjp_binned(e1_->handle(), e2_->handle(), coords, w, gx, npts, coef, gcoef,
+                                                   coef_batch_stride, st1, st2, nbatch, work, work_bytes, stream));
+        else
+            check(ddcx_evaluator2d_eval_vjp(e1_->handle(), e2_->handle(), coords, w, gx, npts, coef, gcoef,
+                                            coef_batch_stride, st1, st2, nbatch, stream));
+    }
+
+private:
+    SplineEvaluator const* e1_;
+    SplineEvaluator const* e2_;
+};
 
 // SplineEvaluatorND<Exec, Mem, TypeSeq<BSplines...>, TypeSeq<EvalDDim...>,
 // TypeSeq<Rules...>> (spline_evaluator_nd.hpp:49): one 1D evaluator per

@@ -391,6 +391,70 @@ def splinend_build(splines, vals):
     return cur
 
 
+def _build_axis(s: Spline, src, ax, dmin=None, dmax=None):
+    """SplineBuilder::operator() along axis `ax` of a C-contiguous array,
+    batched over every other axis (lines in row-major order).  dmin / dmax:
+    the Hermite data, arrays shaped like src with nbv rows along `ax`."""
+    src = np.ascontiguousarray(src, dtype=np.float64)
+    shape = list(src.shape)
+    outer = int(np.prod(shape[:ax]))
+    inner = int(np.prod(shape[ax + 1:]))
+    shape[ax] = s.size
+    out = np.empty(shape)
+    dmin = None if dmin is None or s.nbv_l == 0 else np.ascontiguousarray(dmin, dtype=np.float64)
+    dmax = None if dmax is None or s.nbv_r == 0 else np.ascontiguousarray(dmax, dtype=np.float64)
+    if (s.bc_lower == HERMITE and s.nbv_l and dmin is None) or (s.bc_upper == HERMITE and s.nbv_r and dmax is None):
+        raise ValueError("Hermite closure needs boundary derivatives")
+    nbe = (dmin if dmin is not None else dmax).shape[ax] if (dmin is not None or dmax is not None) else 1
+    lib().or_spline_build(s.h, src, inner, inner, 1, src.shape[ax] * inner, out, inner, inner, 1, s.size * inner,
+                          None if dmin is None else dmin.ctypes.data, None if dmax is None else dmax.ctypes.data,
+                          inner, inner, 1, nbe * inner, outer * inner)
+    return out
+
+
+def spline3d_build(s1: Spline, s2: Spline, s3: Spline, vals, derivs=None):
+    """SplineBuilder3D::operator() (spline_builder_3d.hpp:889-1090), with the
+    reference's Hermite data: vals (nbatch, n1, n2, n3) -> coef (nbatch,
+    size1, size2, size3).  derivs maps the reference's argument names to
+    arrays: derivs_min1/max1 (nbatch, nbe1, n2, n3), derivs_min2/max2
+    (nbatch, n1, nbe2, n3), derivs_min3/max3 (nbatch, n1, n2, nbe3),
+    mixed_derivs_<a>1_<b>2 (nbatch, nbe1, nbe2, n3), mixed_derivs_<b>2_<c>3
+    (nbatch, n1, nbe2, nbe3), mixed_derivs_<a>1_<c>3 (nbatch, nbe1, n2,
+    nbe3) and mixed_derivs_<a>1_<b>2_<c>3 (nbatch, nbe1, nbe2, nbe3).
+    As in the reference: the dimension-2 and dimension-3 derivatives and the
+    2-3 cross derivatives are built along dimension 1 first (:889-1038, with
+    the mixed derivatives involving dimension 1 as their Hermite data), the
+    values along dimension 1 (:1058), then SplineBuilder2D over dimensions 2
+    and 3 of the result with the built derivatives (:1060-1090)."""
+    d = derivs or {}
+    vals = np.ascontiguousarray(vals, dtype=np.float64)
+    nb = vals.shape[0]
+
+    def b1(name, lo, hi):
+        a = d.get(name)
+        return None if a is None else _build_axis(s1, a, 1, d.get(lo), d.get(hi))
+
+    d2min = b1("derivs_min2", "mixed_derivs_min1_min2", "mixed_derivs_max1_min2") if s2.bc_lower == HERMITE else None
+    d2max = b1("derivs_max2", "mixed_derivs_min1_max2", "mixed_derivs_max1_max2") if s2.bc_upper == HERMITE else None
+    d3min = b1("derivs_min3", "mixed_derivs_min1_min3", "mixed_derivs_max1_min3") if s3.bc_lower == HERMITE else None
+    d3max = b1("derivs_max3", "mixed_derivs_min1_max3", "mixed_derivs_max1_max3") if s3.bc_upper == HERMITE else None
+    m23 = {}
+    for a2, bc2 in (("min", s2.bc_lower), ("max", s2.bc_upper)):
+        for a3, bc3 in (("min", s3.bc_lower), ("max", s3.bc_upper)):
+            if bc2 == HERMITE and bc3 == HERMITE:
+                m23[a2, a3] = b1(f"mixed_derivs_{a2}2_{a3}3", f"mixed_derivs_min1_{a2}2_{a3}3",
+                                 f"mixed_derivs_max1_{a2}2_{a3}3")
+    spline1 = _build_axis(s1, vals, 1, d.get("derivs_min1"), d.get("derivs_max1"))
+
+    def flat(a):
+        return None if a is None else a.reshape((nb * s1.size,) + a.shape[2:])
+
+    coef = spline2d_build(s2, s3, flat(spline1), flat(d2min), flat(d2max), flat(d3min), flat(d3max),
+                          flat(m23.get(("min", "min"))), flat(m23.get(("max", "min"))),
+                          flat(m23.get(("min", "max"))), flat(m23.get(("max", "max"))))
+    return coef.reshape(nb, s1.size, s2.size, s3.size)
+
+
 def _nd_args(splines, coef):
     nd = len(splines)
     hs = (C.c_void_p * nd)(*[s.h for s in splines])

@@ -1,0 +1,215 @@
+// Drives device arrays through the C++ facade (include/ddcx.hpp) the way a
+// caller of the reference's SplineBuilder / SplineEvaluator /
+// SplineBuilder2D / SplineEvaluator2D would, and writes inputs and outputs
+// as raw little-endian doubles into the directory given as argv[1]; the
+// test test_gpu_cpp_facade.py checks them against the oracle.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+#include "ddcx.hpp"
+
+namespace {
+
+std::string g_dir;
+
+void hip_ok(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) {
+        std::fprintf(stderr, "%s: %s\n", what, hipGetErrorString(e));
+        std::exit(3);
+    }
+}
+
+struct DeviceArray {
+    double* p = nullptr;
+    size_t n = 0;
+    explicit DeviceArray(size_t count) : n(count) { hip_ok(hipMalloc(&p, n * sizeof(double)), "hipMalloc"); }
+    explicit DeviceArray(std::vector<double> const& h) : DeviceArray(h.size())
+    {
+        hip_ok(hipMemcpy(p, h.data(), n * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy H2D");
+    }
+    ~DeviceArray() { (void)hipFree(p); }
+    DeviceArray(DeviceArray const&) = delete;
+    std::vector<double> get() const
+    {
+        std::vector<double> h(n);
+        hip_ok(hipDeviceSynchronize(), "sync");
+        hip_ok(hipMemcpy(h.data(), p, n * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+        return h;
+    }
+};
+
+void dump(std::string const& name, std::vector<double> const& v)
+{
+    std::FILE* f = std::fopen((g_dir + "/" + name + ".bin").c_str(), "wb");
+    if (!f || std::fwrite(v.data(), sizeof(double), v.size(), f) != v.size()) {
+        std::fprintf(stderr, "cannot write %s\n", name.c_str());
+        std::exit(4);
+    }
+    std::fclose(f);
+}
+
+// deterministic uniform [0, 1) numbers (64-bit LCG, top 53 bits)
+struct Lcg {
+    uint64_t s;
+    double next()
+    {
+        s = s * 6364136223846793005ull + 1442695040888963407ull;
+        return double(s >> 11) * 0x1p-53;
+    }
+};
+
+// 1D: uniform periodic cubic, 64 cells, 33 lines, in one layout; build, eval,
+// deriv(1), integrate
+void case_1d(bool batch_contiguous)
+{
+    int const nc = 64, B = 33, P = 100;
+    ddcx::UniformBSplines bs(3, 0.0, 1.0, nc, true);
+    ddcx::SplineBuilder builder(bs, ddcx::SplineBuilderClosure::PERIODIC, ddcx::SplineBuilderClosure::PERIODIC);
+    ddcx::SplineEvaluator ev(bs, ddcx::PeriodicExtrapolationRule(), ddcx::PeriodicExtrapolationRule());
+    int const m = builder.npoints(), s = bs.size();
+    std::vector<double> x = builder.interpolation_points();
+    Lcg r{uint64_t(batch_contiguous ? 1 : 2)};
+    std::vector<double> vals(size_t(m) * B), coords(size_t(P) * B);
+    for (int i = 0; i < m; ++i)
+        for (int l = 0; l < B; ++l) {
+            double const v = std::sin(2 * M_PI * x[i] * (1 + l % 8)) + 0.1 * (r.next() - 0.5);
+            vals[batch_contiguous ? size_t(i) * B + l : size_t(l) * m + i] = v;
+        }
+    for (auto& c : coords) c = -0.5 + 2.0 * r.next();
+    int const ax = batch_contiguous ? 0 : 1;
+    std::vector<int64_t> const ev_ext = batch_contiguous ? std::vector<int64_t>{m, B} : std::vector<int64_t>{B, m};
+    std::vector<int64_t> const ec_ext = batch_contiguous ? std::vector<int64_t>{s, B} : std::vector<int64_t>{B, s};
+    std::vector<int64_t> const ep_ext = batch_contiguous ? std::vector<int64_t>{P, B} : std::vector<int64_t>{B, P};
+    int64_t nb = 0;
+    ddcx_layout const lv = ddcx::batched_layout(ev_ext, ax, &nb);
+    ddcx_layout const lc = ddcx::batched_layout(ec_ext, ax);
+    ddcx_layout const lp = ddcx::batched_layout(ep_ext, ax);
+    DeviceArray dv(vals), dc(size_t(s) * B), dx(coords), dout(size_t(P) * B), dder(size_t(P) * B), dint(B);
+    builder(ddcx::Span<double>{dc.p, lc, nb}, ddcx::Span<double const>{dv.p, lv, nb});
+    ev(ddcx::Span<double>{dout.p, lp, nb}, ddcx::Span<double const>{dx.p, lp, nb},
+       ddcx::Span<double const>{dc.p, lc, nb}, P);
+    ev.deriv(1, ddcx::Span<double>{dder.p, lp, nb}, ddcx::Span<double const>{dx.p, lp, nb},
+             ddcx::Span<double const>{dc.p, lc, nb}, P);
+    ev.integrate(dint.p, 1, ddcx::Span<double const>{dc.p, lc, nb});
+    std::string const tag = batch_contiguous ? "1d_bc_" : "1d_ic_";
+    dump(tag + "vals", vals);
+    dump(tag + "coords", coords);
+    dump(tag + "coef", dc.get());
+    dump(tag + "eval", dout.get());
+    dump(tag + "deriv1", dder.get());
+    dump(tag + "integrals", dint.get());
+}
+
+// 1D: non-uniform cubic Greville with ConstantExtrapolationRule(eval_pos)
+// on both sides, points outside the domain
+void case_rules()
+{
+    int const nc = 40, B = 5, P = 200;
+    Lcg r{3};
+    std::vector<double> br(nc + 1);
+    br[0] = 0.0;
+    for (int i = 1; i <= nc; ++i) br[i] = br[i - 1] + 0.5 + r.next();
+    for (auto& b : br) b /= br[nc];
+    ddcx::NonUniformBSplines bs(3, br, false);
+    ddcx::SplineBuilder builder(bs, ddcx::SplineBuilderClosure::GREVILLE, ddcx::SplineBuilderClosure::GREVILLE);
+    ddcx::SplineEvaluator ev(bs, ddcx::ConstantExtrapolationRule(0.3), ddcx::ConstantExtrapolationRule(0.8));
+    int const m = builder.npoints(), s = bs.size();
+    std::vector<double> vals(size_t(m) * B), coords(size_t(P) * B);
+    for (auto& v : vals) v = r.next() - 0.5;
+    for (auto& c : coords) c = -0.3 + 1.6 * r.next();
+    int64_t nb = 0;
+    ddcx_layout const lv = ddcx::batched_layout({m, B}, 0, &nb);
+    ddcx_layout const lc = ddcx::batched_layout({s, B}, 0);
+    ddcx_layout const lp = ddcx::batched_layout({P, B}, 0);
+    DeviceArray dv(vals), dc(size_t(s) * B), dx(coords), dout(size_t(P) * B);
+    builder(ddcx::Span<double>{dc.p, lc, nb}, ddcx::Span<double const>{dv.p, lv, nb});
+    ev(ddcx::Span<double>{dout.p, lp, nb}, ddcx::Span<double const>{dx.p, lp, nb},
+       ddcx::Span<double const>{dc.p, lc, nb}, P);
+    dump("rules_breaks", br);
+    dump("rules_vals", vals);
+    dump("rules_coords", coords);
+    dump("rules_eval", dout.get());
+}
+
+// 2D: non-uniform cubic Greville (20 cells) x uniform periodic cubic (16
+// cells), 2 batch entries: SplineBuilder2D, SplineEvaluator2D values,
+// deriv_dim_1 and the adjoint
+void case_2d()
+{
+    int const nc1 = 20, nc2 = 16, NB = 2, P = 500;
+    Lcg r{4};
+    std::vector<double> br(nc1 + 1);
+    br[0] = 0.0;
+    for (int i = 1; i <= nc1; ++i) br[i] = br[i - 1] + 0.5 + r.next();
+    for (auto& b : br) b /= br[nc1];
+    ddcx::NonUniformBSplines bs1(3, br, false);
+    ddcx::UniformBSplines bs2(3, 0.0, 1.0, nc2, true);
+    ddcx::SplineBuilder b1(bs1, ddcx::SplineBuilderClosure::GREVILLE, ddcx::SplineBuilderClosure::GREVILLE);
+    ddcx::SplineBuilder b2(bs2, ddcx::SplineBuilderClosure::PERIODIC, ddcx::SplineBuilderClosure::PERIODIC);
+    ddcx::SplineEvaluator e1(bs1, ddcx::NullExtrapolationRule(), ddcx::NullExtrapolationRule());
+    ddcx::SplineEvaluator e2(bs2, ddcx::PeriodicExtrapolationRule(), ddcx::PeriodicExtrapolationRule());
+    ddcx::SplineBuilder2D builder(b1, b2, bs1.size());
+    ddcx::SplineEvaluator2D ev(e1, e2);
+    int const n1 = b1.npoints(), n2 = b2.npoints(), s1 = bs1.size(), s2 = bs2.size();
+    std::vector<double> vals(size_t(NB) * n1 * n2), coords(size_t(NB) * P * 2), w(size_t(NB) * P);
+    for (auto& v : vals) v = r.next() - 0.5;
+    for (size_t i = 0; i < coords.size(); i += 2) {
+        coords[i] = 0.01 + 0.98 * r.next();
+        coords[i + 1] = -0.5 + 2.0 * r.next();
+    }
+    for (auto& x : w) x = r.next() - 0.5;
+    DeviceArray dv(vals), dc(size_t(NB) * s1 * s2), dws(size_t(builder.workspace_size(NB))), dx(coords), dw(w);
+    DeviceArray dout(size_t(NB) * P), dd1(size_t(NB) * P), dgx(size_t(NB) * P * 2), dgc(size_t(NB) * s1 * s2);
+    builder(dc.p, dv.p, NB, dws.p);
+    int64_t const cbs = int64_t(s1) * s2;
+    ev(dout.p, dx.p, P, dc.p, cbs, s2, 1, NB);
+    ev.deriv_dim_1(dd1.p, dx.p, P, dc.p, cbs, s2, 1, NB);
+    int64_t const wb = ev.eval_vjp_workspace(P, NB);
+    DeviceArray dwork(size_t(std::max<int64_t>(wb, 8) + 7) / 8);
+    ev.eval_vjp(dx.p, dw.p, dgx.p, P, dc.p, dgc.p, cbs, s2, 1, NB, wb > 0 ? dwork.p : nullptr, wb);
+    dump("2d_breaks", br);
+    dump("2d_vals", vals);
+    dump("2d_coords", coords);
+    dump("2d_w", w);
+    dump("2d_coef", dc.get());
+    dump("2d_eval", dout.get());
+    dump("2d_deriv1", dd1.get());
+    dump("2d_gx", dgx.get());
+    dump("2d_gcoef", dgc.get());
+}
+
+} // namespace
+
+int main(int argc, char** argv)
+{
+    if (argc != 2) {
+        std::fprintf(stderr, "usage: %s OUTPUT_DIR\n", argv[0]);
+        return 2;
+    }
+    g_dir = argv[1];
+    try {
+        std::printf("facade_device: 1D batch-contiguous\n");
+        std::fflush(stdout);
+        case_1d(true);
+        std::printf("facade_device: 1D interpolation-contiguous\n");
+        std::fflush(stdout);
+        case_1d(false);
+        std::printf("facade_device: constant rules with eval_pos\n");
+        std::fflush(stdout);
+        case_rules();
+        std::printf("facade_device: 2D\n");
+        std::fflush(stdout);
+        case_2d();
+    } catch (std::exception const& e) {
+        std::fprintf(stderr, "facade_device: %s\n", e.what());
+        return 1;
+    }
+    std::printf("facade_device: done\n");
+    return 0;
+}

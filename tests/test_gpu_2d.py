@@ -54,6 +54,39 @@ def test_build2d_matches_oracle(n1, n2, nb, cuda):
     assert rel_err(got, ref) <= TOL
 
 
+@pytest.mark.parametrize("n1,n2,nb", [(1024, 1024, 2), (300, 256, 3)])
+def test_build2d_fp32_refine_matches_oracle(n1, n2, nb, cuda):
+    """The tensor-product workload (non-uniform cubic Greville x random
+    breakpoints x uniform periodic cubic, 1024 x 1024 at full size) built
+    by SplineBuilder2D over two FP32-sweep + FP64-refinement builders,
+    against the FP64 oracle: <= 1e-12 relative on every line of the final
+    (dimension-2) pass and on every batch entry."""
+    import torch
+    import ddc_amd as D
+    import oracle as O
+
+    br = breaks_for("random", n1, seed=4)
+    o1 = O.Spline(3, False, G, G, breaks=br)
+    o2 = O.Spline(3, True, P, P, ncells=n2, rmin=0.0, rmax=1.0)
+    bs1 = D.NonUniformBSplines(3, br)
+    bs2 = D.UniformBSplines(3, 0.0, 1.0, n2, periodic=True)
+    F = D.Precision.FP32_REFINE
+    b1 = D.SplineBuilder(bs1, D.SplineBuilderClosure.GREVILLE, D.SplineBuilderClosure.GREVILLE, precision=F)
+    b2 = D.SplineBuilder(bs2, precision=F)
+    assert b1.info.kernel == 2 and b2.info.kernel == 2, "mixed-precision kernel not selected"
+    vals = _vals(o1, o2, nb, 3)
+    ref = O.spline2d_build(o1, o2, vals)
+    coef = torch.full((nb, bs1.size(), bs2.size()), np.nan, dtype=torch.float64, device=cuda)
+    D.SplineBuilder2D(b1, b2)(coef, torch.from_numpy(vals).to(cuda))
+    got = coef.cpu().numpy()
+    assert np.all(np.isfinite(got))
+    line = (np.abs(got - ref).max(axis=2) / np.abs(ref).max(axis=2)).max()
+    entry = max(rel_err(got[l], ref[l]) for l in range(nb))
+    print(f"2D FP32+refine {n1}x{n2}: steps {b1.info.refine_steps}/{b2.info.refine_steps}, "
+          f"max line err {line:.2e}, max entry err {entry:.2e}")
+    assert line <= 1e-12 and entry <= 1e-12
+
+
 @pytest.mark.parametrize("orders", [(0, 0), (1, 0), (0, 1), (1, 1)])
 def test_eval2d_matches_oracle(orders, cuda):
     import torch

@@ -1,0 +1,82 @@
+"""The C++ facade (include/ddcx.hpp) on device arrays: tests/cpp/facade_device,
+a compiled C++ program, builds and evaluates through SplineBuilder /
+SplineEvaluator (both layouts, periodic and constant rules with eval_pos)
+and SplineBuilder2D / SplineEvaluator2D (values, deriv_dim_1, adjoint), and
+writes its inputs and outputs; they are checked here against the oracle at
+the parity bars (1e-13; 1e-12 for accumulated coefficient gradients)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+EXE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "facade_device")
+
+
+@pytest.fixture(scope="module")
+def outputs(cuda, tmp_path_factory):
+    if not os.path.exists(EXE):
+        pytest.fail(f"{EXE} is not built (make -C tests/cpp)")
+    d = tmp_path_factory.mktemp("facade")
+    r = subprocess.run([EXE, str(d)], capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stderr
+    return lambda name: np.fromfile(os.path.join(d, name + ".bin"), dtype=np.float64)
+
+
+@pytest.mark.parametrize("tag", ["1d_bc_", "1d_ic_"])
+def test_facade_1d(outputs, tag):
+    nc, B, P = 64, 33, 100
+    orc = O.Spline(3, True, O.PERIODIC, O.PERIODIC, ncells=nc)
+    m, s = orc.npts, orc.size
+    bc = tag == "1d_bc_"
+    lay = "batch_contiguous" if bc else "interp_contiguous"
+    shp = (lambda r: (r, B)) if bc else (lambda r: (B, r))
+    vals, coords = outputs(tag + "vals").reshape(shp(m)), outputs(tag + "coords").reshape(shp(P))
+    coef = orc.build(vals, layout=lay)
+    assert rel_err(outputs(tag + "coef").reshape(shp(s)), coef) <= 1e-13
+    assert rel_err(outputs(tag + "eval").reshape(shp(P)), orc.eval(coords, coef, layout=lay)) <= 1e-13
+    d1 = orc.eval(coords, coef, deriv=1, layout=lay)
+    assert rel_err(outputs(tag + "deriv1").reshape(shp(P)), d1) <= 1e-13
+    assert rel_err(outputs(tag + "integrals"), orc.integrate(coef, layout=lay)) <= 1e-13
+
+
+def test_facade_constant_rules(outputs):
+    B, P = 5, 200
+    orc = O.Spline(3, False, O.GREVILLE, O.GREVILLE, breaks=outputs("rules_breaks"))
+    vals = outputs("rules_vals").reshape(orc.npts, B)
+    coords = outputs("rules_coords").reshape(P, B)
+    coef = orc.build(vals)
+    ref = orc.eval(coords, coef, extrap=("constant", "constant"), eval_pos=(0.3, 0.8))
+    assert rel_err(outputs("rules_eval").reshape(P, B), ref) <= 1e-13
+
+
+def test_facade_2d(outputs):
+    NB, P = 2, 500
+    o1 = O.Spline(3, False, O.GREVILLE, O.GREVILLE, breaks=outputs("2d_breaks"))
+    o2 = O.Spline(3, True, O.PERIODIC, O.PERIODIC, ncells=16)
+    s1, s2 = o1.size, o2.size
+    vals = outputs("2d_vals").reshape(NB, o1.npts, o2.npts)
+    coords = outputs("2d_coords").reshape(NB, P, 2)
+    w = outputs("2d_w").reshape(NB, P)
+    coef = O.spline2d_build(o1, o2, vals)
+    assert rel_err(outputs("2d_coef").reshape(NB, s1, s2), coef) <= 1e-13
+    assert rel_err(outputs("2d_eval").reshape(NB, P), O.spline2d_eval(o1, o2, coords, coef)) <= 1e-13
+    assert rel_err(outputs("2d_deriv1").reshape(NB, P), O.spline2d_eval(o1, o2, coords, coef, 1, 0)) <= 1e-13
+    # adjoint: gcoef against the Jacobian of the oracle's evaluation, gx
+    # against w times the partials at the wrapped coordinates
+    gc = outputs("2d_gcoef").reshape(NB, s1, s2)
+    units = np.eye(s1 * s2).reshape(s1 * s2, s1, s2)
+    for l in range(NB):
+        J = O.spline2d_eval(o1, o2, np.broadcast_to(coords[l], (s1 * s2, P, 2)).copy(), units)
+        assert rel_err(gc[l], (J @ w[l]).reshape(s1, s2)) <= 1e-12
+    wrapped = coords.copy()
+    wrapped[..., 1] -= np.floor(wrapped[..., 1])
+    gx = outputs("2d_gx").reshape(NB, P, 2)
+    assert rel_err(gx[..., 0], w * O.spline2d_eval(o1, o2, wrapped, coef, 1, 0)) <= 1e-13
+    assert rel_err(gx[..., 1], w * O.spline2d_eval(o1, o2, wrapped, coef, 0, 1)) <= 1e-13

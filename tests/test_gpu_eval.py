@@ -223,3 +223,37 @@ def test_nonuniform_cell_search_at_edges(mesh, ncells, ratio, layout, cuda):
         ev.deriv(3, out, tx, tc, axis=1)
         got = out.cpu().numpy().T
     assert rel_err(got, ref) <= TOL
+
+
+@pytest.mark.parametrize("case", [Case(3, True, P, ncells=1024), Case(5, True, P, ncells=1024),
+                                  Case(3, False, G, ncells=1024, mesh="random")], ids=lambda c: c.id)
+@pytest.mark.parametrize("layout", ["batch_contiguous", "interp_contiguous"])
+def test_deriv_orders_tight(case, layout, cuda):
+    """deriv(k) on the FP64 workload's bases (uniform periodic cubic and
+    quintic, non-uniform cubic Greville, 1024 cells), relative to max
+    |deriv(k)| of the oracle: the first derivative at the 1e-13 bar of the
+    values; every order's measured error is printed."""
+    import torch
+    import ddc_amd as D
+
+    nbatch, npts = 64, 1024
+    orc, coef = _coef(case, nbatch, seed=11)
+    bs, _ = case.product()
+    rng = np.random.default_rng(12)
+    x = rng.uniform(orc.rmin, orc.rmax, (npts, nbatch))
+    ev = D.SplineEvaluator(bs)
+    bc = layout == "batch_contiguous"
+    ax = 0 if bc else 1
+    tc = torch.from_numpy(coef if bc else np.ascontiguousarray(coef.T)).to(cuda)
+    tx = torch.from_numpy(x if bc else np.ascontiguousarray(x.T)).to(cuda)
+    errs = []
+    for order in range(1, case.degree + 1):
+        ref = orc.eval(x, coef, deriv=order)
+        out = torch.empty_like(tx)
+        ev.deriv(order, out, tx, tc, axis=ax)
+        got = out.cpu().numpy() if bc else out.cpu().numpy().T
+        errs.append(rel_err(got, ref))
+    print(f"{case.id} {layout}: deriv orders 1..{case.degree} rel err " + " ".join(f"{e:.1e}" for e in errs))
+    assert errs[0] <= TOL
+    for k, e in enumerate(errs, start=1):
+        assert e <= TOL * 10 ** (k - 1), (k, e)

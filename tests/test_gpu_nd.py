@@ -123,6 +123,35 @@ def test_eval3d_matches_oracle(spec, orders, cuda):
     assert rel_err(out.cpu().numpy(), ref) <= TOL * 10 ** sum(orders)
 
 
+@pytest.mark.parametrize("spec,orders", [
+    ([(2, "up", 9), (5, "nug", 14), (4, "ug", 7)], (2, 5, 4)),
+    ([(2, "up", 9), (5, "nug", 14), (4, "ug", 7)], (1, 4, 3)),
+    ([(5, "up", 12), (5, "nug", 14), (5, "ug", 10)], (5, 4, 5)),
+    ([(4, "ug", 8), (4, "nug", 9)], (4, 4)),
+], ids=["mixed_top", "mixed_high", "d5_fixed", "d4_2d"])
+def test_evalnd_orders_up_to_degree(spec, orders, cuda):
+    """deriv() with every dimension's order up to its degree
+    (spline_evaluator_nd.hpp:735 asserts order <= degree), on the mixed-degree
+    kernel and the fixed-degree one, against the oracle."""
+    import torch
+    import ddc_amd as D
+    import oracle as O
+
+    bases = _bases(spec)
+    ss = [x[0] for x in bases]
+    rng = np.random.default_rng(21)
+    nb, npts = 2, 300
+    coef = _coef(ss, nb, rng)
+    coords = np.stack([rng.uniform(o.rmin, o.rmax, (nb, npts)) for o in ss], axis=-1)
+    ref = O.splinend_eval(ss, coords, coef, orders=list(orders))
+    ev = D.SplineEvaluatorND(*[x[3] for x in bases])
+    out = torch.full((nb, npts), np.nan, dtype=torch.float64, device=cuda)
+    ev.deriv(orders, out, torch.from_numpy(coords).to(cuda), torch.from_numpy(coef).to(cuda))
+    err = rel_err(out.cpu().numpy(), ref)
+    print(f"orders {orders}: rel err {err:.2e}")
+    assert err <= 1e-12
+
+
 @pytest.mark.parametrize("spec", [[(3, "nug", 20)], [(2, "up", 9), (5, "nug", 14)],
                                   [(1, "ug", 5), (3, "up", 8), (2, "nug", 6), (4, "up", 7)],
                                   [(4, "nug", 11), (4, "up", 9)], [(5, "up", 7), (5, "nug", 12)],
@@ -216,3 +245,54 @@ def test_build_eval3d_reproduces_interpolation_points(cuda):
     d2 = torch.empty_like(out)
     ev.deriv((1, 0, 0), d2, torch.from_numpy(coords).to(cuda), coef)
     assert torch.equal(d1, d2)
+
+
+@pytest.mark.parametrize("closures", [("H", "H", "H"), ("H", "G", "P"), ("G", "H", "H"), ("P", "H", "G")])
+def test_build3d_hermite_matches_oracle(closures, cuda):
+    """SplineBuilder3D with Hermite closures (spline_builder_3d.hpp:889-1090):
+    the derivative, edge and corner cross-derivative inputs as the reference
+    takes them, random data, against the oracle's restatement (pinned by
+    tensor-polynomial reproduction in test_oracle_nd.py)."""
+    import torch
+    import ddc_amd as D
+    import oracle as O
+    from test_oracle_nd import hermite_3d_data
+
+    CL = {"H": (O.HERMITE, D.SplineBuilderClosure.HERMITE), "G": (O.GREVILLE, D.SplineBuilderClosure.GREVILLE),
+          "P": (O.PERIODIC, D.SplineBuilderClosure.PERIODIC)}
+    specs = [(3, False, 9), (5, True, 8), (4, True, 7)]
+    ss, bl = [], []
+    for k, ((deg, uni, n), c) in enumerate(zip(specs, closures)):
+        oc, dc = CL[c]
+        per = c == "P"
+        if uni:
+            ss.append(O.Spline(deg, True, oc, oc, ncells=n, rmin=0.0, rmax=1.0))
+            bs = D.UniformBSplines(deg, 0.0, 1.0, n, periodic=per)
+        else:
+            br = breaks_for("random", n, seed=30 + k)
+            ss.append(O.Spline(deg, False, oc, oc, breaks=br))
+            bs = D.NonUniformBSplines(deg, br, periodic=per)
+        bl.append(D.SplineBuilder(bs, dc, dc))
+    polys = [np.polynomial.Polynomial(np.random.default_rng(k).standard_normal(s.degree + 1))
+             for k, s in enumerate(ss)]
+    nb = 2
+    vals, d = hermite_3d_data(ss, polys, nb)
+    rng = np.random.default_rng(7)
+    vals = vals + 0.1 * rng.standard_normal(vals.shape)  # not a polynomial: every input matters
+    d = {k: v + 0.1 * rng.standard_normal(v.shape) for k, v in d.items()}
+    # inputs the closures do not take are not passed (the reference passes nullopt)
+    used = {k: v for k, v in d.items() if _takes(k, closures)}
+    ref = O.spline3d_build(*ss, vals, used)
+    t = {k: torch.from_numpy(np.ascontiguousarray(v)).to(cuda) for k, v in used.items()}
+    coef = torch.full((nb,) + tuple(s.size for s in ss), np.nan, dtype=torch.float64, device=cuda)
+    D.SplineBuilder3D(*bl)(coef, torch.from_numpy(vals).to(cuda), **t)
+    got = coef.cpu().numpy()
+    assert np.all(np.isfinite(got))
+    assert rel_err(got, ref) <= TOL
+
+
+def _takes(name, closures):
+    """Whether SplineBuilder3D reads argument `name` for these closures:
+    every dimension the argument carries derivatives of must be Hermite."""
+    dims = [int(part[-1]) for part in name.split("_")[1:] if part[:3] in ("min", "max")]
+    return all(closures[k - 1] == "H" for k in dims)

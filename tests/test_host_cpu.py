@@ -152,6 +152,19 @@ int main()
     }
     ddcx::SplineBuilder3D b3(b, pb, b, bs.size(), ub.size(), bs.size());
     std::printf("%lld %lld %d\n", (long long)b3.work1_size(2), (long long)b3.work2_size(2), e3.dimension());
+    // a permuted (strided) array: extents (6, 5, 7), strides (7, 42, 1)
+    ddcx_layout ps = ddcx::batched_layout({6, 5, 7}, {7, 42, 1}, 1, &nbatch);
+    std::printf("%lld %lld %lld %lld %lld\n", (long long)ps.row_stride, (long long)ps.inner_count,
+                (long long)ps.inner_stride, (long long)ps.outer_stride, (long long)nbatch);
+    try {
+        ddcx::SplineEvaluator bad(bs, ddcx::ConstantExtrapolationRule(2.0), ddcx::NullExtrapolationRule());
+        std::printf("no error\n");
+    } catch (std::runtime_error const& e) {
+        std::printf("error: %s\n", e.what());
+    }
+    ddcx::SplineEvaluator ok(bs, ddcx::ConstantExtrapolationRule(0.25, 0.0, 1.0), ddcx::ConstantExtrapolationRule(0.75));
+    ddcx::SplineBuilder2D b2d(b, pb, bs.size());
+    std::printf("%lld\n", (long long)b2d.workspace_size(3));
     return 0;
 }
 """
@@ -200,6 +213,12 @@ def test_cpp_facade(tmp_path):
     assert out[7].startswith("error:") and "order" in out[7]
     n1, s1 = b.npoints(), bs.size()
     assert out[8].split() == [str(2 * s1 * pb.npoints() * n1), str(2 * s1 * ub.size() * n1), "3"]
+    tp = torch.empty((5, 6, 7), dtype=torch.float64).permute(1, 0, 2)
+    lay, nb = D.batched_layout(tp, 1)
+    assert out[9].split() == [str(v) for v in (lay.row_stride, lay.inner_count, lay.inner_stride,
+                                               lay.outer_stride, nb)]
+    assert out[10].startswith("error:") and "inside the domain" in out[10]
+    assert out[11] == str(3 * s1 * pb.npoints())
 
 
 def test_knots_as_interpolation_points():

@@ -133,3 +133,84 @@ def test_3d_build_is_three_1d_builds():
         out = s.build(flat).reshape((s.size,) + moved.shape[1:])
         cur = np.moveaxis(out, 0, ax)
     assert np.array_equal(got, cur)
+
+
+def hermite_3d_data(ss, polys, nb=1):
+    """Interpolation data of P1(x) P2(y) P3(z) for the 3D builder with the
+    reference's argument names: values, boundary derivatives per dimension
+    (row k is order k + degree % 2), and the cross derivatives on edges and
+    corners, each taken from the polynomials' exact derivatives."""
+    pts = [s.points() for s in ss]
+    bnd = [{"min": s.rmin, "max": s.rmax} for s in ss]
+
+    def dv(k, p, order, t):
+        q = polys[k].deriv(order) if order else polys[k]
+        return q(t)
+
+    def field(spec):
+        # spec per dimension: None (the interpolation points) or (side, orders list)
+        axes = []
+        for k in range(3):
+            if spec[k] is None:
+                axes.append(np.array([dv(k, None, 0, t) for t in pts[k]]))
+            else:
+                side, ords = spec[k]
+                axes.append(np.array([dv(k, None, o, bnd[k][side]) for o in ords]))
+        a = np.einsum("i,j,k->ijk", *axes)
+        return np.repeat(a[None], nb, axis=0)
+
+    ords = [[i + s.degree % 2 for i in range(s.nbv_l)] for s in ss]
+    out = {"vals": field([None, None, None])}
+    for k in range(3):
+        for side in ("min", "max"):
+            spec = [None, None, None]
+            spec[k] = (side, ords[k])
+            out[f"derivs_{side}{k + 1}"] = field(spec)
+    for (a, b) in ((0, 1), (1, 2), (0, 2)):
+        for sa in ("min", "max"):
+            for sb in ("min", "max"):
+                spec = [None, None, None]
+                spec[a], spec[b] = (sa, ords[a]), (sb, ords[b])
+                out[f"mixed_derivs_{sa}{a + 1}_{sb}{b + 1}"] = field(spec)
+    for s1 in ("min", "max"):
+        for s2 in ("min", "max"):
+            for s3 in ("min", "max"):
+                out[f"mixed_derivs_{s1}1_{s2}2_{s3}3"] = field([(s1, ords[0]), (s2, ords[1]), (s3, ords[2])])
+    vals = out.pop("vals")
+    return vals, out
+
+
+def test_3d_hermite_build_reproduces_tensor_polynomials():
+    """SplineBuilder3D with Hermite closures in all three dimensions (boundary
+    derivatives, edge and corner cross derivatives, spline_builder_3d.hpp:
+    889-1090) reproduces P1(x) P2(y) P3(z) of degree <= each spline degree
+    exactly; the triple corner derivatives are used (changing one changes
+    the coefficients), and an all-Greville build ignores the derivative data."""
+    from cases import breaks_for
+
+    H, G = O.HERMITE, O.GREVILLE
+    ss = [O.Spline(3, False, H, H, breaks=breaks_for("random", 6, seed=1)),
+          O.Spline(5, True, H, H, ncells=5, rmin=-0.5, rmax=1.5),
+          O.Spline(3, True, H, H, ncells=7, rmin=0.0, rmax=2.0)]
+    polys = [np.polynomial.Polynomial([0.4, -1.1, 0.6, 0.9]),
+             np.polynomial.Polynomial([1.0, 0.3, -0.7, 0.2, 0.5, -0.4]),
+             np.polynomial.Polynomial([-0.2, 0.8, 0.5, -0.3])]
+    vals, d = hermite_3d_data(ss, polys)
+    coef = O.spline3d_build(*ss, vals, d)
+    rng = np.random.default_rng(4)
+    xyz = np.stack([rng.uniform(s.rmin, s.rmax, 400) for s in ss], axis=-1)[None]
+    ref = polys[0](xyz[0, :, 0]) * polys[1](xyz[0, :, 1]) * polys[2](xyz[0, :, 2])
+    got = O.splinend_eval(ss, xyz, coef)[0]
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+    dref = polys[0].deriv()(xyz[0, :, 0]) * polys[1].deriv(2)(xyz[0, :, 1]) * polys[2](xyz[0, :, 2])
+    dgot = O.splinend_eval(ss, xyz, coef, orders=[1, 2, 0])[0]
+    assert np.abs(dgot - dref).max() <= 1e-10 * np.abs(dref).max()
+    bad = dict(d)
+    bad["mixed_derivs_max1_min2_max3"] = d["mixed_derivs_max1_min2_max3"].copy()
+    bad["mixed_derivs_max1_min2_max3"][0, 0, 0, 0] += 1.0
+    assert np.abs(O.spline3d_build(*ss, vals, bad) - coef).max() > 1e-6
+    gs = [O.Spline(3, False, G, G, breaks=breaks_for("random", 6, seed=1)),
+          O.Spline(3, True, G, G, ncells=5, rmin=-0.5, rmax=1.5),
+          O.Spline(3, True, G, G, ncells=7, rmin=0.0, rmax=2.0)]
+    gv = np.random.default_rng(5).standard_normal((2,) + tuple(s.npts for s in gs))
+    assert np.array_equal(O.spline3d_build(*gs, gv, {}), O.splinend_build(gs, gv))
