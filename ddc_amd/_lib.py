@@ -85,6 +85,7 @@ SIGNATURES = {
     "ddcx_evaluator2d_eval_vjp_workspace": [_P, _P, _I64, _I64, _P],
     "ddcx_evaluator2d_eval_vjp_binned": [_P, _P, _P, _P, _P, _I64, _P, _P, _I64, _I64, _I64, _I64, _P, _I64, _P],
     "ddcx_evaluatornd_integrate": [_P, C.c_int, _P, _I64, _P, _I64, _P, _I64, _P],
+    "ddcx_evaluator2d_eval_vjp_binned_static": [_P, _P, _P, _P, _P, _I64, _P, _P, _I64, _I64, _I64, _I64, _P],
     "ddcx_problem_create": [C.POINTER(ddcx_problem_desc), _PP],
     "ddcx_problem_destroy": [_P],
     "ddcx_problem_set_element": [_P, C.c_int, C.c_int, C.c_double],

@@ -67,7 +67,7 @@ __global__ void __launch_bounds__(LB* K) build_refine_kernel(RefineView rv, Buil
     int const r0 = seg * L;
     int const nq = rv.nq;
     int const nb = rv.nb;
-    int const RS = rv.rs;
+    constexpr int RS = KF + KB + 1 + (FDIV ? 1 : 0); // row record length, fixed by the band shape
     const double* __restrict__ vwg = vals + lay(vl, 0, line0);
     double* __restrict__ cwg = coef + lay(cl, 0, line0);
     uint32_t const vlo = uint32_t(vb - lay(vl, 0, line0)) + uint32_t(rv.q0 + r0) * uint32_t(vl.rs);
@@ -433,6 +433,8 @@ template <int K, int L, int KF, int KB, bool FDIV, bool BDIVF>
 void launch_cfg(BuilderPlan const& p, double* coef, Layout cl, const double* vals, Layout vl, const double* dl,
                 const double* dr, Layout dlay, int64_t nbatch, hipStream_t stream)
 {
+    if (p.refine.rs != KF + KB + 1 + (FDIV ? 1 : 0))
+        throw Error(DDCX_ERR_INVALID, "row table record length does not match the band shape");
     constexpr int NT = 1024, LB = NT / K;
     constexpr int KE = KF > KB ? KF : KB;
     constexpr int KX = 2 * KF > KE ? 2 * KF : KE;

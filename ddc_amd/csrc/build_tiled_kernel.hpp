@@ -30,7 +30,9 @@ namespace tiled_build {
 
 using namespace tiled;
 
-template <int LB, int K, int L, int KF, int KB, bool FDIV, bool BDIVF, bool TLDS, bool IC>
+// TMODE: 0 row table read from global memory (L2), 1 staged in LDS whole,
+// 2 staged in LDS one sweep's fields at a time (tables too large for 1)
+template <int LB, int K, int L, int KF, int KB, bool FDIV, bool BDIVF, int TMODE, bool IC>
 __global__ void __launch_bounds__(LB* K) build_tiled_kernel(TiledView tv, BuildView bv,
                                                            const double* __restrict__ vals, Layout vl,
                                                            const double* __restrict__ dl,
@@ -39,6 +41,8 @@ __global__ void __launch_bounds__(LB* K) build_tiled_kernel(TiledView tv, BuildV
 {
     constexpr int KE = KF > KB ? KF : KB;
     constexpr int NT = LB * K;
+    constexpr bool TLDS = TMODE != 0;
+    constexpr int RSF = KF + (FDIV ? 1 : 0), RSB = KB + 1; // split records
     __shared__ double ex[KE][K][LB];
     __shared__ double lamx[kLamMax][LB];
     __shared__ double x2s[kNbMax][LB];
@@ -56,7 +60,7 @@ __global__ void __launch_bounds__(LB* K) build_tiled_kernel(TiledView tv, BuildV
     int64_t const cb = lay(cl, 0, lc);
     int const r0 = seg * L;
     int const nq = tv.nq;
-    int const RS = tv.rs;
+    constexpr int RS = KF + KB + 1 + (FDIV ? 1 : 0); // row record length, fixed by the band shape
     // Workgroup-uniform bases plus 32-bit per-lane offsets (the host checks
     // that every offset of a tile fits), so that each load/store needs one
     // address register instead of two.
@@ -69,8 +73,9 @@ __global__ void __launch_bounds__(LB* K) build_tiled_kernel(TiledView tv, BuildV
     // dynamic LDS: the row table and the padded Q^-1 gamma blocks (TLDS),
     // then the staging area, LB x kStagePitch doubles, and the per-line
     // offsets of vals and coef (IC)
-    double* gpad = rt_lds + K * L * RS;
-    double* stg = rt_lds + (TLDS ? K * L * RS + tv.ngb * L * kNbMax : 0);
+    int const tab = TMODE == 2 ? K * L * (RSF > RSB ? RSF : RSB) : K * L * RS; // doubles of the table region
+    double* gpad = rt_lds + tab;
+    double* stg = rt_lds + (TLDS ? tab + tv.ngb * L * kNbMax : 0);
     uint32_t* loff = reinterpret_cast<uint32_t*>(stg + LB * kStagePitch);
     int const nl = int(min<int64_t>(LB, nbatch - line0));
 
@@ -135,23 +140,34 @@ __global__ void __launch_bounds__(LB* K) build_tiled_kernel(TiledView tv, BuildV
 
     // per-row table and border data: staged in LDS when they fit (one copy
     // per workgroup)
-    const double* T;
-    if constexpr (TLDS) {
-        // a few double2 per thread (RS <= 10 doubles per row, K * L <= 2 * NT rows)
-        int const n2 = (K * L * RS) / 2;
-        const double2* src = reinterpret_cast<const double2*>(tv.rt);
-        double2* dst = reinterpret_cast<double2*>(rt_lds);
-        for (int idx = tid; idx < n2; idx += NT) dst[idx] = src[idx];
+    if constexpr (TMODE == 2) {
+        // the forward sweep's fields, then (after its closing barrier) the
+        // backward sweep's, in the same LDS region
+        stage_half<NT, KF, KB, FDIV, true>(tv.rt, RS, K * L, rt_lds, tid);
         stage_border<NT, true>(tv, bl, gpad, tid);
         __syncthreads();
-        T = rt_lds + r0 * RS;
-    } else {
-        stage_border<NT, false>(tv, bl, gpad, tid); // couplings only
+        tiled_forward<LB, K, L, KF, FDIV, 0, KF>(y, rt_lds + r0 * RSF, RSF, tv, ex, seg, ln);
+        stage_half<NT, KF, KB, FDIV, false>(tv.rt, RS, K * L, rt_lds, tid);
         __syncthreads();
-        T = tv.rt + size_t(r0) * RS;
+        tiled_backward<LB, K, L, KB, BDIVF, 0, KB>(y, rt_lds + r0 * RSB, RSB, tv, ex, seg, ln);
+    } else {
+        const double* T;
+        if constexpr (TLDS) {
+            // a few double2 per thread (RS <= 10 doubles per row, K * L <= 2 * NT rows)
+            int const n2 = (K * L * RS) / 2;
+            const double2* src = reinterpret_cast<const double2*>(tv.rt);
+            double2* dst = reinterpret_cast<double2*>(rt_lds);
+            for (int idx = tid; idx < n2; idx += NT) dst[idx] = src[idx];
+            stage_border<NT, true>(tv, bl, gpad, tid);
+            __syncthreads();
+            T = rt_lds + r0 * RS;
+        } else {
+            stage_border<NT, false>(tv, bl, gpad, tid); // couplings only
+            __syncthreads();
+            T = tv.rt + size_t(r0) * RS;
+        }
+        tiled_sweeps<LB, K, L, KF, KB, FDIV, BDIVF, KE>(y, T, RS, tv, ex, seg, ln);
     }
-
-    tiled_sweeps<LB, K, L, KF, KB, FDIV, BDIVF, KE>(y, T, RS, tv, ex, seg, ln);
 
     // ---------------- border (2x2 blocks) ----------------
     int const nb = tv.nb;
@@ -267,12 +283,12 @@ __global__ void __launch_bounds__(LB* K) build_tiled_kernel(TiledView tv, BuildV
     }
 }
 
-template <int K, int L, int KF, int KB, bool FDIV, bool BDIVF, bool TLDS, bool IC>
+template <int K, int L, int KF, int KB, bool FDIV, bool BDIVF, int TMODE, bool IC>
 void launch_one(BuilderPlan const& p, size_t lds, double* coef, Layout cl, const double* vals, Layout vl,
                 const double* dl, const double* dr, Layout dlay, int64_t nbatch, hipStream_t stream)
 {
     constexpr int NT = 1024, LB = NT / K;
-    auto kern = build_tiled_kernel<LB, K, L, KF, KB, FDIV, BDIVF, TLDS, IC>;
+    auto kern = build_tiled_kernel<LB, K, L, KF, KB, FDIV, BDIVF, TMODE, IC>;
     if (lds > 0)
         hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                       int(lds)),
@@ -287,25 +303,33 @@ template <int K, int L, int KF, int KB, bool FDIV, bool BDIVF>
 void launch_cfg(BuilderPlan const& p, double* coef, Layout cl, const double* vals, Layout vl, const double* dl,
                 const double* dr, Layout dlay, int64_t nbatch, hipStream_t stream)
 {
+    if (p.tiled.rs != KF + KB + 1 + (FDIV ? 1 : 0))
+        throw Error(DDCX_ERR_INVALID, "row table record length does not match the band shape");
     constexpr int NT = 1024, LB = NT / K;
     constexpr int KE = KF > KB ? KF : KB;
     constexpr size_t static_lds =
         size_t(KE * K * LB + kLamMax * LB + kNbMax * LB) * sizeof(double) + sizeof(BorderLds);
     bool const ic = vl.rs == 1 && cl.rs == 1;
     size_t const stage = ic ? size_t(LB) * kStagePitch * sizeof(double) + 2 * LB * sizeof(uint32_t) : 0;
-    size_t const table = (size_t(K) * L * p.tiled.rs + gamma_lds_doubles(p.tview)) * sizeof(double);
-    bool const tlds = static_lds + table + stage <= size_t(160 * 1024) && p.tiled.rs <= 10;
-    size_t const lds = (tlds ? table : 0) + stage;
-    auto go = [&]<bool TL, bool I>() {
-        launch_one<K, L, KF, KB, FDIV, BDIVF, TL, I>(p, lds, coef, cl, vals, vl, dl, dr, dlay, nbatch, stream);
+    constexpr int RSH = (KF + (FDIV ? 1 : 0)) > (KB + 1) ? (KF + (FDIV ? 1 : 0)) : (KB + 1);
+    size_t const gam = gamma_lds_doubles(p.tview) * sizeof(double);
+    size_t const table = size_t(K) * L * p.tiled.rs * sizeof(double) + gam;
+    size_t const half = size_t(K) * L * RSH * sizeof(double) + gam;
+    size_t constexpr budget = 160 * 1024;
+    int const mode = (static_lds + table + stage <= budget && p.tiled.rs <= 10) ? 1
+                     : (static_lds + half + stage <= budget)                   ? 2
+                                                                               : 0;
+    size_t const lds = (mode == 1 ? table : mode == 2 ? half : 0) + stage;
+    auto go = [&]<int TM, bool I>() {
+        launch_one<K, L, KF, KB, FDIV, BDIVF, TM, I>(p, lds, coef, cl, vals, vl, dl, dr, dlay, nbatch, stream);
     };
-    if (ic) {
-        if (tlds) go.template operator()<true, true>();
-        else go.template operator()<false, true>();
-    } else {
-        if (tlds) go.template operator()<true, false>();
-        else go.template operator()<false, false>();
-    }
+    auto pick = [&]<bool I>() {
+        if (mode == 1) go.template operator()<1, I>();
+        else if (mode == 2) go.template operator()<2, I>();
+        else go.template operator()<0, I>();
+    };
+    if (ic) pick.template operator()<true>();
+    else pick.template operator()<false>();
 }
 
 template <int K, int L>

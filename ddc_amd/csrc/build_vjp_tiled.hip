@@ -85,7 +85,7 @@ __global__ void __launch_bounds__(LB* K) build_vjp_tiled_kernel(TiledView tv, Bu
     int const r0 = seg * L;
     int const nq = tv.nq, nb = tv.nb, top = tv.top;
     int const off = bv.offset, nbas = bv.nbasis, nsz = bv.size;
-    int const RS = tv.rs;
+    constexpr int RS = KF + KB + 1 + (FDIV ? 1 : 0); // row record length, fixed by the band shape
     const double* __restrict__ cwg = gcoef + lay(cl, 0, line0);
     double* __restrict__ vwg = gvals + lay(vl, 0, line0);
     uint32_t const clo = uint32_t(cb - lay(cl, 0, line0));
@@ -310,6 +310,8 @@ template <int K, int L, int KF, int KB, bool FDIV, bool BDIVF>
 void launch_cfg(BuilderPlan const& p, const double* gcoef, Layout cl, double* gvals, Layout vl, double* gdl,
                 double* gdr, Layout dlay, int64_t nbatch, hipStream_t stream)
 {
+    if (p.tiled_t.rs != KF + KB + 1 + (FDIV ? 1 : 0))
+        throw Error(DDCX_ERR_INVALID, "row table record length does not match the band shape");
     constexpr int NT = 1024, LB = NT / K;
     constexpr int KE = KF > KB ? KF : KB;
     constexpr size_t static_lds =

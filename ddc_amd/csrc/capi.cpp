@@ -466,6 +466,31 @@ ddcx_status ddcx_evaluator2d_eval_vjp_binned(const ddcx_evaluator* e1, const ddc
     });
 }
 
+ddcx_status ddcx_evaluator2d_eval_vjp_binned_static(const ddcx_evaluator* e1, const ddcx_evaluator* e2,
+                                                    const double* coords, const double* w, double* gx, int64_t npts,
+                                                    const double* coef, double* gcoef, int64_t coef_batch_stride,
+                                                    int64_t st1, int64_t st2, int64_t nbatch, void* stream)
+{
+    return guarded([&] {
+        require(e1 && e2, "null evaluator");
+        require(npts >= 0 && nbatch >= 0, "negative size");
+        if (nbatch == 0) return;
+        require(coef && gcoef, "null data pointer");
+        require(npts == 0 || (coords && w && gx), "null data pointer");
+        int64_t const need = ddcx::eval2d_vjp_workspace(*e1->bs, *e2->bs, npts, nbatch);
+        if (need == 0)
+            throw ddcx::Error(DDCX_ERR_UNSUPPORTED, "binned 2D adjoint: grid or degree out of range");
+        if (need > int64_t(DDCX_STATIC_SCRATCH_BYTES))
+            throw ddcx::Error(DDCX_ERR_INVALID, ("binned 2D adjoint: the call needs " + std::to_string(need) +
+                                                 " bytes of workspace, above the static workspace of " +
+                                                 std::to_string(int64_t(DDCX_STATIC_SCRATCH_BYTES)) + " bytes"));
+        ddcx::Rules2D const rules = rules2d(e1, e2);
+        ddcx::launch_eval2d_vjp_binned(*e1->bs, *e2->bs, rules, coords, w, gx, npts, coef, gcoef, coef_batch_stride,
+                                       st1, st2, nbatch, ddcx::static_scratch(), int64_t(DDCX_STATIC_SCRATCH_BYTES),
+                                       static_cast<hipStream_t>(stream));
+    });
+}
+
 ddcx_status ddcx_evaluatornd_eval(const ddcx_evaluator* const* evs, int nd, const int* orders, double* out,
                                   const double* coords, int64_t npts, const double* coef, int64_t coef_batch_stride,
                                   const int64_t* coef_strides, int64_t nbatch, void* stream)

@@ -264,6 +264,29 @@ Work2D carve(Bin2D const& g, int64_t npts, int64_t nbatch, void* base)
 
 } // namespace
 
+// The workspace of the allocation-free entry point
+// (ddcx_evaluator2d_eval_vjp_binned_static): a file-scope device array, one
+// copy per GPU, held until the process ends.  Host code never uses the
+// array's name as an address: static_scratch() resolves each device's copy
+// once with hipGetSymbolAddress and keeps it.
+static __device__ __align__(256) unsigned char scratch[DDCX_STATIC_SCRATCH_BYTES];
+
+void* static_scratch()
+{
+    static std::mutex m;
+    static std::vector<void*> ptrs;
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    std::lock_guard<std::mutex> lock(m);
+    if (int(ptrs.size()) <= dev) ptrs.resize(size_t(dev) + 1, nullptr);
+    if (!ptrs[size_t(dev)]) {
+        void* p = nullptr;
+        hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        ptrs[size_t(dev)] = p;
+    }
+    return ptrs[size_t(dev)];
+}
+
 int64_t eval2d_vjp_workspace(BSplinesPlan const& bp1, BSplinesPlan const& bp2, int64_t npts, int64_t nbatch)
 {
     Bin2D const g = bins_for(bp1, bp2, nbatch);

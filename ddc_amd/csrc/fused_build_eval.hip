@@ -50,7 +50,7 @@ __global__ void __launch_bounds__(LB* K) fused_build_eval_kernel(TiledView tv, B
     int64_t const cb = lay(cl, 0, lc);
     int const r0 = seg * L;
     int const nq = tv.nq;
-    int const RS = tv.rs;
+    constexpr int RS = KF + KB + 1 + (FDIV ? 1 : 0); // row record length, fixed by the band shape
     const double* __restrict__ vwg = vals + lay(vl, 0, line0);
     uint32_t const vlo = uint32_t(vb - lay(vl, 0, line0)) + uint32_t(tv.q0 + r0) * uint32_t(vl.rs);
     uint32_t const vrs = uint32_t(vl.rs);
@@ -215,6 +215,8 @@ void launch_one(BuilderPlan const& p, BSplinesView const& b, Rules1D rules, doub
                 const double* coords, Layout pl, int64_t npts, int64_t nbatch, hipStream_t stream, size_t lds,
                 const double* fpts, const double* fshift)
 {
+    if (p.tiled_f.rs != KF + KB + 1 + (FDIV ? 1 : 0))
+        throw Error(DDCX_ERR_INVALID, "row table record length does not match the band shape");
     constexpr int NT = 1024, LB = NT / K, L = 16;
     auto kern = fused_build_eval_kernel<LB, K, L, KF, KB, FDIV, BDIVF, D>;
     hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,

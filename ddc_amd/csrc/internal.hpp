@@ -388,6 +388,9 @@ void launch_eval2d(BSplinesPlan const& b1, BSplinesPlan const& b2, Rules2D const
                    int64_t st2, int64_t nbatch, hipStream_t stream);
 // binned 2D adjoint (eval2d_binned.hip): workspace bytes, 0 if unsupported
 int64_t eval2d_vjp_workspace(BSplinesPlan const& bp1, BSplinesPlan const& bp2, int64_t npts, int64_t nbatch);
+// device address of the current GPU's copy of the static workspace
+// (DDCX_STATIC_SCRATCH_BYTES), resolved once per device
+void* static_scratch();
 void launch_eval2d_vjp_binned(BSplinesPlan const& bp1, BSplinesPlan const& bp2, Rules2D const& rules,
                               const double* coords, const double* w, double* gx, int64_t npts, const double* coef,
                               double* gcoef, int64_t cbs, int64_t st1, int64_t st2, int64_t nbatch, void* work,

@@ -139,130 +139,153 @@ inline bool offsets_fit(Layout const& l, int64_t rows, int64_t nbatch)
 // The two sweeps of the main-block solve on a K x L tiling (one thread per
 // segment and line, L rows in registers): local pass with a zero incoming
 // state, exchange of the outgoing states through LDS, exact incoming state
-// from the transfer products, correction by the homogeneous recurrence.  The
-// row table T holds [Lf(KF) | Ub(KB) | rDb | rDf (FDIV)] per row.  No barrier
-// follows the backward sweep.
-template <int LB, int K, int L, int KF, int KB, bool FDIV, bool BDIVF, int KE>
-__device__ __forceinline__ void tiled_sweeps(double (&y)[L], const double* T, int RS, TiledView const& tv,
-                                             double (*ex)[K][LB], int seg, int ln)
+// from the transfer products, correction by the homogeneous recurrence.
+// Row records: the forward sweep reads Lf(KF) at OLF and 1/d at ORDF (FDIV),
+// the backward sweep Ub(KB) at OUB and 1/d at ORDB; the records of the full
+// row table are [Lf(KF) | Ub(KB) | rDb | rDf (FDIV)].  The forward sweep ends
+// with a barrier; the backward sweep does not.
+template <int LB, int K, int L, int KF, bool FDIV, int OLF, int ORDF>
+__device__ __forceinline__ void tiled_forward(double (&y)[L], const double* T, int RS, TiledView const& tv,
+                                              double (*ex)[K][LB], int seg, int ln)
 {
-    constexpr int oLf = 0, oUb = KF, orDb = KF + KB, orDf = KF + KB + 1;
-    // ---------------- forward sweep: local pass, exchange, fixup ----------------
-    {
-        double s[KF];
+    double s[KF];
 #pragma unroll
-        for (int k = 0; k < KF; ++k) s[k] = 0.0;
+    for (int k = 0; k < KF; ++k) s[k] = 0.0;
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+        const double* R = T + i * RS;
+        double t = y[i];
+#pragma unroll
+        for (int k = KF; k >= 1; --k) t = fma(-s[k - 1], R[OLF + k - 1], t);
+        if constexpr (FDIV) t = t * R[ORDF];
+        y[i] = t;
+#pragma unroll
+        for (int k = KF - 1; k > 0; --k) s[k] = s[k - 1];
+        s[0] = t;
+        chunk_fence(i);
+    }
+#pragma unroll
+    for (int k = 0; k < KF; ++k) ex[k][seg][ln] = s[k];
+    __syncthreads();
+    double in[KF];
+#pragma unroll
+    for (int k = 0; k < KF; ++k) in[k] = 0.0;
+    for (int w = 1; w <= tv.wf; ++w) {
+        int const src = seg - w;
+        if (src >= 0) {
+            const double* m = tv.Mf + (size_t(seg) * tv.wf + (w - 1)) * (KF * KF);
+#pragma unroll
+            for (int a = 0; a < KF; ++a)
+#pragma unroll
+                for (int b = 0; b < KF; ++b) in[a] = fma(m[a * KF + b], ex[b][src][ln], in[a]);
+        }
+    }
+    if (seg > 0) {
+        // rows += homogeneous forward recurrence started from the incoming state
 #pragma unroll
         for (int i = 0; i < L; ++i) {
             const double* R = T + i * RS;
-            double t = y[i];
+            double t = 0.0;
 #pragma unroll
-            for (int k = KF; k >= 1; --k) t = fma(-s[k - 1], R[oLf + k - 1], t);
-            if constexpr (FDIV) t = t * R[orDf];
-            y[i] = t;
+            for (int k = KF; k >= 1; --k) t = fma(-in[k - 1], R[OLF + k - 1], t);
+            if constexpr (FDIV) t = t * R[ORDF];
+            y[i] += t;
 #pragma unroll
-            for (int k = KF - 1; k > 0; --k) s[k] = s[k - 1];
-            s[0] = t;
+            for (int k = KF - 1; k > 0; --k) in[k] = in[k - 1];
+            in[0] = t;
             chunk_fence(i);
         }
-#pragma unroll
-        for (int k = 0; k < KF; ++k) ex[k][seg][ln] = s[k];
-        __syncthreads();
-        double in[KF];
-#pragma unroll
-        for (int k = 0; k < KF; ++k) in[k] = 0.0;
-        for (int w = 1; w <= tv.wf; ++w) {
-            int const src = seg - w;
-            if (src >= 0) {
-                const double* m = tv.Mf + (size_t(seg) * tv.wf + (w - 1)) * (KF * KF);
-#pragma unroll
-                for (int a = 0; a < KF; ++a)
-#pragma unroll
-                    for (int b = 0; b < KF; ++b) in[a] = fma(m[a * KF + b], ex[b][src][ln], in[a]);
-            }
-        }
-        if (seg > 0) {
-            // rows += homogeneous forward recurrence started from the incoming state
-#pragma unroll
-            for (int i = 0; i < L; ++i) {
-                const double* R = T + i * RS;
-                double t = 0.0;
-#pragma unroll
-                for (int k = KF; k >= 1; --k) t = fma(-in[k - 1], R[oLf + k - 1], t);
-                if constexpr (FDIV) t = t * R[orDf];
-                y[i] += t;
-#pragma unroll
-                for (int k = KF - 1; k > 0; --k) in[k] = in[k - 1];
-                in[0] = t;
-                chunk_fence(i);
-            }
-        }
-        __syncthreads();
     }
+    __syncthreads();
+}
 
-    // ---------------- backward sweep: local pass, exchange, fixup ----------------
-    {
-        double s[KB];
+template <int LB, int K, int L, int KB, bool BDIVF, int OUB, int ORDB>
+__device__ __forceinline__ void tiled_backward(double (&y)[L], const double* T, int RS, TiledView const& tv,
+                                               double (*ex)[K][LB], int seg, int ln)
+{
+    double s[KB];
 #pragma unroll
-        for (int k = 0; k < KB; ++k) s[k] = 0.0;
+    for (int k = 0; k < KB; ++k) s[k] = 0.0;
+#pragma unroll
+    for (int i = L - 1; i >= 0; --i) {
+        const double* R = T + i * RS;
+        double t;
+        if constexpr (BDIVF) {
+            t = fma(-s[0], R[OUB], y[i] * R[ORDB]);
+        } else {
+            t = y[i];
+#pragma unroll
+            for (int k = KB; k >= 1; --k) t = fma(-s[k - 1], R[OUB + k - 1], t);
+            t = t * R[ORDB];
+        }
+        y[i] = t;
+#pragma unroll
+        for (int k = KB - 1; k > 0; --k) s[k] = s[k - 1];
+        s[0] = t;
+        chunk_fence(i);
+    }
+#pragma unroll
+    for (int k = 0; k < KB; ++k) ex[k][seg][ln] = s[k];
+    __syncthreads();
+    double in[KB];
+#pragma unroll
+    for (int k = 0; k < KB; ++k) in[k] = 0.0;
+    for (int w = 1; w <= tv.wb; ++w) {
+        int const src = seg + w;
+        if (src < K) {
+            const double* m = tv.Mb + (size_t(seg) * tv.wb + (w - 1)) * (KB * KB);
+#pragma unroll
+            for (int a = 0; a < KB; ++a)
+#pragma unroll
+                for (int b = 0; b < KB; ++b) in[a] = fma(m[a * KB + b], ex[b][src][ln], in[a]);
+        }
+    }
+    if (seg + 1 < K) {
+        // rows += homogeneous backward recurrence started from the incoming state
 #pragma unroll
         for (int i = L - 1; i >= 0; --i) {
             const double* R = T + i * RS;
             double t;
             if constexpr (BDIVF) {
-                t = fma(-s[0], R[oUb], y[i] * R[orDb]);
+                t = -in[0] * R[OUB];
             } else {
-                t = y[i];
+                t = 0.0;
 #pragma unroll
-                for (int k = KB; k >= 1; --k) t = fma(-s[k - 1], R[oUb + k - 1], t);
-                t = t * R[orDb];
+                for (int k = KB; k >= 1; --k) t = fma(-in[k - 1], R[OUB + k - 1], t);
+                t = t * R[ORDB];
             }
-            y[i] = t;
+            y[i] += t;
 #pragma unroll
-            for (int k = KB - 1; k > 0; --k) s[k] = s[k - 1];
-            s[0] = t;
+            for (int k = KB - 1; k > 0; --k) in[k] = in[k - 1];
+            in[0] = t;
             chunk_fence(i);
         }
-#pragma unroll
-        for (int k = 0; k < KB; ++k) ex[k][seg][ln] = s[k];
-        __syncthreads();
-        double in[KB];
-#pragma unroll
-        for (int k = 0; k < KB; ++k) in[k] = 0.0;
-        for (int w = 1; w <= tv.wb; ++w) {
-            int const src = seg + w;
-            if (src < K) {
-                const double* m = tv.Mb + (size_t(seg) * tv.wb + (w - 1)) * (KB * KB);
-#pragma unroll
-                for (int a = 0; a < KB; ++a)
-#pragma unroll
-                    for (int b = 0; b < KB; ++b) in[a] = fma(m[a * KB + b], ex[b][src][ln], in[a]);
-            }
-        }
-        if (seg + 1 < K) {
-            // rows += homogeneous backward recurrence started from the incoming state
-#pragma unroll
-            for (int i = L - 1; i >= 0; --i) {
-                const double* R = T + i * RS;
-                double t;
-                if constexpr (BDIVF) {
-                    t = -in[0] * R[oUb];
-                } else {
-                    t = 0.0;
-#pragma unroll
-                    for (int k = KB; k >= 1; --k) t = fma(-in[k - 1], R[oUb + k - 1], t);
-                    t = t * R[orDb];
-                }
-                y[i] += t;
-#pragma unroll
-                for (int k = KB - 1; k > 0; --k) in[k] = in[k - 1];
-                in[0] = t;
-                chunk_fence(i);
-            }
-        }
     }
-
 }
 
+template <int LB, int K, int L, int KF, int KB, bool FDIV, bool BDIVF, int KE>
+__device__ __forceinline__ void tiled_sweeps(double (&y)[L], const double* T, int RS, TiledView const& tv,
+                                             double (*ex)[K][LB], int seg, int ln)
+{
+    tiled_forward<LB, K, L, KF, FDIV, 0, KF + KB + 1>(y, T, RS, tv, ex, seg, ln);
+    tiled_backward<LB, K, L, KB, BDIVF, KF, KF + KB>(y, T, RS, tv, ex, seg, ln);
+}
+
+// Split staging of the row table, for tables that do not fit LDS whole:
+// the fields one sweep reads, copied from the full records (stride RS) into
+// compact records in LDS ([Lf(KF) | rDf] for the forward sweep, [Ub(KB) |
+// rDb] for the backward one).  The caller synchronises.
+template <int NT, int KF, int KB, bool FDIV, bool FORWARD>
+__device__ __forceinline__ void stage_half(const double* __restrict__ rt, int RS, int rows, double* dst, int tid)
+{
+    constexpr int W = FORWARD ? KF + (FDIV ? 1 : 0) : KB + 1;
+    for (int e = tid; e < rows * W; e += NT) {
+        int const r = e / W, f = e % W;
+        int src;
+        if constexpr (FORWARD) src = f < KF ? f : KF + KB + 1;
+        else src = f < KB ? KF + f : KF + KB;
+        dst[e] = rt[size_t(r) * RS + src];
+    }
+}
 } // namespace tiled
 } // namespace ddcx

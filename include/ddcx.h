@@ -267,6 +267,21 @@ ddcx_status ddcx_evaluator2d_eval_vjp_binned(const ddcx_evaluator* e1, const ddc
                                              int64_t st1, int64_t st2, int64_t nbatch, void* work,
                                              int64_t work_bytes, void* stream);
 
+/* ddcx_evaluator2d_eval_vjp_binned without a workspace argument: no
+ * reference counterpart; for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The workspace is a static device array of the library,
+ * DDCX_STATIC_SCRATCH_BYTES per GPU (about 4 B per point: up to ~16M points
+ * per call), held by every GPU the library runs on until the process ends;
+ * DDCX_ERR_INVALID when the call needs more.  Nothing is allocated.  All
+ * calls on one GPU share that array: calls that overlap on one GPU (other
+ * streams, other threads) race and must be serialised by the caller. */
+#define DDCX_STATIC_SCRATCH_BYTES (64u << 20)
+ddcx_status ddcx_evaluator2d_eval_vjp_binned_static(const ddcx_evaluator* e1, const ddcx_evaluator* e2,
+                                                    const double* coords, const double* w, double* gx, int64_t npts,
+                                                    const double* coef, double* gcoef, int64_t coef_batch_stride,
+                                                    int64_t st1, int64_t st2, int64_t nbatch, void* stream);
+
 /* SplineEvaluatorND::operator() / deriv(), spline_evaluator_nd.hpp:300-590
  * (eval :684-720, eval_no_bc :782-826); SplineEvaluator3D
  * (spline_evaluator_3d.hpp:443-1019) is its nd = 3 case.  evs: one evaluator

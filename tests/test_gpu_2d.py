@@ -54,7 +54,7 @@ def test_build2d_matches_oracle(n1, n2, nb, cuda):
     assert rel_err(got, ref) <= TOL
 
 
-@pytest.mark.parametrize("n1,n2,nb", [(1024, 1024, 2), (300, 256, 3)])
+@pytest.mark.parametrize("n1,n2,nb", [(1024, 1024, 2), (300, 320, 3)])
 def test_build2d_fp32_refine_matches_oracle(n1, n2, nb, cuda):
     """The tensor-product workload (non-uniform cubic Greville x random
     breakpoints x uniform periodic cubic, 1024 x 1024 at full size) built

@@ -16,13 +16,13 @@ from matrices import SIZES, nonsym, offset_banded, pds, periodic_band
 pytestmark = pytest.mark.gpu
 
 
-def _load(prob, A):
+def _load(setter, A):
+    """Sets the nonzero entries of A through setter(i, j, a_ij)."""
     N = A.shape[0]
     for i in range(N):
         for j in range(N):
             if A[i, j] != 0.0:
-                prob.set_element(i, j, A[i, j])
-    return prob
+                setter(i, j, A[i, j])
 
 
 def solve_and_validate(prob, orc, A, cuda, nrhs_extra=300):
@@ -30,12 +30,12 @@ def solve_and_validate(prob, orc, A, cuda, nrhs_extra=300):
     import torch
 
     N = A.shape[0]
-    _load(prob, A)
+    _load(prob.set_element, A)
     for i in range(N):
         for j in range(N):
             assert prob.get_element(i, j) == A[i, j]
     prob.setup_solver()
-    _load(orc, A)
+    _load(orc.set, A)
     orc.setup()
     rows = prob.required_number_of_rhs_rows()
     assert rows == orc.rhs_rows and prob.size() == N
