@@ -339,8 +339,11 @@ def measure_2d(D, timed, gen):
     ev.eval_vjp(coords, w, coef, gx, gc)
     t_a = timed(lambda: ev.eval_vjp(coords, w, coef, gx, gc), 2)
     by_b = 8 * nb * (n1 * n2 + bs1.size() * bs2.size())
+    # the two 1D passes also write and read the dimension-1 spline once each
+    by_b2 = by_b + 2 * 8 * nb * bs1.size() * n2
     by_e = 24 * nb * P + 8 * coef.numel()
     return {"build_us": round(t_b * 1e6, 1), "build_GBps": round(by_b / t_b / 1e9, 1),
+            "build_GBps_with_intermediate": round(by_b2 / t_b / 1e9, 1),
             "fp32_refine_build_us": round(t_bf * 1e6, 1), "fp32_refine_build_GBps": round(by_b / t_bf / 1e9, 1),
             "fp32_refine_max_line_rel_err_vs_fp64": float(f"{err32:.3e}"),
             "eval_us": round(t_e * 1e6, 1), "eval_GBps": round(by_e / t_e / 1e9, 1),

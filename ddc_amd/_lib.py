@@ -79,6 +79,8 @@ SIGNATURES = {
     "ddcx_evaluator_eval_vjp": [_P, _P, _P, _P, _L, _I64, _P, _P, _L, _I64, _P],
     "ddcx_evaluator2d_eval": [_P, _P, C.c_int, C.c_int, _P, _P, _I64, _P, _I64, _I64, _I64, _I64, _P],
     "ddcx_evaluator2d_eval_vjp": [_P, _P, _P, _P, _P, _I64, _P, _P, _I64, _I64, _I64, _I64, _P],
+    "ddcx_evaluator2d_eval_workspace": [_P, _P, _I64, _P],
+    "ddcx_evaluator2d_eval_packed": [_P, _P, C.c_int, C.c_int, _P, _P, _I64, _P, _I64, _I64, _I64, _I64, _P, _I64, _P],
     "ddcx_build_eval": [_P, _P, _P, _L, _P, _L, _P, _P, _L, _P, _P, _L, _I64, _I64, _P],
     "ddcx_advect": [_P, _P, _P, _L, _P, _L, _P, _P, _L, _P, _P, _L, _I64, _P],
     "ddcx_evaluatornd_eval": [_P, C.c_int, _P, _P, _P, _I64, _P, _I64, _P, _I64, _P],

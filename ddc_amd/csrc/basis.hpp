@@ -35,6 +35,12 @@ struct BSplinesView {
     const uint32_t* cells;
     int nbucket;
     double inv_bucket; // nbucket / (rmax - rmin)
+    // Non-uniform device evaluation: per cell, the reciprocals of the knot
+    // differences the Cox-de Boor recursion divides by, degree*(degree+1)/2
+    // of them (cell c at inv_dk + c * degree*(degree+1)/2, entry
+    // j*(j+1)/2 + r = 1 / (t[c+degree+r+1] - t[c+degree+r-j])).  nullptr:
+    // the reciprocals are formed per point.
+    const double* inv_dk;
 };
 
 __host__ __device__ inline double knot_coord(BSplinesView const& b, int k)
@@ -199,6 +205,7 @@ __device__ __forceinline__ double fast_rcp(double d)
 // degree+2).
 // FAST (device evaluation only): multiplications by the constants 1/j
 // instead of divisions by j.
+// FAST also fuses the update saved + xx * temp into one fma.
 template <int NV, bool FAST = false>
 __host__ __device__ inline void uniform_basis_from_offset(double offset, double* values)
 {
@@ -211,7 +218,7 @@ __host__ __device__ inline void uniform_basis_from_offset(double offset, double*
         for (int r = 0; r < j; ++r) {
             xx += 1;
             double const temp = FAST ? values[r] * (1.0 / j) : values[r] / j;
-            values[r] = saved + xx * temp;
+            values[r] = FAST ? fma(xx, temp, saved) : saved + xx * temp;
             saved = (j - xx) * temp;
         }
         values[j] = saved;
@@ -220,9 +227,13 @@ __host__ __device__ inline void uniform_basis_from_offset(double offset, double*
 
 // de Boor with left/right arrays, NonUniformBSplines eval_basis,
 // bsplines_non_uniform.hpp:461-497; `icell` is the knot index from
-// nonuniform_cell.
+// nonuniform_cell.  FAST (device evaluation): the divisor
+// right[r] + left[j-r] = t[icell+r+1] - t[icell+r-j] is taken as the cell's
+// tabulated reciprocal `inv` when given (else the hardware reciprocal
+// refined), and the update is one fma.
 template <int D, bool FAST = false>
-__host__ __device__ inline void nonuniform_basis(const double* knots, int icell, double x, double* values)
+__host__ __device__ inline void nonuniform_basis(const double* knots, int icell, double x, double* values,
+                                                 const double* inv = nullptr)
 {
     double left[D > 0 ? D : 1];
     double right[D > 0 ? D : 1];
@@ -235,13 +246,24 @@ __host__ __device__ inline void nonuniform_basis(const double* knots, int icell,
 #pragma unroll
         for (int r = 0; r < j + 1; ++r) {
             double temp;
-            if constexpr (FAST) temp = values[r] * fast_rcp(right[r] + left[j - r]);
-            else temp = values[r] / (right[r] + left[j - r]);
-            values[r] = saved + right[r] * temp;
+            if constexpr (FAST) {
+                temp = values[r] * (inv ? inv[j * (j + 1) / 2 + r] : fast_rcp(right[r] + left[j - r]));
+                values[r] = fma(right[r], temp, saved);
+            } else {
+                temp = values[r] / (right[r] + left[j - r]);
+                values[r] = saved + right[r] * temp;
+            }
             saved = left[j - r] * temp;
         }
         values[j + 1] = saved;
     }
+}
+
+// the cell's row of BSplinesView::inv_dk (FAST device evaluation only)
+template <int D>
+__host__ __device__ inline const double* cell_inv(BSplinesView const& b, int icell)
+{
+    return b.inv_dk ? b.inv_dk + size_t(icell - D) * (D * (D + 1) / 2) : nullptr;
 }
 
 // Basis kind of a kernel instantiation: decided by BSplinesView::uniform at
@@ -264,7 +286,7 @@ __host__ __device__ inline int eval_basis(BSplinesView const& b, const double* k
         return jmin;
     }
     int const icell = nonuniform_cell(b, knots, x);
-    nonuniform_basis<D, FAST>(knots, icell, x, values);
+    nonuniform_basis<D, FAST>(knots, icell, x, values, FAST ? cell_inv<D>(b, icell) : nullptr);
     return icell - D;
 }
 
@@ -291,7 +313,7 @@ __host__ __device__ inline int eval_basis_and_n_derivs(BSplinesView const& b, co
             for (int r = 0; r < j; ++r) {
                 xx += 1.0;
                 double const temp = FAST ? ndu[j - 1][r] * (1.0 / j) : ndu[j - 1][r] / j;
-                ndu[j][r] = saved + xx * temp;
+                ndu[j][r] = FAST ? fma(xx, temp, saved) : saved + xx * temp;
                 saved = (j - xx) * temp;
             }
             ndu[j][j] = saved;
@@ -301,6 +323,7 @@ __host__ __device__ inline int eval_basis_and_n_derivs(BSplinesView const& b, co
         double right[D > 0 ? D : 1];
         int const icell = nonuniform_cell(b, knots, x);
         jmin = icell - D;
+        const double* inv = FAST ? cell_inv<D>(b, icell) : nullptr;
         ndu[0][0] = 1.0;
 #pragma unroll
         for (int j = 0; j < D; ++j) {
@@ -309,10 +332,11 @@ __host__ __device__ inline int eval_basis_and_n_derivs(BSplinesView const& b, co
             double saved = 0.0;
 #pragma unroll
             for (int r = 0; r < j + 1; ++r) {
-                if constexpr (FAST) ndu[r][j + 1] = fast_rcp(right[r] + left[j - r]);
+                if constexpr (FAST)
+                    ndu[r][j + 1] = inv ? inv[j * (j + 1) / 2 + r] : fast_rcp(right[r] + left[j - r]);
                 else ndu[r][j + 1] = 1.0 / (right[r] + left[j - r]);
                 double const temp = ndu[j][r] * ndu[r][j + 1];
-                ndu[j + 1][r] = saved + right[r] * temp;
+                ndu[j + 1][r] = FAST ? fma(right[r], temp, saved) : saved + right[r] * temp;
                 saved = left[j - r] * temp;
             }
             ndu[j + 1][j + 1] = saved;
@@ -343,13 +367,13 @@ __host__ __device__ inline int eval_basis_and_n_derivs(BSplinesView const& b, co
                     a[j][s2] = uni ? (FAST ? (a[j][s1] - a[j - 1][s1]) * (1.0 / (pk + 1))
                                           : (a[j][s1] - a[j - 1][s1]) / (pk + 1))
                                    : (a[j][s1] - a[j - 1][s1]) * ndu[rk + j][pk + 1];
-                    d += a[j][s2] * ndu[pk][rk + j];
+                    d = FAST ? fma(a[j][s2], ndu[pk][rk + j], d) : d + a[j][s2] * ndu[pk][rk + j];
                 }
             }
             if (r <= pk) {
                 a[k][s2] = uni ? (FAST ? -a[k - 1][s1] * (1.0 / (pk + 1)) : -a[k - 1][s1] / (pk + 1))
                                : -a[k - 1][s1] * ndu[r][pk + 1];
-                d += a[k][s2] * ndu[pk][r];
+                d = FAST ? fma(a[k][s2], ndu[pk][r], d) : d + a[k][s2] * ndu[pk][r];
             }
             derivs[r * (N + 1) + k] = d;
             int const t = s1;

@@ -421,6 +421,34 @@ ddcx_status ddcx_evaluator2d_eval(const ddcx_evaluator* e1, const ddcx_evaluator
     });
 }
 
+ddcx_status ddcx_evaluator2d_eval_workspace(const ddcx_evaluator* e1, const ddcx_evaluator* e2, int64_t nbatch,
+                                            int64_t* bytes)
+{
+    return guarded([&] {
+        require(e1 && e2 && bytes, "null argument");
+        require(nbatch >= 0, "negative size");
+        *bytes = ddcx::eval2d_packed_record_bytes(*e1->bs, *e2->bs) * nbatch;
+    });
+}
+
+ddcx_status ddcx_evaluator2d_eval_packed(const ddcx_evaluator* e1, const ddcx_evaluator* e2, int order1, int order2,
+                                         double* out, const double* coords, int64_t npts, const double* coef,
+                                         int64_t coef_batch_stride, int64_t st1, int64_t st2, int64_t nbatch,
+                                         void* work, int64_t work_bytes, void* stream)
+{
+    return guarded([&] {
+        require(e1 && e2, "null evaluator");
+        require(order1 >= 0 && order2 >= 0, "negative derivative order");
+        require(npts >= 0 && nbatch >= 0, "negative size");
+        if (npts * nbatch == 0) return;
+        require(out && coords && coef && work, "null data pointer");
+        ddcx::Rules2D const rules = rules2d(e1, e2);
+        ddcx::launch_eval2d_packed(*e1->bs, *e2->bs, rules, order1, order2, out, coords, npts, coef,
+                                   coef_batch_stride, st1, st2, nbatch, work, work_bytes,
+                                   static_cast<hipStream_t>(stream));
+    });
+}
+
 ddcx_status ddcx_evaluator2d_eval_vjp(const ddcx_evaluator* e1, const ddcx_evaluator* e2, const double* coords,
                                       const double* w, double* gx, int64_t npts, const double* coef, double* gcoef,
                                       int64_t coef_batch_stride, int64_t st1, int64_t st2, int64_t nbatch,
@@ -558,7 +586,12 @@ ddcx_status ddcx_problem_create(const ddcx_problem_desc* desc, ddcx_problem** ou
             pr.q.init(ddcx::QType::Dense, desc->n, 0, 0);
             break;
         case DDCX_PROBLEM_BAND:
-            require(desc->kl >= 0 && desc->ku >= 0 && desc->kl < desc->n && desc->ku < desc->n, "bad bandwidths");
+            // the maker's dense fallback (2 kl + ku + 1 >= n) takes any bandwidths
+            // (splines_linear_problem_maker.hpp:56-77); symmetric storage needs kl < n
+            require(desc->kl >= 0 && desc->ku >= 0 &&
+                        ((desc->kl < desc->n && desc->ku < desc->n) ||
+                         (!(pds && desc->kl == desc->ku) && 2 * desc->kl + desc->ku + 1 >= desc->n)),
+                    "bad bandwidths");
             pr.n = desc->n;
             ddcx::make_band_q(pr.q, desc->n, desc->kl, desc->ku, pds);
             break;

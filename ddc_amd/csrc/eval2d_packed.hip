@@ -1,0 +1,227 @@
+// Batched 2D evaluation (SplineEvaluator2D::operator() / deriv*,
+// spline_evaluator_2d.hpp:370-825) for many points per batch entry, through a
+// packed copy of the coefficients.
+//
+// Scattered points gather a (d1+1) x (d2+1) coefficient patch each.  From the
+// plain (size1, size2) array that is d1+1 separate rows, each its own cache
+// line, and with 2^24 points per batch entry the 8 MiB array does not stay in
+// one XCD's L2.  So the call first packs, for every cell (i1, i2), the patch
+// that cell's points use into one contiguous, 16-byte aligned record of
+// P = (d1+1)(d2+1) doubles (padded to even), in a caller workspace; then
+//   1. each lane of a wave takes one point: rules, cell indices and basis
+//      values (the same routines as eval2d_kernel), parked in LDS;
+//   2. groups of 8 lanes take one point each and load its record as 16-byte
+//      pairs (one cache line per point for cubic x cubic), multiply by the
+//      basis products and reduce over the group with cross-lane adds;
+//   3. each lane stores its point's value (coalesced).
+// The terms are summed in a fixed order, so results do not depend on
+// scheduling; they differ from eval2d_kernel's by rounding only.
+#include <algorithm>
+
+#include "internal.hpp"
+
+namespace ddcx {
+
+namespace {
+
+constexpr int kPackThreads = 256;
+constexpr int kEvalThreads = 256;
+constexpr int kGroup = 8; // lanes per point in the gather
+
+__host__ __device__ constexpr int patch_len(int d1, int d2)
+{
+    return ((d1 + 1) * (d2 + 1) + 1) / 2 * 2;
+}
+
+// One thread per (batch entry, cell1, cell2, patch row): copies the row's
+// d2+1 coefficients into the cell's record.
+template <int D1, int D2>
+__global__ void __launch_bounds__(kPackThreads) pack2d_kernel(const double* __restrict__ coef, int64_t cbs,
+                                                              int64_t st1, int64_t st2, int nc1, int nc2,
+                                                              int64_t nbatch, double* __restrict__ packed)
+{
+    constexpr int P = patch_len(D1, D2);
+    int64_t const t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    int64_t const per = int64_t(nc1) * nc2 * (D1 + 1);
+    if (t >= per * nbatch) return;
+    int64_t const l = t / per;
+    int64_t r = t - l * per;
+    int const i = int(r % (D1 + 1));
+    r /= (D1 + 1);
+    int const i2 = int(r % nc2), i1 = int(r / nc2);
+    const double* src = coef + l * cbs + int64_t(i1 + i) * st1 + int64_t(i2) * st2;
+    double* dst = packed + ((l * nc1 + i1) * int64_t(nc2) + i2) * P + i * (D2 + 1);
+#pragma unroll
+    for (int j = 0; j < D2 + 1; ++j) dst[j] = src[int64_t(j) * st2];
+    if constexpr (P != (D1 + 1) * (D2 + 1)) {
+        if (i == D1) dst[D2 + 1] = 0.0; // the pad element
+    }
+}
+
+template <int D1, int D2, int O1, int O2>
+__global__ void __launch_bounds__(kEvalThreads) eval2d_packed_kernel(BSplinesView b1, BSplinesView b2,
+                                                                     Rules2D rules, double* __restrict__ out,
+                                                                     const double* __restrict__ coords,
+                                                                     int64_t npts, const double* __restrict__ packed,
+                                                                     int nc1, int nc2, int64_t batch0)
+{
+    constexpr int P = patch_len(D1, D2);
+    constexpr int NPAIR = P / 2;
+    constexpr int NB = D1 + 1 + D2 + 1;
+    constexpr int WAVES = kEvalThreads / 64;
+    __shared__ double sb[WAVES][64][NB];  // basis values of dimension 1, then 2
+    __shared__ int64_t srec[WAVES][64];   // record index, -1: the Null rule gives 0
+    __shared__ double so[WAVES][64];
+    int const wv = threadIdx.x / 64, lane = threadIdx.x % 64;
+    int64_t const l = batch0 + blockIdx.y;
+    int64_t const p0 = (int64_t(blockIdx.x) * WAVES + wv) * 64;
+    if (p0 >= npts) return; // wave-uniform
+    {
+        int64_t const p = p0 + lane < npts ? p0 + lane : npts - 1;
+        int64_t const t = l * npts + p;
+        double x1 = coords[2 * t], x2 = coords[2 * t + 1];
+        bool keep = true;
+        if constexpr (O1 == 0 && O2 == 0) {
+            bool in1, in2;
+            keep = place2d(b1, b2, rules, x1, x2, in1, in2);
+        }
+        double v1[D1 + 1], v2[D2 + 1];
+        int j1, j2;
+        if constexpr (O1 == 0) j1 = eval_basis<D1, true>(b1, b1.knots, x1, v1);
+        else {
+            double der[(D1 + 1) * (O1 + 1)];
+            j1 = eval_basis_and_n_derivs<D1, O1, true>(b1, b1.knots, x1, der);
+#pragma unroll
+            for (int i = 0; i < D1 + 1; ++i) v1[i] = der[i * (O1 + 1) + O1];
+        }
+        if constexpr (O2 == 0) j2 = eval_basis<D2, true>(b2, b2.knots, x2, v2);
+        else {
+            double der[(D2 + 1) * (O2 + 1)];
+            j2 = eval_basis_and_n_derivs<D2, O2, true>(b2, b2.knots, x2, der);
+#pragma unroll
+            for (int i = 0; i < D2 + 1; ++i) v2[i] = der[i * (O2 + 1) + O2];
+        }
+#pragma unroll
+        for (int i = 0; i < D1 + 1; ++i) sb[wv][lane][i] = v1[i];
+#pragma unroll
+        for (int j = 0; j < D2 + 1; ++j) sb[wv][lane][D1 + 1 + j] = v2[j];
+        srec[wv][lane] = keep ? (l * nc1 + j1) * int64_t(nc2) + j2 : -1;
+    }
+    __builtin_amdgcn_wave_barrier();
+    int const sub = lane % kGroup;
+#pragma unroll 2
+    for (int it = 0; it < 64 / kGroup; ++it) {
+        int const q = it * kGroup + lane / kGroup;
+        int64_t const rec = srec[wv][q];
+        const double2* r2 = reinterpret_cast<const double2*>(packed + (rec < 0 ? 0 : rec) * P);
+        double part = 0.0;
+#pragma unroll
+        for (int e2 = sub; e2 < NPAIR; e2 += kGroup) {
+            double2 const c = r2[e2];
+            int const e = 2 * e2;
+            // element e is (row e / (D2+1), column e % (D2+1)); the pad element has a zero coefficient
+            int const ia = e / (D2 + 1), ja = e % (D2 + 1);
+            int const ib = (e + 1) / (D2 + 1), jb = (e + 1) % (D2 + 1);
+            double const wa = sb[wv][q][ia] * sb[wv][q][D1 + 1 + ja];
+            double const wb = (e + 1 < (D1 + 1) * (D2 + 1)) ? sb[wv][q][ib] * sb[wv][q][D1 + 1 + jb] : 0.0;
+            part = fma(c.x, wa, part);
+            part = fma(c.y, wb, part);
+        }
+#pragma unroll
+        for (int m = 1; m < kGroup; m *= 2) part += __shfl_xor(part, m, kGroup);
+        if (sub == 0) so[wv][q] = rec < 0 ? 0.0 : part;
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (p0 + lane < npts) out[l * npts + p0 + lane] = so[wv][lane];
+}
+
+template <class F>
+bool dispatch_2d(int d1, int d2, int o1, int o2, F&& f)
+{
+    bool done = false;
+    auto go = [&]<int D1, int D2>() {
+        auto sel = [&]<int O1, int O2>() {
+            if constexpr (O1 <= D1 && O2 <= D2) {
+                if (!done && o1 == O1 && o2 == O2) {
+                    f.template operator()<D1, D2, O1, O2>();
+                    done = true;
+                }
+            }
+        };
+        sel.template operator()<0, 0>();
+        sel.template operator()<1, 0>();
+        sel.template operator()<0, 1>();
+        sel.template operator()<1, 1>();
+        sel.template operator()<2, 0>();
+        sel.template operator()<0, 2>();
+        sel.template operator()<2, 1>();
+        sel.template operator()<1, 2>();
+        sel.template operator()<2, 2>();
+    };
+    auto outer = [&]<int D1>() {
+        switch (d2) {
+        case 1: go.template operator()<D1, 1>(); break;
+        case 2: go.template operator()<D1, 2>(); break;
+        case 3: go.template operator()<D1, 3>(); break;
+        case 4: go.template operator()<D1, 4>(); break;
+        case 5: go.template operator()<D1, 5>(); break;
+        default: break;
+        }
+    };
+    switch (d1) {
+    case 1: outer.template operator()<1>(); break;
+    case 2: outer.template operator()<2>(); break;
+    case 3: outer.template operator()<3>(); break;
+    case 4: outer.template operator()<4>(); break;
+    case 5: outer.template operator()<5>(); break;
+    default: break;
+    }
+    return done;
+}
+
+} // namespace
+
+int64_t eval2d_packed_record_bytes(BSplinesPlan const& bp1, BSplinesPlan const& bp2)
+{
+    int const d1 = bp1.view.degree, d2 = bp2.view.degree;
+    return int64_t(bp1.view.ncells) * bp2.view.ncells * patch_len(d1, d2) * int64_t(sizeof(double));
+}
+
+void launch_eval2d_packed(BSplinesPlan const& bp1, BSplinesPlan const& bp2, Rules2D const& rules, int o1, int o2,
+                          double* out, const double* coords, int64_t npts, const double* coef, int64_t cbs,
+                          int64_t st1, int64_t st2, int64_t nbatch, void* work, int64_t work_bytes,
+                          hipStream_t stream)
+{
+    if (npts * nbatch == 0) return;
+    bp1.ensure_device();
+    bp2.ensure_device();
+    BSplinesView const b1 = bp1.device_view(), b2 = bp2.device_view();
+    int const nc1 = b1.ncells, nc2 = b2.ncells;
+    int64_t const rec_bytes = eval2d_packed_record_bytes(bp1, bp2);
+    if (reinterpret_cast<uintptr_t>(work) % 16 != 0)
+        throw Error(DDCX_ERR_INVALID, "packed 2D evaluation: workspace must be 16-byte aligned");
+    int64_t const chunk = work_bytes / rec_bytes; // batch entries packed at a time
+    if (chunk < 1) throw Error(DDCX_ERR_INVALID, "packed 2D evaluation: workspace smaller than one batch entry");
+    double* packed = static_cast<double*>(work);
+    unsigned const gx = unsigned((npts + kEvalThreads - 1) / kEvalThreads);
+    if (npts > int64_t(0x7fffffff) * kEvalThreads) throw Error(DDCX_ERR_INVALID, "too many points for one launch");
+    bool const ok = dispatch_2d(b1.degree, b2.degree, o1, o2, [&]<int D1, int D2, int O1, int O2>() {
+        for (int64_t l0 = 0; l0 < nbatch; l0 += chunk) {
+            int64_t const nb = std::min(chunk, nbatch - l0);
+            int64_t const nthreads = int64_t(nc1) * nc2 * (D1 + 1) * nb;
+            hipLaunchKernelGGL((pack2d_kernel<D1, D2>), dim3(unsigned((nthreads + kPackThreads - 1) / kPackThreads)),
+                               dim3(kPackThreads), 0, stream, coef + l0 * cbs, cbs, st1, st2, nc1, nc2, nb, packed);
+            for (int64_t l1 = 0; l1 < nb; l1 += 65535) {
+                unsigned const gy = unsigned(std::min<int64_t>(65535, nb - l1));
+                // the kernel indexes batch entries from the packed chunk's start
+                hipLaunchKernelGGL((eval2d_packed_kernel<D1, D2, O1, O2>), dim3(gx, gy), dim3(kEvalThreads), 0,
+                                   stream, b1, b2, rules, out + l0 * npts, coords + 2 * l0 * npts, npts, packed, nc1,
+                                   nc2, l1);
+            }
+        }
+    });
+    if (!ok) throw Error(DDCX_ERR_UNSUPPORTED, "2D evaluation supports degrees 1..5 and derivative orders <= 2");
+    hip_check(hipGetLastError(), "packed 2D evaluation launch");
+}
+
+} // namespace ddcx

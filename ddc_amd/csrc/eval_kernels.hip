@@ -4,6 +4,8 @@
 // point -> thread mapping follows the memory layout so that coordinate loads
 // and output stores are coalesced in both batch-contiguous and
 // interpolation-contiguous layouts.
+#include <algorithm>
+
 #include "internal.hpp"
 
 namespace ddcx {
@@ -115,18 +117,23 @@ __global__ void __launch_bounds__(256) integrate_kernel(const double* __restrict
     out[l * ostride] = acc;
 }
 
-// SplineEvaluator2D::eval / eval_no_bc, spline_evaluator_2d.hpp:679-825
-template <int D1, int D2, int O1, int O2>
+// SplineEvaluator2D::eval / eval_no_bc, spline_evaluator_2d.hpp:679-825.
+// Grid: x over the points of one batch entry, y over batch entries.  The
+// patch sum is y = sum_i v1[i] * (sum_j c(i, j) v2[j]) with fused
+// multiply-adds (a rounding-level reordering of eval_no_bc's
+// sum_ij c v1 v2).  UNIT: unit coefficient stride along dimension 2, so each
+// patch row is one contiguous run (wide loads).
+template <int D1, int D2, int O1, int O2, bool UNIT>
 __global__ void __launch_bounds__(256) eval2d_kernel(BSplinesView b1, BSplinesView b2, Rules2D rules,
                                                      double* __restrict__ out, const double* __restrict__ coords,
                                                      int64_t npts, const double* __restrict__ coef, int64_t cbs,
-                                                     int64_t st1, int64_t st2, int64_t nbatch)
+                                                     int64_t st1, int64_t st2, int64_t batch0)
 {
-    int64_t const t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (t >= npts * nbatch) return;
-    int64_t const l = t / npts;
-    double x1 = coords[2 * t];
-    double x2 = coords[2 * t + 1];
+    int64_t const p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= npts) return;
+    int64_t const l = batch0 + blockIdx.y;
+    int64_t const t = l * npts + p;
+    double x1 = coords[2 * t], x2 = coords[2 * t + 1];
     if constexpr (O1 == 0 && O2 == 0) {
         bool in1, in2;
         if (!place2d(b1, b2, rules, x1, x2, in1, in2)) {
@@ -150,12 +157,16 @@ __global__ void __launch_bounds__(256) eval2d_kernel(BSplinesView b1, BSplinesVi
 #pragma unroll
         for (int i = 0; i < D2 + 1; ++i) v2[i] = der[i * (O2 + 1) + O2];
     }
-    const double* c = coef + l * cbs;
+    const double* c = coef + l * cbs + int64_t(j1) * st1 + (UNIT ? int64_t(j2) : int64_t(j2) * st2);
     double y = 0.0;
 #pragma unroll
-    for (int i = 0; i < D1 + 1; ++i)
+    for (int i = 0; i < D1 + 1; ++i) {
+        const double* row = c + int64_t(i) * st1;
+        double ri = 0.0;
 #pragma unroll
-        for (int j = 0; j < D2 + 1; ++j) y += c[int64_t(j1 + i) * st1 + int64_t(j2 + j) * st2] * v1[i] * v2[j];
+        for (int j = 0; j < D2 + 1; ++j) ri = fma(row[UNIT ? j : int64_t(j) * st2], v2[j], ri);
+        y = fma(v1[i], ri, y);
+    }
     out[t] = y;
 }
 
@@ -302,12 +313,20 @@ void launch_eval2d(BSplinesPlan const& bp1, BSplinesPlan const& bp2, Rules2D con
     bp1.ensure_device();
     bp2.ensure_device();
     BSplinesView const b1 = bp1.device_view(), b2 = bp2.device_view();
-    unsigned const grid = grid_for(npts * nbatch, 256);
+    unsigned const gx = grid_for(npts, 256);
+    bool const unit = st2 == 1;
     bool launched = false;
     auto go = [&]<int D1, int D2>() {
         auto ord = [&]<int O1, int O2>() {
-            hipLaunchKernelGGL((eval2d_kernel<D1, D2, O1, O2>), dim3(grid), dim3(256), 0, stream, b1, b2, rules, out,
-                               coords, npts, coef, cbs, st1, st2, nbatch);
+            for (int64_t l0 = 0; l0 < nbatch; l0 += 65535) {
+                unsigned const gy = unsigned(std::min<int64_t>(65535, nbatch - l0));
+                if (unit)
+                    hipLaunchKernelGGL((eval2d_kernel<D1, D2, O1, O2, true>), dim3(gx, gy), dim3(256), 0, stream, b1,
+                                       b2, rules, out, coords, npts, coef, cbs, st1, st2, l0);
+                else
+                    hipLaunchKernelGGL((eval2d_kernel<D1, D2, O1, O2, false>), dim3(gx, gy), dim3(256), 0, stream,
+                                       b1, b2, rules, out, coords, npts, coef, cbs, st1, st2, l0);
+            }
             launched = true;
         };
         auto sel = [&]<int O1, int O2>() {

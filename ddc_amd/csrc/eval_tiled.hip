@@ -140,6 +140,7 @@ __global__ void __launch_bounds__(NT) eval_tiled_kernel(BSplinesView b, Rules1D 
     const double* knots = is_uniform_kind(KIND, b.uniform) ? nullptr : kn;
     BSplinesView bl = b;
     bl.cells = b.nbucket > 0 ? cells : nullptr;
+    bl.inv_dk = nullptr; // knots are in LDS: the refined reciprocal beats a global table load
     const double* tl = tile + ln;
     // software pipeline: the next batch of coordinates is in flight while the
     // current one is evaluated and stored
@@ -240,6 +241,7 @@ __global__ void __launch_bounds__(NT) eval_stream_kernel(BSplinesView b, Rules1D
     const double* knots = is_uniform_kind(KIND, b.uniform) ? nullptr : kn;
     BSplinesView bl = b;
     bl.cells = b.nbucket > 0 ? cells : nullptr;
+    bl.inv_dk = nullptr; // knots are in LDS: the refined reciprocal beats a global table load
 
     int64_t const G = gridDim.x;
     int64_t t = blockIdx.x;
@@ -380,6 +382,7 @@ __global__ void __launch_bounds__(NT) eval_vjp_tiled_kernel(BSplinesView b, Rule
     const double* knots = b.uniform ? nullptr : kn;
     BSplinesView bl = b;
     bl.cells = b.nbucket > 0 ? cells : nullptr;
+    bl.inv_dk = nullptr; // knots are in LDS: the refined reciprocal beats a global table load
     if (active) {
         const double* tl = tile + ln;
         double* gl = gt + ln;

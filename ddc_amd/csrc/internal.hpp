@@ -116,6 +116,9 @@ struct BSplinesPlan {
     DeviceBuffer cells_dev;
     std::vector<double> integrals; // integrals() over the full spline domain (size() entries)
     DeviceBuffer integrals_dev;
+    // non-uniform bases: BSplinesView::inv_dk of the device view
+    std::vector<double> inv_dk_host;
+    DeviceBuffer inv_dk_dev;
     bool on_device = false; // uploaded at creation when a HIP device is present
     void upload();
     void ensure_device() const; // throws unless the device copies exist
@@ -128,6 +131,7 @@ struct BSplinesPlan {
         v.cells = cells_host.empty() ? nullptr : cells_dev.as<uint32_t>();
         v.nbucket = int(cells_host.size());
         v.inv_bucket = inv_bucket;
+        v.inv_dk = inv_dk_host.empty() ? nullptr : inv_dk_dev.as<double>();
         return v;
     }
 };
@@ -388,6 +392,12 @@ void launch_eval2d(BSplinesPlan const& b1, BSplinesPlan const& b2, Rules2D const
                    int64_t st2, int64_t nbatch, hipStream_t stream);
 // binned 2D adjoint (eval2d_binned.hip): workspace bytes, 0 if unsupported
 int64_t eval2d_vjp_workspace(BSplinesPlan const& bp1, BSplinesPlan const& bp2, int64_t npts, int64_t nbatch);
+// packed 2D evaluation (eval2d_packed.hip): workspace bytes per batch entry
+int64_t eval2d_packed_record_bytes(BSplinesPlan const& bp1, BSplinesPlan const& bp2);
+void launch_eval2d_packed(BSplinesPlan const& bp1, BSplinesPlan const& bp2, Rules2D const& rules, int o1, int o2,
+                          double* out, const double* coords, int64_t npts, const double* coef, int64_t cbs,
+                          int64_t st1, int64_t st2, int64_t nbatch, void* work, int64_t work_bytes,
+                          hipStream_t stream);
 // device address of the current GPU's copy of the static workspace
 // (DDCX_STATIC_SCRATCH_BYTES), resolved once per device
 void* static_scratch();

@@ -554,15 +554,43 @@ class SplineEvaluator2D:
     def __init__(self, ev1: SplineEvaluator, ev2: SplineEvaluator):
         self.ev1, self.ev2 = ev1, ev2
 
-    def __call__(self, spline_eval, coords_eval, spline_coef, order1=0, order2=0):
-        """coords_eval: (batch, npts, 2); spline_eval: (batch, npts); spline_coef: (batch, s1, s2)."""
+    # workspace of the packed evaluation: at most this many bytes are kept
+    PACKED_WORKSPACE_MAX = 1 << 30
+
+    def __call__(self, spline_eval, coords_eval, spline_coef, order1=0, order2=0, packed=None):
+        """coords_eval: (batch, npts, 2); spline_eval: (batch, npts); spline_coef: (batch, s1, s2).
+        packed: None picks the packed-coefficient path (ddcx_evaluator2d_eval_packed)
+        when there are at least twice as many points as cells per batch entry;
+        True / False forces it on / off."""
+        _check_f64_cuda(coords_eval, "coords_eval")
+        _check_f64_cuda(spline_eval, "spline_eval")
+        _check_f64_cuda(spline_coef, "spline_coef")
         c = coords_eval.contiguous()
         squeeze = spline_coef.dim() == 2
         coef = spline_coef.unsqueeze(0) if squeeze else spline_coef
         nb = coef.shape[0]
+        if c.shape[-1] != 2 or c.numel() % (2 * nb):
+            raise ValueError("coords_eval must be (batch, npts, 2)")
         npts = c.numel() // (2 * nb)
-        if not spline_eval.is_contiguous():
-            raise ValueError("spline_eval must be contiguous")
+        if not spline_eval.is_contiguous() or spline_eval.numel() != nb * npts:
+            raise ValueError("spline_eval must be contiguous with one value per point")
+        b1, b2 = self.ev1.bsplines, self.ev2.bsplines
+        if coef.shape[1] != b1.size() or coef.shape[2] != b2.size():
+            raise ValueError("spline_coef has the wrong number of coefficients")
+        if packed is None:
+            packed = npts >= 2 * b1.ncells() * b2.ncells()
+        if packed and nb * npts > 0:
+            per = C.c_int64(0)
+            call("ddcx_evaluator2d_eval_workspace", self.ev1._h, self.ev2._h, 1, C.byref(per))
+            entries = max(1, min(nb, self.PACKED_WORKSPACE_MAX // max(per.value, 1)))
+            need = per.value * entries
+            ws = getattr(self, "_pack_ws", None)
+            if ws is None or ws.numel() < need or ws.device != c.device:
+                ws = self._pack_ws = torch.empty(need, dtype=torch.uint8, device=c.device)
+            call("ddcx_evaluator2d_eval_packed", self.ev1._h, self.ev2._h, int(order1), int(order2),
+                 _ptr(spline_eval), _ptr(c), npts, _ptr(coef), coef.stride(0), coef.stride(1), coef.stride(2), nb,
+                 _ptr(ws), ws.numel(), _stream())
+            return spline_eval
         call("ddcx_evaluator2d_eval", self.ev1._h, self.ev2._h, int(order1), int(order2), _ptr(spline_eval), _ptr(c),
              npts, _ptr(coef), coef.stride(0), coef.stride(1), coef.stride(2), nb, _stream())
         return spline_eval

@@ -253,6 +253,21 @@ ddcx_status ddcx_evaluator2d_eval(const ddcx_evaluator* e1, const ddcx_evaluator
                                   int64_t coef_batch_stride, int64_t st1, int64_t st2, int64_t nbatch,
                                   void* stream);
 
+/* ddcx_evaluator2d_eval through a packed copy of the coefficients (no
+ * reference counterpart; the same values to rounding): for many points per
+ * batch entry, each cell's (d1+1) x (d2+1) coefficient patch is first copied
+ * into one contiguous record in `work`, so that a point gathers one record
+ * instead of d1+1 separate rows.  work: device memory, 16-byte aligned, of at
+ * least the bytes ddcx_evaluator2d_eval_workspace gives for one batch entry;
+ * with less than all of nbatch, the batch entries are packed and evaluated a
+ * workspace-full at a time.  Calls that share a workspace must not overlap. */
+ddcx_status ddcx_evaluator2d_eval_workspace(const ddcx_evaluator* e1, const ddcx_evaluator* e2, int64_t nbatch,
+                                            int64_t* bytes);
+ddcx_status ddcx_evaluator2d_eval_packed(const ddcx_evaluator* e1, const ddcx_evaluator* e2, int order1, int order2,
+                                         double* out, const double* coords, int64_t npts, const double* coef,
+                                         int64_t coef_batch_stride, int64_t st1, int64_t st2, int64_t nbatch,
+                                         void* work, int64_t work_bytes, void* stream);
+
 /* ddcx_evaluator2d_eval_vjp with the points binned by cell tile (no
  * reference counterpart): the same results, with per-tile accumulation in
  * LDS instead of one global atomic per point and coefficient.  work: device

@@ -289,3 +289,48 @@ def test_eval2d_vjp_binned_matches_atomic(rules, cuda):
     # a smaller call reuses the workspace
     ev.eval_vjp(xy[:, :100].contiguous(), w[:, :100].contiguous(), coef, gx_b[:, :100].contiguous(), gc_b)
     assert np.all(np.isfinite(gc_b.cpu().numpy()))
+
+
+@pytest.mark.parametrize("degs,orders", [((3, 3), (0, 0)), ((3, 3), (1, 1)), ((2, 5), (0, 2)), ((5, 4), (2, 0)),
+                                         ((1, 2), (0, 0))])
+def test_eval2d_packed_matches_oracle(degs, orders, cuda):
+    """The packed-coefficient evaluation (ddcx_evaluator2d_eval_packed):
+    values and partial derivatives against the oracle, with points outside
+    the domain on every side (Null x periodic); a workspace of one batch
+    entry (entries packed and evaluated one at a time) gives the same
+    values as a workspace of all."""
+    import ctypes as C
+    import torch
+    import ddc_amd as D
+    import oracle as O
+    from ddc_amd._lib import call
+
+    d1, d2 = degs
+    br = breaks_for("random", 23, seed=9)
+    o1 = O.Spline(d1, False, G, G, breaks=br)
+    o2 = O.Spline(d2, True, P, P, ncells=17, rmin=0.0, rmax=1.0)
+    bs1 = D.NonUniformBSplines(d1, br)
+    bs2 = D.UniformBSplines(d2, 0.0, 1.0, 17, periodic=True)
+    ev = D.SplineEvaluator2D(D.SplineEvaluator(bs1, D.NullExtrapolationRule(), D.NullExtrapolationRule()),
+                             D.SplineEvaluator(bs2, D.PeriodicExtrapolationRule(), D.PeriodicExtrapolationRule()))
+    nb, npts = 3, 2000
+    rng = np.random.default_rng(13)
+    coef = rng.standard_normal((nb, bs1.size(), bs2.size()))
+    coef[:, :, bs2.nbasis():] = coef[:, :, :d2]
+    lo = 0.0 if orders != (0, 0) else -0.3
+    coords = np.stack([rng.uniform(lo, 1 - lo, (nb, npts)), rng.uniform(-0.5, 1.5, (nb, npts))], axis=-1)
+    if orders != (0, 0):  # derivatives apply no wrap: keep dimension 2 inside
+        coords[..., 1] = rng.uniform(0.0, 1.0, (nb, npts))
+    ref = O.spline2d_eval(o1, o2, coords, coef, *orders)
+    tc, tx = torch.from_numpy(coef).to(cuda), torch.from_numpy(coords).to(cuda)
+    out = torch.full((nb, npts), np.nan, dtype=torch.float64, device=cuda)
+    ev(out, tx, tc, *orders, packed=True)
+    assert rel_err(out.cpu().numpy(), ref) <= TOL
+    per = C.c_int64(0)
+    call("ddcx_evaluator2d_eval_workspace", ev.ev1._h, ev.ev2._h, 1, C.byref(per))
+    ws = torch.empty(per.value + 100, dtype=torch.uint8, device=cuda)  # one entry: packed one at a time
+    out1 = torch.full_like(out, np.nan)
+    call("ddcx_evaluator2d_eval_packed", ev.ev1._h, ev.ev2._h, orders[0], orders[1], out1.data_ptr(), tx.data_ptr(),
+         npts, tc.data_ptr(), tc.stride(0), tc.stride(1), tc.stride(2), nb, ws.data_ptr(), ws.numel(), None)
+    torch.cuda.synchronize()
+    assert np.array_equal(out1.cpu().numpy(), out.cpu().numpy())

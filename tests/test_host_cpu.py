@@ -253,6 +253,9 @@ def test_linear_problem_host_side():
     assert LP.make_new_band(20, 1, 1, True).main_kind() == "pds_tridiag"
     assert LP.make_new_band(20, 2, 2, True).main_kind() == "pds_band"
     assert LP.make_new_band(20, 0, 4, True).main_kind() == "band"  # kl != ku: not symmetric storage
+    assert LP.make_new_band(10, 0, 12, True).main_kind() == "dense"  # the maker's fallback takes ku >= n
+    with pytest.raises(DDCXError):
+        LP.make_new_band(10, 12, 12, True)  # symmetric band storage wider than the matrix
     p = LP.make_new_block_matrix_with_band_main_block(20, 2, 2, False, 3, 2)
     assert p.blocked() == 2 and p.required_number_of_rhs_rows() == 22
     assert LP.make_new_periodic_band_matrix(10, 4, 4, False).blocked() == 0  # dense is smaller
