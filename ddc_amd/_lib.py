@@ -87,6 +87,8 @@ SIGNATURES = {
     "ddcx_evaluator2d_eval_vjp_workspace": [_P, _P, _I64, _I64, _P],
     "ddcx_evaluator2d_eval_vjp_binned": [_P, _P, _P, _P, _P, _I64, _P, _P, _I64, _I64, _I64, _I64, _P, _I64, _P],
     "ddcx_evaluatornd_integrate": [_P, C.c_int, _P, _I64, _P, _I64, _P, _I64, _P],
+    "ddcx_evaluatornd_integrate_workspace": [_P, C.c_int, _I64, _P],
+    "ddcx_evaluatornd_integrate_ws": [_P, C.c_int, _P, _I64, _P, _I64, _P, _I64, _P, _I64, _P],
     "ddcx_evaluator2d_eval_vjp_binned_static": [_P, _P, _P, _P, _P, _I64, _P, _P, _I64, _I64, _I64, _I64, _P],
     "ddcx_problem_create": [C.POINTER(ddcx_problem_desc), _PP],
     "ddcx_problem_destroy": [_P],

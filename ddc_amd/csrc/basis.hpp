@@ -35,12 +35,6 @@ struct BSplinesView {
     const uint32_t* cells;
     int nbucket;
     double inv_bucket; // nbucket / (rmax - rmin)
-    // Non-uniform device evaluation: per cell, the reciprocals of the knot
-    // differences the Cox-de Boor recursion divides by, degree*(degree+1)/2
-    // of them (cell c at inv_dk + c * degree*(degree+1)/2, entry
-    // j*(j+1)/2 + r = 1 / (t[c+degree+r+1] - t[c+degree+r-j])).  nullptr:
-    // the reciprocals are formed per point.
-    const double* inv_dk;
 };
 
 __host__ __device__ inline double knot_coord(BSplinesView const& b, int k)
@@ -227,13 +221,10 @@ __host__ __device__ inline void uniform_basis_from_offset(double offset, double*
 
 // de Boor with left/right arrays, NonUniformBSplines eval_basis,
 // bsplines_non_uniform.hpp:461-497; `icell` is the knot index from
-// nonuniform_cell.  FAST (device evaluation): the divisor
-// right[r] + left[j-r] = t[icell+r+1] - t[icell+r-j] is taken as the cell's
-// tabulated reciprocal `inv` when given (else the hardware reciprocal
-// refined), and the update is one fma.
+// nonuniform_cell.  FAST (device evaluation): the divisor is inverted by the
+// refined hardware reciprocal and the update is one fma.
 template <int D, bool FAST = false>
-__host__ __device__ inline void nonuniform_basis(const double* knots, int icell, double x, double* values,
-                                                 const double* inv = nullptr)
+__host__ __device__ inline void nonuniform_basis(const double* knots, int icell, double x, double* values)
 {
     double left[D > 0 ? D : 1];
     double right[D > 0 ? D : 1];
@@ -247,7 +238,7 @@ __host__ __device__ inline void nonuniform_basis(const double* knots, int icell,
         for (int r = 0; r < j + 1; ++r) {
             double temp;
             if constexpr (FAST) {
-                temp = values[r] * (inv ? inv[j * (j + 1) / 2 + r] : fast_rcp(right[r] + left[j - r]));
+                temp = values[r] * fast_rcp(right[r] + left[j - r]);
                 values[r] = fma(right[r], temp, saved);
             } else {
                 temp = values[r] / (right[r] + left[j - r]);
@@ -257,13 +248,6 @@ __host__ __device__ inline void nonuniform_basis(const double* knots, int icell,
         }
         values[j + 1] = saved;
     }
-}
-
-// the cell's row of BSplinesView::inv_dk (FAST device evaluation only)
-template <int D>
-__host__ __device__ inline const double* cell_inv(BSplinesView const& b, int icell)
-{
-    return b.inv_dk ? b.inv_dk + size_t(icell - D) * (D * (D + 1) / 2) : nullptr;
 }
 
 // Basis kind of a kernel instantiation: decided by BSplinesView::uniform at
@@ -286,7 +270,7 @@ __host__ __device__ inline int eval_basis(BSplinesView const& b, const double* k
         return jmin;
     }
     int const icell = nonuniform_cell(b, knots, x);
-    nonuniform_basis<D, FAST>(knots, icell, x, values, FAST ? cell_inv<D>(b, icell) : nullptr);
+    nonuniform_basis<D, FAST>(knots, icell, x, values);
     return icell - D;
 }
 
@@ -323,7 +307,6 @@ __host__ __device__ inline int eval_basis_and_n_derivs(BSplinesView const& b, co
         double right[D > 0 ? D : 1];
         int const icell = nonuniform_cell(b, knots, x);
         jmin = icell - D;
-        const double* inv = FAST ? cell_inv<D>(b, icell) : nullptr;
         ndu[0][0] = 1.0;
 #pragma unroll
         for (int j = 0; j < D; ++j) {
@@ -332,8 +315,7 @@ __host__ __device__ inline int eval_basis_and_n_derivs(BSplinesView const& b, co
             double saved = 0.0;
 #pragma unroll
             for (int r = 0; r < j + 1; ++r) {
-                if constexpr (FAST)
-                    ndu[r][j + 1] = inv ? inv[j * (j + 1) / 2 + r] : fast_rcp(right[r] + left[j - r]);
+                if constexpr (FAST) ndu[r][j + 1] = fast_rcp(right[r] + left[j - r]);
                 else ndu[r][j + 1] = 1.0 / (right[r] + left[j - r]);
                 double const temp = ndu[j][r] * ndu[r][j + 1];
                 ndu[j + 1][r] = FAST ? fma(right[r], temp, saved) : saved + right[r] * temp;

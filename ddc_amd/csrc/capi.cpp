@@ -566,8 +566,50 @@ ddcx_status ddcx_evaluatornd_integrate(const ddcx_evaluator* const* evs, int nd,
         }
         if (nbatch == 0) return;
         require(out && coef, "null data pointer");
-        ddcx::launch_integratend(bs, nd, out, out_stride, coef, coef_batch_stride, coef_strides, nbatch,
+        ddcx::launch_integratend(bs, nd, out, out_stride, coef, coef_batch_stride, coef_strides, nbatch, nullptr, 1,
                                  static_cast<hipStream_t>(stream));
+    });
+}
+
+ddcx_status ddcx_evaluatornd_integrate_workspace(const ddcx_evaluator* const* evs, int nd, int64_t nbatch,
+                                                 int64_t* bytes)
+{
+    return guarded([&] {
+        require(evs && nd >= 1 && nd <= ddcx::kMaxNd, "nd must be between 1 and 4");
+        require(bytes, "null argument");
+        require(nbatch >= 0, "negative size");
+        ddcx::BSplinesPlan const* bs[ddcx::kMaxNd];
+        for (int k = 0; k < nd; ++k) {
+            require(evs[k], "null evaluator");
+            bs[k] = evs[k]->bs.get();
+        }
+        int const S = ddcx::integratend_split(bs, nd, nbatch);
+        *bytes = S > 1 ? nbatch * S * int64_t(sizeof(double)) : 0;
+    });
+}
+
+ddcx_status ddcx_evaluatornd_integrate_ws(const ddcx_evaluator* const* evs, int nd, double* out, int64_t out_stride,
+                                          const double* coef, int64_t coef_batch_stride, const int64_t* coef_strides,
+                                          int64_t nbatch, void* work, int64_t work_bytes, void* stream)
+{
+    return guarded([&] {
+        require(evs && nd >= 1 && nd <= ddcx::kMaxNd, "nd must be between 1 and 4");
+        require(coef_strides, "null coefficient strides");
+        require(nbatch >= 0, "negative size");
+        ddcx::BSplinesPlan const* bs[ddcx::kMaxNd];
+        for (int k = 0; k < nd; ++k) {
+            require(evs[k], "null evaluator");
+            bs[k] = evs[k]->bs.get();
+        }
+        if (nbatch == 0) return;
+        require(out && coef, "null data pointer");
+        int const S = ddcx::integratend_split(bs, nd, nbatch);
+        int64_t const need = S > 1 ? nbatch * S * int64_t(sizeof(double)) : 0;
+        require(work_bytes >= need, "integrate workspace smaller than ddcx_evaluatornd_integrate_workspace");
+        require(need == 0 || (work && reinterpret_cast<uintptr_t>(work) % alignof(double) == 0),
+                "integrate workspace null or misaligned");
+        ddcx::launch_integratend(bs, nd, out, out_stride, coef, coef_batch_stride, coef_strides, nbatch,
+                                 static_cast<double*>(work), S, static_cast<hipStream_t>(stream));
     });
 }
 

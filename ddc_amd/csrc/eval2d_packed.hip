@@ -9,7 +9,8 @@
 // that cell's points use into one contiguous, 16-byte aligned record of
 // P = (d1+1)(d2+1) doubles (padded to even), in a caller workspace; then
 //   1. each lane of a wave takes one point: rules, cell indices and basis
-//      values (the same routines as eval2d_kernel), parked in LDS;
+//      values (the same routines as eval2d_kernel, with the knots and cell
+//      brackets of a non-uniform dimension staged in LDS), parked in LDS;
 //   2. groups of 8 lanes take one point each and load its record as 16-byte
 //      pairs (one cache line per point for cubic x cubic), multiply by the
 //      basis products and reduce over the group with cross-lane adds;
@@ -27,6 +28,7 @@ namespace {
 constexpr int kPackThreads = 256;
 constexpr int kEvalThreads = 256;
 constexpr int kGroup = 8; // lanes per point in the gather
+constexpr size_t kTabLds = 64 * 1024; // LDS for staged knots and brackets
 
 __host__ __device__ constexpr int patch_len(int d1, int d2)
 {
@@ -58,12 +60,16 @@ __global__ void __launch_bounds__(kPackThreads) pack2d_kernel(const double* __re
     }
 }
 
+// Persistent: workgroup g stages the knot vector and bucket brackets of each
+// non-uniform dimension in LDS (when they fit, else they are read from
+// global memory), then takes tiles g, g + G, ... of kEvalThreads points, tile
+// t being points (t % tiles) * kEvalThreads ... of batch entry t / tiles.
 template <int D1, int D2, int O1, int O2>
-__global__ void __launch_bounds__(kEvalThreads) eval2d_packed_kernel(BSplinesView b1, BSplinesView b2,
-                                                                     Rules2D rules, double* __restrict__ out,
-                                                                     const double* __restrict__ coords,
-                                                                     int64_t npts, const double* __restrict__ packed,
-                                                                     int nc1, int nc2, int64_t batch0)
+__global__ void __launch_bounds__(kEvalThreads) eval2d_packed_kernel(BSplinesView b1, BSplinesView b2, Rules2D rules,
+                                                                     double* __restrict__ out,
+                                                                     const double* __restrict__ coords, int64_t npts,
+                                                                     const double* __restrict__ packed, int nc1,
+                                                                     int nc2, int64_t nbatch, int stage1, int stage2)
 {
     constexpr int P = patch_len(D1, D2);
     constexpr int NPAIR = P / 2;
@@ -72,67 +78,97 @@ __global__ void __launch_bounds__(kEvalThreads) eval2d_packed_kernel(BSplinesVie
     __shared__ double sb[WAVES][64][NB];  // basis values of dimension 1, then 2
     __shared__ int64_t srec[WAVES][64];   // record index, -1: the Null rule gives 0
     __shared__ double so[WAVES][64];
+    extern __shared__ double tabs[];      // staged knots 1 | knots 2 | brackets 1 | brackets 2
+    int const nk1 = stage1 ? b1.ncells + 1 + 2 * D1 : 0, nk2 = stage2 ? b2.ncells + 1 + 2 * D2 : 0;
+    double* const kn1 = tabs;
+    double* const kn2 = tabs + nk1;
+    uint32_t* const cl1 = reinterpret_cast<uint32_t*>(tabs + nk1 + nk2);
+    uint32_t* const cl2 = cl1 + (stage1 ? b1.nbucket : 0);
+    if (stage1) {
+        for (int i = threadIdx.x; i < nk1; i += kEvalThreads) kn1[i] = b1.knots[i];
+        for (int i = threadIdx.x; i < b1.nbucket; i += kEvalThreads) cl1[i] = b1.cells[i];
+    }
+    if (stage2) {
+        for (int i = threadIdx.x; i < nk2; i += kEvalThreads) kn2[i] = b2.knots[i];
+        for (int i = threadIdx.x; i < b2.nbucket; i += kEvalThreads) cl2[i] = b2.cells[i];
+    }
+    __syncthreads();
+    if (stage1) {
+        b1.knots = kn1;
+        b1.cells = b1.nbucket > 0 ? cl1 : nullptr;
+    }
+    if (stage2) {
+        b2.knots = kn2;
+        b2.cells = b2.nbucket > 0 ? cl2 : nullptr;
+    }
     int const wv = threadIdx.x / 64, lane = threadIdx.x % 64;
-    int64_t const l = batch0 + blockIdx.y;
-    int64_t const p0 = (int64_t(blockIdx.x) * WAVES + wv) * 64;
-    if (p0 >= npts) return; // wave-uniform
-    {
-        int64_t const p = p0 + lane < npts ? p0 + lane : npts - 1;
-        int64_t const t = l * npts + p;
-        double x1 = coords[2 * t], x2 = coords[2 * t + 1];
-        bool keep = true;
-        if constexpr (O1 == 0 && O2 == 0) {
-            bool in1, in2;
-            keep = place2d(b1, b2, rules, x1, x2, in1, in2);
-        }
-        double v1[D1 + 1], v2[D2 + 1];
-        int j1, j2;
-        if constexpr (O1 == 0) j1 = eval_basis<D1, true>(b1, b1.knots, x1, v1);
-        else {
-            double der[(D1 + 1) * (O1 + 1)];
-            j1 = eval_basis_and_n_derivs<D1, O1, true>(b1, b1.knots, x1, der);
-#pragma unroll
-            for (int i = 0; i < D1 + 1; ++i) v1[i] = der[i * (O1 + 1) + O1];
-        }
-        if constexpr (O2 == 0) j2 = eval_basis<D2, true>(b2, b2.knots, x2, v2);
-        else {
-            double der[(D2 + 1) * (O2 + 1)];
-            j2 = eval_basis_and_n_derivs<D2, O2, true>(b2, b2.knots, x2, der);
-#pragma unroll
-            for (int i = 0; i < D2 + 1; ++i) v2[i] = der[i * (O2 + 1) + O2];
-        }
-#pragma unroll
-        for (int i = 0; i < D1 + 1; ++i) sb[wv][lane][i] = v1[i];
-#pragma unroll
-        for (int j = 0; j < D2 + 1; ++j) sb[wv][lane][D1 + 1 + j] = v2[j];
-        srec[wv][lane] = keep ? (l * nc1 + j1) * int64_t(nc2) + j2 : -1;
-    }
-    __builtin_amdgcn_wave_barrier();
     int const sub = lane % kGroup;
-#pragma unroll 2
-    for (int it = 0; it < 64 / kGroup; ++it) {
-        int const q = it * kGroup + lane / kGroup;
-        int64_t const rec = srec[wv][q];
-        const double2* r2 = reinterpret_cast<const double2*>(packed + (rec < 0 ? 0 : rec) * P);
-        double part = 0.0;
+    int64_t const tiles = (npts + kEvalThreads - 1) / kEvalThreads;
+    for (int64_t t = blockIdx.x; t < tiles * nbatch; t += gridDim.x) {
+        int64_t const l = t / tiles;
+        int64_t const p0 = (t - l * tiles) * kEvalThreads + wv * 64;
+        if (p0 >= npts) continue; // wave-uniform
+        {
+            int64_t const p = p0 + lane < npts ? p0 + lane : npts - 1;
+            int64_t const q = l * npts + p;
+            double x1 = coords[2 * q], x2 = coords[2 * q + 1];
+            bool keep = true;
+            if constexpr (O1 == 0 && O2 == 0) {
+                bool in1, in2;
+                keep = place2d(b1, b2, rules, x1, x2, in1, in2);
+            }
+            double v1[D1 + 1], v2[D2 + 1];
+            int j1, j2;
+            if constexpr (O1 == 0) j1 = eval_basis<D1, true>(b1, b1.knots, x1, v1);
+            else {
+                double der[(D1 + 1) * (O1 + 1)];
+                j1 = eval_basis_and_n_derivs<D1, O1, true>(b1, b1.knots, x1, der);
 #pragma unroll
-        for (int e2 = sub; e2 < NPAIR; e2 += kGroup) {
-            double2 const c = r2[e2];
-            int const e = 2 * e2;
-            // element e is (row e / (D2+1), column e % (D2+1)); the pad element has a zero coefficient
-            int const ia = e / (D2 + 1), ja = e % (D2 + 1);
-            int const ib = (e + 1) / (D2 + 1), jb = (e + 1) % (D2 + 1);
-            double const wa = sb[wv][q][ia] * sb[wv][q][D1 + 1 + ja];
-            double const wb = (e + 1 < (D1 + 1) * (D2 + 1)) ? sb[wv][q][ib] * sb[wv][q][D1 + 1 + jb] : 0.0;
-            part = fma(c.x, wa, part);
-            part = fma(c.y, wb, part);
+                for (int i = 0; i < D1 + 1; ++i) v1[i] = der[i * (O1 + 1) + O1];
+            }
+            if constexpr (O2 == 0) j2 = eval_basis<D2, true>(b2, b2.knots, x2, v2);
+            else {
+                double der[(D2 + 1) * (O2 + 1)];
+                j2 = eval_basis_and_n_derivs<D2, O2, true>(b2, b2.knots, x2, der);
+#pragma unroll
+                for (int i = 0; i < D2 + 1; ++i) v2[i] = der[i * (O2 + 1) + O2];
+            }
+#pragma unroll
+            for (int i = 0; i < D1 + 1; ++i) sb[wv][lane][i] = v1[i];
+#pragma unroll
+            for (int j = 0; j < D2 + 1; ++j) sb[wv][lane][D1 + 1 + j] = v2[j];
+            srec[wv][lane] = keep ? (l * nc1 + j1) * int64_t(nc2) + j2 : -1;
         }
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll 2
+        for (int it = 0; it < 64 / kGroup; ++it) {
+            int const q = it * kGroup + lane / kGroup;
+            int64_t const rec = srec[wv][q];
+            const double2* r2 = reinterpret_cast<const double2*>(packed + (rec < 0 ? 0 : rec) * P);
+            double part = 0.0;
 #pragma unroll
-        for (int m = 1; m < kGroup; m *= 2) part += __shfl_xor(part, m, kGroup);
-        if (sub == 0) so[wv][q] = rec < 0 ? 0.0 : part;
+            for (int k = 0; k < (NPAIR + kGroup - 1) / kGroup; ++k) {
+                int const e2 = sub + k * kGroup;
+                if (e2 >= NPAIR) break;
+                double2 const c = r2[e2];
+                int const e = 2 * e2;
+                // element e is (row e / (D2+1), column e % (D2+1)); the pad element has a zero coefficient
+                int const ia = e / (D2 + 1), ja = e % (D2 + 1);
+                int const ib = (e + 1) / (D2 + 1), jb = (e + 1) % (D2 + 1);
+                double const wa = sb[wv][q][ia] * sb[wv][q][D1 + 1 + ja];
+                double const wb = (e + 1 < (D1 + 1) * (D2 + 1)) ? sb[wv][q][ib] * sb[wv][q][D1 + 1 + jb] : 0.0;
+                part = fma(c.x, wa, part);
+                part = fma(c.y, wb, part);
+            }
+#pragma unroll
+            for (int m = 1; m < kGroup; m *= 2) part += __shfl_xor(part, m, kGroup);
+            if (sub == 0) so[wv][q] = rec < 0 ? 0.0 : part;
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (p0 + lane < npts) out[l * npts + p0 + lane] = so[wv][lane];
+        // the next tile's LDS writes come after every lane's read of this one
+        __builtin_amdgcn_wave_barrier();
     }
-    __builtin_amdgcn_wave_barrier();
-    if (p0 + lane < npts) out[l * npts + p0 + lane] = so[wv][lane];
 }
 
 template <class F>
@@ -203,21 +239,32 @@ void launch_eval2d_packed(BSplinesPlan const& bp1, BSplinesPlan const& bp2, Rule
     int64_t const chunk = work_bytes / rec_bytes; // batch entries packed at a time
     if (chunk < 1) throw Error(DDCX_ERR_INVALID, "packed 2D evaluation: workspace smaller than one batch entry");
     double* packed = static_cast<double*>(work);
-    unsigned const gx = unsigned((npts + kEvalThreads - 1) / kEvalThreads);
-    if (npts > int64_t(0x7fffffff) * kEvalThreads) throw Error(DDCX_ERR_INVALID, "too many points for one launch");
+    // knots and brackets of the non-uniform dimensions go to LDS when they fit
+    auto tab_bytes = [](BSplinesView const& b) {
+        return b.uniform ? size_t(0) : size_t(b.ncells + 1 + 2 * b.degree) * sizeof(double) + size_t(b.nbucket) * 4;
+    };
+    size_t const t1 = tab_bytes(b1), t2 = tab_bytes(b2);
+    int const stage1 = !b1.uniform && t1 <= kTabLds, stage2 = !b2.uniform && t1 * stage1 + t2 <= kTabLds;
+    size_t const lds = (stage1 ? t1 : 0) + (stage2 ? t2 : 0);
+    int64_t const tiles = (npts + kEvalThreads - 1) / kEvalThreads;
     bool const ok = dispatch_2d(b1.degree, b2.degree, o1, o2, [&]<int D1, int D2, int O1, int O2>() {
+        auto kern = eval2d_packed_kernel<D1, D2, O1, O2>;
+        if (lds > 0)
+            hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)),
+                      "hipFuncSetAttribute");
+        int per_cu = 0;
+        hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kEvalThreads, lds),
+                  "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+        int64_t const resident = int64_t(std::max(per_cu, 1)) * cu_count();
         for (int64_t l0 = 0; l0 < nbatch; l0 += chunk) {
             int64_t const nb = std::min(chunk, nbatch - l0);
             int64_t const nthreads = int64_t(nc1) * nc2 * (D1 + 1) * nb;
             hipLaunchKernelGGL((pack2d_kernel<D1, D2>), dim3(unsigned((nthreads + kPackThreads - 1) / kPackThreads)),
                                dim3(kPackThreads), 0, stream, coef + l0 * cbs, cbs, st1, st2, nc1, nc2, nb, packed);
-            for (int64_t l1 = 0; l1 < nb; l1 += 65535) {
-                unsigned const gy = unsigned(std::min<int64_t>(65535, nb - l1));
-                // the kernel indexes batch entries from the packed chunk's start
-                hipLaunchKernelGGL((eval2d_packed_kernel<D1, D2, O1, O2>), dim3(gx, gy), dim3(kEvalThreads), 0,
-                                   stream, b1, b2, rules, out + l0 * npts, coords + 2 * l0 * npts, npts, packed, nc1,
-                                   nc2, l1);
-            }
+            unsigned const grid = unsigned(std::min<int64_t>(tiles * nb, resident));
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(kEvalThreads), lds, stream, b1, b2, rules, out + l0 * npts,
+                               coords + 2 * l0 * npts, npts, packed, nc1, nc2, nb, stage1, stage2);
         }
     });
     if (!ok) throw Error(DDCX_ERR_UNSUPPORTED, "2D evaluation supports degrees 1..5 and derivative orders <= 2");

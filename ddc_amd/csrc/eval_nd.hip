@@ -7,11 +7,13 @@
 // basis routines by a wave-uniform switch), then the sum over the
 // (d_1+1) x ... x (d_N+1) coefficient patch in row-major order with the
 // basis product folded from the right, as eval_no_bc does
-// (spline_evaluator_nd.hpp:782-826).  Integration is one workgroup per batch
-// entry reducing coef(i) * (I_1(i_1) * (...)) over the whole coefficient
-// block (spline_evaluator_nd.hpp:597-632).
+// (spline_evaluator_nd.hpp:782-826).  Integration reduces
+// coef(i) * (I_1(i_1) * (...)) over the coefficient block of each batch entry
+// (spline_evaluator_nd.hpp:597-632): one workgroup per entry, or, with a
+// workspace for the partial sums, S workgroups per entry and a second pass.
 #include "internal.hpp"
 
+#include <algorithm>
 #include <utility>
 
 namespace ddcx {
@@ -238,21 +240,24 @@ __global__ void __launch_bounds__(256) evalnd_fixed_kernel(NdArgs a, double* __r
 
 constexpr int kIntThreads = 1024;
 
-// One workgroup per batch entry.  Wave w takes the rows (all indices but the
-// last) in groups of RW (w*RW .. w*RW+RW-1, then the next group of the next
-// round), and its lanes sweep the last dimension, so the loads of a row are
-// contiguous when the last dimension is.  Each term is
-// coef(i) * (I_1(i_1) * (... * I_N(i_N))) as the reference forms it; the
-// per-lane sums are combined by a fixed LDS tree, so results do not depend on
-// scheduling.
+// Workgroup g takes batch entry l = g / S and the rows (all indices but the
+// last) [rows*s/S, rows*(s+1)/S) of it, s = g % S.  Wave w takes those rows
+// in groups of RW (then the next group of the next round), and its lanes
+// sweep the last dimension, so the loads of a row are contiguous when the
+// last dimension is.  Each term is coef(i) * (I_1(i_1) * (... * I_N(i_N))) as
+// the reference forms it; the per-lane sums are combined by a fixed LDS tree
+// into dst[l*dst_ls + s], and the S partial sums of an entry by
+// integratend_sum_kernel in index order, so results do not depend on
+// scheduling (they do depend on S, which is a function of the sizes only).
 template <int N>
-__global__ void __launch_bounds__(kIntThreads) integratend_kernel(NdArgs a, double* __restrict__ out,
-                                                                  int64_t ostride, const double* __restrict__ coef,
-                                                                  int64_t cbs, int64_t nbatch)
+__global__ void __launch_bounds__(kIntThreads) integratend_kernel(NdArgs a, double* __restrict__ dst, int64_t dst_ls,
+                                                                  const double* __restrict__ coef, int64_t cbs,
+                                                                  int S)
 {
     constexpr int W = 64, NW = kIntThreads / W;
     __shared__ double part[kIntThreads];
-    int64_t const l = blockIdx.x;
+    int64_t const l = blockIdx.x / unsigned(S);
+    int const s = int(blockIdx.x - l * S);
     int const lane = threadIdx.x % W;
     int const wave = threadIdx.x / W;
     int n[N];
@@ -262,6 +267,7 @@ __global__ void __launch_bounds__(kIntThreads) integratend_kernel(NdArgs a, doub
         n[k] = a.b[k].ncells + a.b[k].degree;
         if (k < N - 1) rows *= n[k];
     }
+    int64_t const r_lo = rows * s / S, r_hi = rows * (s + 1) / S;
     const double* c = coef + l * cbs;
     const double* qn = a.integrals[N - 1];
     int64_t const sn = a.st[N - 1];
@@ -270,12 +276,12 @@ __global__ void __launch_bounds__(kIntThreads) integratend_kernel(NdArgs a, doub
     double acc[RW];
 #pragma unroll
     for (int u = 0; u < RW; ++u) acc[u] = 0.0;
-    for (int64_t r0 = int64_t(wave) * RW; r0 < rows; r0 += int64_t(NW) * RW) {
+    for (int64_t r0 = r_lo + int64_t(wave) * RW; r0 < r_hi; r0 += int64_t(NW) * RW) {
         int64_t off[RW];
         double outer[RW][N > 1 ? N - 1 : 1];
 #pragma unroll
         for (int u = 0; u < RW; ++u) {
-            uint32_t rr = uint32_t(min(r0 + u, rows - 1)); // rows < 2^31, checked at launch
+            uint32_t rr = uint32_t(min(r0 + u, r_hi - 1)); // rows < 2^31, checked at launch
             off[u] = 0;
 #pragma unroll
             for (int k = N - 2; k >= 0; --k) {
@@ -293,7 +299,7 @@ __global__ void __launch_bounds__(kIntThreads) integratend_kernel(NdArgs a, doub
 #pragma unroll
                 for (int k = N - 2; k >= 0; --k) p = outer[u][k] * p;
                 double const t = c[off[u] + i * sn] * p;
-                if (r0 + u < rows) acc[u] += t;
+                if (r0 + u < r_hi) acc[u] += t;
             }
         }
     }
@@ -302,11 +308,22 @@ __global__ void __launch_bounds__(kIntThreads) integratend_kernel(NdArgs a, doub
     for (int u = 0; u < RW; ++u) sum += acc[u];
     part[threadIdx.x] = sum;
     __syncthreads();
-    for (int s = kIntThreads / 2; s > 0; s /= 2) {
-        if (int(threadIdx.x) < s) part[threadIdx.x] += part[threadIdx.x + s];
+    for (int h = kIntThreads / 2; h > 0; h /= 2) {
+        if (int(threadIdx.x) < h) part[threadIdx.x] += part[threadIdx.x + h];
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[l * ostride] = part[0];
+    if (threadIdx.x == 0) dst[l * dst_ls + s] = part[0];
+}
+
+// out[l*ostride] = sum of the S partial sums of entry l, in index order
+__global__ void integratend_sum_kernel(const double* __restrict__ part, int S, double* __restrict__ out,
+                                       int64_t ostride, int64_t nbatch)
+{
+    int64_t const l = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (l >= nbatch) return;
+    double y = 0.0;
+    for (int s = 0; s < S; ++s) y += part[l * S + s];
+    out[l * ostride] = y;
 }
 
 NdArgs make_args(BSplinesPlan const* const* bs, int nd, const int* orders, const int* lower, const int* upper,
@@ -388,20 +405,50 @@ void launch_evalnd(BSplinesPlan const* const* bs, int nd, const int* orders, con
     hip_check(hipGetLastError(), "evalnd_kernel launch");
 }
 
+namespace {
+
+int64_t integratend_rows(BSplinesPlan const* const* bs, int nd)
+{
+    int64_t rows = 1;
+    for (int k = 0; k + 1 < nd; ++k) rows *= bs[k]->size();
+    return rows;
+}
+
+} // namespace
+
+int integratend_split(BSplinesPlan const* const* bs, int nd, int64_t nbatch)
+{
+    // workgroups per batch entry: about 8 per CU over the whole batch, each
+    // with at least 64 rows
+    int64_t const rows = integratend_rows(bs, nd);
+    int64_t s = (2048 + std::max<int64_t>(nbatch, 1) - 1) / std::max<int64_t>(nbatch, 1);
+    s = std::min<int64_t>(s, std::max<int64_t>(1, rows / 64));
+    return int(std::max<int64_t>(1, std::min<int64_t>(s, 1024)));
+}
+
 void launch_integratend(BSplinesPlan const* const* bs, int nd, double* out, int64_t ostride, const double* coef,
-                        int64_t cbs, const int64_t* st, int64_t nbatch, hipStream_t stream)
+                        int64_t cbs, const int64_t* st, int64_t nbatch, double* part, int S, hipStream_t stream)
 {
     if (nbatch == 0) return;
     NdArgs const a = make_args(bs, nd, nullptr, nullptr, nullptr, st);
-    if (nbatch > int64_t(0x7fffffff)) throw Error(DDCX_ERR_INVALID, "batch too large for one launch");
-    int64_t rows = 1;
-    for (int k = 0; k + 1 < nd; ++k) rows *= bs[k]->size();
+    int64_t const rows = integratend_rows(bs, nd);
     if (rows >= (int64_t(1) << 31)) throw Error(DDCX_ERR_UNSUPPORTED, "too many coefficient rows to integrate");
+    if (S < 1 || (S > 1 && !part)) throw Error(DDCX_ERR_INVALID, "integrate: bad split");
+    if (nbatch * S > int64_t(0x7fffffff)) throw Error(DDCX_ERR_INVALID, "batch too large for one launch");
     dispatch_nd(nd, [&]<int N>() {
-        hipLaunchKernelGGL((integratend_kernel<N>), dim3(unsigned(nbatch)), dim3(kIntThreads), 0, stream, a, out,
-                           ostride, coef, cbs, nbatch);
+        if (S == 1)
+            hipLaunchKernelGGL((integratend_kernel<N>), dim3(unsigned(nbatch)), dim3(kIntThreads), 0, stream, a, out,
+                               ostride, coef, cbs, 1);
+        else
+            hipLaunchKernelGGL((integratend_kernel<N>), dim3(unsigned(nbatch * S)), dim3(kIntThreads), 0, stream, a,
+                               part, int64_t(S), coef, cbs, S);
     });
     hip_check(hipGetLastError(), "integratend_kernel launch");
+    if (S > 1) {
+        hipLaunchKernelGGL(integratend_sum_kernel, dim3(unsigned((nbatch + 255) / 256)), dim3(256), 0, stream, part, S,
+                           out, ostride, nbatch);
+        hip_check(hipGetLastError(), "integratend_sum_kernel launch");
+    }
 }
 
 } // namespace ddcx

@@ -140,7 +140,6 @@ __global__ void __launch_bounds__(NT) eval_tiled_kernel(BSplinesView b, Rules1D 
     const double* knots = is_uniform_kind(KIND, b.uniform) ? nullptr : kn;
     BSplinesView bl = b;
     bl.cells = b.nbucket > 0 ? cells : nullptr;
-    bl.inv_dk = nullptr; // knots are in LDS: the refined reciprocal beats a global table load
     const double* tl = tile + ln;
     // software pipeline: the next batch of coordinates is in flight while the
     // current one is evaluated and stored
@@ -241,7 +240,6 @@ __global__ void __launch_bounds__(NT) eval_stream_kernel(BSplinesView b, Rules1D
     const double* knots = is_uniform_kind(KIND, b.uniform) ? nullptr : kn;
     BSplinesView bl = b;
     bl.cells = b.nbucket > 0 ? cells : nullptr;
-    bl.inv_dk = nullptr; // knots are in LDS: the refined reciprocal beats a global table load
 
     int64_t const G = gridDim.x;
     int64_t t = blockIdx.x;
@@ -382,7 +380,6 @@ __global__ void __launch_bounds__(NT) eval_vjp_tiled_kernel(BSplinesView b, Rule
     const double* knots = b.uniform ? nullptr : kn;
     BSplinesView bl = b;
     bl.cells = b.nbucket > 0 ? cells : nullptr;
-    bl.inv_dk = nullptr; // knots are in LDS: the refined reciprocal beats a global table load
     if (active) {
         const double* tl = tile + ln;
         double* gl = gt + ln;
@@ -429,19 +426,6 @@ __global__ void __launch_bounds__(NT) eval_vjp_tiled_kernel(BSplinesView b, Rule
 
 constexpr size_t kMaxLds = 160 * 1024;
 
-int cu_count()
-{
-    static int cached[64] = {};
-    int dev = 0;
-    hip_check(hipGetDevice(&dev), "hipGetDevice");
-    if (dev < 0 || dev >= 64) dev = 0;
-    if (cached[dev] == 0) {
-        int n = 0;
-        hip_check(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
-        cached[dev] = n > 0 ? n : 1;
-    }
-    return cached[dev];
-}
 
 // eval_stream_kernel for the value form when its shape fits: plain
 // batch-contiguous points and coefficients (element (row, line) at row * rs +
@@ -526,6 +510,20 @@ bool launch_orders(BSplinesPlan const& bs, Rules1D rules, double* out, const dou
 }
 
 } // namespace
+
+int cu_count()
+{
+    static int cached[64] = {};
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= 64) dev = 0;
+    if (cached[dev] == 0) {
+        int n = 0;
+        hip_check(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+        cached[dev] = n > 0 ? n : 1;
+    }
+    return cached[dev];
+}
 
 bool launch_eval_tiled(BSplinesPlan const& bs, Rules1D rules, int order, double* out, const double* coords,
                        Layout pl, int64_t npts, const double* coef, Layout cl, int64_t nbatch, hipStream_t stream)

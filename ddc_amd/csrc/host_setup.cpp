@@ -554,22 +554,6 @@ std::shared_ptr<BSplinesPlan> make_bsplines(ddcx_bsplines_desc const& d)
         }
     }
 
-    v.inv_dk = nullptr;
-    if (!v.uniform) {
-        // reciprocals of the knot differences of each cell's Cox-de Boor
-        // recursion, for the device evaluators (BSplinesView::inv_dk)
-        int const nr = D * (D + 1) / 2;
-        std::vector<double> const& t = p->knots_host;
-        p->inv_dk_host.assign(size_t(d.ncells) * nr, 0.0);
-        for (int c = 0; c < d.ncells; ++c) {
-            int const icell = c + D;
-            for (int j = 0; j < D; ++j)
-                for (int r = 0; r <= j; ++r)
-                    p->inv_dk_host[size_t(c) * nr + j * (j + 1) / 2 + r] =
-                        1.0 / (t[icell + r + 1] - t[icell + r - j]);
-        }
-    }
-
     // integrals(), integrals.hpp:30-153
     int const sz = p->size(), nb = p->nbasis();
     p->integrals.assign(sz, 0.0);
@@ -615,7 +599,6 @@ void BSplinesPlan::upload()
 {
     if (!view.uniform) knots_dev.upload(knots_host);
     if (!cells_host.empty()) cells_dev.upload(cells_host);
-    if (!inv_dk_host.empty()) inv_dk_dev.upload(inv_dk_host);
     integrals_dev.upload(integrals);
     on_device = true;
 }

@@ -116,9 +116,6 @@ struct BSplinesPlan {
     DeviceBuffer cells_dev;
     std::vector<double> integrals; // integrals() over the full spline domain (size() entries)
     DeviceBuffer integrals_dev;
-    // non-uniform bases: BSplinesView::inv_dk of the device view
-    std::vector<double> inv_dk_host;
-    DeviceBuffer inv_dk_dev;
     bool on_device = false; // uploaded at creation when a HIP device is present
     void upload();
     void ensure_device() const; // throws unless the device copies exist
@@ -131,7 +128,6 @@ struct BSplinesPlan {
         v.cells = cells_host.empty() ? nullptr : cells_dev.as<uint32_t>();
         v.nbucket = int(cells_host.size());
         v.inv_bucket = inv_bucket;
-        v.inv_dk = inv_dk_host.empty() ? nullptr : inv_dk_dev.as<double>();
         return v;
     }
 };
@@ -385,8 +381,14 @@ constexpr int kMaxNd = 4;
 void launch_evalnd(BSplinesPlan const* const* bs, int nd, const int* orders, const int* lower, const int* upper,
                    double* out, const double* coords, int64_t npts, const double* coef, int64_t cbs,
                    const int64_t* st, int64_t nbatch, hipStream_t stream);
+// compute units of the current device (cached per device)
+int cu_count();
+
+// workgroups per batch entry of the integration (1: no partial sums)
+int integratend_split(BSplinesPlan const* const* bs, int nd, int64_t nbatch);
+// part: nbatch * S doubles of partial sums when S > 1
 void launch_integratend(BSplinesPlan const* const* bs, int nd, double* out, int64_t ostride, const double* coef,
-                        int64_t cbs, const int64_t* st, int64_t nbatch, hipStream_t stream);
+                        int64_t cbs, const int64_t* st, int64_t nbatch, double* part, int S, hipStream_t stream);
 void launch_eval2d(BSplinesPlan const& b1, BSplinesPlan const& b2, Rules2D const& rules, int d1, int d2, double* out,
                    const double* coords, int64_t npts, const double* coef, int64_t coef_batch_stride, int64_t st1,
                    int64_t st2, int64_t nbatch, hipStream_t stream);

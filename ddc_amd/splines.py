@@ -782,8 +782,12 @@ class SplineEvaluatorND:
         out = integrals_out.reshape(-1)
         if out.numel() != nb:
             raise ValueError("integrals_out must hold one value per batch entry")
-        call("ddcx_evaluatornd_integrate", self._hs, nd, _ptr(out), out.stride(0), _ptr(coef), coef.stride(0), st,
-             nb, _stream())
+        # the partial sums of the split reduction go to a device workspace
+        need = C.c_int64()
+        call("ddcx_evaluatornd_integrate_workspace", self._hs, nd, nb, C.byref(need))
+        ws = torch.empty(max(need.value, 1), dtype=torch.uint8, device=coef.device)
+        call("ddcx_evaluatornd_integrate_ws", self._hs, nd, _ptr(out), out.stride(0), _ptr(coef), coef.stride(0), st,
+             nb, _ptr(ws), need.value, _stream())
         return integrals_out
 
 

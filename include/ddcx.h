@@ -318,6 +318,18 @@ ddcx_status ddcx_evaluatornd_integrate(const ddcx_evaluator* const* evs, int nd,
                                        const double* coef, int64_t coef_batch_stride, const int64_t* coef_strides,
                                        int64_t nbatch, void* stream);
 
+/* The same integral with a caller workspace (no reference counterpart): the
+ * rows of each batch entry are split over several workgroups whose partial
+ * sums go to the workspace and are then added in index order, so a small
+ * batch still spreads over the whole GPU.  *bytes = the workspace size for
+ * nbatch entries (0 when the split is not needed; work may then be NULL).
+ * Results equal ddcx_evaluatornd_integrate's up to rounding. */
+ddcx_status ddcx_evaluatornd_integrate_workspace(const ddcx_evaluator* const* evs, int nd, int64_t nbatch,
+                                                 int64_t* bytes);
+ddcx_status ddcx_evaluatornd_integrate_ws(const ddcx_evaluator* const* evs, int nd, double* out, int64_t out_stride,
+                                          const double* coef, int64_t coef_batch_stride, const int64_t* coef_strides,
+                                          int64_t nbatch, void* work, int64_t work_bytes, void* stream);
+
 /* Vector-Jacobian product of ddcx_evaluator2d_eval (value form; no reference
  * counterpart): gcoef (same strides as coef, overwritten) = sum_p w_p
  * B1(x1_p) (x) B2(x2_p) (tensor scatter-add); gx (interleaved pairs like

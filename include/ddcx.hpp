@@ -534,6 +534,22 @@ public:
         check(ddcx_evaluatornd_integrate(hs_.data(), dimension(), out, out_stride, coef, coef_batch_stride,
                                          strides.data(), nbatch, stream));
     }
+    // the same with a caller workspace of integrate_workspace_size(nbatch)
+    // bytes, over which each batch entry's reduction is split
+    int64_t integrate_workspace_size(int64_t nbatch) const
+    {
+        int64_t bytes = 0;
+        check(ddcx_evaluatornd_integrate_workspace(hs_.data(), dimension(), nbatch, &bytes));
+        return bytes;
+    }
+    void integrate(double* out, int64_t out_stride, const double* coef, int64_t coef_batch_stride,
+                   std::vector<int64_t> const& strides, int64_t nbatch, void* work, int64_t work_bytes,
+                   void* stream = nullptr) const
+    {
+        if (int(strides.size()) != dimension()) throw std::runtime_error("ddcx: one stride per dimension");
+        check(ddcx_evaluatornd_integrate_ws(hs_.data(), dimension(), out, out_stride, coef, coef_batch_stride,
+                                            strides.data(), nbatch, work, work_bytes, stream));
+    }
 
 private:
     std::vector<const ddcx_evaluator*> hs_;

@@ -334,3 +334,33 @@ def test_eval2d_packed_matches_oracle(degs, orders, cuda):
          npts, tc.data_ptr(), tc.stride(0), tc.stride(1), tc.stride(2), nb, ws.data_ptr(), ws.numel(), None)
     torch.cuda.synchronize()
     assert np.array_equal(out1.cpu().numpy(), out.cpu().numpy())
+
+
+@pytest.mark.parametrize("n1,n2", [(23, 19), (6000, 21)], ids=["both_staged", "dim1_too_large_for_lds"])
+def test_eval2d_packed_nonuniform_both(n1, n2, cuda):
+    """Packed evaluation with both dimensions non-uniform: the knots and cell
+    brackets of both staged in LDS, and of dimension 1 read from global
+    memory when they do not fit (6000 cells); constant rules on dimension 2
+    with points outside on every side, against the oracle."""
+    import torch
+    import ddc_amd as D
+    import oracle as O
+
+    br1 = breaks_for("random", n1, seed=21)
+    br2 = breaks_for("random", n2, seed=22)
+    o1 = O.Spline(3, False, G, G, breaks=br1)
+    o2 = O.Spline(2, False, G, G, breaks=br2)
+    bs1, bs2 = D.NonUniformBSplines(3, br1), D.NonUniformBSplines(2, br2)
+    ev = D.SplineEvaluator2D(D.SplineEvaluator(bs1, D.ConstantExtrapolationRule(), D.ConstantExtrapolationRule()),
+                             D.SplineEvaluator(bs2, D.ConstantExtrapolationRule(), D.ConstantExtrapolationRule()))
+    nb, npts = 2, 3000
+    rng = np.random.default_rng(23)
+    coef = rng.standard_normal((nb, bs1.size(), bs2.size()))
+    lo1, hi1, lo2, hi2 = br1[0], br1[-1], br2[0], br2[-1]
+    w1, w2 = hi1 - lo1, hi2 - lo2
+    coords = np.stack([rng.uniform(lo1 - 0.2 * w1, hi1 + 0.2 * w1, (nb, npts)),
+                       rng.uniform(lo2 - 0.2 * w2, hi2 + 0.2 * w2, (nb, npts))], axis=-1)
+    ref = O.spline2d_eval(o1, o2, coords, coef, 0, 0, rules=(1, 1, 1, 1))
+    out = torch.full((nb, npts), np.nan, dtype=torch.float64, device=cuda)
+    ev(out, torch.from_numpy(coords).to(cuda), torch.from_numpy(coef).to(cuda), packed=True)
+    assert rel_err(out.cpu().numpy(), ref) <= TOL

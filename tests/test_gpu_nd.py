@@ -221,6 +221,50 @@ def test_integratend_matches_oracle(spec, cuda):
     assert np.all(np.abs(out.cpu().numpy() - ref) <= TOL * scale)
 
 
+@pytest.mark.parametrize("spec,nb", [([(3, "nug", 64), (3, "up", 64), (3, "ug", 64)], 1),
+                                     ([(5, "up", 40), (2, "ug", 30)], 3), (SPECS_3D[0], 700)],
+                         ids=["64cubed_nb1", "2d_nb3", "d3mix_nb700"])
+def test_integratend_split_matches_single_pass(spec, nb, cuda):
+    """The split reduction (workspace entry point, several workgroups per
+    batch entry) against the one-workgroup-per-entry entry point and the
+    oracle, on shapes where the split is 1, a few, and many workgroups."""
+    import ctypes as C
+    import torch
+    import ddc_amd as D
+    import oracle as O
+    from ddc_amd._lib import call
+
+    bases = _bases(spec)
+    ss = [x[0] for x in bases]
+    rng = np.random.default_rng(16)
+    coef = _coef(ss, nb, rng)
+    ev = D.SplineEvaluatorND(*[x[3] for x in bases])
+    tc = torch.from_numpy(coef).to(cuda)
+    nd = len(spec)
+    st = (C.c_int64 * nd)(*[tc.stride(k + 1) for k in range(nd)])
+    need = C.c_int64()
+    call("ddcx_evaluatornd_integrate_workspace", ev._hs, nd, nb, C.byref(need))
+    rows = int(np.prod([o.size for o in ss[:-1]]))
+    if nb < 8 and rows >= 128:
+        assert need.value > 0  # the split is in use
+    one = torch.full((nb,), np.nan, dtype=torch.float64, device=cuda)
+    call("ddcx_evaluatornd_integrate", ev._hs, nd, C.c_void_p(one.data_ptr()), 1, C.c_void_p(tc.data_ptr()),
+         tc.stride(0), st, nb, C.c_void_p(torch.cuda.current_stream().cuda_stream))
+    split = torch.full((nb,), np.nan, dtype=torch.float64, device=cuda)
+    ev.integrate(split, tc)
+    ref = O.splinend_integrate(ss, coef)
+    scale = O.splinend_integrate(ss, np.abs(coef))
+    assert np.all(np.abs(split.cpu().numpy() - ref) <= TOL * scale)
+    assert np.all(np.abs(split.cpu().numpy() - one.cpu().numpy()) <= 1e-14 * scale)
+    # a workspace one byte short is refused
+    if need.value > 0:
+        ws = torch.empty(need.value, dtype=torch.uint8, device=cuda)
+        with pytest.raises(D.DDCXError, match="workspace"):
+            call("ddcx_evaluatornd_integrate_ws", ev._hs, nd, C.c_void_p(split.data_ptr()), 1,
+                 C.c_void_p(tc.data_ptr()), tc.stride(0), st, nb, C.c_void_p(ws.data_ptr()), need.value - 1,
+                 C.c_void_p(torch.cuda.current_stream().cuda_stream))
+
+
 def test_build_eval3d_reproduces_interpolation_points(cuda):
     """Evaluating the 3D spline at its interpolation grid returns the values."""
     import torch
