@@ -6,14 +6,18 @@
 //               points that the Null rule sends to 0 get gx = 0 here
 //   B  scan:    per batch entry, tile start offsets
 //   C  scatter: each workgroup reserves one range per tile it saw and writes
-//               its points' indices there (perm)
+//               its points there in tile order: index (perm) and the record
+//               (x1, x2, w)
 //   D  tiles:   one workgroup per (batch entry, tile) stages the tile's
-//               coefficient patch ((T + d1) x (T + d2)) in LDS, evaluates
-//               gx = w * grad f for its points from it, accumulates
+//               coefficient patch ((T + d1) x (T + d2)) in LDS, reads its
+//               points' records in order, evaluates gx = w * grad f from
+//               the patch (stored through perm), accumulates
 //               w * B1 (x) B2 in an LDS gradient patch with LDS atomics and
 //               adds the patch to gcoef (zeroed first) with one global atomic
 //               per nonzero entry (neighbouring patches overlap by d rows)
 //
+// The knots and cell brackets of non-uniform dimensions are staged in LDS by
+// every kernel that locates points.
 // The point semantics (periodic wrap, Null and Constant rules, zero
 // coordinate gradient along a clamped dimension) are those of
 // eval2d_vjp_kernel.  The order of the additions inside a patch depends on
@@ -27,7 +31,8 @@ namespace {
 constexpr int kBinT = 256;       // threads of the count / scatter workgroups
 constexpr int kBinPts = 16;      // points per thread there (4096 per workgroup)
 constexpr int kMaxBins = 4096;   // tiles per batch entry
-constexpr int kTileThreads = 256;
+constexpr int kTileThreads = 1024;
+constexpr size_t kTabLds = 48 * 1024; // LDS for staged knots and brackets
 
 struct Bin2D {
     int T;        // tile side in cells
@@ -79,12 +84,15 @@ __global__ void __launch_bounds__(256) zero_patch_kernel(double* __restrict__ g,
 
 __global__ void __launch_bounds__(kBinT) bin_count_kernel(BSplinesView b1, BSplinesView b2, Rules2D rules, Bin2D g,
                                                           const double* __restrict__ coords, double* __restrict__ gx,
-                                                          int64_t npts, int64_t bpb, int* __restrict__ counts)
+                                                          int64_t npts, int64_t bpb, int* __restrict__ counts,
+                                                          StagedTables2D tabs)
 {
     __shared__ int hist[kMaxBins];
+    extern __shared__ double lds_tabs[];
     int64_t const l = blockIdx.x / bpb;
     int64_t const p0 = (blockIdx.x % bpb) * int64_t(kBinT * kBinPts);
     for (int k = threadIdx.x; k < g.nbins; k += kBinT) hist[k] = 0;
+    stage_tables_2d(tabs, b1, b2, lds_tabs, threadIdx.x, kBinT);
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < kBinPts; ++u) {
@@ -136,43 +144,91 @@ __global__ void __launch_bounds__(1024) bin_scan_kernel(Bin2D g, const int* __re
     if (t == 1023) starts[l * (g.nbins + 1) + g.nbins] = part[1023];
 }
 
-__global__ void __launch_bounds__(kBinT) bin_scatter_kernel(BSplinesView b1, BSplinesView b2, Rules2D rules, Bin2D g,
-                                                            const double* __restrict__ coords, int64_t npts,
-                                                            int64_t bpb, int* __restrict__ cursor,
-                                                            int* __restrict__ perm)
+// LDS of the scatter workgroup after the staged tables: the tile histogram,
+// local and global tile offsets (nbins ints each) and the workgroup's points
+// in tile order (kBinT * kBinPts ints)
+size_t scatter_lds(StagedTables2D const& tabs, Bin2D const& g)
 {
-    __shared__ int hist[kMaxBins];
-    __shared__ int base[kMaxBins];
+    return (tabs.bytes + 7) / 8 * 8 + (size_t(3) * g.nbins + size_t(kBinT) * kBinPts) * sizeof(int);
+}
+
+// The workgroup's points are first ordered by tile in LDS (slot j holds
+// tile << 12 | local index), so the records of one tile go out as one run of
+// consecutive lanes and addresses.
+__global__ void __launch_bounds__(kBinT) bin_scatter_kernel(BSplinesView b1, BSplinesView b2, Rules2D rules, Bin2D g,
+                                                            const double* __restrict__ coords,
+                                                            const double* __restrict__ w, int64_t npts, int64_t bpb,
+                                                            int* __restrict__ cursor, int* __restrict__ perm,
+                                                            double* __restrict__ recs, StagedTables2D tabs)
+{
+    static_assert(kBinT * kBinPts <= 4096 && kMaxBins <= 4096, "tile and local index share one int");
+    extern __shared__ double lds_tabs[];
+    int* const hist = reinterpret_cast<int*>(lds_tabs + (tabs.bytes + 7) / 8);
+    int* const lofs = hist + g.nbins;
+    int* const base = lofs + g.nbins;
+    int* const slot = base + g.nbins;
+    __shared__ int part[kBinT];
+    int const tid = threadIdx.x;
     int64_t const l = blockIdx.x / bpb;
     int64_t const p0 = (blockIdx.x % bpb) * int64_t(kBinT * kBinPts);
-    for (int k = threadIdx.x; k < g.nbins; k += kBinT) hist[k] = 0;
+    for (int k = tid; k < g.nbins; k += kBinT) hist[k] = 0;
+    stage_tables_2d(tabs, b1, b2, lds_tabs, tid, kBinT);
     __syncthreads();
     int bin[kBinPts], rank[kBinPts];
 #pragma unroll
     for (int u = 0; u < kBinPts; ++u) {
-        int64_t const p = p0 + u * kBinT + threadIdx.x;
+        int64_t const p = p0 + u * kBinT + tid;
         bin[u] = p < npts ? bin_of(b1, b2, rules, g, coords, nullptr, l * npts + p, false) : -1;
         rank[u] = bin[u] >= 0 ? atomicAdd(&hist[bin[u]], 1) : 0;
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < g.nbins; k += kBinT)
+    // local exclusive offsets: thread t scans tiles [t*per, (t+1)*per), then
+    // the kBinT chunk sums are scanned
+    int const per = (g.nbins + kBinT - 1) / kBinT;
+    int const kb = tid * per, ke = min(kb + per, g.nbins);
+    int sum = 0;
+    for (int k = kb; k < ke; ++k) sum += hist[k];
+    part[tid] = sum;
+    __syncthreads();
+    for (int s = 1; s < kBinT; s *= 2) { // inclusive Hillis-Steele scan
+        int const v = tid >= s ? part[tid - s] : 0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    int run = part[tid] - sum;
+    for (int k = kb; k < ke; ++k) {
+        lofs[k] = run;
+        run += hist[k];
         if (hist[k]) base[k] = atomicAdd(&cursor[l * g.nbins + k], hist[k]);
+    }
+    int const total = part[kBinT - 1];
     __syncthreads();
 #pragma unroll
-    for (int u = 0; u < kBinPts; ++u) {
-        int64_t const p = p0 + u * kBinT + threadIdx.x;
-        if (bin[u] >= 0) perm[l * npts + base[bin[u]] + rank[u]] = int(p);
+    for (int u = 0; u < kBinPts; ++u)
+        if (bin[u] >= 0) slot[lofs[bin[u]] + rank[u]] = (bin[u] << 12) | (u * kBinT + tid);
+    __syncthreads();
+    for (int j = tid; j < total; j += kBinT) {
+        int const v = slot[j];
+        int const b = v >> 12;
+        int64_t const p = p0 + (v & 4095);
+        int64_t const k = l * npts + base[b] + (j - lofs[b]);
+        int64_t const t = l * npts + p;
+        perm[k] = int(p);
+        recs[3 * k] = coords[2 * t];
+        recs[3 * k + 1] = coords[2 * t + 1];
+        recs[3 * k + 2] = w[t];
     }
 }
 
 template <int D1, int D2>
 __global__ void __launch_bounds__(kTileThreads) bin_tile_kernel(BSplinesView b1, BSplinesView b2, Rules2D rules, Bin2D g,
-                                                                const double* __restrict__ coords,
-                                                                const double* __restrict__ w, double* __restrict__ gx,
-                                                                int64_t npts, const double* __restrict__ coef,
+                                                                const double* __restrict__ recs,
+                                                                double* __restrict__ gx, int64_t npts,
+                                                                const double* __restrict__ coef,
                                                                 double* __restrict__ gcoef, int64_t cbs, int64_t st1,
                                                                 int64_t st2, const int* __restrict__ starts,
-                                                                const int* __restrict__ perm)
+                                                                const int* __restrict__ perm, StagedTables2D tabs)
 {
     extern __shared__ double sh[];
     int const P1 = g.T + D1, P2 = g.T + D2;
@@ -189,28 +245,47 @@ __global__ void __launch_bounds__(kTileThreads) bin_tile_kernel(BSplinesView b1,
         cp[k] = in ? c[int64_t(r0 + a) * st1 + int64_t(c0 + b) * st2] : 0.0;
         gp[k] = 0.0;
     }
+    stage_tables_2d(tabs, b1, b2, sh + 2 * P1 * P2, threadIdx.x, kTileThreads);
     __syncthreads();
     int const k0 = starts[l * (g.nbins + 1) + bin], k1 = starts[l * (g.nbins + 1) + bin + 1];
+    // the next record is loaded while the current one is processed
+    double n1 = 0.0, n2 = 0.0, nw = 0.0;
+    int np = 0;
+    auto fetch = [&](int k) {
+        int64_t const q = l * npts + k;
+        n1 = recs[3 * q];
+        n2 = recs[3 * q + 1];
+        nw = recs[3 * q + 2];
+        np = perm[q];
+    };
+    if (k0 + int(threadIdx.x) < k1) fetch(k0 + int(threadIdx.x));
     for (int k = k0 + int(threadIdx.x); k < k1; k += kTileThreads) {
-        int64_t const t = l * npts + perm[l * npts + k];
-        double x1 = coords[2 * t], x2 = coords[2 * t + 1];
-        double const wt = w[t];
+        double x1 = n1, x2 = n2;
+        double const wt = nw;
+        int64_t const t = l * npts + np;
+        if (k + kTileThreads < k1) fetch(k + kTileThreads);
         bool in1, in2;
         place2d(b1, b2, rules, x1, x2, in1, in2); // true for every binned point
         double e1[(D1 + 1) * 2], e2[(D2 + 1) * 2];
         int const j1 = eval_basis_and_n_derivs<D1, 1, true>(b1, b1.knots, x1, e1) - r0;
         int const j2 = eval_basis_and_n_derivs<D2, 1, true>(b2, b2.knots, x2, e2) - c0;
+        // f1 = sum_i B1'_i sum_j c_ij B2_j, f2 = sum_i B1_i sum_j c_ij B2'_j
         double f1 = 0.0, f2 = 0.0;
 #pragma unroll
-        for (int i = 0; i < D1 + 1; ++i)
+        for (int i = 0; i < D1 + 1; ++i) {
+            double s0 = 0.0, s1 = 0.0;
+            double const we = wt * e1[i * 2];
 #pragma unroll
             for (int j = 0; j < D2 + 1; ++j) {
                 int const o = (j1 + i) * P2 + (j2 + j);
                 double const cij = cp[o];
-                f1 += cij * e1[i * 2 + 1] * e2[j * 2];
-                f2 += cij * e1[i * 2] * e2[j * 2 + 1];
-                atomicAdd(&gp[o], wt * e1[i * 2] * e2[j * 2]);
+                s0 = fma(cij, e2[j * 2], s0);
+                s1 = fma(cij, e2[j * 2 + 1], s1);
+                atomicAdd(&gp[o], we * e2[j * 2]);
             }
+            f1 = fma(e1[i * 2 + 1], s0, f1);
+            f2 = fma(e1[i * 2], s1, f2);
+        }
         gx[2 * t] = in1 ? wt * f1 : 0.0;
         gx[2 * t + 1] = in2 ? wt * f2 : 0.0;
     }
@@ -249,17 +324,19 @@ struct Work2D {
     int* starts;
     int* cursor;
     int* perm;
+    double* recs; // (x1, x2, w) per point in tile order
     int64_t bytes;
 };
 
 Work2D carve(Bin2D const& g, int64_t npts, int64_t nbatch, void* base)
 {
-    auto up = [](int64_t n) { return (n * int64_t(sizeof(int)) + 255) / 256 * 256; };
-    int64_t const a = up(nbatch * g.nbins), b = up(nbatch * (g.nbins + 1)), c = up(nbatch * g.nbins),
-                  d = up(nbatch * npts);
+    auto up = [](int64_t n, int64_t size) { return (n * size + 255) / 256 * 256; };
+    int64_t const a = up(nbatch * g.nbins, 4), b = up(nbatch * (g.nbins + 1), 4), c = up(nbatch * g.nbins, 4),
+                  d = up(nbatch * npts, 4), e = up(3 * nbatch * npts, 8);
     char* p = static_cast<char*>(base);
-    return Work2D{reinterpret_cast<int*>(p), reinterpret_cast<int*>(p + a), reinterpret_cast<int*>(p + a + b),
-                  reinterpret_cast<int*>(p + a + b + c), a + b + c + d};
+    return Work2D{reinterpret_cast<int*>(p),         reinterpret_cast<int*>(p + a),
+                  reinterpret_cast<int*>(p + a + b), reinterpret_cast<int*>(p + a + b + c),
+                  reinterpret_cast<double*>(p + a + b + c + d), a + b + c + d + e};
 }
 
 } // namespace
@@ -319,23 +396,29 @@ void launch_eval2d_vjp_binned(BSplinesPlan const& bp1, BSplinesPlan const& bp2, 
                        nbatch * g.nbins);
     if (npts > 0) {
         int64_t const bpb = (npts + kBinT * kBinPts - 1) / (kBinT * kBinPts);
-        hipLaunchKernelGGL(bin_count_kernel, dim3(blocks(bpb * nbatch, 1)), dim3(kBinT), 0, stream, b1, b2, rules, g,
-                           coords, gx, npts, bpb, ws.counts);
+        StagedTables2D const tabs = staged_tables_2d(b1, b2, kTabLds);
+        hipLaunchKernelGGL(bin_count_kernel, dim3(blocks(bpb * nbatch, 1)), dim3(kBinT), tabs.bytes, stream, b1, b2,
+                           rules, g, coords, gx, npts, bpb, ws.counts, tabs);
         hipLaunchKernelGGL(bin_scan_kernel, dim3(unsigned(nbatch)), dim3(1024), 0, stream, g, ws.counts, ws.starts,
                            ws.cursor);
-        hipLaunchKernelGGL(bin_scatter_kernel, dim3(blocks(bpb * nbatch, 1)), dim3(kBinT), 0, stream, b1, b2, rules, g,
-                           coords, npts, bpb, ws.cursor, ws.perm);
+        size_t const slds = scatter_lds(tabs, g);
+        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(bin_scatter_kernel),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(slds)),
+                  "hipFuncSetAttribute");
+        hipLaunchKernelGGL(bin_scatter_kernel, dim3(blocks(bpb * nbatch, 1)), dim3(kBinT), slds, stream, b1, b2, rules,
+                           g, coords, w, npts, bpb, ws.cursor, ws.perm, ws.recs, tabs);
         bool launched = false;
         dispatch_degree(b1.degree, [&]<int D1>() {
             dispatch_degree(b2.degree, [&]<int D2>() {
                 if constexpr (D1 <= 5 && D2 <= 5) {
-                    size_t const lds = size_t(2) * (g.T + D1) * (g.T + D2) * sizeof(double);
+                    size_t const lds = size_t(2) * (g.T + D1) * (g.T + D2) * sizeof(double) + tabs.bytes;
                     auto kern = bin_tile_kernel<D1, D2>;
                     hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)),
                               "hipFuncSetAttribute");
                     hipLaunchKernelGGL(kern, dim3(blocks(nbatch * g.nbins, 1)), dim3(kTileThreads), lds, stream, b1,
-                                       b2, rules, g, coords, w, gx, npts, coef, gcoef, cbs, st1, st2, ws.starts, ws.perm);
+                                       b2, rules, g, ws.recs, gx, npts, coef, gcoef, cbs, st1, st2, ws.starts, ws.perm,
+                                       tabs);
                     launched = true;
                 }
             });

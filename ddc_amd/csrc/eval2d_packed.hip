@@ -69,7 +69,7 @@ __global__ void __launch_bounds__(kEvalThreads) eval2d_packed_kernel(BSplinesVie
                                                                      double* __restrict__ out,
                                                                      const double* __restrict__ coords, int64_t npts,
                                                                      const double* __restrict__ packed, int nc1,
-                                                                     int nc2, int64_t nbatch, int stage1, int stage2)
+                                                                     int nc2, int64_t nbatch, StagedTables2D tabs)
 {
     constexpr int P = patch_len(D1, D2);
     constexpr int NPAIR = P / 2;
@@ -78,29 +78,9 @@ __global__ void __launch_bounds__(kEvalThreads) eval2d_packed_kernel(BSplinesVie
     __shared__ double sb[WAVES][64][NB];  // basis values of dimension 1, then 2
     __shared__ int64_t srec[WAVES][64];   // record index, -1: the Null rule gives 0
     __shared__ double so[WAVES][64];
-    extern __shared__ double tabs[];      // staged knots 1 | knots 2 | brackets 1 | brackets 2
-    int const nk1 = stage1 ? b1.ncells + 1 + 2 * D1 : 0, nk2 = stage2 ? b2.ncells + 1 + 2 * D2 : 0;
-    double* const kn1 = tabs;
-    double* const kn2 = tabs + nk1;
-    uint32_t* const cl1 = reinterpret_cast<uint32_t*>(tabs + nk1 + nk2);
-    uint32_t* const cl2 = cl1 + (stage1 ? b1.nbucket : 0);
-    if (stage1) {
-        for (int i = threadIdx.x; i < nk1; i += kEvalThreads) kn1[i] = b1.knots[i];
-        for (int i = threadIdx.x; i < b1.nbucket; i += kEvalThreads) cl1[i] = b1.cells[i];
-    }
-    if (stage2) {
-        for (int i = threadIdx.x; i < nk2; i += kEvalThreads) kn2[i] = b2.knots[i];
-        for (int i = threadIdx.x; i < b2.nbucket; i += kEvalThreads) cl2[i] = b2.cells[i];
-    }
+    extern __shared__ double lds_tabs[]; // staged knots and cell brackets
+    stage_tables_2d(tabs, b1, b2, lds_tabs, threadIdx.x, kEvalThreads);
     __syncthreads();
-    if (stage1) {
-        b1.knots = kn1;
-        b1.cells = b1.nbucket > 0 ? cl1 : nullptr;
-    }
-    if (stage2) {
-        b2.knots = kn2;
-        b2.cells = b2.nbucket > 0 ? cl2 : nullptr;
-    }
     int const wv = threadIdx.x / 64, lane = threadIdx.x % 64;
     int const sub = lane % kGroup;
     int64_t const tiles = (npts + kEvalThreads - 1) / kEvalThreads;
@@ -240,12 +220,8 @@ void launch_eval2d_packed(BSplinesPlan const& bp1, BSplinesPlan const& bp2, Rule
     if (chunk < 1) throw Error(DDCX_ERR_INVALID, "packed 2D evaluation: workspace smaller than one batch entry");
     double* packed = static_cast<double*>(work);
     // knots and brackets of the non-uniform dimensions go to LDS when they fit
-    auto tab_bytes = [](BSplinesView const& b) {
-        return b.uniform ? size_t(0) : size_t(b.ncells + 1 + 2 * b.degree) * sizeof(double) + size_t(b.nbucket) * 4;
-    };
-    size_t const t1 = tab_bytes(b1), t2 = tab_bytes(b2);
-    int const stage1 = !b1.uniform && t1 <= kTabLds, stage2 = !b2.uniform && t1 * stage1 + t2 <= kTabLds;
-    size_t const lds = (stage1 ? t1 : 0) + (stage2 ? t2 : 0);
+    StagedTables2D const tabs = staged_tables_2d(b1, b2, kTabLds);
+    size_t const lds = tabs.bytes;
     int64_t const tiles = (npts + kEvalThreads - 1) / kEvalThreads;
     bool const ok = dispatch_2d(b1.degree, b2.degree, o1, o2, [&]<int D1, int D2, int O1, int O2>() {
         auto kern = eval2d_packed_kernel<D1, D2, O1, O2>;
@@ -264,7 +240,7 @@ void launch_eval2d_packed(BSplinesPlan const& bp1, BSplinesPlan const& bp2, Rule
                                dim3(kPackThreads), 0, stream, coef + l0 * cbs, cbs, st1, st2, nc1, nc2, nb, packed);
             unsigned const grid = unsigned(std::min<int64_t>(tiles * nb, resident));
             hipLaunchKernelGGL(kern, dim3(grid), dim3(kEvalThreads), lds, stream, b1, b2, rules, out + l0 * npts,
-                               coords + 2 * l0 * npts, npts, packed, nc1, nc2, nb, stage1, stage2);
+                               coords + 2 * l0 * npts, npts, packed, nc1, nc2, nb, tabs);
         }
     });
     if (!ok) throw Error(DDCX_ERR_UNSUPPORTED, "2D evaluation supports degrees 1..5 and derivative orders <= 2");

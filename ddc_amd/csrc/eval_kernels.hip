@@ -199,17 +199,24 @@ __global__ void __launch_bounds__(256) eval2d_vjp_kernel(BSplinesView b1, BSplin
     int const j2 = eval_basis_and_n_derivs<D2, 1, true>(b2, b2.knots, x2, e2);
     const double* c = coef + l * cbs;
     double* gc = gcoef + l * cbs;
+    // f1 = sum_i B1'_i sum_j c_ij B2_j, f2 = sum_i B1_i sum_j c_ij B2'_j (the
+    // binned adjoint forms the same sums in the same order)
     double f1 = 0.0, f2 = 0.0;
 #pragma unroll
-    for (int i = 0; i < D1 + 1; ++i)
+    for (int i = 0; i < D1 + 1; ++i) {
+        double s0 = 0.0, s1 = 0.0;
+        double const we = wt * e1[i * 2];
 #pragma unroll
         for (int j = 0; j < D2 + 1; ++j) {
             int64_t const o = int64_t(j1 + i) * st1 + int64_t(j2 + j) * st2;
             double const cij = c[o];
-            f1 += cij * e1[i * 2 + 1] * e2[j * 2];
-            f2 += cij * e1[i * 2] * e2[j * 2 + 1];
-            atomicAdd(&gc[o], wt * e1[i * 2] * e2[j * 2]);
+            s0 = fma(cij, e2[j * 2], s0);
+            s1 = fma(cij, e2[j * 2 + 1], s1);
+            atomicAdd(&gc[o], we * e2[j * 2]);
         }
+        f1 = fma(e1[i * 2 + 1], s0, f1);
+        f2 = fma(e1[i * 2], s1, f2);
+    }
     gx[2 * t] = in1 ? wt * f1 : 0.0;
     gx[2 * t + 1] = in2 ? wt * f2 : 0.0;
 }
