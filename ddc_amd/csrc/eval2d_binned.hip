@@ -2,8 +2,9 @@
 // cell tile, instead of one global atomic per point and coefficient.
 //
 //   A  count:   each point's tile (T x T cells) -> per-tile counts (LDS
-//               histogram per workgroup, one global add per tile it saw);
-//               points that the Null rule sends to 0 get gx = 0 here
+//               histogram per workgroup, one global add per tile it saw),
+//               and the tile itself (16 bits per point, for C); points that
+//               the Null rule sends to 0 get gx = 0 here
 //   B  scan:    per batch entry, tile start offsets
 //   C  scatter: each workgroup reserves one range per tile it saw and writes
 //               its points there in tile order: index (perm) and the record
@@ -31,8 +32,10 @@ namespace {
 constexpr int kBinT = 256;       // threads of the count / scatter workgroups
 constexpr int kBinPts = 16;      // points per thread there (4096 per workgroup)
 constexpr int kMaxBins = 4096;   // tiles per batch entry
+constexpr int kNoBin = 0xffff;   // stored tile of a point the Null rule sends to 0
 constexpr int kTileThreads = 1024;
 constexpr size_t kTabLds = 48 * 1024; // LDS for staged knots and brackets
+constexpr size_t kLdsBytes = 160 * 1024;
 
 struct Bin2D {
     int T;        // tile side in cells
@@ -50,20 +53,26 @@ __device__ __forceinline__ int cell_of(BSplinesView const& b, double x)
     return nonuniform_cell(b, b.knots, x) - b.degree;
 }
 
+// tile of the point (x1, x2) as given (before wrap and rules), or -1 when the
+// Null rule sends it to 0
+__device__ __forceinline__ int bin_of_xy(BSplinesView const& b1, BSplinesView const& b2, Rules2D rules,
+                                         Bin2D const& g, double x1, double x2)
+{
+    bool in1, in2;
+    if (!place2d(b1, b2, rules, x1, x2, in1, in2)) return -1;
+    return (cell_of(b1, x1) / g.T) * g.nt2 + cell_of(b2, x2) / g.T;
+}
+
 // tile of point t, or -1 (Null rule: gx = 0 written here when `write_gx`)
 __device__ __forceinline__ int bin_of(BSplinesView const& b1, BSplinesView const& b2, Rules2D rules, Bin2D const& g,
                                       const double* coords, double* gx, int64_t t, bool write_gx)
 {
-    double x1 = coords[2 * t], x2 = coords[2 * t + 1];
-    bool in1, in2;
-    if (!place2d(b1, b2, rules, x1, x2, in1, in2)) {
-        if (write_gx) {
-            gx[2 * t] = 0.0;
-            gx[2 * t + 1] = 0.0;
-        }
-        return -1;
+    int const bin = bin_of_xy(b1, b2, rules, g, coords[2 * t], coords[2 * t + 1]);
+    if (bin < 0 && write_gx) {
+        gx[2 * t] = 0.0;
+        gx[2 * t + 1] = 0.0;
     }
-    return (cell_of(b1, x1) / g.T) * g.nt2 + cell_of(b2, x2) / g.T;
+    return bin;
 }
 
 __global__ void __launch_bounds__(256) fill_int_kernel(int* __restrict__ p, int64_t n)
@@ -85,7 +94,7 @@ __global__ void __launch_bounds__(256) zero_patch_kernel(double* __restrict__ g,
 __global__ void __launch_bounds__(kBinT) bin_count_kernel(BSplinesView b1, BSplinesView b2, Rules2D rules, Bin2D g,
                                                           const double* __restrict__ coords, double* __restrict__ gx,
                                                           int64_t npts, int64_t bpb, int* __restrict__ counts,
-                                                          StagedTables2D tabs)
+                                                          uint16_t* __restrict__ bins, StagedTables2D tabs)
 {
     __shared__ int hist[kMaxBins];
     extern __shared__ double lds_tabs[];
@@ -99,6 +108,7 @@ __global__ void __launch_bounds__(kBinT) bin_count_kernel(BSplinesView b1, BSpli
         int64_t const p = p0 + u * kBinT + threadIdx.x;
         if (p < npts) {
             int const bin = bin_of(b1, b2, rules, g, coords, gx, l * npts + p, true);
+            bins[l * npts + p] = uint16_t(bin < 0 ? kNoBin : bin);
             if (bin >= 0) atomicAdd(&hist[bin], 1);
         }
     }
@@ -144,53 +154,64 @@ __global__ void __launch_bounds__(1024) bin_scan_kernel(Bin2D g, const int* __re
     if (t == 1023) starts[l * (g.nbins + 1) + g.nbins] = part[1023];
 }
 
-// LDS of the scatter workgroup after the staged tables: the tile histogram,
-// local and global tile offsets (nbins ints each) and the workgroup's points
-// in tile order (kBinT * kBinPts ints)
-size_t scatter_lds(StagedTables2D const& tabs, Bin2D const& g)
+constexpr int kScatT = 1024; // threads of the scatter workgroups
+
+// LDS of a scatter workgroup of kScatT * PTS points: the points' records (x1, x2, w) in tile order, the tile histogram, local and
+// global tile offsets (nbins ints each), and per slot its tile and local index
+size_t scatter_lds(Bin2D const& g, int pts)
 {
-    return (tabs.bytes + 7) / 8 * 8 + (size_t(3) * g.nbins + size_t(kBinT) * kBinPts) * sizeof(int);
+    size_t const np = size_t(kScatT) * pts;
+    return np * 3 * sizeof(double) + size_t(3) * g.nbins * sizeof(int) + np * 2 * sizeof(uint16_t);
 }
 
-// The workgroup's points are first ordered by tile in LDS (slot j holds
-// tile << 12 | local index), so the records of one tile go out as one run of
-// consecutive lanes and addresses.
-__global__ void __launch_bounds__(kBinT) bin_scatter_kernel(BSplinesView b1, BSplinesView b2, Rules2D rules, Bin2D g,
-                                                            const double* __restrict__ coords,
-                                                            const double* __restrict__ w, int64_t npts, int64_t bpb,
-                                                            int* __restrict__ cursor, int* __restrict__ perm,
-                                                            double* __restrict__ recs, StagedTables2D tabs)
+// Each thread takes PTS points (read once, kept in registers) and their tiles
+// from the count pass, the workgroup orders the records by tile in LDS, and the records of one tile go
+// out as one run of consecutive lanes and addresses.
+template <int PTS>
+__global__ void __launch_bounds__(kScatT) bin_scatter_kernel(Bin2D g, const double* __restrict__ coords,
+                                                             const double* __restrict__ w, int64_t npts, int64_t bpb,
+                                                             int* __restrict__ cursor, int* __restrict__ perm,
+                                                             double* __restrict__ recs,
+                                                             const uint16_t* __restrict__ bins)
 {
-    static_assert(kBinT * kBinPts <= 4096 && kMaxBins <= 4096, "tile and local index share one int");
-    extern __shared__ double lds_tabs[];
-    int* const hist = reinterpret_cast<int*>(lds_tabs + (tabs.bytes + 7) / 8);
+    constexpr int NP = kScatT * PTS;
+    static_assert(NP <= 65536 && kMaxBins <= 65536, "slot tile and local index are 16-bit");
+    extern __shared__ double srec[];
+    int* const hist = reinterpret_cast<int*>(srec + 3 * NP);
     int* const lofs = hist + g.nbins;
     int* const base = lofs + g.nbins;
-    int* const slot = base + g.nbins;
-    __shared__ int part[kBinT];
+    uint16_t* const sbin = reinterpret_cast<uint16_t*>(base + g.nbins);
+    uint16_t* const sloc = sbin + NP;
+    __shared__ int part[kScatT];
     int const tid = threadIdx.x;
     int64_t const l = blockIdx.x / bpb;
-    int64_t const p0 = (blockIdx.x % bpb) * int64_t(kBinT * kBinPts);
-    for (int k = tid; k < g.nbins; k += kBinT) hist[k] = 0;
-    stage_tables_2d(tabs, b1, b2, lds_tabs, tid, kBinT);
-    __syncthreads();
-    int bin[kBinPts], rank[kBinPts];
+    int64_t const p0 = (blockIdx.x % bpb) * int64_t(NP);
+    for (int k = tid; k < g.nbins; k += kScatT) hist[k] = 0;
+    double x1[PTS], x2[PTS], wt[PTS];
+    int bin[PTS], rank[PTS];
 #pragma unroll
-    for (int u = 0; u < kBinPts; ++u) {
-        int64_t const p = p0 + u * kBinT + tid;
-        bin[u] = p < npts ? bin_of(b1, b2, rules, g, coords, nullptr, l * npts + p, false) : -1;
-        rank[u] = bin[u] >= 0 ? atomicAdd(&hist[bin[u]], 1) : 0;
+    for (int u = 0; u < PTS; ++u) {
+        int64_t const p = p0 + u * kScatT + tid;
+        int64_t const t = l * npts + (p < npts ? p : npts - 1);
+        x1[u] = coords[2 * t];
+        x2[u] = coords[2 * t + 1];
+        wt[u] = w[t];
+        int const b = bins[t];
+        bin[u] = p < npts && b != kNoBin ? b : -1;
     }
     __syncthreads();
+#pragma unroll
+    for (int u = 0; u < PTS; ++u) rank[u] = bin[u] >= 0 ? atomicAdd(&hist[bin[u]], 1) : 0;
+    __syncthreads();
     // local exclusive offsets: thread t scans tiles [t*per, (t+1)*per), then
-    // the kBinT chunk sums are scanned
-    int const per = (g.nbins + kBinT - 1) / kBinT;
+    // the kScatT chunk sums are scanned
+    int const per = (g.nbins + kScatT - 1) / kScatT;
     int const kb = tid * per, ke = min(kb + per, g.nbins);
     int sum = 0;
     for (int k = kb; k < ke; ++k) sum += hist[k];
     part[tid] = sum;
     __syncthreads();
-    for (int s = 1; s < kBinT; s *= 2) { // inclusive Hillis-Steele scan
+    for (int s = 1; s < kScatT; s *= 2) { // inclusive Hillis-Steele scan
         int const v = tid >= s ? part[tid - s] : 0;
         __syncthreads();
         part[tid] += v;
@@ -202,22 +223,26 @@ __global__ void __launch_bounds__(kBinT) bin_scatter_kernel(BSplinesView b1, BSp
         run += hist[k];
         if (hist[k]) base[k] = atomicAdd(&cursor[l * g.nbins + k], hist[k]);
     }
-    int const total = part[kBinT - 1];
+    int const total = part[kScatT - 1];
     __syncthreads();
 #pragma unroll
-    for (int u = 0; u < kBinPts; ++u)
-        if (bin[u] >= 0) slot[lofs[bin[u]] + rank[u]] = (bin[u] << 12) | (u * kBinT + tid);
+    for (int u = 0; u < PTS; ++u)
+        if (bin[u] >= 0) {
+            int const j = lofs[bin[u]] + rank[u];
+            srec[3 * j] = x1[u];
+            srec[3 * j + 1] = x2[u];
+            srec[3 * j + 2] = wt[u];
+            sbin[j] = uint16_t(bin[u]);
+            sloc[j] = uint16_t(u * kScatT + tid);
+        }
     __syncthreads();
-    for (int j = tid; j < total; j += kBinT) {
-        int const v = slot[j];
-        int const b = v >> 12;
-        int64_t const p = p0 + (v & 4095);
+    for (int j = tid; j < total; j += kScatT) {
+        int const b = sbin[j];
         int64_t const k = l * npts + base[b] + (j - lofs[b]);
-        int64_t const t = l * npts + p;
-        perm[k] = int(p);
-        recs[3 * k] = coords[2 * t];
-        recs[3 * k + 1] = coords[2 * t + 1];
-        recs[3 * k + 2] = w[t];
+        perm[k] = int(p0 + sloc[j]);
+        recs[3 * k] = srec[3 * j];
+        recs[3 * k + 1] = srec[3 * j + 1];
+        recs[3 * k + 2] = srec[3 * j + 2];
     }
 }
 
@@ -301,7 +326,8 @@ __global__ void __launch_bounds__(kTileThreads) bin_tile_kernel(BSplinesView b1,
 // Tiles of 64 x 64 cells (few tiles per batch entry, so the count and scatter
 // workgroups make few global additions), or 32 x 32 when 64 would leave fewer
 // than 2048 tile workgroups in the launch (or more tiles than the LDS
-// histograms hold).  Two (T + 5)^2 patches of doubles fit in LDS for T <= 64.
+// histograms hold).  Two (T + 5)^2 patches of doubles and the staged tables
+// fit in LDS for T <= 64.
 Bin2D bins_for(BSplinesPlan const& bp1, BSplinesPlan const& bp2, int64_t nbatch)
 {
     auto make = [&](int T) {
@@ -324,7 +350,8 @@ struct Work2D {
     int* starts;
     int* cursor;
     int* perm;
-    double* recs; // (x1, x2, w) per point in tile order
+    double* recs;   // (x1, x2, w) per point in tile order
+    uint16_t* bins; // tile of each point (kNoBin: Null rule)
     int64_t bytes;
 };
 
@@ -332,11 +359,15 @@ Work2D carve(Bin2D const& g, int64_t npts, int64_t nbatch, void* base)
 {
     auto up = [](int64_t n, int64_t size) { return (n * size + 255) / 256 * 256; };
     int64_t const a = up(nbatch * g.nbins, 4), b = up(nbatch * (g.nbins + 1), 4), c = up(nbatch * g.nbins, 4),
-                  d = up(nbatch * npts, 4), e = up(3 * nbatch * npts, 8);
+                  d = up(nbatch * npts, 4), e = up(3 * nbatch * npts, 8), f = up(nbatch * npts, 2);
     char* p = static_cast<char*>(base);
-    return Work2D{reinterpret_cast<int*>(p),         reinterpret_cast<int*>(p + a),
-                  reinterpret_cast<int*>(p + a + b), reinterpret_cast<int*>(p + a + b + c),
-                  reinterpret_cast<double*>(p + a + b + c + d), a + b + c + d + e};
+    return Work2D{reinterpret_cast<int*>(p),
+                  reinterpret_cast<int*>(p + a),
+                  reinterpret_cast<int*>(p + a + b),
+                  reinterpret_cast<int*>(p + a + b + c),
+                  reinterpret_cast<double*>(p + a + b + c + d),
+                  reinterpret_cast<uint16_t*>(p + a + b + c + d + e),
+                  a + b + c + d + e + f};
 }
 
 } // namespace
@@ -398,15 +429,25 @@ void launch_eval2d_vjp_binned(BSplinesPlan const& bp1, BSplinesPlan const& bp2, 
         int64_t const bpb = (npts + kBinT * kBinPts - 1) / (kBinT * kBinPts);
         StagedTables2D const tabs = staged_tables_2d(b1, b2, kTabLds);
         hipLaunchKernelGGL(bin_count_kernel, dim3(blocks(bpb * nbatch, 1)), dim3(kBinT), tabs.bytes, stream, b1, b2,
-                           rules, g, coords, gx, npts, bpb, ws.counts, tabs);
+                           rules, g, coords, gx, npts, bpb, ws.counts, ws.bins, tabs);
         hipLaunchKernelGGL(bin_scan_kernel, dim3(unsigned(nbatch)), dim3(1024), 0, stream, g, ws.counts, ws.starts,
                            ws.cursor);
-        size_t const slds = scatter_lds(tabs, g);
-        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(bin_scatter_kernel),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(slds)),
-                  "hipFuncSetAttribute");
-        hipLaunchKernelGGL(bin_scatter_kernel, dim3(blocks(bpb * nbatch, 1)), dim3(kBinT), slds, stream, b1, b2, rules,
-                           g, coords, w, npts, bpb, ws.cursor, ws.perm, ws.recs, tabs);
+        // the most points per scatter workgroup whose records fit in LDS
+        auto scatter = [&]<int PTS>() {
+            size_t const slds = scatter_lds(g, PTS);
+            if (slds + kScatT * sizeof(int) > kLdsBytes) return false;
+            auto kern = bin_scatter_kernel<PTS>;
+            hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, int(slds)),
+                      "hipFuncSetAttribute");
+            int64_t const sbpb = (npts + kScatT * PTS - 1) / (kScatT * PTS);
+            hipLaunchKernelGGL(kern, dim3(blocks(sbpb * nbatch, 1)), dim3(kScatT), slds, stream, g, coords, w, npts,
+                               sbpb, ws.cursor, ws.perm, ws.recs, ws.bins);
+            return true;
+        };
+        if (!scatter.template operator()<4>() && !scatter.template operator()<2>() &&
+            !scatter.template operator()<1>())
+            throw Error(DDCX_ERR_UNSUPPORTED, "binned 2D adjoint: scatter does not fit in LDS");
         bool launched = false;
         dispatch_degree(b1.degree, [&]<int D1>() {
             dispatch_degree(b2.degree, [&]<int D2>() {
