@@ -384,53 +384,48 @@ __host__ __device__ inline int eval_basis_and_n_derivs(BSplinesView const& b, co
 }
 
 // LDS copies of the knot vectors and cell brackets of the non-uniform
-// dimensions of a 2D kernel: which dimensions are staged (those that fit in
-// the byte budget, dimension 1 first) and the bytes they take.
-struct StagedTables2D {
-    int stage1, stage2;
-    size_t bytes;
-};
-
+// dimensions of a 2D kernel, laid out knots 1 | knots 2 | brackets 1 |
+// brackets 2 (a uniform dimension has none).
 inline size_t staged_table_bytes(BSplinesView const& b)
 {
     return b.uniform ? 0
                      : size_t(b.ncells + 1 + 2 * b.degree) * sizeof(double) + size_t(b.nbucket) * sizeof(uint32_t);
 }
 
-inline StagedTables2D staged_tables_2d(BSplinesView const& b1, BSplinesView const& b2, size_t budget)
+inline size_t staged_tables_bytes_2d(BSplinesView const& b1, BSplinesView const& b2)
 {
-    size_t const t1 = staged_table_bytes(b1), t2 = staged_table_bytes(b2);
-    StagedTables2D s{};
-    s.stage1 = !b1.uniform && t1 <= budget;
-    s.stage2 = !b2.uniform && (s.stage1 ? t1 : 0) + t2 <= budget;
-    s.bytes = (s.stage1 ? t1 : 0) + (s.stage2 ? t2 : 0);
-    return s;
+    return staged_table_bytes(b1) + staged_table_bytes(b2);
+}
+
+// whether the tables of both dimensions can be staged within `budget` bytes
+// (a non-uniform dimension needs its cell brackets)
+inline bool staged_tables_fit_2d(BSplinesView const& b1, BSplinesView const& b2, size_t budget)
+{
+    return (b1.uniform || b1.nbucket > 0) && (b2.uniform || b2.nbucket > 0) &&
+           staged_tables_bytes_2d(b1, b2) <= budget;
 }
 
 #ifdef __HIPCC__
-// The workgroup's threads copy the staged tables to `lds` (8-byte aligned,
-// s.bytes long) and point b1 / b2 at the copies; the caller synchronises the
-// workgroup before the first use.
-__device__ inline void stage_tables_2d(StagedTables2D const& s, BSplinesView& b1, BSplinesView& b2, double* lds,
-                                       int tid, int nthreads)
+// The workgroup's threads copy the tables to `lds` (8-byte aligned,
+// staged_tables_bytes_2d long) and point b1 / b2 at the copies, for either
+// basis kind, so that every knot and bracket read of the kernel is an LDS
+// read; the caller synchronises the workgroup before the first use.
+__device__ inline void stage_tables_2d(BSplinesView& b1, BSplinesView& b2, double* lds, int tid, int nthreads)
 {
-    int const nk1 = s.stage1 ? b1.ncells + 1 + 2 * b1.degree : 0, nk2 = s.stage2 ? b2.ncells + 1 + 2 * b2.degree : 0;
+    int const nk1 = b1.uniform ? 0 : b1.ncells + 1 + 2 * b1.degree, nk2 = b2.uniform ? 0 : b2.ncells + 1 + 2 * b2.degree;
+    int const nc1 = b1.uniform ? 0 : b1.nbucket, nc2 = b2.uniform ? 0 : b2.nbucket;
     double* const kn1 = lds;
     double* const kn2 = lds + nk1;
     uint32_t* const cl1 = reinterpret_cast<uint32_t*>(lds + nk1 + nk2);
-    uint32_t* const cl2 = cl1 + (s.stage1 ? b1.nbucket : 0);
-    if (s.stage1) {
-        for (int i = tid; i < nk1; i += nthreads) kn1[i] = b1.knots[i];
-        for (int i = tid; i < b1.nbucket; i += nthreads) cl1[i] = b1.cells[i];
-        b1.knots = kn1;
-        b1.cells = b1.nbucket > 0 ? cl1 : nullptr;
-    }
-    if (s.stage2) {
-        for (int i = tid; i < nk2; i += nthreads) kn2[i] = b2.knots[i];
-        for (int i = tid; i < b2.nbucket; i += nthreads) cl2[i] = b2.cells[i];
-        b2.knots = kn2;
-        b2.cells = b2.nbucket > 0 ? cl2 : nullptr;
-    }
+    uint32_t* const cl2 = cl1 + nc1;
+    for (int i = tid; i < nk1; i += nthreads) kn1[i] = b1.knots[i];
+    for (int i = tid; i < nc1; i += nthreads) cl1[i] = b1.cells[i];
+    for (int i = tid; i < nk2; i += nthreads) kn2[i] = b2.knots[i];
+    for (int i = tid; i < nc2; i += nthreads) cl2[i] = b2.cells[i];
+    b1.knots = kn1;
+    b1.cells = cl1;
+    b2.knots = kn2;
+    b2.cells = cl2;
 }
 #endif
 

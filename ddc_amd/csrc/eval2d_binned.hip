@@ -18,7 +18,8 @@
 //               per nonzero entry (neighbouring patches overlap by d rows)
 //
 // The knots and cell brackets of non-uniform dimensions are staged in LDS by
-// every kernel that locates points.
+// every kernel that locates points (grids whose tables do not fit take the
+// one-atomic-per-term kernel instead: the workspace query returns 0).
 // The point semantics (periodic wrap, Null and Constant rules, zero
 // coordinate gradient along a clamped dimension) are those of
 // eval2d_vjp_kernel.  The order of the additions inside a patch depends on
@@ -94,14 +95,14 @@ __global__ void __launch_bounds__(256) zero_patch_kernel(double* __restrict__ g,
 __global__ void __launch_bounds__(kBinT) bin_count_kernel(BSplinesView b1, BSplinesView b2, Rules2D rules, Bin2D g,
                                                           const double* __restrict__ coords, double* __restrict__ gx,
                                                           int64_t npts, int64_t bpb, int* __restrict__ counts,
-                                                          uint16_t* __restrict__ bins, StagedTables2D tabs)
+                                                          uint16_t* __restrict__ bins)
 {
     __shared__ int hist[kMaxBins];
     extern __shared__ double lds_tabs[];
     int64_t const l = blockIdx.x / bpb;
     int64_t const p0 = (blockIdx.x % bpb) * int64_t(kBinT * kBinPts);
     for (int k = threadIdx.x; k < g.nbins; k += kBinT) hist[k] = 0;
-    stage_tables_2d(tabs, b1, b2, lds_tabs, threadIdx.x, kBinT);
+    stage_tables_2d(b1, b2, lds_tabs, threadIdx.x, kBinT);
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < kBinPts; ++u) {
@@ -253,7 +254,7 @@ __global__ void __launch_bounds__(kTileThreads) bin_tile_kernel(BSplinesView b1,
                                                                 const double* __restrict__ coef,
                                                                 double* __restrict__ gcoef, int64_t cbs, int64_t st1,
                                                                 int64_t st2, const int* __restrict__ starts,
-                                                                const int* __restrict__ perm, StagedTables2D tabs)
+                                                                const int* __restrict__ perm, int stage)
 {
     extern __shared__ double sh[];
     int const P1 = g.T + D1, P2 = g.T + D2;
@@ -270,7 +271,11 @@ __global__ void __launch_bounds__(kTileThreads) bin_tile_kernel(BSplinesView b1,
         cp[k] = in ? c[int64_t(r0 + a) * st1 + int64_t(c0 + b) * st2] : 0.0;
         gp[k] = 0.0;
     }
-    stage_tables_2d(tabs, b1, b2, sh + 2 * P1 * P2, threadIdx.x, kTileThreads);
+    // The knots and brackets are staged in LDS behind a run-time flag (set by
+    // the launcher), so their reads compile to generic (flat) loads: measured
+    // faster in this kernel than LDS instructions, which share the LDS pipe
+    // with the 32 patch accesses per point, and than global loads.
+    if (stage) stage_tables_2d(b1, b2, sh + 2 * P1 * P2, threadIdx.x, kTileThreads);
     __syncthreads();
     int const k0 = starts[l * (g.nbins + 1) + bin], k1 = starts[l * (g.nbins + 1) + bin + 1];
     // the next record is loaded while the current one is processed
@@ -398,7 +403,9 @@ void* static_scratch()
 int64_t eval2d_vjp_workspace(BSplinesPlan const& bp1, BSplinesPlan const& bp2, int64_t npts, int64_t nbatch)
 {
     Bin2D const g = bins_for(bp1, bp2, nbatch);
-    if (g.nbins == 0 || npts >= (int64_t(1) << 31) || bp1.view.degree > 5 || bp2.view.degree > 5) return 0;
+    if (g.nbins == 0 || npts >= (int64_t(1) << 31) || bp1.view.degree > 5 || bp2.view.degree > 5 ||
+        !staged_tables_fit_2d(bp1.device_view(), bp2.device_view(), kTabLds))
+        return 0;
     return carve(g, npts, nbatch, nullptr).bytes;
 }
 
@@ -427,9 +434,9 @@ void launch_eval2d_vjp_binned(BSplinesPlan const& bp1, BSplinesPlan const& bp2, 
                        nbatch * g.nbins);
     if (npts > 0) {
         int64_t const bpb = (npts + kBinT * kBinPts - 1) / (kBinT * kBinPts);
-        StagedTables2D const tabs = staged_tables_2d(b1, b2, kTabLds);
-        hipLaunchKernelGGL(bin_count_kernel, dim3(blocks(bpb * nbatch, 1)), dim3(kBinT), tabs.bytes, stream, b1, b2,
-                           rules, g, coords, gx, npts, bpb, ws.counts, ws.bins, tabs);
+        size_t const tabs = staged_tables_bytes_2d(b1, b2);
+        hipLaunchKernelGGL(bin_count_kernel, dim3(blocks(bpb * nbatch, 1)), dim3(kBinT), tabs, stream, b1, b2, rules,
+                           g, coords, gx, npts, bpb, ws.counts, ws.bins);
         hipLaunchKernelGGL(bin_scan_kernel, dim3(unsigned(nbatch)), dim3(1024), 0, stream, g, ws.counts, ws.starts,
                            ws.cursor);
         // the most points per scatter workgroup whose records fit in LDS
@@ -452,14 +459,14 @@ void launch_eval2d_vjp_binned(BSplinesPlan const& bp1, BSplinesPlan const& bp2, 
         dispatch_degree(b1.degree, [&]<int D1>() {
             dispatch_degree(b2.degree, [&]<int D2>() {
                 if constexpr (D1 <= 5 && D2 <= 5) {
-                    size_t const lds = size_t(2) * (g.T + D1) * (g.T + D2) * sizeof(double) + tabs.bytes;
+                    size_t const lds = size_t(2) * (g.T + D1) * (g.T + D2) * sizeof(double) + tabs;
                     auto kern = bin_tile_kernel<D1, D2>;
                     hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)),
                               "hipFuncSetAttribute");
                     hipLaunchKernelGGL(kern, dim3(blocks(nbatch * g.nbins, 1)), dim3(kTileThreads), lds, stream, b1,
                                        b2, rules, g, ws.recs, gx, npts, coef, gcoef, cbs, st1, st2, ws.starts, ws.perm,
-                                       tabs);
+                                       1);
                     launched = true;
                 }
             });

@@ -61,15 +61,14 @@ __global__ void __launch_bounds__(kPackThreads) pack2d_kernel(const double* __re
 }
 
 // Persistent: workgroup g stages the knot vector and bucket brackets of each
-// non-uniform dimension in LDS (when they fit, else they are read from
-// global memory), then takes tiles g, g + G, ... of kEvalThreads points, tile
+// non-uniform dimension in LDS, then takes tiles g, g + G, ... of kEvalThreads points, tile
 // t being points (t % tiles) * kEvalThreads ... of batch entry t / tiles.
 template <int D1, int D2, int O1, int O2>
 __global__ void __launch_bounds__(kEvalThreads) eval2d_packed_kernel(BSplinesView b1, BSplinesView b2, Rules2D rules,
                                                                      double* __restrict__ out,
                                                                      const double* __restrict__ coords, int64_t npts,
                                                                      const double* __restrict__ packed, int nc1,
-                                                                     int nc2, int64_t nbatch, StagedTables2D tabs)
+                                                                     int nc2, int64_t nbatch)
 {
     constexpr int P = patch_len(D1, D2);
     constexpr int NPAIR = P / 2;
@@ -79,7 +78,7 @@ __global__ void __launch_bounds__(kEvalThreads) eval2d_packed_kernel(BSplinesVie
     __shared__ int64_t srec[WAVES][64];   // record index, -1: the Null rule gives 0
     __shared__ double so[WAVES][64];
     extern __shared__ double lds_tabs[]; // staged knots and cell brackets
-    stage_tables_2d(tabs, b1, b2, lds_tabs, threadIdx.x, kEvalThreads);
+    stage_tables_2d(b1, b2, lds_tabs, threadIdx.x, kEvalThreads);
     __syncthreads();
     int const wv = threadIdx.x / 64, lane = threadIdx.x % 64;
     int const sub = lane % kGroup;
@@ -219,9 +218,13 @@ void launch_eval2d_packed(BSplinesPlan const& bp1, BSplinesPlan const& bp2, Rule
     int64_t const chunk = work_bytes / rec_bytes; // batch entries packed at a time
     if (chunk < 1) throw Error(DDCX_ERR_INVALID, "packed 2D evaluation: workspace smaller than one batch entry");
     double* packed = static_cast<double*>(work);
-    // knots and brackets of the non-uniform dimensions go to LDS when they fit
-    StagedTables2D const tabs = staged_tables_2d(b1, b2, kTabLds);
-    size_t const lds = tabs.bytes;
+    // the knots and brackets of the non-uniform dimensions go to LDS; grids
+    // whose tables do not fit take the direct kernel
+    if (!staged_tables_fit_2d(b1, b2, kTabLds)) {
+        launch_eval2d(bp1, bp2, rules, o1, o2, out, coords, npts, coef, cbs, st1, st2, nbatch, stream);
+        return;
+    }
+    size_t const lds = staged_tables_bytes_2d(b1, b2);
     int64_t const tiles = (npts + kEvalThreads - 1) / kEvalThreads;
     bool const ok = dispatch_2d(b1.degree, b2.degree, o1, o2, [&]<int D1, int D2, int O1, int O2>() {
         auto kern = eval2d_packed_kernel<D1, D2, O1, O2>;
@@ -240,7 +243,7 @@ void launch_eval2d_packed(BSplinesPlan const& bp1, BSplinesPlan const& bp2, Rule
                                dim3(kPackThreads), 0, stream, coef + l0 * cbs, cbs, st1, st2, nc1, nc2, nb, packed);
             unsigned const grid = unsigned(std::min<int64_t>(tiles * nb, resident));
             hipLaunchKernelGGL(kern, dim3(grid), dim3(kEvalThreads), lds, stream, b1, b2, rules, out + l0 * npts,
-                               coords + 2 * l0 * npts, npts, packed, nc1, nc2, nb, tabs);
+                               coords + 2 * l0 * npts, npts, packed, nc1, nc2, nb);
         }
     });
     if (!ok) throw Error(DDCX_ERR_UNSUPPORTED, "2D evaluation supports degrees 1..5 and derivative orders <= 2");
