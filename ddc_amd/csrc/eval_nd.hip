@@ -315,15 +315,21 @@ __global__ void __launch_bounds__(kIntThreads) integratend_kernel(NdArgs a, doub
     if (threadIdx.x == 0) dst[l * dst_ls + s] = part[0];
 }
 
-// out[l*ostride] = sum of the S partial sums of entry l, in index order
-__global__ void integratend_sum_kernel(const double* __restrict__ part, int S, double* __restrict__ out,
-                                       int64_t ostride, int64_t nbatch)
+// out[l*ostride] = sum of the S partial sums of entry l: one wave per entry,
+// lane i adds partials i, i + 64, ... in order, then a fixed butterfly
+// combines the lanes, so the order of the additions is a function of S only
+__global__ void __launch_bounds__(256) integratend_sum_kernel(const double* __restrict__ part, int S,
+                                                              double* __restrict__ out, int64_t ostride,
+                                                              int64_t nbatch)
 {
-    int64_t const l = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (l >= nbatch) return;
+    int64_t const l = int64_t(blockIdx.x) * 4 + threadIdx.x / 64;
+    int const lane = threadIdx.x % 64;
+    if (l >= nbatch) return; // wave-uniform
     double y = 0.0;
-    for (int s = 0; s < S; ++s) y += part[l * S + s];
-    out[l * ostride] = y;
+    for (int s = lane; s < S; s += 64) y += part[l * S + s];
+#pragma unroll
+    for (int m = 32; m >= 1; m /= 2) y += __shfl_xor(y, m, 64);
+    if (lane == 0) out[l * ostride] = y;
 }
 
 NdArgs make_args(BSplinesPlan const* const* bs, int nd, const int* orders, const int* lower, const int* upper,
@@ -445,7 +451,7 @@ void launch_integratend(BSplinesPlan const* const* bs, int nd, double* out, int6
     });
     hip_check(hipGetLastError(), "integratend_kernel launch");
     if (S > 1) {
-        hipLaunchKernelGGL(integratend_sum_kernel, dim3(unsigned((nbatch + 255) / 256)), dim3(256), 0, stream, part, S,
+        hipLaunchKernelGGL(integratend_sum_kernel, dim3(unsigned((nbatch + 3) / 4)), dim3(256), 0, stream, part, S,
                            out, ostride, nbatch);
         hip_check(hipGetLastError(), "integratend_sum_kernel launch");
     }

@@ -286,7 +286,7 @@ ddcx_status ddcx_evaluator2d_eval_vjp_binned(const ddcx_evaluator* e1, const ddc
  * reference counterpart; for callers that cannot allocate device memory
  * (code under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc).  The workspace is a static device array of the library,
- * DDCX_STATIC_SCRATCH_BYTES per GPU (about 4 B per point: up to ~16M points
+ * DDCX_STATIC_SCRATCH_BYTES per GPU (about 30 B per point: up to ~2.2M points
  * per call), held by every GPU the library runs on until the process ends;
  * DDCX_ERR_INVALID when the call needs more.  Nothing is allocated.  All
  * calls on one GPU share that array: calls that overlap on one GPU (other

@@ -454,6 +454,25 @@ public:
         check(ddcx_evaluator2d_eval(e1_->handle(), e2_->handle(), order1, order2, out, coords, npts, coef,
                                     coef_batch_stride, st1, st2, nbatch, stream));
     }
+    // bytes of device workspace the packed evaluation needs for nbatch
+    // entries (it takes any amount from one entry's worth up; less than
+    // nbatch entries' worth packs and evaluates a workspace-full at a time)
+    int64_t eval_workspace(int64_t nbatch) const
+    {
+        int64_t bytes = 0;
+        check(ddcx_evaluator2d_eval_workspace(e1_->handle(), e2_->handle(), nbatch, &bytes));
+        return bytes;
+    }
+    // deriv (or operator() with orders 0, 0) through packed per-cell
+    // coefficient records in `work` (16-byte aligned): for many points per
+    // batch entry; the same values to rounding
+    void deriv(int order1, int order2, double* out, const double* coords, int64_t npts, const double* coef,
+               int64_t coef_batch_stride, int64_t st1, int64_t st2, int64_t nbatch, void* work, int64_t work_bytes,
+               void* stream) const
+    {
+        check(ddcx_evaluator2d_eval_packed(e1_->handle(), e2_->handle(), order1, order2, out, coords, npts, coef,
+                                           coef_batch_stride, st1, st2, nbatch, work, work_bytes, stream));
+    }
     void deriv_dim_1(double* out, const double* coords, int64_t npts, const double* coef, int64_t cbs, int64_t st1,
                      int64_t st2, int64_t nbatch, void* stream = nullptr) const
     {

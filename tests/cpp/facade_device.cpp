@@ -139,7 +139,7 @@ void case_rules()
 
 // 2D: non-uniform cubic Greville (20 cells) x uniform periodic cubic (16
 // cells), 2 batch entries: SplineBuilder2D, SplineEvaluator2D values,
-// deriv_dim_1 and the adjoint
+// deriv_dim_1, the packed evaluation and the adjoint
 void case_2d()
 {
     int const nc1 = 20, nc2 = 16, NB = 2, P = 500;
@@ -170,6 +170,9 @@ void case_2d()
     int64_t const cbs = int64_t(s1) * s2;
     ev(dout.p, dx.p, P, dc.p, cbs, s2, 1, NB);
     ev.deriv_dim_1(dd1.p, dx.p, P, dc.p, cbs, s2, 1, NB);
+    // the packed form with a workspace of one batch entry (entries packed one at a time)
+    DeviceArray dpk(size_t(NB) * P), dpw(size_t(ev.eval_workspace(1) + 7) / 8);
+    ev.deriv(0, 0, dpk.p, dx.p, P, dc.p, cbs, s2, 1, NB, dpw.p, ev.eval_workspace(1), nullptr);
     int64_t const wb = ev.eval_vjp_workspace(P, NB);
     DeviceArray dwork(size_t(std::max<int64_t>(wb, 8) + 7) / 8);
     ev.eval_vjp(dx.p, dw.p, dgx.p, P, dc.p, dgc.p, cbs, s2, 1, NB, wb > 0 ? dwork.p : nullptr, wb);
@@ -180,6 +183,7 @@ void case_2d()
     dump("2d_coef", dc.get());
     dump("2d_eval", dout.get());
     dump("2d_deriv1", dd1.get());
+    dump("2d_eval_packed", dpk.get());
     dump("2d_gx", dgx.get());
     dump("2d_gcoef", dgc.get());
 }

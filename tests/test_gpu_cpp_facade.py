@@ -68,6 +68,7 @@ def test_facade_2d(outputs):
     assert rel_err(outputs("2d_coef").reshape(NB, s1, s2), coef) <= 1e-13
     assert rel_err(outputs("2d_eval").reshape(NB, P), O.spline2d_eval(o1, o2, coords, coef)) <= 1e-13
     assert rel_err(outputs("2d_deriv1").reshape(NB, P), O.spline2d_eval(o1, o2, coords, coef, 1, 0)) <= 1e-13
+    assert rel_err(outputs("2d_eval_packed").reshape(NB, P), O.spline2d_eval(o1, o2, coords, coef)) <= 1e-13
     # adjoint: gcoef against the Jacobian of the oracle's evaluation, gx
     # against w times the partials at the wrapped coordinates
     gc = outputs("2d_gcoef").reshape(NB, s1, s2)
