@@ -127,9 +127,22 @@ __global__ void __launch_bounds__(LB* K) build_refine_kernel(RefineView rv, Buil
     __syncthreads();
     int const e0 = lexp[ln] == kNoExp ? 0 : lexp[ln];
 
-    double x[L];
+    // With one refinement step the solution after the first solve is
+    // d0 / sc0: an FP32 value times a power of two, held as such (16 VGPRs
+    // instead of 32); the FP64 additions are the same, so are the results.
+    constexpr bool XF = STEPS == 1;
+    double x[XF ? 1 : L];
+    float x0[XF ? L : 1];
+    double isc0 = 0.0, isc_last = 0.0;
+    if constexpr (!XF) {
 #pragma unroll
-    for (int i = 0; i < L; ++i) x[i] = 0.0;
+        for (int i = 0; i < L; ++i) x[i] = 0.0;
+    }
+    auto xval = [&](int i) -> double {
+        if constexpr (XF) return double(x0[i]) * isc0;
+        else return x[i];
+    };
+    float d[L];
 
     // a compile-time step count lets the steps unroll, which keeps the
     // register allocation of the loop body from spilling
@@ -138,7 +151,7 @@ __global__ void __launch_bounds__(LB* K) build_refine_kernel(RefineView rv, Buil
         // ---------------- scaled residual sc (b - A x) ----------------
         int const ex = min(max(it * rv.shift - e0, -1000), 1000);
         double const sc = ldexp(1.0, ex), isc = ldexp(1.0, -ex);
-        float d[L];
+        isc_last = isc;
         if (it == 0) {
 #pragma unroll
             for (int i = 0; i < L; ++i) d[i] = float(bq[i] * sc);
@@ -149,8 +162,8 @@ __global__ void __launch_bounds__(LB* K) build_refine_kernel(RefineView rv, Buil
         } else {
 #pragma unroll
             for (int k = 0; k < KQ; ++k) {
-                xch[k][seg][ln] = x[k];
-                xch[KQ + k][seg][ln] = x[L - KQ + k];
+                xch[k][seg][ln] = xval(k);
+                xch[KQ + k][seg][ln] = xval(L - KQ + k);
             }
             for (int z = 0; z < rv.bl_nnz; ++z) {
                 int const c = rv.bl_c[z];
@@ -158,7 +171,7 @@ __global__ void __launch_bounds__(LB* K) build_refine_kernel(RefineView rv, Buil
                     double v = 0.0;
 #pragma unroll
                     for (int i = 0; i < L; ++i)
-                        if (r0 + i == c) v = x[i];
+                        if (r0 + i == c) v = xval(i);
                     lamx[z][ln] = v;
                 }
             }
@@ -174,7 +187,7 @@ __global__ void __launch_bounds__(LB* K) build_refine_kernel(RefineView rv, Buil
                     double v;
                     if (c < 0) v = seg > 0 ? xch[KQ + KQ + c][seg - 1][ln] : 0.0;
                     else if (c >= L) v = seg + 1 < K ? xch[c - L][seg + 1][ln] : 0.0;
-                    else v = x[c];
+                    else v = xval(c);
                     t = fma(-Q[j], v, t);
                 }
                 for (int z = g0; z < g1; ++z)
@@ -366,9 +379,22 @@ __global__ void __launch_bounds__(LB* K) build_refine_kernel(RefineView rv, Buil
                 d[i] = t;
             }
         }
+        if constexpr (XF) {
+            if (it == 0) {
 #pragma unroll
-        for (int i = 0; i < L; ++i) x[i] = x[i] + double(d[i]) * isc;
+                for (int i = 0; i < L; ++i) x0[i] = d[i];
+                isc0 = isc;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < L; ++i) x[i] = x[i] + double(d[i]) * isc;
+        }
     }
+    // the solution: x + d / sc of the last step
+    auto xout = [&](int i) -> double {
+        if constexpr (XF) return double(x0[i]) * isc0 + double(d[i]) * isc_last;
+        else return x[i];
+    };
 
     // ---------------- store ----------------
     if (!active) return;
@@ -377,11 +403,11 @@ __global__ void __launch_bounds__(LB* K) build_refine_kernel(RefineView rv, Buil
     uint32_t const c0 = clo + uint32_t(rv.top + r0 + off) * crs;
     if (r0 + L <= nq) {
 #pragma unroll
-        for (int i = 0; i < L; ++i) __builtin_nontemporal_store(x[i], &cwg[c0 + uint32_t(i) * crs]);
+        for (int i = 0; i < L; ++i) __builtin_nontemporal_store(xout(i), &cwg[c0 + uint32_t(i) * crs]);
     } else {
 #pragma unroll
         for (int i = 0; i < L; ++i)
-            if (r0 + i < nq) __builtin_nontemporal_store(x[i], &cwg[c0 + uint32_t(i) * crs]);
+            if (r0 + i < nq) __builtin_nontemporal_store(xout(i), &cwg[c0 + uint32_t(i) * crs]);
     }
     if (bv.periodic && (r0 < bv.degree || r0 + L > nbas - off)) {
 #pragma unroll
@@ -389,9 +415,9 @@ __global__ void __launch_bounds__(LB* K) build_refine_kernel(RefineView rv, Buil
             int const prow = rv.top + r0 + i;
             if (prow < nq) {
                 int const j0 = prow + off - nbas;
-                if (j0 >= 0) cwg[clo + uint32_t(j0) * crs] = x[i];
+                if (j0 >= 0) cwg[clo + uint32_t(j0) * crs] = xout(i);
                 int const j2 = prow + off + nbas;
-                if (j2 < nbas + bv.degree) cwg[clo + uint32_t(j2) * crs] = x[i];
+                if (j2 < nbas + bv.degree) cwg[clo + uint32_t(j2) * crs] = xout(i);
             }
         }
     }
