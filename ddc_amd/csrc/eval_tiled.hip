@@ -429,8 +429,12 @@ constexpr size_t kMaxLds = 160 * 1024;
 
 // eval_stream_kernel for the value form when its shape fits: plain
 // batch-contiguous points and coefficients (element (row, line) at row * rs +
-// line), a uniform basis, npts <= 1024, size() <= 1056, buffers below 4 GiB
-// including the offsets of the point slots past the end of a line.
+// line), npts <= NST * U * ROWS, size() <= PR * ROWS, buffers below 4 GiB
+// including the offsets of the point slots past the end of a line.  Uniform
+// bases: 512 threads, 8 points per thread and stage, 33 prefetch rows; non-
+// uniform bases (whose cell search wants more waves to hide its dependent
+// LDS reads): 768 threads, 4 points per thread and stage, 22 prefetch rows,
+// degrees up to 3.
 template <int D>
 bool launch_stream(BSplinesPlan const& bs, Rules1D rules, double* out, const double* coords, Layout pl,
                    int64_t npts, const double* coef, Layout cl, int64_t nbatch, hipStream_t stream)
@@ -438,29 +442,28 @@ bool launch_stream(BSplinesPlan const& bs, Rules1D rules, double* out, const dou
     int const ncoef = bs.size();
     int const nknots = bs.view.uniform ? 0 : int(bs.knots_host.size());
     BSplinesView const b = bs.device_view();
-    constexpr int LB = 16, NT = 512, ROWS = NT / LB, U = 8, pitch = 17;
+    constexpr int LB = 16, pitch = 17;
     auto plain = [&](Layout const& L) { return L.is == 1 && L.inner >= nbatch && L.rs >= nbatch; };
     if (!plain(pl) || !plain(cl)) return false;
-    int const nst = npts <= 2 * U * ROWS ? 2 : 4;
-    // the instantiations below hold their prefetch rows and 8-point stages in
-    // registers without spilling.  Non-uniform bases keep eval_tiled_kernel
-    // (their cell search runs better with 16 waves per CU), and so do batches
-    // of fewer than two tiles per CU, where nothing is left to prefetch.
-    if (npts > 4 * U * ROWS || ncoef > 33 * ROWS) return false;
-    if (!b.uniform) return false;
+    // batches of fewer than two tiles per CU keep eval_tiled_kernel: nothing
+    // is left to prefetch
     int64_t const ntiles = (nbatch + LB - 1) / LB;
     if (ntiles < 2 * int64_t(cu_count())) return false;
-    uint64_t const lim = uint64_t(1) << 32;
-    // byte offsets up to the last slot of the last stage must fit 32 bits
-    if ((uint64_t(nst) * U * ROWS * uint64_t(pl.rs) + uint64_t(nbatch)) * 8 >= lim) return false;
-    if ((uint64_t(ncoef) * uint64_t(cl.rs) + uint64_t(nbatch)) * 8 >= lim) return false;
-    uint32_t const pbytes = uint32_t((uint64_t(npts - 1) * uint64_t(pl.rs) + uint64_t(nbatch)) * 8);
-    uint32_t const cbytes = uint32_t((uint64_t(ncoef - 1) * uint64_t(cl.rs) + uint64_t(nbatch)) * 8);
     size_t const lds = size_t(ncoef) * pitch * sizeof(double) + nknots * sizeof(double) +
                        size_t(b.nbucket) * sizeof(uint32_t);
     if (lds > kMaxLds) return false;
     unsigned const grid = unsigned(std::min<int64_t>(ntiles, cu_count()));
-    auto go = [&]<int KIND, int NST, int PR>() {
+    // the instantiations below hold their prefetch rows and point stages in
+    // registers without spilling
+    auto go = [&]<int KIND, int NT, int U, int NST, int PR>() {
+        constexpr int ROWS = NT / LB;
+        if (npts > int64_t(NST) * U * ROWS || ncoef > PR * ROWS) return false;
+        uint64_t const lim = uint64_t(1) << 32;
+        // byte offsets up to the last slot of the last stage must fit 32 bits
+        if ((uint64_t(NST) * U * ROWS * uint64_t(pl.rs) + uint64_t(nbatch)) * 8 >= lim) return false;
+        if ((uint64_t(ncoef) * uint64_t(cl.rs) + uint64_t(nbatch)) * 8 >= lim) return false;
+        uint32_t const pbytes = uint32_t((uint64_t(npts - 1) * uint64_t(pl.rs) + uint64_t(nbatch)) * 8);
+        uint32_t const cbytes = uint32_t((uint64_t(ncoef - 1) * uint64_t(cl.rs) + uint64_t(nbatch)) * 8);
         auto kern = eval_stream_kernel<D, KIND, NT, U, NST, PR>;
         hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)),
@@ -470,7 +473,13 @@ bool launch_stream(BSplinesPlan const& bs, Rules1D rules, double* out, const dou
         hip_check(hipGetLastError(), "eval_stream_kernel launch");
         return true;
     };
-    return nst == 2 ? go.template operator()<kUniformBasis, 2, 33>() : go.template operator()<kUniformBasis, 4, 33>();
+    if (b.uniform)
+        return npts <= 2 * 8 * 32 ? go.template operator()<kUniformBasis, 512, 8, 2, 33>()
+                                  : go.template operator()<kUniformBasis, 512, 8, 4, 33>();
+    if constexpr (D <= 3) // higher degrees spill in this form and keep eval_tiled_kernel
+        return npts <= 4 * 4 * 48 ? go.template operator()<kNonUniformBasis, 768, 4, 4, 22>()
+                                  : go.template operator()<kNonUniformBasis, 768, 4, 6, 22>();
+    return false;
 }
 
 template <int D, int ORDER>
