@@ -84,6 +84,8 @@ SIGNATURES = {
     "ddcx_build_eval": [_P, _P, _P, _L, _P, _L, _P, _P, _L, _P, _P, _L, _I64, _I64, _P],
     "ddcx_advect": [_P, _P, _P, _L, _P, _L, _P, _P, _L, _P, _P, _L, _I64, _P],
     "ddcx_evaluatornd_eval": [_P, C.c_int, _P, _P, _P, _I64, _P, _I64, _P, _I64, _P],
+    "ddcx_evaluatornd_eval_workspace": [_P, C.c_int, _I64, _I64, _P],
+    "ddcx_evaluatornd_eval_binned": [_P, C.c_int, _P, _P, _P, _I64, _P, _I64, _P, _I64, _P, _I64, _P],
     "ddcx_evaluator2d_eval_vjp_workspace": [_P, _P, _I64, _I64, _P],
     "ddcx_evaluator2d_eval_vjp_binned": [_P, _P, _P, _P, _P, _I64, _P, _P, _I64, _I64, _I64, _I64, _P, _I64, _P],
     "ddcx_evaluatornd_integrate": [_P, C.c_int, _P, _I64, _P, _I64, _P, _I64, _P],

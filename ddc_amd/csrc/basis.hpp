@@ -274,6 +274,17 @@ __host__ __device__ inline int eval_basis(BSplinesView const& b, const double* k
     return icell - D;
 }
 
+// First B-spline index of the cell holding x: the jmin that eval_basis and
+// eval_basis_and_n_derivs return for the same x, without the basis values.
+__host__ __device__ inline int cell_jmin(BSplinesView const& b, double x)
+{
+    if (b.uniform) {
+        double offset;
+        return uniform_cell(b, x, offset);
+    }
+    return nonuniform_cell(b, b.knots, x) - b.degree;
+}
+
 // eval_basis_and_n_derivs (NURBS book A2.3), uniform bsplines_uniform.hpp:475-564
 // and non-uniform bsplines_non_uniform.hpp:561-664.  derivs is (D+1) x (N+1)
 // row-major.  Returns jmin.

@@ -519,35 +519,81 @@ ddcx_status ddcx_evaluator2d_eval_vjp_binned_static(const ddcx_evaluator* e1, co
     });
 }
 
+namespace {
+
+// the evaluators' plans and rules, checked as ddcx_evaluatornd_eval requires
+void nd_setup(const ddcx_evaluator* const* evs, int nd, const int* orders, ddcx::BSplinesPlan const** bs, int* lower,
+              int* upper)
+{
+    require(evs && nd >= 1 && nd <= ddcx::kMaxNd, "nd must be between 1 and 4");
+    for (int k = 0; k < nd; ++k) {
+        require(evs[k], "null evaluator");
+        bs[k] = evs[k]->bs.get();
+        lower[k] = rule_of(evs[k]->lower);
+        upper[k] = rule_of(evs[k]->upper);
+        if (!bs[k]->view.periodic && (lower[k] == DDCX_EXTRAP_CONSTANT || upper[k] == DDCX_EXTRAP_CONSTANT))
+            throw ddcx::Error(DDCX_ERR_UNSUPPORTED, "the N-dimensional evaluator takes Null or Periodic extrapolation rules");
+        if (bs[k]->view.degree > 5)
+            throw ddcx::Error(DDCX_ERR_UNSUPPORTED, "N-dimensional evaluation supports degrees 1..5");
+        if (orders) {
+            require(orders[k] >= 0, "negative derivative order");
+            require(orders[k] <= bs[k]->view.degree, "derivative order above the degree");
+        }
+    }
+}
+
+} // namespace
+
 ddcx_status ddcx_evaluatornd_eval(const ddcx_evaluator* const* evs, int nd, const int* orders, double* out,
                                   const double* coords, int64_t npts, const double* coef, int64_t coef_batch_stride,
                                   const int64_t* coef_strides, int64_t nbatch, void* stream)
 {
     return guarded([&] {
-        require(evs && nd >= 1 && nd <= ddcx::kMaxNd, "nd must be between 1 and 4");
-        require(coef_strides, "null coefficient strides");
-        require(npts >= 0 && nbatch >= 0, "negative size");
         ddcx::BSplinesPlan const* bs[ddcx::kMaxNd];
         int lower[ddcx::kMaxNd], upper[ddcx::kMaxNd];
-        for (int k = 0; k < nd; ++k) {
-            require(evs[k], "null evaluator");
-            bs[k] = evs[k]->bs.get();
-            lower[k] = rule_of(evs[k]->lower);
-            upper[k] = rule_of(evs[k]->upper);
-            if (!bs[k]->view.periodic && (lower[k] == DDCX_EXTRAP_CONSTANT || upper[k] == DDCX_EXTRAP_CONSTANT))
-                throw ddcx::Error(DDCX_ERR_UNSUPPORTED,
-                                  "the N-dimensional evaluator takes Null or Periodic extrapolation rules");
-            if (bs[k]->view.degree > 5)
-                throw ddcx::Error(DDCX_ERR_UNSUPPORTED, "N-dimensional evaluation supports degrees 1..5");
-            if (orders) {
-                require(orders[k] >= 0, "negative derivative order");
-                require(orders[k] <= bs[k]->view.degree, "derivative order above the degree");
-            }
-        }
+        nd_setup(evs, nd, orders, bs, lower, upper);
+        require(coef_strides, "null coefficient strides");
+        require(npts >= 0 && nbatch >= 0, "negative size");
         if (npts * nbatch == 0) return;
         require(out && coords && coef, "null data pointer");
         ddcx::launch_evalnd(bs, nd, orders, lower, upper, out, coords, npts, coef, coef_batch_stride, coef_strides,
                             nbatch, static_cast<hipStream_t>(stream));
+    });
+}
+
+ddcx_status ddcx_evaluatornd_eval_workspace(const ddcx_evaluator* const* evs, int nd, int64_t npts, int64_t nbatch,
+                                            int64_t* bytes)
+{
+    return guarded([&] {
+        ddcx::BSplinesPlan const* bs[ddcx::kMaxNd];
+        int lower[ddcx::kMaxNd], upper[ddcx::kMaxNd];
+        nd_setup(evs, nd, nullptr, bs, lower, upper);
+        require(bytes, "null argument");
+        require(npts >= 0 && nbatch >= 0, "negative size");
+        *bytes = ddcx::evalnd_binned_workspace(bs, nd, npts, nbatch);
+    });
+}
+
+ddcx_status ddcx_evaluatornd_eval_binned(const ddcx_evaluator* const* evs, int nd, const int* orders, double* out,
+                                         const double* coords, int64_t npts, const double* coef,
+                                         int64_t coef_batch_stride, const int64_t* coef_strides, int64_t nbatch,
+                                         void* work, int64_t work_bytes, void* stream)
+{
+    return guarded([&] {
+        ddcx::BSplinesPlan const* bs[ddcx::kMaxNd];
+        int lower[ddcx::kMaxNd], upper[ddcx::kMaxNd];
+        nd_setup(evs, nd, orders, bs, lower, upper);
+        require(coef_strides, "null coefficient strides");
+        require(npts >= 0 && nbatch >= 0, "negative size");
+        int64_t const need = ddcx::evalnd_binned_workspace(bs, nd, npts, nbatch);
+        if (need == 0)
+            throw ddcx::Error(DDCX_ERR_UNSUPPORTED, "binned N-D evaluation: 3 dimensions of one degree <= 5 only");
+        require(work_bytes >= need, "binned N-D evaluation: workspace smaller than ddcx_evaluatornd_eval_workspace");
+        require(work && reinterpret_cast<uintptr_t>(work) % 8 == 0, "binned N-D evaluation: workspace null or misaligned");
+        if (npts * nbatch == 0) return;
+        require(out && coords && coef, "null data pointer");
+        ddcx::launch_evalnd_binned(bs, nd, orders, lower, upper, out, coords, npts, coef, coef_batch_stride,
+                                   coef_strides, nbatch, work, work_bytes, static_cast<hipStream_t>(stream));
     });
 }
 

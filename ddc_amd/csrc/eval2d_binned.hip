@@ -24,35 +24,24 @@
 // coordinate gradient along a clamped dimension) are those of
 // eval2d_vjp_kernel.  The order of the additions inside a patch depends on
 // scheduling, as with any atomic accumulation.
-#include "internal.hpp"
+#include "binning.hpp"
 
 namespace ddcx {
 
 namespace {
 
-constexpr int kBinT = 256;       // threads of the count / scatter workgroups
+using namespace binning;
+
+constexpr int kBinT = 256;       // threads of the count workgroups
 constexpr int kBinPts = 16;      // points per thread there (4096 per workgroup)
-constexpr int kMaxBins = 4096;   // tiles per batch entry
-constexpr int kNoBin = 0xffff;   // stored tile of a point the Null rule sends to 0
 constexpr int kTileThreads = 1024;
 constexpr size_t kTabLds = 48 * 1024; // LDS for staged knots and brackets
-constexpr size_t kLdsBytes = 160 * 1024;
 
 struct Bin2D {
     int T;        // tile side in cells
     int nt1, nt2; // tiles per dimension
     int nbins;
 };
-
-// first B-spline index of the cell holding x (the jmin of eval_basis)
-__device__ __forceinline__ int cell_of(BSplinesView const& b, double x)
-{
-    if (b.uniform) {
-        double off;
-        return uniform_cell(b, x, off);
-    }
-    return nonuniform_cell(b, b.knots, x) - b.degree;
-}
 
 // tile of the point (x1, x2) as given (before wrap and rules), or -1 when the
 // Null rule sends it to 0
@@ -61,7 +50,7 @@ __device__ __forceinline__ int bin_of_xy(BSplinesView const& b1, BSplinesView co
 {
     bool in1, in2;
     if (!place2d(b1, b2, rules, x1, x2, in1, in2)) return -1;
-    return (cell_of(b1, x1) / g.T) * g.nt2 + cell_of(b2, x2) / g.T;
+    return (cell_jmin(b1, x1) / g.T) * g.nt2 + cell_jmin(b2, x2) / g.T;
 }
 
 // tile of point t, or -1 (Null rule: gx = 0 written here when `write_gx`)
@@ -74,12 +63,6 @@ __device__ __forceinline__ int bin_of(BSplinesView const& b1, BSplinesView const
         gx[2 * t + 1] = 0.0;
     }
     return bin;
-}
-
-__global__ void __launch_bounds__(256) fill_int_kernel(int* __restrict__ p, int64_t n)
-{
-    int64_t const i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0;
 }
 
 __global__ void __launch_bounds__(256) zero_patch_kernel(double* __restrict__ g, int s1, int s2, int64_t cbs,
@@ -116,135 +99,6 @@ __global__ void __launch_bounds__(kBinT) bin_count_kernel(BSplinesView b1, BSpli
     __syncthreads();
     for (int k = threadIdx.x; k < g.nbins; k += kBinT)
         if (hist[k]) atomicAdd(&counts[l * g.nbins + k], hist[k]);
-}
-
-// exclusive scan of one batch entry's tile counts (nbins <= 4096, 1024 threads,
-// 4 tiles each); starts gets nbins + 1 entries, cursor a copy of the first nbins
-__global__ void __launch_bounds__(1024) bin_scan_kernel(Bin2D g, const int* __restrict__ counts,
-                                                        int* __restrict__ starts, int* __restrict__ cursor)
-{
-    __shared__ int part[1024];
-    int64_t const l = blockIdx.x;
-    int const t = threadIdx.x;
-    int c[4];
-    int sum = 0;
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-        int const k = 4 * t + u;
-        c[u] = k < g.nbins ? counts[l * g.nbins + k] : 0;
-        sum += c[u];
-    }
-    part[t] = sum;
-    __syncthreads();
-    for (int s = 1; s < 1024; s *= 2) { // inclusive Hillis-Steele scan
-        int const v = t >= s ? part[t - s] : 0;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
-    }
-    int run = part[t] - sum;
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-        int const k = 4 * t + u;
-        if (k < g.nbins) {
-            starts[l * (g.nbins + 1) + k] = run;
-            cursor[l * g.nbins + k] = run;
-        }
-        run += c[u];
-    }
-    if (t == 1023) starts[l * (g.nbins + 1) + g.nbins] = part[1023];
-}
-
-constexpr int kScatT = 1024; // threads of the scatter workgroups
-
-// LDS of a scatter workgroup of kScatT * PTS points: the points' records (x1, x2, w) in tile order, the tile histogram, local and
-// global tile offsets (nbins ints each), and per slot its tile and local index
-size_t scatter_lds(Bin2D const& g, int pts)
-{
-    size_t const np = size_t(kScatT) * pts;
-    return np * 3 * sizeof(double) + size_t(3) * g.nbins * sizeof(int) + np * 2 * sizeof(uint16_t);
-}
-
-// Each thread takes PTS points (read once, kept in registers) and their tiles
-// from the count pass, the workgroup orders the records by tile in LDS, and the records of one tile go
-// out as one run of consecutive lanes and addresses.
-template <int PTS>
-__global__ void __launch_bounds__(kScatT) bin_scatter_kernel(Bin2D g, const double* __restrict__ coords,
-                                                             const double* __restrict__ w, int64_t npts, int64_t bpb,
-                                                             int* __restrict__ cursor, int* __restrict__ perm,
-                                                             double* __restrict__ recs,
-                                                             const uint16_t* __restrict__ bins)
-{
-    constexpr int NP = kScatT * PTS;
-    static_assert(NP <= 65536 && kMaxBins <= 65536, "slot tile and local index are 16-bit");
-    extern __shared__ double srec[];
-    int* const hist = reinterpret_cast<int*>(srec + 3 * NP);
-    int* const lofs = hist + g.nbins;
-    int* const base = lofs + g.nbins;
-    uint16_t* const sbin = reinterpret_cast<uint16_t*>(base + g.nbins);
-    uint16_t* const sloc = sbin + NP;
-    __shared__ int part[kScatT];
-    int const tid = threadIdx.x;
-    int64_t const l = blockIdx.x / bpb;
-    int64_t const p0 = (blockIdx.x % bpb) * int64_t(NP);
-    for (int k = tid; k < g.nbins; k += kScatT) hist[k] = 0;
-    double x1[PTS], x2[PTS], wt[PTS];
-    int bin[PTS], rank[PTS];
-#pragma unroll
-    for (int u = 0; u < PTS; ++u) {
-        int64_t const p = p0 + u * kScatT + tid;
-        int64_t const t = l * npts + (p < npts ? p : npts - 1);
-        x1[u] = coords[2 * t];
-        x2[u] = coords[2 * t + 1];
-        wt[u] = w[t];
-        int const b = bins[t];
-        bin[u] = p < npts && b != kNoBin ? b : -1;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int u = 0; u < PTS; ++u) rank[u] = bin[u] >= 0 ? atomicAdd(&hist[bin[u]], 1) : 0;
-    __syncthreads();
-    // local exclusive offsets: thread t scans tiles [t*per, (t+1)*per), then
-    // the kScatT chunk sums are scanned
-    int const per = (g.nbins + kScatT - 1) / kScatT;
-    int const kb = tid * per, ke = min(kb + per, g.nbins);
-    int sum = 0;
-    for (int k = kb; k < ke; ++k) sum += hist[k];
-    part[tid] = sum;
-    __syncthreads();
-    for (int s = 1; s < kScatT; s *= 2) { // inclusive Hillis-Steele scan
-        int const v = tid >= s ? part[tid - s] : 0;
-        __syncthreads();
-        part[tid] += v;
-        __syncthreads();
-    }
-    int run = part[tid] - sum;
-    for (int k = kb; k < ke; ++k) {
-        lofs[k] = run;
-        run += hist[k];
-        if (hist[k]) base[k] = atomicAdd(&cursor[l * g.nbins + k], hist[k]);
-    }
-    int const total = part[kScatT - 1];
-    __syncthreads();
-#pragma unroll
-    for (int u = 0; u < PTS; ++u)
-        if (bin[u] >= 0) {
-            int const j = lofs[bin[u]] + rank[u];
-            srec[3 * j] = x1[u];
-            srec[3 * j + 1] = x2[u];
-            srec[3 * j + 2] = wt[u];
-            sbin[j] = uint16_t(bin[u]);
-            sloc[j] = uint16_t(u * kScatT + tid);
-        }
-    __syncthreads();
-    for (int j = tid; j < total; j += kScatT) {
-        int const b = sbin[j];
-        int64_t const k = l * npts + base[b] + (j - lofs[b]);
-        perm[k] = int(p0 + sloc[j]);
-        recs[3 * k] = srec[3 * j];
-        recs[3 * k + 1] = srec[3 * j + 1];
-        recs[3 * k + 2] = srec[3 * j + 2];
-    }
 }
 
 template <int D1, int D2>
@@ -350,31 +204,6 @@ Bin2D bins_for(BSplinesPlan const& bp1, BSplinesPlan const& bp2, int64_t nbatch)
     return none;
 }
 
-struct Work2D {
-    int* counts;
-    int* starts;
-    int* cursor;
-    int* perm;
-    double* recs;   // (x1, x2, w) per point in tile order
-    uint16_t* bins; // tile of each point (kNoBin: Null rule)
-    int64_t bytes;
-};
-
-Work2D carve(Bin2D const& g, int64_t npts, int64_t nbatch, void* base)
-{
-    auto up = [](int64_t n, int64_t size) { return (n * size + 255) / 256 * 256; };
-    int64_t const a = up(nbatch * g.nbins, 4), b = up(nbatch * (g.nbins + 1), 4), c = up(nbatch * g.nbins, 4),
-                  d = up(nbatch * npts, 4), e = up(3 * nbatch * npts, 8), f = up(nbatch * npts, 2);
-    char* p = static_cast<char*>(base);
-    return Work2D{reinterpret_cast<int*>(p),
-                  reinterpret_cast<int*>(p + a),
-                  reinterpret_cast<int*>(p + a + b),
-                  reinterpret_cast<int*>(p + a + b + c),
-                  reinterpret_cast<double*>(p + a + b + c + d),
-                  reinterpret_cast<uint16_t*>(p + a + b + c + d + e),
-                  a + b + c + d + e + f};
-}
-
 } // namespace
 
 // The workspace of the allocation-free entry point
@@ -406,7 +235,7 @@ int64_t eval2d_vjp_workspace(BSplinesPlan const& bp1, BSplinesPlan const& bp2, i
     if (g.nbins == 0 || npts >= (int64_t(1) << 31) || bp1.view.degree > 5 || bp2.view.degree > 5 ||
         !staged_tables_fit_2d(bp1.device_view(), bp2.device_view(), kTabLds))
         return 0;
-    return carve(g, npts, nbatch, nullptr).bytes;
+    return carve(g.nbins, 3, npts, nbatch, nullptr).bytes;
 }
 
 void launch_eval2d_vjp_binned(BSplinesPlan const& bp1, BSplinesPlan const& bp2, Rules2D const& rules,
@@ -421,12 +250,8 @@ void launch_eval2d_vjp_binned(BSplinesPlan const& bp1, BSplinesPlan const& bp2, 
     int64_t const need = eval2d_vjp_workspace(bp1, bp2, npts, nbatch);
     if (need == 0) throw Error(DDCX_ERR_UNSUPPORTED, "binned 2D adjoint: grid or degree out of range");
     if (work_bytes < need) throw Error(DDCX_ERR_INVALID, "binned 2D adjoint: workspace too small");
-    Work2D const ws = carve(g, npts, nbatch, work);
-    auto blocks = [](int64_t n, int t) {
-        int64_t const b = (n + t - 1) / t;
-        if (b > int64_t(0x7fffffff)) throw Error(DDCX_ERR_INVALID, "batch too large for one launch");
-        return unsigned(b);
-    };
+    Work const ws = carve(g.nbins, 3, npts, nbatch, work);
+    auto blocks = grid_blocks;
     int const s1 = bp1.size(), s2 = bp2.size();
     hipLaunchKernelGGL(zero_patch_kernel, dim3(blocks(int64_t(s1) * s2 * nbatch, 256)), dim3(256), 0, stream, gcoef,
                        s1, s2, cbs, st1, st2, nbatch);
@@ -437,24 +262,7 @@ void launch_eval2d_vjp_binned(BSplinesPlan const& bp1, BSplinesPlan const& bp2, 
         size_t const tabs = staged_tables_bytes_2d(b1, b2);
         hipLaunchKernelGGL(bin_count_kernel, dim3(blocks(bpb * nbatch, 1)), dim3(kBinT), tabs, stream, b1, b2, rules,
                            g, coords, gx, npts, bpb, ws.counts, ws.bins);
-        hipLaunchKernelGGL(bin_scan_kernel, dim3(unsigned(nbatch)), dim3(1024), 0, stream, g, ws.counts, ws.starts,
-                           ws.cursor);
-        // the most points per scatter workgroup whose records fit in LDS
-        auto scatter = [&]<int PTS>() {
-            size_t const slds = scatter_lds(g, PTS);
-            if (slds + kScatT * sizeof(int) > kLdsBytes) return false;
-            auto kern = bin_scatter_kernel<PTS>;
-            hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, int(slds)),
-                      "hipFuncSetAttribute");
-            int64_t const sbpb = (npts + kScatT * PTS - 1) / (kScatT * PTS);
-            hipLaunchKernelGGL(kern, dim3(blocks(sbpb * nbatch, 1)), dim3(kScatT), slds, stream, g, coords, w, npts,
-                               sbpb, ws.cursor, ws.perm, ws.recs, ws.bins);
-            return true;
-        };
-        if (!scatter.template operator()<4>() && !scatter.template operator()<2>() &&
-            !scatter.template operator()<1>())
-            throw Error(DDCX_ERR_UNSUPPORTED, "binned 2D adjoint: scatter does not fit in LDS");
+        scan_and_scatter<2, true>(g.nbins, coords, w, npts, nbatch, ws, stream);
         bool launched = false;
         dispatch_degree(b1.degree, [&]<int D1>() {
             dispatch_degree(b2.degree, [&]<int D2>() {

@@ -11,7 +11,7 @@
 // coef(i) * (I_1(i_1) * (...)) over the coefficient block of each batch entry
 // (spline_evaluator_nd.hpp:597-632): one workgroup per entry, or, with a
 // workspace for the partial sums, S workgroups per entry and a second pass.
-#include "internal.hpp"
+#include "binning.hpp"
 
 #include <algorithm>
 #include <utility>
@@ -332,6 +332,171 @@ __global__ void __launch_bounds__(256) integratend_sum_kernel(const double* __re
     if (lane == 0) out[l * ostride] = y;
 }
 
+// ---------------- binned 3D evaluation ----------------
+// With many scattered points per batch entry, a point's (D+1)^3 patch is
+// (D+1)^2 separate rows of an array far beyond one XCD's L2.  The points are
+// binned by tile of T^3 cells (binning.hpp); one workgroup per (batch entry,
+// tile) stages the tile's (T+D)^3 coefficient patch and the knots and cell
+// brackets of the non-uniform dimensions in LDS and evaluates its points
+// from there, with the basis routines and the term order of
+// evalnd_fixed_kernel: the values are bit-identical to it.
+constexpr int kNdBinT = 256;      // threads of the count workgroups
+constexpr int kNdBinPts = 16;     // points per thread there
+constexpr int kNdTileThreads = 512;
+constexpr size_t kNdPatchLds = 64 * 1024;
+constexpr size_t kNdTabLds = 32 * 1024;
+
+struct Bin3D {
+    int T;
+    int nt[3];
+    int nbins;
+};
+
+// the LDS copies of the knots and cell brackets of the non-uniform
+// dimensions: knots of every dimension, then brackets of every dimension
+size_t nd_tables_bytes(NdArgs const& a, int N)
+{
+    size_t n = 0;
+    for (int k = 0; k < N; ++k)
+        if (!a.b[k].uniform)
+            n += size_t(a.b[k].ncells + 1 + 2 * a.b[k].degree) * sizeof(double) + size_t(a.b[k].nbucket) * 4;
+    return n;
+}
+
+template <int N>
+__device__ void stage_tables_nd(NdArgs& a, double* lds, int tid, int nthreads)
+{
+    double* kn = lds;
+    int nk_total = 0;
+#pragma unroll
+    for (int k = 0; k < N; ++k) nk_total += a.b[k].uniform ? 0 : a.b[k].ncells + 1 + 2 * a.b[k].degree;
+    uint32_t* cl = reinterpret_cast<uint32_t*>(lds + nk_total);
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        BSplinesView& b = a.b[k];
+        int const nk = b.uniform ? 0 : b.ncells + 1 + 2 * b.degree, nc = b.uniform ? 0 : b.nbucket;
+        for (int i = tid; i < nk; i += nthreads) kn[i] = b.knots[i];
+        for (int i = tid; i < nc; i += nthreads) cl[i] = b.cells[i];
+        b.knots = kn;
+        b.cells = cl;
+        kn += nk;
+        cl += nc;
+    }
+}
+
+// SplineEvaluatorND::eval's point handling (evalnd_fixed_kernel): periodic
+// wrap, then the Null rule (0, no tile) unless some order is nonzero
+template <int N>
+__device__ __forceinline__ bool nd_place(NdArgs const& a, double (&x)[N], int any_order)
+{
+    if (any_order) return true;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        BSplinesView const& b = a.b[k];
+        if (b.periodic && (x[k] < b.rmin || x[k] > b.rmax)) x[k] -= floor((x[k] - b.rmin) / b.length) * b.length;
+    }
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        BSplinesView const& b = a.b[k];
+        if (!b.periodic && ((x[k] < b.rmin && a.lower[k] == DDCX_EXTRAP_NULL) ||
+                            (x[k] > b.rmax && a.upper[k] == DDCX_EXTRAP_NULL)))
+            return false;
+    }
+    return true;
+}
+
+// each point's tile and the per-tile counts; points the Null rule sends to 0
+// get their value here
+__global__ void __launch_bounds__(kNdBinT) ndbin_count_kernel(NdArgs a, Bin3D g, const double* __restrict__ coords,
+                                                              double* __restrict__ out, int64_t npts, int64_t bpb,
+                                                              int* __restrict__ counts, uint16_t* __restrict__ bins,
+                                                              int any_order)
+{
+    __shared__ int hist[binning::kMaxBins];
+    extern __shared__ double lds_tabs[];
+    int64_t const l = blockIdx.x / bpb;
+    int64_t const p0 = (blockIdx.x % bpb) * int64_t(kNdBinT * kNdBinPts);
+    for (int k = threadIdx.x; k < g.nbins; k += kNdBinT) hist[k] = 0;
+    stage_tables_nd<3>(a, lds_tabs, threadIdx.x, kNdBinT);
+    __syncthreads();
+#pragma unroll 4
+    for (int u = 0; u < kNdBinPts; ++u) {
+        int64_t const p = p0 + u * kNdBinT + threadIdx.x;
+        if (p < npts) {
+            int64_t const t = l * npts + p;
+            double x[3] = {coords[3 * t], coords[3 * t + 1], coords[3 * t + 2]};
+            if (!nd_place<3>(a, x, any_order)) {
+                out[t] = 0.0;
+                bins[t] = uint16_t(binning::kNoBin);
+                continue;
+            }
+            int bin = 0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) bin = bin * g.nt[k] + cell_jmin(a.b[k], x[k]) / g.T;
+            bins[t] = uint16_t(bin);
+            atomicAdd(&hist[bin], 1);
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < g.nbins; k += kNdBinT)
+        if (hist[k]) atomicAdd(&counts[l * g.nbins + k], hist[k]);
+}
+
+template <int D, bool ANY>
+__global__ void __launch_bounds__(kNdTileThreads) ndbin_tile_kernel(NdArgs a, Bin3D g,
+                                                                    const double* __restrict__ recs,
+                                                                    double* __restrict__ out, int64_t npts,
+                                                                    const double* __restrict__ coef, int64_t cbs,
+                                                                    const int* __restrict__ starts,
+                                                                    const int* __restrict__ perm)
+{
+    constexpr int N = 3;
+    extern __shared__ double sh[];
+    int const P = g.T + D;
+    double* const cp = sh; // coefficient patch, P^3, dimension 3 fastest
+    int64_t const l = blockIdx.x / g.nbins;
+    int const bin = int(blockIdx.x % g.nbins);
+    int a0[N];
+    {
+        int r = bin;
+#pragma unroll
+        for (int k = N - 1; k >= 0; --k) {
+            a0[k] = (r % g.nt[k]) * g.T;
+            r /= g.nt[k];
+        }
+    }
+    int sz[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) sz[k] = a.b[k].ncells + a.b[k].degree;
+    const double* c = coef + l * cbs;
+    int const P3 = P * P * P;
+    for (int idx = threadIdx.x; idx < P3; idx += kNdTileThreads) {
+        int const i0 = idx / (P * P), i1 = (idx / P) % P, i2 = idx % P;
+        bool const in = a0[0] + i0 < sz[0] && a0[1] + i1 < sz[1] && a0[2] + i2 < sz[2];
+        cp[idx] = in ? c[int64_t(a0[0] + i0) * a.st[0] + int64_t(a0[1] + i1) * a.st[1] + int64_t(a0[2] + i2) * a.st[2]]
+                     : 0.0;
+    }
+    stage_tables_nd<N>(a, sh + P3, threadIdx.x, kNdTileThreads);
+    __syncthreads();
+    int64_t const pst[N] = {int64_t(P) * P, P, 1};
+    int const k0 = starts[l * (g.nbins + 1) + bin], k1 = starts[l * (g.nbins + 1) + bin + 1];
+    for (int k = k0 + int(threadIdx.x); k < k1; k += kNdTileThreads) {
+        int64_t const q = l * npts + k;
+        double x[N] = {recs[3 * q], recs[3 * q + 1], recs[3 * q + 2]};
+        nd_place<N>(a, x, ANY ? 1 : 0); // the wrap; every binned point passes the rules
+        double v[N][kM];
+        int64_t base = 0;
+#pragma unroll
+        for (int d = 0; d < N; ++d) {
+            int const j = basis_fixed<D, ANY>(a.b[d], x[d], ANY ? a.order[d] : 0, v[d]);
+            base += int64_t(j - a0[d]) * pst[d];
+        }
+        double y = 0.0;
+        tensor_acc_fixed<N, D>(v, pst, cp + base, y);
+        out[l * npts + perm[q]] = y;
+    }
+}
+
 NdArgs make_args(BSplinesPlan const* const* bs, int nd, const int* orders, const int* lower, const int* upper,
                  const int64_t* st)
 {
@@ -409,6 +574,86 @@ void launch_evalnd(BSplinesPlan const* const* bs, int nd, const int* orders, con
                            coef, cbs, nbatch, any);
     });
     hip_check(hipGetLastError(), "evalnd_kernel launch");
+}
+
+namespace {
+
+// tile side T (the patch of (T+D)^3 doubles fits kNdPatchLds) and tiles; none
+// (nbins 0) when the grid is not 3D with one degree, the tiles exceed the
+// histograms or the knots and brackets do not fit in LDS
+Bin3D bins3d(NdArgs const& a, int nd)
+{
+    Bin3D g{};
+    if (nd != 3) return g;
+    int const D = a.b[0].degree;
+    for (int k = 0; k < 3; ++k)
+        if (a.b[k].degree != D || D > 5 || (!a.b[k].uniform && a.b[k].nbucket == 0)) return g;
+    if (nd_tables_bytes(a, 3) > kNdTabLds) return g;
+    int T = 16;
+    while (T > 4 && size_t(T + D) * (T + D) * (T + D) * sizeof(double) > kNdPatchLds) --T;
+    int64_t nbins = 1;
+    for (int k = 0; k < 3; ++k) {
+        g.nt[k] = (a.b[k].ncells + T - 1) / T;
+        nbins *= g.nt[k];
+    }
+    if (nbins > binning::kMaxBins) return Bin3D{};
+    g.T = T;
+    g.nbins = int(nbins);
+    return g;
+}
+
+} // namespace
+
+int64_t evalnd_binned_workspace(BSplinesPlan const* const* bs, int nd, int64_t npts, int64_t nbatch)
+{
+    // sizes only: the views' shapes, no device access
+    NdArgs a{};
+    for (int k = 0; k < nd && k < kMaxNd; ++k) a.b[k] = bs[k]->device_view();
+    Bin3D const g = bins3d(a, nd);
+    if (g.nbins == 0 || npts >= (int64_t(1) << 31)) return 0;
+    return binning::carve(g.nbins, 3, npts, nbatch, nullptr).bytes;
+}
+
+void launch_evalnd_binned(BSplinesPlan const* const* bs, int nd, const int* orders, const int* lower,
+                          const int* upper, double* out, const double* coords, int64_t npts, const double* coef,
+                          int64_t cbs, const int64_t* st, int64_t nbatch, void* work, int64_t work_bytes,
+                          hipStream_t stream)
+{
+    using namespace binning;
+    NdArgs const a = make_args(bs, nd, orders, lower, upper, st);
+    Bin3D const g = bins3d(a, nd);
+    int64_t const need = evalnd_binned_workspace(bs, nd, npts, nbatch);
+    if (need == 0) throw Error(DDCX_ERR_UNSUPPORTED, "binned N-D evaluation: 3 dimensions of one degree <= 5 only");
+    if (work_bytes < need) throw Error(DDCX_ERR_INVALID, "binned N-D evaluation: workspace too small");
+    if (npts * nbatch == 0) return;
+    int any = 0;
+    for (int k = 0; k < nd; ++k) any |= a.order[k];
+    Work const ws = carve(g.nbins, 3, npts, nbatch, work);
+    size_t const tabs = nd_tables_bytes(a, 3);
+    hipLaunchKernelGGL(fill_int_kernel, dim3(grid_blocks(nbatch * g.nbins, 256)), dim3(256), 0, stream, ws.counts,
+                       nbatch * g.nbins);
+    int64_t const bpb = (npts + kNdBinT * kNdBinPts - 1) / (kNdBinT * kNdBinPts);
+    hipLaunchKernelGGL(ndbin_count_kernel, dim3(grid_blocks(bpb * nbatch, 1)), dim3(kNdBinT), tabs, stream, a, g,
+                       coords, out, npts, bpb, ws.counts, ws.bins, any);
+    scan_and_scatter<3, false>(g.nbins, coords, nullptr, npts, nbatch, ws, stream);
+    bool launched = false;
+    dispatch_degree(a.b[0].degree, [&]<int D>() {
+        if constexpr (D <= 5) {
+            size_t const lds = size_t(g.T + D) * (g.T + D) * (g.T + D) * sizeof(double) + tabs;
+            auto go = [&](auto kern) {
+                hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)),
+                          "hipFuncSetAttribute");
+                hipLaunchKernelGGL(kern, dim3(grid_blocks(nbatch * g.nbins, 1)), dim3(kNdTileThreads), lds, stream, a,
+                                   g, ws.recs, out, npts, coef, cbs, ws.starts, ws.perm);
+            };
+            if (any) go(ndbin_tile_kernel<D, true>);
+            else go(ndbin_tile_kernel<D, false>);
+            launched = true;
+        }
+    });
+    if (!launched) throw Error(DDCX_ERR_UNSUPPORTED, "binned N-D evaluation supports degrees 1..5");
+    hip_check(hipGetLastError(), "binned N-D evaluation launch");
 }
 
 namespace {

@@ -381,6 +381,14 @@ constexpr int kMaxNd = 4;
 void launch_evalnd(BSplinesPlan const* const* bs, int nd, const int* orders, const int* lower, const int* upper,
                    double* out, const double* coords, int64_t npts, const double* coef, int64_t cbs,
                    const int64_t* st, int64_t nbatch, hipStream_t stream);
+// binned 3D evaluation (eval_nd.hip): workspace bytes (0: not supported for
+// this grid, use launch_evalnd), and the launch
+int64_t evalnd_binned_workspace(BSplinesPlan const* const* bs, int nd, int64_t npts, int64_t nbatch);
+void launch_evalnd_binned(BSplinesPlan const* const* bs, int nd, const int* orders, const int* lower,
+                          const int* upper, double* out, const double* coords, int64_t npts, const double* coef,
+                          int64_t cbs, const int64_t* st, int64_t nbatch, void* work, int64_t work_bytes,
+                          hipStream_t stream);
+
 // compute units of the current device (cached per device)
 int cu_count();
 

@@ -752,11 +752,16 @@ class SplineEvaluatorND:
         st = (C.c_int64 * nd)(*[coef.stride(k + 1) for k in range(nd)])
         return coef, st
 
-    def deriv(self, orders, spline_eval, coords_eval, spline_coef):
+    def deriv(self, orders, spline_eval, coords_eval, spline_coef, binned=None):
         """SplineEvaluatorND::deriv (spline_evaluator_nd.hpp:445-590): orders
-        is one derivative order per dimension."""
+        is one derivative order per dimension.  binned: None takes the
+        binned-by-tile evaluation (ddcx_evaluatornd_eval_binned, the same
+        values bit for bit) when the grid allows it and a batch entry has at
+        least one point per eight cells; True requires it; False never."""
         nd = len(self.evaluators)
         coef, st = self._coef(spline_coef)
+        _check_f64_cuda(coords_eval, "coords_eval")
+        _check_f64_cuda(spline_eval, "spline_eval")
         c = coords_eval.contiguous()
         if c.shape[-1] != nd:
             raise ValueError(f"coords_eval must end with a dimension of size {nd}")
@@ -764,14 +769,32 @@ class SplineEvaluatorND:
             raise ValueError("spline_eval must be contiguous")
         nb = coef.shape[0]
         npts = c.numel() // (nd * nb)
+        if spline_eval.numel() != npts * nb:
+            raise ValueError("spline_eval must hold one value per point")
         ords = None if orders is None else (C.c_int * nd)(*[int(o) for o in orders])
-        call("ddcx_evaluatornd_eval", self._hs, nd, ords, _ptr(spline_eval), _ptr(c), npts, _ptr(coef),
-             coef.stride(0), st, nb, _stream())
+        need = C.c_int64(0)
+        if binned is not False:
+            call("ddcx_evaluatornd_eval_workspace", self._hs, nd, npts, nb, C.byref(need))
+            if binned is True and need.value == 0:
+                raise NotImplementedError("the binned evaluation takes three dimensions of one degree <= 5")
+            if binned is None:
+                cells = int(np.prod([e.bsplines.ncells() for e in self.evaluators]))
+                if 8 * npts < cells:
+                    need.value = 0
+        if need.value > 0:
+            ws = getattr(self, "_work", None)
+            if ws is None or ws.numel() < need.value or ws.device != c.device:
+                ws = self._work = torch.empty(need.value, dtype=torch.uint8, device=c.device)
+            call("ddcx_evaluatornd_eval_binned", self._hs, nd, ords, _ptr(spline_eval), _ptr(c), npts, _ptr(coef),
+                 coef.stride(0), st, nb, _ptr(ws), ws.numel(), _stream())
+        else:
+            call("ddcx_evaluatornd_eval", self._hs, nd, ords, _ptr(spline_eval), _ptr(c), npts, _ptr(coef),
+                 coef.stride(0), st, nb, _stream())
         return spline_eval
 
-    def __call__(self, spline_eval, coords_eval, spline_coef):
+    def __call__(self, spline_eval, coords_eval, spline_coef, binned=None):
         """SplineEvaluatorND::operator() (spline_evaluator_nd.hpp:300-440)."""
-        return self.deriv(None, spline_eval, coords_eval, spline_coef)
+        return self.deriv(None, spline_eval, coords_eval, spline_coef, binned=binned)
 
     def integrate(self, integrals_out, spline_coef):
         """SplineEvaluatorND::integrate (spline_evaluator_nd.hpp:597-632):

@@ -311,6 +311,21 @@ ddcx_status ddcx_evaluatornd_eval(const ddcx_evaluator* const* evs, int nd, cons
                                   const double* coords, int64_t npts, const double* coef, int64_t coef_batch_stride,
                                   const int64_t* coef_strides, int64_t nbatch, void* stream);
 
+/* ddcx_evaluatornd_eval with the points binned by tile of cells (no
+ * reference counterpart; the same values bit for bit): for three dimensions
+ * of one degree and many scattered points per batch entry, each tile's
+ * coefficient patch is read once into on-chip memory instead of every point
+ * gathering (d+1)^2 rows of the coefficient array.  *bytes = the device
+ * workspace for npts x nbatch points (about 34 B per point), 0 when the grid
+ * is outside the binned kernels' range (then use ddcx_evaluatornd_eval).
+ * work: 8-byte aligned; calls that share a workspace must not overlap. */
+ddcx_status ddcx_evaluatornd_eval_workspace(const ddcx_evaluator* const* evs, int nd, int64_t npts, int64_t nbatch,
+                                            int64_t* bytes);
+ddcx_status ddcx_evaluatornd_eval_binned(const ddcx_evaluator* const* evs, int nd, const int* orders, double* out,
+                                         const double* coords, int64_t npts, const double* coef,
+                                         int64_t coef_batch_stride, const int64_t* coef_strides, int64_t nbatch,
+                                         void* work, int64_t work_bytes, void* stream);
+
 /* SplineEvaluatorND::integrate(integrals, spline_coef),
  * spline_evaluator_nd.hpp:597-632: out[l*out_stride] = integral of batch l's
  * spline over the whole domain. */

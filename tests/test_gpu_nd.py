@@ -340,3 +340,68 @@ def _takes(name, closures):
     every dimension the argument carries derivatives of must be Hermite."""
     dims = [int(part[-1]) for part in name.split("_")[1:] if part[:3] in ("min", "max")]
     return all(closures[k - 1] == "H" for k in dims)
+
+
+@pytest.mark.parametrize("spec,orders", [
+    ([(3, "nug", 40), (3, "up", 36), (3, "ug", 34)], (0, 0, 0)),
+    ([(3, "nug", 40), (3, "up", 36), (3, "ug", 34)], (1, 0, 2)),
+    ([(5, "up", 33), (5, "nug", 17), (5, "ug", 20)], (0, 0, 0)),
+    ([(1, "ug", 50), (1, "up", 3), (1, "nug", 70)], (0, 1, 0)),
+    ([(2, "nug", 18), (2, "nug", 21), (2, "up", 16)], (0, 0, 0)),
+], ids=["d3", "d3_deriv", "d5", "d1_one_tile_dim", "d2_two_nonuniform"])
+def test_evalnd_binned_matches_direct(spec, orders, cuda):
+    """The binned 3D evaluation (ddcx_evaluatornd_eval_binned: points grouped
+    by tile of cells, patch in LDS) against the one-thread-per-point kernel
+    bit for bit, and against the oracle; points outside every side of the
+    domain on the value form (periodic wrap, Null rule); several tiles per
+    dimension; 3 batch entries of different point densities."""
+    import torch
+    import ddc_amd as D
+    import oracle as O
+
+    bases = _bases(spec)
+    ss = [x[0] for x in bases]
+    rng = np.random.default_rng(17)
+    nb, npts = 3, 30000
+    coef = _coef(ss, nb, rng)
+    ext = 0.2 if orders == (0, 0, 0) else 0.0
+    coords = np.stack([rng.uniform(o.rmin - ext * (o.rmax - o.rmin), o.rmax + ext * (o.rmax - o.rmin), (nb, npts))
+                       for o in ss], axis=-1)
+    coords[1, : npts // 2] = coords[1, :1]  # one batch entry with half its points in one cell
+    ev = D.SplineEvaluator3D(*[x[3] for x in bases])
+    tc, tx = torch.from_numpy(coef).to(cuda), torch.from_numpy(coords).to(cuda)
+    direct = torch.full((nb, npts), np.nan, dtype=torch.float64, device=cuda)
+    ev.deriv(orders, direct, tx, tc, binned=False)
+    binned = torch.full_like(direct, np.nan)
+    ev.deriv(orders, binned, tx, tc, binned=True)
+    torch.cuda.synchronize()
+    assert np.array_equal(binned.cpu().numpy(), direct.cpu().numpy())
+    ref = O.splinend_eval(ss, coords, coef, orders=list(orders))
+    assert rel_err(binned.cpu().numpy(), ref) <= TOL * 10 ** sum(orders)
+    # a second, smaller call reuses the workspace
+    small = torch.full((nb, 100), np.nan, dtype=torch.float64, device=cuda)
+    ev.deriv(orders, small, tx[:, :100].contiguous(), tc, binned=True)
+    assert np.array_equal(small.cpu().numpy(), direct[:, :100].cpu().numpy())
+
+
+def test_evalnd_binned_refuses_mixed_degrees(cuda):
+    """Mixed degrees (or other than three dimensions) have no binned form:
+    the workspace query gives 0 and binned=True raises; the default takes the
+    direct kernel."""
+    import ctypes as C
+    import torch
+    import ddc_amd as D
+    from ddc_amd._lib import call
+
+    bases = _bases(SPECS_3D[1])
+    ev = D.SplineEvaluator3D(*[x[3] for x in bases])
+    need = C.c_int64(-1)
+    call("ddcx_evaluatornd_eval_workspace", ev._hs, 3, 1000, 2, C.byref(need))
+    assert need.value == 0
+    ss = [x[0] for x in bases]
+    coef = torch.from_numpy(_coef(ss, 2, np.random.default_rng(1))).to(cuda)
+    x = torch.rand((2, 1000, 3), dtype=torch.float64, device=cuda)
+    out = torch.empty((2, 1000), dtype=torch.float64, device=cuda)
+    with pytest.raises(NotImplementedError):
+        ev(out, x, coef, binned=True)
+    ev(out, x, coef)
