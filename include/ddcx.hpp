@@ -545,6 +545,27 @@ public:
         check(ddcx_evaluatornd_eval(hs_.data(), dimension(), orders.empty() ? nullptr : orders.data(), out, coords,
                                     npts, coef, coef_batch_stride, strides.data(), nbatch, stream));
     }
+    // bytes of device workspace the binned evaluation needs for npts x
+    // nbatch points (0: not available for this grid)
+    int64_t eval_workspace(int64_t npts, int64_t nbatch) const
+    {
+        int64_t bytes = 0;
+        check(ddcx_evaluatornd_eval_workspace(hs_.data(), dimension(), npts, nbatch, &bytes));
+        return bytes;
+    }
+    // deriv (orders empty: operator()) with the points binned by tile of
+    // cells through `work` (8-byte aligned): for many points per batch entry;
+    // the same values bit for bit
+    void deriv(std::vector<int> const& orders, double* out, const double* coords, int64_t npts, const double* coef,
+               int64_t coef_batch_stride, std::vector<int64_t> const& strides, int64_t nbatch, void* work,
+               int64_t work_bytes, void* stream) const
+    {
+        if (int(strides.size()) != dimension() || (!orders.empty() && int(orders.size()) != dimension()))
+            throw std::runtime_error("ddcx: one stride and one order per dimension");
+        check(ddcx_evaluatornd_eval_binned(hs_.data(), dimension(), orders.empty() ? nullptr : orders.data(), out,
+                                           coords, npts, coef, coef_batch_stride, strides.data(), nbatch, work,
+                                           work_bytes, stream));
+    }
     // integrate(integrals, spline_coef), spline_evaluator_nd.hpp:597-632
     void integrate(double* out, int64_t out_stride, const double* coef, int64_t coef_batch_stride,
                    std::vector<int64_t> const& strides, int64_t nbatch, void* stream = nullptr) const
