@@ -35,6 +35,7 @@ using namespace binning;
 constexpr int kBinT = 256;       // threads of the count workgroups
 constexpr int kBinPts = 16;      // points per thread there (4096 per workgroup)
 constexpr int kTileThreads = 1024;
+constexpr int kTileRecs = 4;    // records per thread in flight in the tile kernel
 constexpr size_t kTabLds = 48 * 1024; // LDS for staged knots and brackets
 
 struct Bin2D {
@@ -132,46 +133,67 @@ __global__ void __launch_bounds__(kTileThreads) bin_tile_kernel(BSplinesView b1,
     if (stage) stage_tables_2d(b1, b2, sh + 2 * P1 * P2, threadIdx.x, kTileThreads);
     __syncthreads();
     int const k0 = starts[l * (g.nbins + 1) + bin], k1 = starts[l * (g.nbins + 1) + bin + 1];
-    // the next record is loaded while the current one is processed
-    double n1 = 0.0, n2 = 0.0, nw = 0.0;
-    int np = 0;
-    auto fetch = [&](int k) {
-        int64_t const q = l * npts + k;
-        n1 = recs[3 * q];
-        n2 = recs[3 * q + 1];
-        nw = recs[3 * q + 2];
-        np = perm[q];
-    };
-    if (k0 + int(threadIdx.x) < k1) fetch(k0 + int(threadIdx.x));
-    for (int k = k0 + int(threadIdx.x); k < k1; k += kTileThreads) {
-        double x1 = n1, x2 = n2;
-        double const wt = nw;
-        int64_t const t = l * npts + np;
-        if (k + kTileThreads < k1) fetch(k + kTileThreads);
-        bool in1, in2;
-        place2d(b1, b2, rules, x1, x2, in1, in2); // true for every binned point
-        double e1[(D1 + 1) * 2], e2[(D2 + 1) * 2];
-        int const j1 = eval_basis_and_n_derivs<D1, 1, true>(b1, b1.knots, x1, e1) - r0;
-        int const j2 = eval_basis_and_n_derivs<D2, 1, true>(b2, b2.knots, x2, e2) - c0;
-        // f1 = sum_i B1'_i sum_j c_ij B2_j, f2 = sum_i B1_i sum_j c_ij B2'_j
-        double f1 = 0.0, f2 = 0.0;
+    // kTileRecs records per thread and step, the next step's in flight while
+    // the current ones are processed (one record per thread left the kernel
+    // waiting on memory latency)
+    constexpr int U = D1 + D2 <= 7 ? kTileRecs : 2; // high degrees: fewer, to stay within the registers
+    double n1[U], n2[U], nw[U];
+    int np[U];
+    auto fetch = [&](int kb) {
 #pragma unroll
-        for (int i = 0; i < D1 + 1; ++i) {
-            double s0 = 0.0, s1 = 0.0;
-            double const we = wt * e1[i * 2];
-#pragma unroll
-            for (int j = 0; j < D2 + 1; ++j) {
-                int const o = (j1 + i) * P2 + (j2 + j);
-                double const cij = cp[o];
-                s0 = fma(cij, e2[j * 2], s0);
-                s1 = fma(cij, e2[j * 2 + 1], s1);
-                atomicAdd(&gp[o], we * e2[j * 2]);
-            }
-            f1 = fma(e1[i * 2 + 1], s0, f1);
-            f2 = fma(e1[i * 2], s1, f2);
+        for (int u = 0; u < U; ++u) {
+            int const k = min(kb + u * kTileThreads, k1 - 1); // past the end: the last record again, unused
+            int64_t const q = l * npts + k;
+            n1[u] = recs[3 * q];
+            n2[u] = recs[3 * q + 1];
+            nw[u] = recs[3 * q + 2];
+            np[u] = perm[q];
         }
-        gx[2 * t] = in1 ? wt * f1 : 0.0;
-        gx[2 * t + 1] = in2 ? wt * f2 : 0.0;
+    };
+    int const kbeg = k0 + int(threadIdx.x);
+    if (kbeg < k1) fetch(kbeg);
+    for (int kb = kbeg; kb < k1; kb += U * kTileThreads) {
+        double c1[U], c2[U], cw[U];
+        int cpn[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            c1[u] = n1[u];
+            c2[u] = n2[u];
+            cw[u] = nw[u];
+            cpn[u] = np[u];
+        }
+        if (kb + U * kTileThreads < k1) fetch(kb + U * kTileThreads);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (kb + u * kTileThreads >= k1) break;
+            double x1 = c1[u], x2 = c2[u];
+            double const wt = cw[u];
+            int64_t const t = l * npts + cpn[u];
+            bool in1, in2;
+            place2d(b1, b2, rules, x1, x2, in1, in2); // true for every binned point
+            double e1[(D1 + 1) * 2], e2[(D2 + 1) * 2];
+            int const j1 = eval_basis_and_n_derivs<D1, 1, true>(b1, b1.knots, x1, e1) - r0;
+            int const j2 = eval_basis_and_n_derivs<D2, 1, true>(b2, b2.knots, x2, e2) - c0;
+            // f1 = sum_i B1'_i sum_j c_ij B2_j, f2 = sum_i B1_i sum_j c_ij B2'_j
+            double f1 = 0.0, f2 = 0.0;
+#pragma unroll
+            for (int i = 0; i < D1 + 1; ++i) {
+                double s0 = 0.0, s1 = 0.0;
+                double const we = wt * e1[i * 2];
+#pragma unroll
+                for (int j = 0; j < D2 + 1; ++j) {
+                    int const o = (j1 + i) * P2 + (j2 + j);
+                    double const cij = cp[o];
+                    s0 = fma(cij, e2[j * 2], s0);
+                    s1 = fma(cij, e2[j * 2 + 1], s1);
+                    atomicAdd(&gp[o], we * e2[j * 2]);
+                }
+                f1 = fma(e1[i * 2 + 1], s0, f1);
+                f2 = fma(e1[i * 2], s1, f2);
+            }
+            gx[2 * t] = in1 ? wt * f1 : 0.0;
+            gx[2 * t + 1] = in2 ? wt * f2 : 0.0;
+        }
     }
     __syncthreads();
     double* gc = gcoef + l * cbs;

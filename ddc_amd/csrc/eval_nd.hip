@@ -480,20 +480,51 @@ __global__ void __launch_bounds__(kNdTileThreads) ndbin_tile_kernel(NdArgs a, Bi
     __syncthreads();
     int64_t const pst[N] = {int64_t(P) * P, P, 1};
     int const k0 = starts[l * (g.nbins + 1) + bin], k1 = starts[l * (g.nbins + 1) + bin + 1];
-    for (int k = k0 + int(threadIdx.x); k < k1; k += kNdTileThreads) {
-        int64_t const q = l * npts + k;
-        double x[N] = {recs[3 * q], recs[3 * q + 1], recs[3 * q + 2]};
-        nd_place<N>(a, x, ANY ? 1 : 0); // the wrap; every binned point passes the rules
-        double v[N][kM];
-        int64_t base = 0;
+    // U records per thread and step, the next step's in flight while the
+    // current ones are evaluated
+    constexpr int U = 4;
+    double nx[U][N];
+    int np[U];
+    auto fetch = [&](int kb) {
 #pragma unroll
-        for (int d = 0; d < N; ++d) {
-            int const j = basis_fixed<D, ANY>(a.b[d], x[d], ANY ? a.order[d] : 0, v[d]);
-            base += int64_t(j - a0[d]) * pst[d];
+        for (int u = 0; u < U; ++u) {
+            int const k = min(kb + u * kNdTileThreads, k1 - 1); // past the end: the last record again, unused
+            int64_t const q = l * npts + k;
+#pragma unroll
+            for (int d = 0; d < N; ++d) nx[u][d] = recs[3 * q + d];
+            np[u] = perm[q];
         }
-        double y = 0.0;
-        tensor_acc_fixed<N, D>(v, pst, cp + base, y);
-        out[l * npts + perm[q]] = y;
+    };
+    int const kbeg = k0 + int(threadIdx.x);
+    if (kbeg < k1) fetch(kbeg);
+    for (int kb = kbeg; kb < k1; kb += U * kNdTileThreads) {
+        double cx[U][N];
+        int cpn[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+#pragma unroll
+            for (int d = 0; d < N; ++d) cx[u][d] = nx[u][d];
+            cpn[u] = np[u];
+        }
+        if (kb + U * kNdTileThreads < k1) fetch(kb + U * kNdTileThreads);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (kb + u * kNdTileThreads >= k1) break;
+            double x[N];
+#pragma unroll
+            for (int d = 0; d < N; ++d) x[d] = cx[u][d];
+            nd_place<N>(a, x, ANY ? 1 : 0); // the wrap; every binned point passes the rules
+            double v[N][kM];
+            int64_t base = 0;
+#pragma unroll
+            for (int d = 0; d < N; ++d) {
+                int const j = basis_fixed<D, ANY>(a.b[d], x[d], ANY ? a.order[d] : 0, v[d]);
+                base += int64_t(j - a0[d]) * pst[d];
+            }
+            double y = 0.0;
+            tensor_acc_fixed<N, D>(v, pst, cp + base, y);
+            out[l * npts + cpn[u]] = y;
+        }
     }
 }
 
