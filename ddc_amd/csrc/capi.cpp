@@ -376,6 +376,19 @@ ddcx_status ddcx_evaluator_deriv(const ddcx_evaluator* h, int order, double* out
     });
 }
 
+ddcx_status ddcx_evaluator_device_view(const ddcx_evaluator* h, ddcx_device_evaluator* out)
+{
+    return guarded([&] {
+        require(h && out, "null argument");
+        h->bs->ensure_device();
+        ddcx::BSplinesView const v = h->bs->device_view();
+        ddcx::Rules1D const r = rules1d(h);
+        *out = ddcx_device_evaluator{v.degree, v.uniform, v.periodic, v.ncells, v.origin, v.step, v.inv_step,
+                                     v.rmin, v.rmax, v.length, v.knots, v.cells, v.nbucket, v.inv_bucket,
+                                     r.lower, r.upper, r.pos_lower, r.pos_upper};
+    });
+}
+
 ddcx_status ddcx_evaluator_integrate(const ddcx_evaluator* h, double* out, int64_t out_stride, const double* coef,
                                      ddcx_layout coef_layout, int64_t nbatch, void* stream)
 {

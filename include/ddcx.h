@@ -232,6 +232,25 @@ ddcx_status ddcx_evaluator_deriv(const ddcx_evaluator* h, int order, double* out
                                  ddcx_layout points_layout, int64_t npts, const double* coef,
                                  ddcx_layout coef_layout, int64_t nbatch, void* stream);
 
+/* The device data of an evaluator, for the device-callable single-point
+ * evaluation of include/ddcx_device.hpp (SplineEvaluator::operator()(coord,
+ * coef_1d) and deriv(order, coord, coef_1d), spline_evaluator.hpp:228-235,
+ * :357-478): the basis description (knots and cell brackets in device
+ * memory of the evaluator's basis, valid while the evaluator exists) and the
+ * extrapolation rules with their positions. */
+typedef struct ddcx_device_evaluator {
+    int degree, uniform, periodic, ncells;
+    double origin, step, inv_step; /* uniform knots: origin + k * step */
+    double rmin, rmax, length;
+    const double* knots;   /* non-uniform knots (ncells + 1 + 2 * degree), device memory */
+    const uint32_t* cells; /* non-uniform cell-search brackets, device memory (NULL: full search) */
+    int nbucket;
+    double inv_bucket;
+    int lower, upper;      /* ddcx_extrap of the lower and upper rule */
+    double pos_lower, pos_upper; /* evaluation positions of Constant rules */
+} ddcx_device_evaluator;
+ddcx_status ddcx_evaluator_device_view(const ddcx_evaluator* h, ddcx_device_evaluator* out);
+
 /* SplineEvaluator::integrate(integrals, spline_coef), spline_evaluator.hpp:493-526. */
 ddcx_status ddcx_evaluator_integrate(const ddcx_evaluator* h, double* out, int64_t out_stride, const double* coef,
                                      ddcx_layout coef_layout, int64_t nbatch, void* stream);

@@ -312,6 +312,14 @@ public:
                                       gcoef.data, coef.layout, coef.nbatch, stream));
     }
     ddcx_evaluator* handle() const { return h_.get(); }
+    // the device data for the device-callable single-point evaluation of
+    // ddcx_device.hpp (operator()(coord, coef_1d), deriv(order, coord, coef_1d))
+    ddcx_device_evaluator device_view() const
+    {
+        ddcx_device_evaluator d{};
+        check(ddcx_evaluator_device_view(h_.get(), &d));
+        return d;
+    }
 
 private:
     static bool same(ddcx_layout const& a, ddcx_layout const& b)

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "ddcx.hpp"
+#include "ddcx_device.hpp"
 
 namespace {
 
@@ -64,6 +65,30 @@ struct Lcg {
     }
 };
 
+// The caller's own kernel, the way a reference user calls the device-callable
+// SplineEvaluator::operator()(coord, coef_1d) / deriv(order, coord, coef_1d)
+// (spline_evaluator.hpp:228-235): one thread per point of a batch-contiguous
+// (P, B) array, each line's coefficients at stride B.
+__global__ void user_point_kernel(ddcx_device_evaluator de, int order, const double* coef, const double* x,
+                                  double* out, int P, int B)
+{
+    int const t = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= P * B) return;
+    int const l = t % B;
+    ddcx::device::Evaluator const e = ddcx::device::evaluator(de);
+    out[t] = ddcx::device::deriv(e, order, x[t], coef + l, B);
+}
+
+void point_eval(ddcx::SplineEvaluator const& ev, int order, DeviceArray const& dc, DeviceArray const& dx,
+                DeviceArray& out, int P, int B)
+{
+    ddcx_device_evaluator const de = ev.device_view();
+    hipLaunchKernelGGL(user_point_kernel, dim3(unsigned((P * B + 255) / 256)), dim3(256), 0, 0, de, order, dc.p,
+                       dx.p, out.p, P, B);
+    hip_ok(hipGetLastError(), "user_point_kernel launch");
+    hip_ok(hipDeviceSynchronize(), "user_point_kernel");
+}
+
 // 1D: uniform periodic cubic, 64 cells, 33 lines, in one layout; build, eval,
 // deriv(1), integrate
 void case_1d(bool batch_contiguous)
@@ -104,6 +129,13 @@ void case_1d(bool batch_contiguous)
     dump(tag + "eval", dout.get());
     dump(tag + "deriv1", dder.get());
     dump(tag + "integrals", dint.get());
+    if (batch_contiguous) {
+        DeviceArray dp0(size_t(P) * B), dp1(size_t(P) * B);
+        point_eval(ev, 0, dc, dx, dp0, P, B);
+        point_eval(ev, 1, dc, dx, dp1, P, B);
+        dump(tag + "point_eval", dp0.get());
+        dump(tag + "point_deriv1", dp1.get());
+    }
 }
 
 // 1D: non-uniform cubic Greville with ConstantExtrapolationRule(eval_pos)
@@ -135,6 +167,11 @@ void case_rules()
     dump("rules_vals", vals);
     dump("rules_coords", coords);
     dump("rules_eval", dout.get());
+    DeviceArray dp0(size_t(P) * B), dp2(size_t(P) * B);
+    point_eval(ev, 0, dc, dx, dp0, P, B);
+    point_eval(ev, 2, dc, dx, dp2, P, B);
+    dump("rules_point_eval", dp0.get());
+    dump("rules_point_deriv2", dp2.get());
 }
 
 // 2D: non-uniform cubic Greville (20 cells) x uniform periodic cubic (16

@@ -44,6 +44,9 @@ def test_facade_1d(outputs, tag):
     d1 = orc.eval(coords, coef, deriv=1, layout=lay)
     assert rel_err(outputs(tag + "deriv1").reshape(shp(P)), d1) <= 1e-13
     assert rel_err(outputs(tag + "integrals"), orc.integrate(coef, layout=lay)) <= 1e-13
+    if bc:  # the device-callable single-point form in the program's own kernel: the batched values bit for bit
+        assert np.array_equal(outputs(tag + "point_eval"), outputs(tag + "eval"))
+        assert np.array_equal(outputs(tag + "point_deriv1"), outputs(tag + "deriv1"))
 
 
 def test_facade_constant_rules(outputs):
@@ -54,6 +57,10 @@ def test_facade_constant_rules(outputs):
     coef = orc.build(vals)
     ref = orc.eval(coords, coef, extrap=("constant", "constant"), eval_pos=(0.3, 0.8))
     assert rel_err(outputs("rules_eval").reshape(P, B), ref) <= 1e-13
+    assert np.array_equal(outputs("rules_point_eval"), outputs("rules_eval"))
+    inside = (coords >= orc.rmin) & (coords <= orc.rmax)  # deriv() applies no rule: compare inside the domain
+    d2 = orc.eval(coords, coef, deriv=2)
+    assert rel_err(outputs("rules_point_deriv2").reshape(P, B)[inside], d2[inside]) <= 1e-12
 
 
 def test_facade_2d(outputs):
