@@ -75,6 +75,8 @@ SIGNATURES = {
     "ddcx_evaluator_destroy": [_P],
     "ddcx_evaluator_eval": [_P, _P, _P, _L, _I64, _P, _L, _I64, _P],
     "ddcx_evaluator_deriv": [_P, C.c_int, _P, _P, _L, _I64, _P, _L, _I64, _P],
+    "ddcx_evaluator_eval_at": [_P, C.c_int, _P, _L, _P, _I64, _P, _L, _I64, _P],
+    "ddcx_evaluator_device_view": [_P, _P],
     "ddcx_evaluator_integrate": [_P, _P, _I64, _P, _L, _I64, _P],
     "ddcx_evaluator_eval_vjp": [_P, _P, _P, _P, _L, _I64, _P, _P, _L, _I64, _P],
     "ddcx_evaluator2d_eval": [_P, _P, C.c_int, C.c_int, _P, _P, _I64, _P, _I64, _I64, _I64, _I64, _P],

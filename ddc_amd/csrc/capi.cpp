@@ -360,6 +360,22 @@ ddcx_status ddcx_evaluator_eval(const ddcx_evaluator* h, double* out, const doub
     });
 }
 
+ddcx_status ddcx_evaluator_eval_at(const ddcx_evaluator* h, int order, double* out, ddcx_layout points_layout,
+                                   const double* points, int64_t npts, const double* coef, ddcx_layout coef_layout,
+                                   int64_t nbatch, void* stream)
+{
+    return guarded([&] {
+        require(h, "null evaluator");
+        require(order >= 0 && order <= h->bs->view.degree, "derivative order must be in [0, degree]");
+        require(npts >= 0 && nbatch >= 0, "negative size");
+        if (npts * nbatch == 0) return;
+        require(out && points && coef, "null data pointer");
+        ddcx::Layout const shared{1, 1, 0, 0}; // point p of every line at points[p]
+        ddcx::launch_eval_xl(*h->bs, rules1d(h), order, out, points, shared, ddcx::to_layout(points_layout), npts,
+                             coef, ddcx::to_layout(coef_layout), nbatch, static_cast<hipStream_t>(stream));
+    });
+}
+
 ddcx_status ddcx_evaluator_deriv(const ddcx_evaluator* h, int order, double* out, const double* coords,
                                  ddcx_layout points_layout, int64_t npts, const double* coef, ddcx_layout coef_layout,
                                  int64_t nbatch, void* stream)

@@ -39,7 +39,8 @@ __device__ __forceinline__ double eval_no_bc(BSplinesView const& b, double x, co
 
 template <int D, int ORDER>
 __global__ void __launch_bounds__(256) eval_kernel(BSplinesView b, Rules1D rules, double* __restrict__ out,
-                                                   const double* __restrict__ coords, Layout pl, int64_t npts,
+                                                   const double* __restrict__ coords, Layout xl, Layout pl,
+                                                   int64_t npts,
                                                    const double* __restrict__ coef, Layout cl, int64_t nbatch,
                                                    int point_fastest)
 {
@@ -48,7 +49,7 @@ __global__ void __launch_bounds__(256) eval_kernel(BSplinesView b, Rules1D rules
     int64_t const i = point_fastest ? t % npts : t / nbatch;
     int64_t const l = point_fastest ? t / npts : t % nbatch;
     int64_t const o = lay(pl, i, l);
-    double x = coords[o];
+    double x = coords[lay(xl, i, l)];
     const double* c = coef + lay(cl, 0, l);
     double y;
     if constexpr (ORDER == 0) {
@@ -239,27 +240,27 @@ unsigned grid_for(int64_t n, int threads)
 }
 
 template <int D, int O>
-void eval_launch(BSplinesView const& b, Rules1D rules, double* out, const double* coords, Layout pl,
+void eval_launch(BSplinesView const& b, Rules1D rules, double* out, const double* coords, Layout xl, Layout pl,
                  int64_t npts, const double* coef, Layout cl, int64_t nbatch, hipStream_t stream)
 {
     int const point_fastest = pl.rs == 1 ? 1 : 0;
     hipLaunchKernelGGL((eval_kernel<D, O>), dim3(grid_for(npts * nbatch, 256)), dim3(256), 0, stream, b, rules,
-                       out, coords, pl, npts, coef, cl, nbatch, point_fastest);
+                       out, coords, xl, pl, npts, coef, cl, nbatch, point_fastest);
 }
 
 template <int D>
-void eval_order(int order, BSplinesView const& b, Rules1D rules, double* out, const double* coords, Layout pl,
-                int64_t npts, const double* coef, Layout cl, int64_t nbatch, hipStream_t stream)
+void eval_order(int order, BSplinesView const& b, Rules1D rules, double* out, const double* coords, Layout xl,
+                Layout pl, int64_t npts, const double* coef, Layout cl, int64_t nbatch, hipStream_t stream)
 {
     switch (order) {
-    case 0: eval_launch<D, 0>(b, rules, out, coords, pl, npts, coef, cl, nbatch, stream); return;
-    case 1: eval_launch<D, 1>(b, rules, out, coords, pl, npts, coef, cl, nbatch, stream); return;
-    case 2: if constexpr (D >= 2) { eval_launch<D, 2>(b, rules, out, coords, pl, npts, coef, cl, nbatch, stream); return; } break;
-    case 3: if constexpr (D >= 3) { eval_launch<D, 3>(b, rules, out, coords, pl, npts, coef, cl, nbatch, stream); return; } break;
-    case 4: if constexpr (D >= 4) { eval_launch<D, 4>(b, rules, out, coords, pl, npts, coef, cl, nbatch, stream); return; } break;
-    case 5: if constexpr (D >= 5) { eval_launch<D, 5>(b, rules, out, coords, pl, npts, coef, cl, nbatch, stream); return; } break;
-    case 6: if constexpr (D >= 6) { eval_launch<D, 6>(b, rules, out, coords, pl, npts, coef, cl, nbatch, stream); return; } break;
-    case 7: if constexpr (D >= 7) { eval_launch<D, 7>(b, rules, out, coords, pl, npts, coef, cl, nbatch, stream); return; } break;
+    case 0: eval_launch<D, 0>(b, rules, out, coords, xl, pl, npts, coef, cl, nbatch, stream); return;
+    case 1: eval_launch<D, 1>(b, rules, out, coords, xl, pl, npts, coef, cl, nbatch, stream); return;
+    case 2: if constexpr (D >= 2) { eval_launch<D, 2>(b, rules, out, coords, xl, pl, npts, coef, cl, nbatch, stream); return; } break;
+    case 3: if constexpr (D >= 3) { eval_launch<D, 3>(b, rules, out, coords, xl, pl, npts, coef, cl, nbatch, stream); return; } break;
+    case 4: if constexpr (D >= 4) { eval_launch<D, 4>(b, rules, out, coords, xl, pl, npts, coef, cl, nbatch, stream); return; } break;
+    case 5: if constexpr (D >= 5) { eval_launch<D, 5>(b, rules, out, coords, xl, pl, npts, coef, cl, nbatch, stream); return; } break;
+    case 6: if constexpr (D >= 6) { eval_launch<D, 6>(b, rules, out, coords, xl, pl, npts, coef, cl, nbatch, stream); return; } break;
+    case 7: if constexpr (D >= 7) { eval_launch<D, 7>(b, rules, out, coords, xl, pl, npts, coef, cl, nbatch, stream); return; } break;
     default: break;
     }
     throw Error(DDCX_ERR_INVALID, "derivative order must be in [1, degree]");
@@ -270,12 +271,19 @@ void eval_order(int order, BSplinesView const& b, Rules1D rules, double* out, co
 void launch_eval(BSplinesPlan const& bs, Rules1D rules, int order, double* out, const double* coords,
                  Layout pl, int64_t npts, const double* coef, Layout cl, int64_t nbatch, hipStream_t stream)
 {
+    launch_eval_xl(bs, rules, order, out, coords, pl, pl, npts, coef, cl, nbatch, stream);
+}
+
+void launch_eval_xl(BSplinesPlan const& bs, Rules1D rules, int order, double* out, const double* coords,
+                    Layout xl, Layout pl, int64_t npts, const double* coef, Layout cl, int64_t nbatch,
+                    hipStream_t stream)
+{
     if (npts * nbatch == 0) return;
     bs.ensure_device();
-    if (launch_eval_tiled(bs, rules, order, out, coords, pl, npts, coef, cl, nbatch, stream)) return;
+    if (launch_eval_tiled(bs, rules, order, out, coords, xl, pl, npts, coef, cl, nbatch, stream)) return;
     BSplinesView const b = bs.device_view();
     bool const ok = dispatch_degree(b.degree, [&]<int D>() {
-        eval_order<D>(order, b, rules, out, coords, pl, npts, coef, cl, nbatch, stream);
+        eval_order<D>(order, b, rules, out, coords, xl, pl, npts, coef, cl, nbatch, stream);
     });
     if (!ok) throw Error(DDCX_ERR_UNSUPPORTED, "degree out of range");
     hip_check(hipGetLastError(), "eval_kernel launch");

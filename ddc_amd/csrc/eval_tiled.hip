@@ -57,7 +57,8 @@ __device__ __forceinline__ double point_value(BSplinesView const& b, const doubl
 
 template <int D, int ORDER, int LB, int NT, bool LINE_MAJOR, int KIND>
 __global__ void __launch_bounds__(NT) eval_tiled_kernel(BSplinesView b, Rules1D rules, double* __restrict__ out,
-                                                        const double* __restrict__ coords, Layout pl, int64_t npts,
+                                                        const double* __restrict__ coords, Layout xl, Layout pl,
+                                                        int64_t npts,
                                                         const double* __restrict__ coef, Layout cl, int64_t nbatch,
                                                         int ncoef, int pitch, int nknots)
 {
@@ -87,15 +88,16 @@ __global__ void __launch_bounds__(NT) eval_tiled_kernel(BSplinesView b, Rules1D 
     // is unconditional and the loads of the next stage stay in flight while
     // the current stage is evaluated.
     int const last = int(npts) - 1;
-    uint32_t const prs = uint32_t(pl.rs);
-    const double* __restrict__ cp = coords + pb;
+    uint32_t const prs = uint32_t(pl.rs), xrs = uint32_t(xl.rs);
+    const double* __restrict__ cp = coords + lay(xl, 0, lc);
     double* __restrict__ op = out + pb;
     auto at = [&](int ii) { return uint64_t(uint32_t(min(ii, last))) * prs; };
+    auto atx = [&](int ii) { return uint64_t(uint32_t(min(ii, last))) * xrs; };
     // first batch of coordinates, then the whole coefficient tile: every load
     // of the tile is in flight before the first LDS write
     double xa[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) xa[u] = __builtin_nontemporal_load(&cp[at(prow + u * ROWS)]);
+    for (int u = 0; u < U; ++u) xa[u] = __builtin_nontemporal_load(&cp[atx(prow + u * ROWS)]);
     // the tile is loaded in chunks of TC elements per thread, every load of a
     // chunk in flight before its LDS writes
     if constexpr (LINE_MAJOR) {
@@ -146,7 +148,7 @@ __global__ void __launch_bounds__(NT) eval_tiled_kernel(BSplinesView b, Rules1D 
     for (int i = prow; i <= last; i += U * ROWS) {
         double xb[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) xb[u] = __builtin_nontemporal_load(&cp[at(i + (U + u) * ROWS)]);
+        for (int u = 0; u < U; ++u) xb[u] = __builtin_nontemporal_load(&cp[atx(i + (U + u) * ROWS)]);
         double y[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) y[u] = point_value<D, ORDER, KIND>(bl, knots, rules, xa[u], tl, pitch);
@@ -483,7 +485,7 @@ bool launch_stream(BSplinesPlan const& bs, Rules1D rules, double* out, const dou
 }
 
 template <int D, int ORDER>
-bool launch_orders(BSplinesPlan const& bs, Rules1D rules, double* out, const double* coords, Layout pl,
+bool launch_orders(BSplinesPlan const& bs, Rules1D rules, double* out, const double* coords, Layout xl, Layout pl,
                    int64_t npts, const double* coef, Layout cl, int64_t nbatch, hipStream_t stream)
 {
     int const ncoef = bs.size();
@@ -502,14 +504,17 @@ bool launch_orders(BSplinesPlan const& bs, Rules1D rules, double* out, const dou
                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)),
                   "hipFuncSetAttribute");
         int64_t const blocks = (nbatch + LB - 1) / LB;
-        hipLaunchKernelGGL(kern, dim3(unsigned(blocks)), dim3(NT), lds, stream, b, rules, out, coords, pl, npts,
+        hipLaunchKernelGGL(kern, dim3(unsigned(blocks)), dim3(NT), lds, stream, b, rules, out, coords, xl, pl, npts,
                            coef, cl, nbatch, ncoef, pitch, nknots);
         hip_check(hipGetLastError(), "eval_tiled_kernel launch");
         return true;
     };
     if (ncoef > 2560) return false;
+    auto same = [](Layout const& a, Layout const& c) {
+        return a.rs == c.rs && a.inner == c.inner && a.is == c.is && a.os == c.os;
+    };
     if constexpr (ORDER == 0) {
-        if (launch_stream<D>(bs, rules, out, coords, pl, npts, coef, cl, nbatch, stream)) return true;
+        if (same(xl, pl) && launch_stream<D>(bs, rules, out, coords, pl, npts, coef, cl, nbatch, stream)) return true;
     }
     if (fits(16, 17)) return go.template operator()<16, 1024>(17);
     if (fits(16, 16)) return go.template operator()<16, 1024>(16);
@@ -535,20 +540,21 @@ int cu_count()
 }
 
 bool launch_eval_tiled(BSplinesPlan const& bs, Rules1D rules, int order, double* out, const double* coords,
-                       Layout pl, int64_t npts, const double* coef, Layout cl, int64_t nbatch, hipStream_t stream)
+                       Layout xl, Layout pl, int64_t npts, const double* coef, Layout cl, int64_t nbatch,
+                       hipStream_t stream)
 {
     // tiny batches, and mixed layouts (points and coefficients not both
     // batch-contiguous or both interpolation-contiguous), take the generic kernel
     if (npts < 64 || nbatch < 8) return false;
     if ((pl.rs == 1) != (cl.rs == 1)) return false;
     // 32-bit point indices and point strides in the kernel
-    if (npts >= (int64_t(1) << 30) || pl.rs >= (int64_t(1) << 32)) return false;
+    if (npts >= (int64_t(1) << 30) || pl.rs >= (int64_t(1) << 32) || xl.rs >= (int64_t(1) << 32)) return false;
     bool launched = false;
     dispatch_degree(bs.view.degree, [&]<int D>() {
         auto ord = [&]<int O>() {
             if constexpr (O <= D) {
                 if (!launched && order == O)
-                    launched = launch_orders<D, O>(bs, rules, out, coords, pl, npts, coef, cl, nbatch, stream);
+                    launched = launch_orders<D, O>(bs, rules, out, coords, xl, pl, npts, coef, cl, nbatch, stream);
             }
         };
         ord.template operator()<0>();

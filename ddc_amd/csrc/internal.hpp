@@ -364,13 +364,19 @@ void launch_problem_solve(SolveView const& sv, double* b, int64_t ldb, int64_t n
                           hipStream_t stream);
 void launch_build_vjp(BuilderPlan const& p, const double* gcoef, Layout cl, double* gvals, Layout vl, double* gdl,
                       double* gdr, Layout dlay, int64_t nbatch, hipStream_t stream);
+// coordinates at coords + lay(xl, point, line), values at out + lay(pl, point, line)
 bool launch_eval_tiled(BSplinesPlan const& bs, Rules1D rules, int order, double* out, const double* coords,
-                       Layout pl, int64_t npts, const double* coef, Layout cl, int64_t nbatch, hipStream_t stream);
+                       Layout xl, Layout pl, int64_t npts, const double* coef, Layout cl, int64_t nbatch,
+                       hipStream_t stream);
 bool launch_eval_vjp_tiled(BSplinesPlan const& bs, Rules1D rules, const double* coords, const double* w,
                            double* gx, Layout pl, int64_t npts, const double* coef, double* gcoef, Layout cl,
                            int64_t nbatch, hipStream_t stream);
 void launch_eval(BSplinesPlan const& bs, Rules1D rules, int order, double* out, const double* coords,
                  Layout pl, int64_t npts, const double* coef, Layout cl, int64_t nbatch, hipStream_t stream);
+// the coordinates in a layout of their own (xl), e.g. one vector shared by every line
+void launch_eval_xl(BSplinesPlan const& bs, Rules1D rules, int order, double* out, const double* coords,
+                    Layout xl, Layout pl, int64_t npts, const double* coef, Layout cl, int64_t nbatch,
+                    hipStream_t stream);
 void launch_eval_vjp(BSplinesPlan const& bs, Rules1D rules, const double* coords, const double* w,
                      double* gx, Layout pl, int64_t npts, const double* coef, double* gcoef, Layout cl,
                      int64_t nbatch, hipStream_t stream);

@@ -406,6 +406,28 @@ class SplineEvaluator:
              _ptr(spline_coef), cl, nbatch, _stream())
         return spline_eval
 
+    def eval_at_points(self, spline_eval, points, spline_coef, axis=0, order=0):
+        """SplineEvaluator::operator()(spline_eval, spline_coef) and
+        deriv(order, spline_eval, spline_coef) (spline_evaluator.hpp:296-342,
+        :432-478): the evaluation points are the evaluation domain's own
+        coordinates, `points` (one per entry of spline_eval along `axis`), the
+        same for every line."""
+        _check_f64_cuda(spline_eval, "spline_eval")
+        _check_f64_cuda(points, "points")
+        _check_f64_cuda(spline_coef, "spline_coef")
+        pts = points.contiguous()
+        if pts.dim() != 1 or pts.numel() != spline_eval.shape[axis]:
+            raise ValueError("points must hold one coordinate per entry of spline_eval along axis")
+        pl, nbatch = batched_layout(spline_eval, axis)
+        cl, nb2 = batched_layout(spline_coef, axis)
+        if nb2 != nbatch:
+            raise ValueError("evaluation and coefficient batch domains differ")
+        if spline_coef.shape[axis] != self.bsplines.size():
+            raise ValueError("spline_coef has the wrong number of coefficients")
+        call("ddcx_evaluator_eval_at", self._h, int(order), _ptr(spline_eval), pl, _ptr(pts), pts.numel(),
+             _ptr(spline_coef), cl, nbatch, _stream())
+        return spline_eval
+
     def integrate(self, integrals_out, spline_coef, axis=0):
         """SplineEvaluator::integrate(), spline_evaluator.hpp:493-526."""
         _check_f64_cuda(integrals_out, "integrals")

@@ -227,6 +227,16 @@ ddcx_status ddcx_evaluator_eval(const ddcx_evaluator* h, double* out, const doub
                                 ddcx_layout points_layout, int64_t npts, const double* coef,
                                 ddcx_layout coef_layout, int64_t nbatch, void* stream);
 
+/* SplineEvaluator::operator()(spline_eval, spline_coef) and
+ * deriv(order, spline_eval, spline_coef), spline_evaluator.hpp:296-342,
+ * :432-478: the evaluation points are the evaluation domain's own
+ * coordinates, the same for every line: `points` holds npts coordinates,
+ * point p of every line at points[p]; out has points_layout.  order 0 is
+ * operator() (with the rules), order > 0 deriv(). */
+ddcx_status ddcx_evaluator_eval_at(const ddcx_evaluator* h, int order, double* out, ddcx_layout points_layout,
+                                   const double* points, int64_t npts, const double* coef, ddcx_layout coef_layout,
+                                   int64_t nbatch, void* stream);
+
 /* SplineEvaluator::deriv(DiscreteElement<Deriv<X>> order, ...), spline_evaluator.hpp:357-430. */
 ddcx_status ddcx_evaluator_deriv(const ddcx_evaluator* h, int order, double* out, const double* coords,
                                  ddcx_layout points_layout, int64_t npts, const double* coef,

@@ -294,6 +294,17 @@ public:
         check(ddcx_evaluator_deriv(h_.get(), order, out.data, coords.data, coords.layout, npts, coef.data,
                                    coef.layout, coef.nbatch, stream));
     }
+    // operator()(spline_eval, spline_coef) / deriv(order, spline_eval,
+    // spline_coef) (spline_evaluator.hpp:296-342, :432-478): at the
+    // evaluation domain's own coordinates `points` (npts of them, device
+    // memory), the same for every line; order 0: operator()
+    void eval_at(int order, Span<double> out, const double* points, int64_t npts, Span<double const> coef,
+                 void* stream = nullptr) const
+    {
+        if (out.nbatch != coef.nbatch) throw std::runtime_error("ddcx: batch sizes differ");
+        check(ddcx_evaluator_eval_at(h_.get(), order, out.data, out.layout, points, npts, coef.data, coef.layout,
+                                     out.nbatch, stream));
+    }
     // integrate(integrals, spline_coef), spline_evaluator.hpp:494
     void integrate(double* out, int64_t out_stride, Span<double const> coef, void* stream = nullptr) const
     {

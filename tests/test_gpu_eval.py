@@ -257,3 +257,40 @@ def test_deriv_orders_tight(case, layout, cuda):
     assert errs[0] <= TOL
     for k, e in enumerate(errs, start=1):
         assert e <= TOL * 10 ** (k - 1), (k, e)
+
+
+@pytest.mark.parametrize("case", [Case(3, True, P, ncells=1024), Case(3, False, G, ncells=500, mesh="random"),
+                                  Case(5, False, H, ncells=300, mesh="random")], ids=lambda c: c.id)
+@pytest.mark.parametrize("layout", ["batch_contiguous", "interp_contiguous"])
+@pytest.mark.parametrize("nbatch", [37, 3], ids=["tiled", "per_point"])
+def test_eval_at_domain_points(case, layout, nbatch, cuda):
+    """operator()(spline_eval, spline_coef) / deriv(order, spline_eval,
+    spline_coef): one coordinate vector shared by every line
+    (ddcx_evaluator_eval_at), bit for bit against the same points repeated on
+    every line, on the tiled kernels (37 lines) and the per-point kernel (3),
+    points outside the domain for the value form, and against the oracle."""
+    import torch
+    import ddc_amd as D
+
+    orc, coef = _coef(case, nbatch, seed=8)
+    bs, _ = case.product()
+    rng = np.random.default_rng(9)
+    npts = 300
+    w = orc.rmax - orc.rmin
+    pts = rng.uniform(orc.rmin - 0.1 * w, orc.rmax + 0.1 * w, npts)
+    ev = D.SplineEvaluator(bs, D.ConstantExtrapolationRule(), D.NullExtrapolationRule())
+    bc = layout == "batch_contiguous"
+    ax = 0 if bc else 1
+    tc = torch.from_numpy(coef if bc else np.ascontiguousarray(coef.T)).to(cuda)
+    for order in (0, 1):
+        p = pts if order == 0 else np.clip(pts, orc.rmin, orc.rmax)
+        full = np.repeat(p[:, None], nbatch, axis=1)
+        tx = torch.from_numpy(full if bc else np.ascontiguousarray(full.T)).to(cuda)
+        ref_out = torch.empty_like(tx)
+        (ev(ref_out, tx, tc, axis=ax) if order == 0 else ev.deriv(order, ref_out, tx, tc, axis=ax))
+        out = torch.full_like(tx, np.nan)
+        ev.eval_at_points(out, torch.from_numpy(p).to(cuda), tc, axis=ax, order=order)
+        assert np.array_equal(out.cpu().numpy(), ref_out.cpu().numpy())
+        got = out.cpu().numpy() if bc else out.cpu().numpy().T
+        ref = orc.eval(full, coef, deriv=order, extrap=("constant", "null"))
+        assert np.abs(got - ref).max() / max(1.0, np.abs(ref).max()) <= TOL * (10 ** order)
