@@ -188,8 +188,8 @@ __device__ __forceinline__ void buffer_store(double v, __amdgpu_buffer_rsrc_t r,
 }
 
 // Persistent form of eval_tiled_kernel for values with plain batch-contiguous
-// arrays (element (row, line) at row * rs + line, 16-line tiles, 1024
-// threads).  Each workgroup walks the tiles blockIdx.x, blockIdx.x +
+// arrays (element (row, line) at row * rs + line, 16-line tiles, NT threads:
+// 512 for uniform bases, 768 for non-uniform ones).  Each workgroup walks the tiles blockIdx.x, blockIdx.x +
 // gridDim.x, ...: while the points of one tile are evaluated, the coefficients
 // of its next tile are loaded into registers (PR rows per thread, one slice per
 // stage of U points), and they go to LDS once the current tile has been read.
