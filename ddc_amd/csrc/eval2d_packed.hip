@@ -12,9 +12,14 @@
 //      values (the same routines as eval2d_kernel, with the knots and cell
 //      brackets of a non-uniform dimension staged in LDS), parked in LDS;
 //   2. groups of 8 lanes take one point each and load its record as 16-byte
-//      pairs (one cache line per point for cubic x cubic), multiply by the
-//      basis products and reduce over the group with cross-lane adds;
-//   3. each lane stores its point's value (coalesced).
+//      pairs (one cache line per point for cubic x cubic); the records of a
+//      wave's 64 points are all requested before the first is used, so each
+//      wave keeps 8 cache-line gathers in flight;
+//   3. each lane multiplies its record pairs by the basis products, one
+//      partial sum per point of its group, and three DPP exchanges within
+//      the group (mirrored lane, then quad permutations) fold the 8 x 8
+//      partial sums so that lane k ends with point k's value;
+//   4. each lane stores its point's value (coalesced).
 // The terms are summed in a fixed order, so results do not depend on
 // scheduling; they differ from eval2d_kernel's by rounding only.
 #include <algorithm>
@@ -60,6 +65,30 @@ __global__ void __launch_bounds__(kPackThreads) pack2d_kernel(const double* __re
     }
 }
 
+// v of the lane that DPP control CTRL names (quad permutations, row mirrors)
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    int const lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xf, 0xf, true);
+    int const hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xf, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+
+// One step of the transposed sum over a group of 8 lanes: each lane pairs
+// with the lane that DPP control CTRL names; the lane with `upper` false
+// keeps the first half of its N partial sums and adds its partner's first
+// half, the other keeps and adds the second half.
+template <int N, int CTRL>
+__device__ __forceinline__ void fold_half(double (&v)[8], bool upper)
+{
+#pragma unroll
+    for (int i = 0; i < N / 2; ++i) {
+        double const send = upper ? v[i] : v[i + N / 2];
+        double const keep = upper ? v[i + N / 2] : v[i];
+        v[i] = keep + dpp_f64<CTRL>(send);
+    }
+}
+
 // Persistent: workgroup g stages the knot vector and bucket brackets of each
 // non-uniform dimension in LDS, then takes tiles g, g + G, ... of kEvalThreads points, tile
 // t being points (t % tiles) * kEvalThreads ... of batch entry t / tiles.
@@ -75,23 +104,29 @@ __global__ void __launch_bounds__(kEvalThreads) eval2d_packed_kernel(BSplinesVie
     constexpr int NB = D1 + 1 + D2 + 1;
     constexpr int WAVES = kEvalThreads / 64;
     __shared__ double sb[WAVES][64][NB];  // basis values of dimension 1, then 2
-    __shared__ int64_t srec[WAVES][64];   // record index, -1: the Null rule gives 0
+    __shared__ int64_t srec[WAVES][64];   // record index
     __shared__ double so[WAVES][64];
     extern __shared__ double lds_tabs[]; // staged knots and cell brackets
     stage_tables_2d(b1, b2, lds_tabs, threadIdx.x, kEvalThreads);
     __syncthreads();
     int const wv = threadIdx.x / 64, lane = threadIdx.x % 64;
-    int const sub = lane % kGroup;
+    int const sub = lane % kGroup, grp = lane / kGroup;
     int64_t const tiles = (npts + kEvalThreads - 1) / kEvalThreads;
-    for (int64_t t = blockIdx.x; t < tiles * nbatch; t += gridDim.x) {
-        int64_t const l = t / tiles;
-        int64_t const p0 = (t - l * tiles) * kEvalThreads + wv * 64;
+    // tile t = l * tiles + tt of this workgroup, advanced by gridDim.x
+    int64_t l = blockIdx.x / tiles, tt = blockIdx.x - (blockIdx.x / tiles) * tiles;
+    for (; l < nbatch; tt += gridDim.x) {
+        while (tt >= tiles && l < nbatch) {
+            tt -= tiles;
+            ++l;
+        }
+        if (l >= nbatch) break;
+        int64_t const p0 = tt * kEvalThreads + wv * 64;
         if (p0 >= npts) continue; // wave-uniform
+        bool keep = true;
         {
             int64_t const p = p0 + lane < npts ? p0 + lane : npts - 1;
-            int64_t const q = l * npts + p;
-            double x1 = coords[2 * q], x2 = coords[2 * q + 1];
-            bool keep = true;
+            double2 const xy = reinterpret_cast<const double2*>(coords)[l * npts + p];
+            double x1 = xy.x, x2 = xy.y;
             if constexpr (O1 == 0 && O2 == 0) {
                 bool in1, in2;
                 keep = place2d(b1, b2, rules, x1, x2, in1, in2);
@@ -116,35 +151,58 @@ __global__ void __launch_bounds__(kEvalThreads) eval2d_packed_kernel(BSplinesVie
             for (int i = 0; i < D1 + 1; ++i) sb[wv][lane][i] = v1[i];
 #pragma unroll
             for (int j = 0; j < D2 + 1; ++j) sb[wv][lane][D1 + 1 + j] = v2[j];
-            srec[wv][lane] = keep ? (l * nc1 + j1) * int64_t(nc2) + j2 : -1;
+            // a point the Null rule sends to 0 reads record 0 and stores 0
+            srec[wv][lane] = keep ? (l * nc1 + j1) * int64_t(nc2) + j2 : 0;
         }
         __builtin_amdgcn_wave_barrier();
-#pragma unroll 2
-        for (int it = 0; it < 64 / kGroup; ++it) {
-            int const q = it * kGroup + lane / kGroup;
-            int64_t const rec = srec[wv][q];
-            const double2* r2 = reinterpret_cast<const double2*>(packed + (rec < 0 ? 0 : rec) * P);
-            double part = 0.0;
+        // lane (grp, sub) forms, for each of the 8 points it = 0..7 of its
+        // group (point it * 8 + grp of the slice), the partial sum over its
+        // record elements; the records of BATCH points are loaded before any
+        // is used
+        constexpr int KL = (NPAIR + kGroup - 1) / kGroup; // 16-byte loads per lane and point
+        constexpr int BATCH = KL == 1 ? 8 : (KL == 2 ? 4 : 2);
+        double part[8];
 #pragma unroll
-            for (int k = 0; k < (NPAIR + kGroup - 1) / kGroup; ++k) {
-                int const e2 = sub + k * kGroup;
-                if (e2 >= NPAIR) break;
-                double2 const c = r2[e2];
-                int const e = 2 * e2;
-                // element e is (row e / (D2+1), column e % (D2+1)); the pad element has a zero coefficient
-                int const ia = e / (D2 + 1), ja = e % (D2 + 1);
-                int const ib = (e + 1) / (D2 + 1), jb = (e + 1) % (D2 + 1);
-                double const wa = sb[wv][q][ia] * sb[wv][q][D1 + 1 + ja];
-                double const wb = (e + 1 < (D1 + 1) * (D2 + 1)) ? sb[wv][q][ib] * sb[wv][q][D1 + 1 + jb] : 0.0;
-                part = fma(c.x, wa, part);
-                part = fma(c.y, wb, part);
+        for (int i0 = 0; i0 < 64 / kGroup; i0 += BATCH) {
+            double2 c[BATCH][KL];
+#pragma unroll
+            for (int u = 0; u < BATCH; ++u) {
+                int const q = (i0 + u) * kGroup + grp;
+                const double2* r2 = reinterpret_cast<const double2*>(packed + srec[wv][q] * P);
+#pragma unroll
+                for (int k = 0; k < KL; ++k) {
+                    int const e2 = sub + k * kGroup;
+                    c[u][k] = e2 < NPAIR ? r2[e2] : make_double2(0.0, 0.0);
+                }
             }
 #pragma unroll
-            for (int m = 1; m < kGroup; m *= 2) part += __shfl_xor(part, m, kGroup);
-            if (sub == 0) so[wv][q] = rec < 0 ? 0.0 : part;
+            for (int u = 0; u < BATCH; ++u) {
+                int const q = (i0 + u) * kGroup + grp;
+                double pt = 0.0;
+#pragma unroll
+                for (int k = 0; k < KL; ++k) {
+                    int const e2 = sub + k * kGroup;
+                    if (e2 >= NPAIR) break;
+                    int const e = 2 * e2;
+                    // element e is (row e / (D2+1), column e % (D2+1)); the pad element has a zero coefficient
+                    int const ia = e / (D2 + 1), ja = e % (D2 + 1);
+                    int const ib = (e + 1) / (D2 + 1), jb = (e + 1) % (D2 + 1);
+                    double const wa = sb[wv][q][ia] * sb[wv][q][D1 + 1 + ja];
+                    double const wb = (e + 1 < (D1 + 1) * (D2 + 1)) ? sb[wv][q][ib] * sb[wv][q][D1 + 1 + jb] : 0.0;
+                    pt = fma(c[u][k].x, wa, pt);
+                    pt = fma(c[u][k].y, wb, pt);
+                }
+                part[i0 + u] = pt;
+            }
         }
+        // transposed sum over the group: lane sub ends with the sum of point
+        // it = sub (partners: mirrored lane 7 - sub, then sub ^ 2, then sub ^ 1)
+        fold_half<8, 0x141>(part, sub >= 4);   // row_half_mirror
+        fold_half<4, 0x4E>(part, (sub & 2) != 0); // quad_perm [2, 3, 0, 1]
+        fold_half<2, 0xB1>(part, (sub & 1) != 0); // quad_perm [1, 0, 3, 2]
+        so[wv][sub * kGroup + grp] = part[0];
         __builtin_amdgcn_wave_barrier();
-        if (p0 + lane < npts) out[l * npts + p0 + lane] = so[wv][lane];
+        if (p0 + lane < npts) out[l * npts + p0 + lane] = keep ? so[wv][lane] : 0.0;
         // the next tile's LDS writes come after every lane's read of this one
         __builtin_amdgcn_wave_barrier();
     }
