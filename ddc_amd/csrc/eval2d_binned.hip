@@ -88,12 +88,24 @@ __global__ void __launch_bounds__(kBinT) bin_count_kernel(BSplinesView b1, BSpli
     for (int k = threadIdx.x; k < g.nbins; k += kBinT) hist[k] = 0;
     stage_tables_2d(b1, b2, lds_tabs, threadIdx.x, kBinT);
     __syncthreads();
+    // every point's coordinates are requested before the first is binned
+    double2 xy[kBinPts];
+#pragma unroll
+    for (int u = 0; u < kBinPts; ++u) {
+        int64_t const p = p0 + u * kBinT + threadIdx.x;
+        xy[u] = reinterpret_cast<const double2*>(coords)[l * npts + (p < npts ? p : npts - 1)];
+    }
 #pragma unroll
     for (int u = 0; u < kBinPts; ++u) {
         int64_t const p = p0 + u * kBinT + threadIdx.x;
         if (p < npts) {
-            int const bin = bin_of(b1, b2, rules, g, coords, gx, l * npts + p, true);
-            bins[l * npts + p] = uint16_t(bin < 0 ? kNoBin : bin);
+            int64_t const t = l * npts + p;
+            int const bin = bin_of_xy(b1, b2, rules, g, xy[u].x, xy[u].y);
+            if (bin < 0) {
+                gx[2 * t] = 0.0;
+                gx[2 * t + 1] = 0.0;
+            }
+            bins[t] = uint16_t(bin < 0 ? kNoBin : bin);
             if (bin >= 0) atomicAdd(&hist[bin], 1);
         }
     }

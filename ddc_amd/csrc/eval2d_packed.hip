@@ -40,29 +40,31 @@ __host__ __device__ constexpr int patch_len(int d1, int d2)
     return ((d1 + 1) * (d2 + 1) + 1) / 2 * 2;
 }
 
-// One thread per (batch entry, cell1, cell2, patch row): copies the row's
-// d2+1 coefficients into the cell's record.
+// One thread per (batch entry, cell, pair of record elements): the record of
+// cell (i1, i2) lists its patch row by row, element k being coefficient
+// (i1 + k / (d2+1), i2 + k % (d2+1)), then the zero pad; consecutive threads
+// write consecutive 16-byte pairs.
 template <int D1, int D2>
 __global__ void __launch_bounds__(kPackThreads) pack2d_kernel(const double* __restrict__ coef, int64_t cbs,
                                                               int64_t st1, int64_t st2, int nc1, int nc2,
                                                               int64_t nbatch, double* __restrict__ packed)
 {
-    constexpr int P = patch_len(D1, D2);
+    constexpr int NPAIR = patch_len(D1, D2) / 2;
+    constexpr int NE = (D1 + 1) * (D2 + 1);
     int64_t const t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    int64_t const per = int64_t(nc1) * nc2 * (D1 + 1);
-    if (t >= per * nbatch) return;
-    int64_t const l = t / per;
-    int64_t r = t - l * per;
-    int const i = int(r % (D1 + 1));
-    r /= (D1 + 1);
-    int const i2 = int(r % nc2), i1 = int(r / nc2);
-    const double* src = coef + l * cbs + int64_t(i1 + i) * st1 + int64_t(i2) * st2;
-    double* dst = packed + ((l * nc1 + i1) * int64_t(nc2) + i2) * P + i * (D2 + 1);
-#pragma unroll
-    for (int j = 0; j < D2 + 1; ++j) dst[j] = src[int64_t(j) * st2];
-    if constexpr (P != (D1 + 1) * (D2 + 1)) {
-        if (i == D1) dst[D2 + 1] = 0.0; // the pad element
-    }
+    int64_t const ncell = int64_t(nc1) * nc2;
+    if (t >= ncell * nbatch * NPAIR) return;
+    int64_t const cell = t / NPAIR; // (l * nc1 + i1) * nc2 + i2
+    int const e = 2 * int(t - cell * NPAIR);
+    int64_t const l = cell / ncell;
+    int64_t const r = cell - l * ncell;
+    int64_t const i1 = r / nc2, i2 = r - i1 * nc2;
+    const double* c = coef + l * cbs;
+    auto elem = [&](int k) {
+        if (k >= NE) return 0.0; // the pad element
+        return c[(i1 + k / (D2 + 1)) * st1 + (i2 + k % (D2 + 1)) * st2];
+    };
+    reinterpret_cast<double2*>(packed)[t] = make_double2(elem(e), elem(e + 1));
 }
 
 // v of the lane that DPP control CTRL names (quad permutations, row mirrors)
@@ -296,7 +298,7 @@ void launch_eval2d_packed(BSplinesPlan const& bp1, BSplinesPlan const& bp2, Rule
         int64_t const resident = int64_t(std::max(per_cu, 1)) * cu_count();
         for (int64_t l0 = 0; l0 < nbatch; l0 += chunk) {
             int64_t const nb = std::min(chunk, nbatch - l0);
-            int64_t const nthreads = int64_t(nc1) * nc2 * (D1 + 1) * nb;
+            int64_t const nthreads = int64_t(nc1) * nc2 * (patch_len(D1, D2) / 2) * nb;
             hipLaunchKernelGGL((pack2d_kernel<D1, D2>), dim3(unsigned((nthreads + kPackThreads - 1) / kPackThreads)),
                                dim3(kPackThreads), 0, stream, coef + l0 * cbs, cbs, st1, st2, nc1, nc2, nb, packed);
             unsigned const grid = unsigned(std::min<int64_t>(tiles * nb, resident));

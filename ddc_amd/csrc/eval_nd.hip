@@ -419,22 +419,35 @@ __global__ void __launch_bounds__(kNdBinT) ndbin_count_kernel(NdArgs a, Bin3D g,
     for (int k = threadIdx.x; k < g.nbins; k += kNdBinT) hist[k] = 0;
     stage_tables_nd<3>(a, lds_tabs, threadIdx.x, kNdBinT);
     __syncthreads();
-#pragma unroll 4
-    for (int u = 0; u < kNdBinPts; ++u) {
-        int64_t const p = p0 + u * kNdBinT + threadIdx.x;
-        if (p < npts) {
-            int64_t const t = l * npts + p;
-            double x[3] = {coords[3 * t], coords[3 * t + 1], coords[3 * t + 2]};
-            if (!nd_place<3>(a, x, any_order)) {
-                out[t] = 0.0;
-                bins[t] = uint16_t(binning::kNoBin);
-                continue;
-            }
-            int bin = 0;
+    // the coordinates of kHold points are requested before the first is binned
+    constexpr int kHold = 8;
+    static_assert(kNdBinPts % kHold == 0, "points per thread in whole batches");
+    for (int u0 = 0; u0 < kNdBinPts; u0 += kHold) {
+        double xs[kHold][3];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) bin = bin * g.nt[k] + cell_jmin(a.b[k], x[k]) / g.T;
-            bins[t] = uint16_t(bin);
-            atomicAdd(&hist[bin], 1);
+        for (int u = 0; u < kHold; ++u) {
+            int64_t const p = p0 + (u0 + u) * kNdBinT + threadIdx.x;
+            int64_t const t = l * npts + (p < npts ? p : npts - 1);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) xs[u][k] = coords[3 * t + k];
+        }
+#pragma unroll
+        for (int u = 0; u < kHold; ++u) {
+            int64_t const p = p0 + (u0 + u) * kNdBinT + threadIdx.x;
+            if (p < npts) {
+                int64_t const t = l * npts + p;
+                double x[3] = {xs[u][0], xs[u][1], xs[u][2]};
+                if (!nd_place<3>(a, x, any_order)) {
+                    out[t] = 0.0;
+                    bins[t] = uint16_t(binning::kNoBin);
+                    continue;
+                }
+                int bin = 0;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) bin = bin * g.nt[k] + cell_jmin(a.b[k], x[k]) / g.T;
+                bins[t] = uint16_t(bin);
+                atomicAdd(&hist[bin], 1);
+            }
         }
     }
     __syncthreads();
