@@ -552,21 +552,53 @@ class SplineBuilder2D:
         b2(s, spline1, d1min, d1max, axis=2)
         return spline
 
-    def build_vjp(self, gspline, gvals):
-        """Adjoint of __call__ with respect to the values: the two transposed 1D
-        solves in reverse order (dim 2, then dim 1).  gspline: (batch, size1,
-        size2); gvals: (batch, n1, n2).  Hermite closures are not covered."""
-        H = SplineBuilderClosure.HERMITE
-        if any(H in (bb.lower, bb.upper) for bb in (self.builder1, self.builder2)):
-            raise NotImplementedError("the 2D adjoint covers closures without boundary derivative inputs")
+    def build_vjp(self, gspline, gvals, gderivs_min1=None, gderivs_max1=None, gderivs_min2=None, gderivs_max2=None,
+                  gmixed_derivs_min1_min2=None, gmixed_derivs_max1_min2=None, gmixed_derivs_min1_max2=None,
+                  gmixed_derivs_max1_max2=None):
+        """Adjoint of __call__: gvals (batch, n1, n2) and, with Hermite
+        closures, the gradients of the derivative inputs (each shaped as the
+        __call__ argument it belongs to, overwritten; None: not returned)
+        from gspline (batch, size1, size2).  __call__'s passes in reverse:
+        the transposed dimension-2 solve gives the gradients of the
+        dimension-1 spline and of the dimension-1 splines of derivs_*2, then
+        transposed dimension-1 solves give the gradients of vals, derivs_*1,
+        derivs_*2 and the corner derivatives."""
         squeeze = gspline.dim() == 2
-        g = gspline.unsqueeze(0) if squeeze else gspline
-        gv = gvals.unsqueeze(0) if squeeze else gvals
+
+        def b(t):
+            return None if t is None else (t.unsqueeze(0) if squeeze else t)
+
+        g, gv = b(gspline), b(gvals)
+        _check_f64_cuda(gv, "gvals")
         nb, n1, n2 = gv.shape
-        shape1 = (nb, self.builder1.bsplines.size(), n2)
-        tmp = torch.empty(shape1, dtype=torch.float64, device=g.device)
-        self.builder2.build_vjp(g, tmp, axis=2)
-        self.builder1.build_vjp(tmp, gv, axis=1)
+        b1, b2 = self.builder1, self.builder2
+        s1 = b1.bsplines.size()
+        H = SplineBuilderClosure.HERMITE
+        dev = g.device
+
+        def out(t, key, shape):
+            # the caller's gradient, or scratch when it is not asked for
+            return b(t) if t is not None else self._scratch(("g",) + key, shape, dev)
+
+        lo1, hi1 = b1.lower == H, b1.upper == H
+        lo2, hi2 = b2.lower == H, b2.upper == H
+        nv1l, nv1u, nv2l, nv2u = b1.s_nbv_xmin, b1.s_nbv_xmax, b2.s_nbv_xmin, b2.s_nbv_xmax
+        gs1 = self._scratch(("g", "s1"), (nb, s1, n2), dev)
+        gd1min = self._scratch(("g", "dmin"), (nb, s1, nv2l), dev) if lo2 else None
+        gd1max = self._scratch(("g", "dmax"), (nb, s1, nv2u), dev) if hi2 else None
+        b2.build_vjp(g, gs1, gd1min, gd1max, axis=2)
+        b1.build_vjp(gs1, gv, out(gderivs_min1, ("d1l",), (nb, nv1l, n2)) if lo1 else None,
+                     out(gderivs_max1, ("d1u",), (nb, nv1u, n2)) if hi1 else None, axis=1)
+        for on, gd1, gd2, gml, gmu, nv2 in ((lo2, gd1min, gderivs_min2, gmixed_derivs_min1_min2,
+                                             gmixed_derivs_max1_min2, nv2l),
+                                            (hi2, gd1max, gderivs_max2, gmixed_derivs_min1_max2,
+                                             gmixed_derivs_max1_max2, nv2u)):
+            if not on:
+                continue
+            tag = "l" if gd1 is gd1min else "u"
+            b1.build_vjp(gd1, out(gd2, ("d2" + tag,), (nb, n1, nv2)),
+                         out(gml, ("ml" + tag,), (nb, nv1l, nv2)) if lo1 else None,
+                         out(gmu, ("mu" + tag,), (nb, nv1u, nv2)) if hi1 else None, axis=1)
         return gvals
 
 
@@ -734,23 +766,72 @@ class SplineBuilder3D:
                   flat(m23.get(("min", "max"))), flat(m23.get(("max", "max"))))
         return spline
 
-    def build_vjp(self, gspline, gvals):
-        """Adjoint of __call__: the transposed 1D solves in reverse order
-        (dimension 3, 2, then 1).  gspline: (batch, size1, size2, size3);
-        gvals: (batch, n1, n2, n3).  Closures with derivative inputs are not
-        covered."""
-        if any(SplineBuilderClosure.HERMITE in (bb.lower, bb.upper) for bb in self.builders):
-            raise NotImplementedError("the 3D adjoint covers closures without boundary derivative inputs")
+    def build_vjp(self, gspline, gvals, **gderivs):
+        """Adjoint of __call__: gvals (batch, n1, n2, n3) from gspline (batch,
+        size1, size2, size3) and, with Hermite closures, the gradients of the
+        derivative inputs, passed by the argument's name with a "g" prefix
+        (gderivs_min1, gmixed_derivs_min1_max2_min3, ...; each shaped as the
+        __call__ argument, overwritten; omitted: not returned).  __call__'s
+        passes in reverse: SplineBuilder2D.build_vjp over dimensions 2 and 3
+        gives the gradients of the dimension-1 splines (of the values, of the
+        dimension-2 and -3 derivatives and of the 2-3 cross derivatives), then
+        transposed dimension-1 solves give the gradients of the inputs."""
+        unknown = set(gderivs) - {"g" + n for n in self.DERIV_NAMES}
+        if unknown:
+            raise TypeError(f"unknown gradient arguments: {sorted(unknown)}")
         squeeze = gspline.dim() == 3
-        g = gspline.unsqueeze(0) if squeeze else gspline
-        gv = gvals.unsqueeze(0) if squeeze else gvals
+
+        def b(t):
+            return None if t is None else (t.unsqueeze(0) if squeeze else t)
+
+        g, gv = b(gspline), b(gvals)
+        _check_f64_cuda(gv, "gvals")
         nb, n1, n2, n3 = gv.shape
         b1, b2, b3 = self.builders
-        t2 = torch.empty((nb, b1.bsplines.size(), b2.bsplines.size(), n3), dtype=torch.float64, device=g.device)
-        t1 = torch.empty((nb, b1.bsplines.size(), n2, n3), dtype=torch.float64, device=g.device)
-        b3.build_vjp(g, t2, axis=3)
-        b2.build_vjp(t2, t1, axis=2)
-        b1.build_vjp(t1, gv, axis=1)
+        s1 = b1.bsplines.size()
+        H = SplineBuilderClosure.HERMITE
+        dev = g.device
+        nv = {(k, "min"): bb.s_nbv_xmin for k, bb in zip((1, 2, 3), self.builders)}
+        nv.update({(k, "max"): bb.s_nbv_xmax for k, bb in zip((1, 2, 3), self.builders)})
+        herm = {(k, sd): (bb.lower if sd == "min" else bb.upper) == H
+                for k, bb in zip((1, 2, 3), self.builders) for sd in ("min", "max")}
+
+        def out(name, shape):
+            # the caller's gradient, or scratch when it is not asked for
+            t = gderivs.get("g" + name)
+            return b(t) if t is not None else self._scratch(("g", name), shape, dev)
+
+        def scratch(key, shape):
+            return self._scratch(("g",) + key, shape, dev)
+
+        def flat(t):
+            return None if t is None else t.reshape((nb * s1,) + tuple(t.shape[2:]))
+
+        sides = ("min", "max")
+        gs1 = scratch(("s1",), (nb, s1, n2, n3))
+        gd2 = {sd: scratch(("d2", sd), (nb, s1, nv[2, sd], n3)) for sd in sides if herm[2, sd]}
+        gd3 = {sd: scratch(("d3", sd), (nb, s1, n2, nv[3, sd])) for sd in sides if herm[3, sd]}
+        gm23 = {(a2, a3): scratch(("m23", a2, a3), (nb, s1, nv[2, a2], nv[3, a3]))
+                for a2 in sides for a3 in sides if herm[2, a2] and herm[3, a3]}
+        self._b23.build_vjp(flat(g), flat(gs1), flat(gd2.get("min")), flat(gd2.get("max")), flat(gd3.get("min")),
+                            flat(gd3.get("max")), flat(gm23.get(("min", "min"))), flat(gm23.get(("max", "min"))),
+                            flat(gm23.get(("min", "max"))), flat(gm23.get(("max", "max"))))
+
+        def along1(src, name, lo, hi, rest):
+            # transposed dimension-1 solve of one of __call__'s dimension-1 builds
+            b1.build_vjp(src, out(name, (nb, n1) + rest),
+                         out(lo, (nb, nv[1, "min"]) + rest) if herm[1, "min"] else None,
+                         out(hi, (nb, nv[1, "max"]) + rest) if herm[1, "max"] else None, axis=1)
+
+        b1.build_vjp(gs1, gv, out("derivs_min1", (nb, nv[1, "min"], n2, n3)) if herm[1, "min"] else None,
+                     out("derivs_max1", (nb, nv[1, "max"], n2, n3)) if herm[1, "max"] else None, axis=1)
+        for sd, t in gd2.items():
+            along1(t, f"derivs_{sd}2", f"mixed_derivs_min1_{sd}2", f"mixed_derivs_max1_{sd}2", (nv[2, sd], n3))
+        for sd, t in gd3.items():
+            along1(t, f"derivs_{sd}3", f"mixed_derivs_min1_{sd}3", f"mixed_derivs_max1_{sd}3", (n2, nv[3, sd]))
+        for (a2, a3), t in gm23.items():
+            along1(t, f"mixed_derivs_{a2}2_{a3}3", f"mixed_derivs_min1_{a2}2_{a3}3",
+                   f"mixed_derivs_max1_{a2}2_{a3}3", (nv[2, a2], nv[3, a3]))
         return gvals
 
 

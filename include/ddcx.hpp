@@ -428,21 +428,39 @@ public:
         check(ddcx_builder_build(b2_->handle(), spline, lo, spline1, li, l2 > 0 ? d1min : nullptr,
                                  u2 > 0 ? d1max : nullptr, ld, nb, stream));
     }
-    // adjoint with respect to the values (closures without derivative inputs):
-    // the transposed 1D solves in reverse order, through the same workspace
+    // adjoint of operator(): gvals and, with Hermite closures, the gradients
+    // of the derivative inputs (shaped as operator()'s; required where the
+    // closures take derivatives), from gspline, through the same workspace:
+    // the transposed dimension-2 solve gives the gradients of the dimension-1
+    // splines, then the transposed dimension-1 solves of operator()'s three
+    // dimension-1 builds give the rest
     void build_vjp(const double* gspline, double* gvals, int64_t nbatch, double* workspace,
-                   void* stream = nullptr) const
+                   double* gderivs_min1 = nullptr, double* gderivs_max1 = nullptr, double* gderivs_min2 = nullptr,
+                   double* gderivs_max2 = nullptr, double* gmixed_min1_min2 = nullptr,
+                   double* gmixed_max1_min2 = nullptr, double* gmixed_min1_max2 = nullptr,
+                   double* gmixed_max1_max2 = nullptr, void* stream = nullptr) const
     {
-        if (nbe1() || nbe2()) throw std::runtime_error("ddcx: the 2D adjoint covers closures without Hermite data");
         int64_t const n1 = b1_->npoints(), n2 = b2_->npoints(), s2 = b2_->info().size;
+        int64_t const l2 = b2_->info().nbv_lower, u2 = b2_->info().nbv_upper, e1 = nbe1();
+        double* gs1 = workspace;
+        double* gd1min = gs1 + nbatch * s1_ * n2;
+        double* gd1max = gd1min + nbatch * s1_ * l2;
         int64_t nb = 0;
         ddcx_layout const lg = batched_layout({nbatch, s1_, s2}, 2, &nb);
         ddcx_layout const lt = batched_layout({nbatch, s1_, n2}, 2);
-        ddcx_layout const none{1, 1, 0, 0};
-        check(ddcx_builder_build_vjp(b2_->handle(), gspline, lg, workspace, lt, nullptr, nullptr, none, nb, stream));
-        ddcx_layout const lt1 = batched_layout({nbatch, s1_, n2}, 1, &nb);
-        ddcx_layout const lv = batched_layout({nbatch, n1, n2}, 1);
-        check(ddcx_builder_build_vjp(b1_->handle(), workspace, lt1, gvals, lv, nullptr, nullptr, none, nb, stream));
+        ddcx_layout const ld = batched_layout({nbatch, s1_, std::max<int64_t>(nbe2(), 1)}, 2);
+        check(ddcx_builder_build_vjp(b2_->handle(), gspline, lg, gs1, lt, l2 > 0 ? gd1min : nullptr,
+                                     u2 > 0 ? gd1max : nullptr, ld, nb, stream));
+        auto dim1 = [&](const double* gin, double* gout, int64_t cols, double* gdmin, double* gdmax) {
+            int64_t nb1 = 0;
+            ddcx_layout const li = batched_layout({nbatch, s1_, cols}, 1, &nb1);
+            ddcx_layout const lo = batched_layout({nbatch, n1, cols}, 1);
+            ddcx_layout const lgd = batched_layout({nbatch, std::max<int64_t>(e1, 1), cols}, 1);
+            check(ddcx_builder_build_vjp(b1_->handle(), gin, li, gout, lo, gdmin, gdmax, lgd, nb1, stream));
+        };
+        dim1(gs1, gvals, n2, gderivs_min1, gderivs_max1);
+        if (l2 > 0) dim1(gd1min, gderivs_min2, l2, gmixed_min1_min2, gmixed_max1_min2);
+        if (u2 > 0) dim1(gd1max, gderivs_max2, u2, gmixed_min1_max2, gmixed_max1_max2);
     }
 
 private:

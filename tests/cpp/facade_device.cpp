@@ -225,6 +225,54 @@ void case_2d()
     dump("2d_gcoef", dgc.get());
 }
 
+// 2D with Hermite closures in both dimensions: non-uniform cubic (12 cells)
+// x uniform quintic (10 cells), 2 batch entries; SplineBuilder2D with the
+// boundary and corner derivatives, and its adjoint with every derivative
+// gradient
+void case_2d_hermite()
+{
+    int const nc1 = 12, nc2 = 10, NB = 2;
+    Lcg r{5};
+    std::vector<double> br(nc1 + 1);
+    br[0] = 0.0;
+    for (int i = 1; i <= nc1; ++i) br[i] = br[i - 1] + 0.5 + r.next();
+    for (auto& b : br) b /= br[nc1];
+    auto const H = ddcx::SplineBuilderClosure::HERMITE;
+    ddcx::NonUniformBSplines bs1(3, br, false);
+    ddcx::UniformBSplines bs2(5, 0.0, 1.0, nc2, false);
+    ddcx::SplineBuilder b1(bs1, H, H);
+    ddcx::SplineBuilder b2(bs2, H, H);
+    ddcx::SplineBuilder2D builder(b1, b2, bs1.size());
+    int64_t const n1 = b1.npoints(), n2 = b2.npoints(), s1 = bs1.size(), s2 = bs2.size();
+    int64_t const e1 = builder.nbe1(), e2 = builder.nbe2();
+    auto rnd = [&](int64_t n) {
+        std::vector<double> v(static_cast<size_t>(n));
+        for (auto& x : v) x = r.next() - 0.5;
+        return v;
+    };
+    std::vector<double> vals = rnd(NB * n1 * n2), gs = rnd(NB * s1 * s2);
+    std::vector<std::vector<double>> d1 = {rnd(NB * e1 * n2), rnd(NB * e1 * n2)}, d2 = {rnd(NB * n1 * e2), rnd(NB * n1 * e2)};
+    std::vector<std::vector<double>> m = {rnd(NB * e1 * e2), rnd(NB * e1 * e2), rnd(NB * e1 * e2), rnd(NB * e1 * e2)};
+    DeviceArray dv(vals), dc(size_t(NB * s1 * s2)), dws(size_t(builder.workspace_size(NB))), dgs(gs);
+    DeviceArray dd1a(d1[0]), dd1b(d1[1]), dd2a(d2[0]), dd2b(d2[1]), dma(m[0]), dmb(m[1]), dmc(m[2]), dmd(m[3]);
+    builder(dc.p, dv.p, NB, dws.p, dd1a.p, dd1b.p, dd2a.p, dd2b.p, dma.p, dmb.p, dmc.p, dmd.p);
+    DeviceArray gv(size_t(NB * n1 * n2)), g1a(size_t(NB * e1 * n2)), g1b(size_t(NB * e1 * n2));
+    DeviceArray g2a(size_t(NB * n1 * e2)), g2b(size_t(NB * n1 * e2));
+    DeviceArray gma(size_t(NB * e1 * e2)), gmb(size_t(NB * e1 * e2)), gmc(size_t(NB * e1 * e2)), gmd(size_t(NB * e1 * e2));
+    builder.build_vjp(dgs.p, gv.p, NB, dws.p, g1a.p, g1b.p, g2a.p, g2b.p, gma.p, gmb.p, gmc.p, gmd.p);
+    dump("2dh_breaks", br);
+    dump("2dh_vals", vals);
+    dump("2dh_gs", gs);
+    char const* dn[] = {"2dh_dmin1", "2dh_dmax1", "2dh_dmin2", "2dh_dmax2", "2dh_mm", "2dh_Mm", "2dh_mM", "2dh_MM"};
+    std::vector<double> const* dv8[] = {&d1[0], &d1[1], &d2[0], &d2[1], &m[0], &m[1], &m[2], &m[3]};
+    for (int k = 0; k < 8; ++k) dump(dn[k], *dv8[k]);
+    dump("2dh_coef", dc.get());
+    DeviceArray const* g9[] = {&gv, &g1a, &g1b, &g2a, &g2b, &gma, &gmb, &gmc, &gmd};
+    char const* gn[] = {"2dh_gvals", "2dh_gdmin1", "2dh_gdmax1", "2dh_gdmin2", "2dh_gdmax2",
+                        "2dh_gmm", "2dh_gMm", "2dh_gmM", "2dh_gMM"};
+    for (int k = 0; k < 9; ++k) dump(gn[k], g9[k]->get());
+}
+
 } // namespace
 
 int main(int argc, char** argv)
@@ -247,6 +295,9 @@ int main(int argc, char** argv)
         std::printf("facade_device: 2D\n");
         std::fflush(stdout);
         case_2d();
+        std::printf("facade_device: 2D Hermite\n");
+        std::fflush(stdout);
+        case_2d_hermite();
     } catch (std::exception const& e) {
         std::fprintf(stderr, "facade_device: %s\n", e.what());
         return 1;

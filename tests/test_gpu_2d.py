@@ -248,6 +248,60 @@ def test_build2d_hermite_matches_oracle(dims, cuda):
     assert rel_err(got, ref) <= TOL
 
 
+@pytest.mark.parametrize("dims", HERMITE_2D, ids=["HHxHH", "PxHG", "HHHxGH", "HHHxP"])
+def test_build2d_hermite_vjp_is_jacobian_transpose(dims, cuda):
+    """SplineBuilder2D.build_vjp with Hermite closures: the gradient of every
+    input block (values, the boundary derivatives of each dimension, the
+    corner cross derivatives) equals the transposed Jacobian of the oracle's
+    2D build for that block, assembled from unit inputs with the other
+    blocks zero.  Bars: 1e-13 for the values, 1e-12 for the derivative
+    blocks, whose gradients pass through the Hermite border rows."""
+    import torch
+    import ddc_amd as D
+    import oracle as O
+
+    bc = {"H": O.HERMITE, "HH": O.HOMOGENEOUS_HERMITE, "G": G, "P": P}
+    orcs, builders = [], []
+    for i, (d, uni, lo, up, n) in enumerate(dims):
+        br = None if uni else breaks_for("random", n, seed=20 + i)
+        periodic = lo == "P"
+        if uni:
+            o = O.Spline(d, True, bc[lo], bc[up], ncells=n, rmin=0.0, rmax=1.0)
+            bs = D.UniformBSplines(d, 0.0, 1.0, n, periodic=periodic)
+        else:
+            o = O.Spline(d, False, bc[lo], bc[up], breaks=br)
+            bs = D.NonUniformBSplines(d, br, periodic=periodic)
+        orcs.append(o)
+        builders.append(D.SplineBuilder(bs, D.SplineBuilderClosure(bc[lo]), D.SplineBuilderClosure(bc[up])))
+    o1, o2 = orcs
+    H1l, H1u = dims[0][2] == "H", dims[0][3] == "H"
+    H2l, H2u = dims[1][2] == "H", dims[1][3] == "H"
+    n1, n2 = o1.npts, o2.npts
+    # block name -> shape of one batch entry (None: the closures take no such input)
+    shapes = {"vals": (n1, n2),
+              "dmin1": (o1.nbv_l, n2) if H1l else None, "dmax1": (o1.nbv_r, n2) if H1u else None,
+              "dmin2": (n1, o2.nbv_l) if H2l else None, "dmax2": (n1, o2.nbv_r) if H2u else None,
+              "mm": (o1.nbv_l, o2.nbv_l) if H1l and H2l else None, "Mm": (o1.nbv_r, o2.nbv_l) if H1u and H2l else None,
+              "mM": (o1.nbv_l, o2.nbv_r) if H1l and H2u else None, "MM": (o1.nbv_r, o2.nbv_r) if H1u and H2u else None}
+    order = ["vals", "dmin1", "dmax1", "dmin2", "dmax2", "mm", "Mm", "mM", "MM"]
+    nb = 2
+    gs = np.random.default_rng(11).standard_normal((nb, o1.size, o2.size))
+    outs = {k: torch.full((nb,) + sh, np.nan, dtype=torch.float64, device=cuda)
+            for k, sh in shapes.items() if sh is not None}
+    D.SplineBuilder2D(*builders).build_vjp(torch.from_numpy(gs).to(cuda), *[outs.get(k) for k in order])
+    for k, sh in shapes.items():
+        if sh is None:
+            continue
+        m = int(np.prod(sh))
+        args = {j: (np.zeros((m,) + s2) if s2 is not None else None) for j, s2 in shapes.items()}
+        args[k] = np.eye(m).reshape((m,) + sh)
+        J = O.spline2d_build(o1, o2, *[args[j] for j in order]).reshape(m, -1)  # row e: build(unit e of block k)
+        ref = gs.reshape(nb, -1) @ J.T
+        got = outs[k].cpu().numpy().reshape(nb, m)
+        assert np.all(np.isfinite(got)), k
+        assert rel_err(got, ref) <= (TOL if k == "vals" else 1e-12), k
+
+
 @pytest.mark.parametrize("rules", [("null", "periodic"), ("constant", "periodic"), ("constant", "constant")])
 def test_eval2d_vjp_binned_matches_atomic(rules, cuda):
     """The binned 2D adjoint (ddcx_evaluator2d_eval_vjp_binned: points

@@ -1,7 +1,8 @@
 """The C++ facade (include/ddcx.hpp) on device arrays: tests/cpp/facade_device,
 a compiled C++ program, builds and evaluates through SplineBuilder /
 SplineEvaluator (both layouts, periodic and constant rules with eval_pos)
-and SplineBuilder2D / SplineEvaluator2D (values, deriv_dim_1, adjoint), and
+and SplineBuilder2D / SplineEvaluator2D (values, deriv_dim_1, adjoint, and
+the builder with Hermite closures and its adjoint), and
 writes its inputs and outputs; they are checked here against the oracle at
 the parity bars (1e-13; 1e-12 for accumulated coefficient gradients)."""
 import os
@@ -88,3 +89,31 @@ def test_facade_2d(outputs):
     gx = outputs("2d_gx").reshape(NB, P, 2)
     assert rel_err(gx[..., 0], w * O.spline2d_eval(o1, o2, wrapped, coef, 1, 0)) <= 1e-13
     assert rel_err(gx[..., 1], w * O.spline2d_eval(o1, o2, wrapped, coef, 0, 1)) <= 1e-13
+
+
+def test_facade_2d_hermite(outputs):
+    """SplineBuilder2D with Hermite closures in both dimensions through the
+    facade: coefficients against the oracle (1e-13), and build_vjp's gradient
+    of every input block against the transposed Jacobian of the oracle's
+    build for that block (1e-13 for the values, 1e-12 for the derivative
+    blocks)."""
+    NB, nc2 = 2, 10
+    H = O.HERMITE
+    o1 = O.Spline(3, False, H, H, breaks=outputs("2dh_breaks"))
+    o2 = O.Spline(5, True, H, H, ncells=nc2, rmin=0.0, rmax=1.0)
+    n1, n2, s1, s2 = o1.npts, o2.npts, o1.size, o2.size
+    e1, e2 = max(o1.nbv_l, o1.nbv_r), max(o2.nbv_l, o2.nbv_r)
+    shapes = {"vals": (n1, n2), "dmin1": (e1, n2), "dmax1": (e1, n2), "dmin2": (n1, e2), "dmax2": (n1, e2),
+              "mm": (e1, e2), "Mm": (e1, e2), "mM": (e1, e2), "MM": (e1, e2)}
+    order = list(shapes)
+    x = {k: outputs("2dh_" + k).reshape((NB,) + sh) for k, sh in shapes.items()}
+    ref = O.spline2d_build(o1, o2, *[x[k] for k in order])
+    assert rel_err(outputs("2dh_coef").reshape(NB, s1, s2), ref) <= 1e-13
+    gs = outputs("2dh_gs").reshape(NB, s1 * s2)
+    for k, sh in shapes.items():
+        m = int(np.prod(sh))
+        args = {j: np.zeros((m,) + s) for j, s in shapes.items()}
+        args[k] = np.eye(m).reshape((m,) + sh)
+        J = O.spline2d_build(o1, o2, *[args[j] for j in order]).reshape(m, -1)
+        got = outputs("2dh_g" + k).reshape(NB, m)
+        assert rel_err(got, gs @ J.T) <= (1e-13 if k == "vals" else 1e-12), k
