@@ -20,7 +20,7 @@ bool launch_build_tiled(BuilderPlan const& p, double* coef, Layout cl, const dou
         return tiled_build::launch_K<32, 32>(p, coef, cl, vals, vl, dl, dr, dlay, nbatch, stream);
     if (p.tiled.K == 64 && p.tiled.L == 32)
         return launch_build_tiled_k64(p, coef, cl, vals, vl, dl, dr, dlay, nbatch, stream);
-    if (p.tiled.K == 32 && p.tiled.L == 8)
+    if ((p.tiled.K == 32 && p.tiled.L == 8) || p.tiled.K == 16)
         return launch_build_tiled_short(p, coef, cl, vals, vl, dl, dr, dlay, nbatch, stream);
     return false;
 }

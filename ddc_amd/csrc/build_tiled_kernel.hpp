@@ -283,11 +283,11 @@ __global__ void __launch_bounds__(LB* K) build_tiled_kernel(TiledView tv, BuildV
     }
 }
 
-template <int K, int L, int KF, int KB, bool FDIV, bool BDIVF, int TMODE, bool IC>
+template <int K, int L, int KF, int KB, bool FDIV, bool BDIVF, int TMODE, bool IC, int NT>
 void launch_one(BuilderPlan const& p, size_t lds, double* coef, Layout cl, const double* vals, Layout vl,
                 const double* dl, const double* dr, Layout dlay, int64_t nbatch, hipStream_t stream)
 {
-    constexpr int NT = 1024, LB = NT / K;
+    constexpr int LB = NT / K;
     auto kern = build_tiled_kernel<LB, K, L, KF, KB, FDIV, BDIVF, TMODE, IC>;
     if (lds > 0)
         hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -299,13 +299,13 @@ void launch_one(BuilderPlan const& p, size_t lds, double* coef, Layout cl, const
     hip_check(hipGetLastError(), "build_tiled_kernel launch");
 }
 
-template <int K, int L, int KF, int KB, bool FDIV, bool BDIVF>
+template <int K, int L, int KF, int KB, bool FDIV, bool BDIVF, int NT>
 void launch_cfg(BuilderPlan const& p, double* coef, Layout cl, const double* vals, Layout vl, const double* dl,
                 const double* dr, Layout dlay, int64_t nbatch, hipStream_t stream)
 {
     if (p.tiled.rs != KF + KB + 1 + (FDIV ? 1 : 0))
         throw Error(DDCX_ERR_INVALID, "row table record length does not match the band shape");
-    constexpr int NT = 1024, LB = NT / K;
+    constexpr int LB = NT / K;
     constexpr int KE = KF > KB ? KF : KB;
     constexpr size_t static_lds =
         size_t(KE * K * LB + kLamMax * LB + kNbMax * LB) * sizeof(double) + sizeof(BorderLds);
@@ -321,7 +321,7 @@ void launch_cfg(BuilderPlan const& p, double* coef, Layout cl, const double* val
                                                                                : 0;
     size_t const lds = (mode == 1 ? table : mode == 2 ? half : 0) + stage;
     auto go = [&]<int TM, bool I>() {
-        launch_one<K, L, KF, KB, FDIV, BDIVF, TM, I>(p, lds, coef, cl, vals, vl, dl, dr, dlay, nbatch, stream);
+        launch_one<K, L, KF, KB, FDIV, BDIVF, TM, I, NT>(p, lds, coef, cl, vals, vl, dl, dr, dlay, nbatch, stream);
     };
     auto pick = [&]<bool I>() {
         if (mode == 1) go.template operator()<1, I>();
@@ -332,13 +332,14 @@ void launch_cfg(BuilderPlan const& p, double* coef, Layout cl, const double* val
     else pick.template operator()<false>();
 }
 
-template <int K, int L>
+// NT threads per workgroup: NT / K lines of K segments of L rows
+template <int K, int L, int NT = 1024>
 bool launch_K(BuilderPlan const& p, double* coef, Layout cl, const double* vals, Layout vl, const double* dl,
               const double* dr, Layout dlay, int64_t nbatch, hipStream_t stream)
 {
     TiledHost const& t = p.tiled;
     auto go = [&]<int KF, int KB, bool FD, bool BD>() {
-        launch_cfg<K, L, KF, KB, FD, BD>(p, coef, cl, vals, vl, dl, dr, dlay, nbatch, stream);
+        launch_cfg<K, L, KF, KB, FD, BD, NT>(p, coef, cl, vals, vl, dl, dr, dlay, nbatch, stream);
         return true;
     };
     if (t.bwd_divfirst) {

@@ -262,8 +262,11 @@ void make_tiled(BuilderPlan& p)
     // (32 lines give 256-byte rows in batch-contiguous layouts)
     int const nq = p.prob.q.n;
     if (nq < 256) {
-        // short lines (64 to 255 rows): 32 segments of 8 rows
-        tile_tables(p.prob, 32, 8, kCarryTol, p.tiled, false, 64);
+        // short lines (64 to 255 rows): 16 segments of 8 or 9 rows up to 144
+        // rows (512-thread workgroups), else 32 segments of 8 rows; the
+        // adjoint keeps 32 x 8
+        if (nq <= 144) tile_tables(p.prob, 16, nq <= 128 ? 8 : 9, kCarryTol, p.tiled, false, 64);
+        else tile_tables(p.prob, 32, 8, kCarryTol, p.tiled, false, 64);
         tile_tables(p.prob, 32, 8, kCarryTol, p.tiled_t, true, 64);
     } else {
         int const K = nq <= 1024 ? 32 : 64;

@@ -112,10 +112,12 @@ TILED_CASES = [
     Case(3, True, P, ncells=1800),
 ]
 
-# Short lines: main blocks of 64 to 255 rows on the 32 x 8 tiling (FP64 build
-# only; the mixed-precision and fused kernels start at 256 rows).
+# Short lines: main blocks of 64 to 255 rows on the short tilings (16 x 8 and
+# 16 x 9 up to 144 rows, 32 x 8 above; FP64 build only; the mixed-precision
+# and fused kernels start at 256 rows).
 SHORT_TILED_CASES = [
     Case(3, True, P, ncells=128),
+    Case(4, False, G, ncells=140, mesh="random"),
     Case(3, False, G, ncells=128, mesh="random"),
     Case(3, True, G, ncells=128),
     Case(5, True, P, ncells=200),
