@@ -41,6 +41,7 @@ __global__ void __launch_bounds__(LB* K) build_tiled_kernel(TiledView tv, BuildV
 {
     constexpr int KE = KF > KB ? KF : KB;
     constexpr int NT = LB * K;
+    constexpr int SR = stage_rows(K * L), SP = stage_pitch(K * L); // staging chunk rows, pitch (IC)
     constexpr bool TLDS = TMODE != 0;
     constexpr int RSF = KF + (FDIV ? 1 : 0), RSB = KB + 1; // split records
     __shared__ double ex[KE][K][LB];
@@ -71,40 +72,40 @@ __global__ void __launch_bounds__(LB* K) build_tiled_kernel(TiledView tv, BuildV
     uint32_t const vrs = uint32_t(vl.rs), crs = uint32_t(cl.rs);
 
     // dynamic LDS: the row table and the padded Q^-1 gamma blocks (TLDS),
-    // then the staging area, LB x kStagePitch doubles, and the per-line
+    // then the staging area, LB x stage_pitch(K * L) doubles, and the per-line
     // offsets of vals and coef (IC)
     int const tab = TMODE == 2 ? K * L * (RSF > RSB ? RSF : RSB) : K * L * RS; // doubles of the table region
     double* gpad = rt_lds + tab;
     double* stg = rt_lds + (TLDS ? tab + tv.ngb * L * kNbMax : 0);
-    uint32_t* loff = reinterpret_cast<uint32_t*>(stg + LB * kStagePitch);
+    uint32_t* loff = reinterpret_cast<uint32_t*>(stg + LB * SP);
     int const nl = int(min<int64_t>(LB, nbatch - line0));
 
     double y[L];
     if constexpr (IC) {
-        constexpr int PER = LB * kStageRows / NT; // staged elements per thread and chunk
+        constexpr int PER = LB * SR / NT; // staged elements per thread and chunk
         if (tid < LB) {
             int64_t const l = line0 + min(tid, nl - 1);
             loff[tid] = uint32_t(lay(vl, 0, l) - lay(vl, 0, line0));
             loff[LB + tid] = uint32_t(lay(cl, 0, l) - lay(cl, 0, line0));
         }
         __syncthreads();
-        for (int rbase = 0; rbase < K * L; rbase += kStageRows) {
+        for (int rbase = 0; rbase < K * L; rbase += SR) {
             double tmp[PER];
 #pragma unroll
             for (int k = 0; k < PER; ++k) {
                 int const e = tid + k * NT;
-                int const l = e / kStageRows, u = rbase + e % kStageRows;
+                int const l = e / SR, u = rbase + e % SR;
                 tmp[k] = (l < nl && u < nq) ? __builtin_nontemporal_load(&vwg[loff[l] + uint32_t(tv.q0 + u)]) : 0.0;
             }
 #pragma unroll
             for (int k = 0; k < PER; ++k) {
                 int const e = tid + k * NT;
-                stg[(e / kStageRows) * kStagePitch + e % kStageRows] = tmp[k];
+                stg[(e / SR) * SP + e % SR] = tmp[k];
             }
             __syncthreads();
-            if (r0 >= rbase && r0 < rbase + kStageRows) {
+            if (r0 >= rbase && r0 < rbase + SR) {
 #pragma unroll
-                for (int i = 0; i < L; ++i) y[i] = stg[ln * kStagePitch + (r0 - rbase) + i];
+                for (int i = 0; i < L; ++i) y[i] = stg[ln * SP + (r0 - rbase) + i];
             }
             __syncthreads();
         }
@@ -238,19 +239,19 @@ __global__ void __launch_bounds__(LB* K) build_tiled_kernel(TiledView tv, BuildV
 
     // ---------------- store ----------------
     if constexpr (IC) {
-        constexpr int PER = LB * kStageRows / NT;
-        for (int rbase = 0; rbase < K * L; rbase += kStageRows) {
-            if (r0 >= rbase && r0 < rbase + kStageRows) {
+        constexpr int PER = LB * SR / NT;
+        for (int rbase = 0; rbase < K * L; rbase += SR) {
+            if (r0 >= rbase && r0 < rbase + SR) {
 #pragma unroll
-                for (int i = 0; i < L; ++i) stg[ln * kStagePitch + (r0 - rbase) + i] = y[i];
+                for (int i = 0; i < L; ++i) stg[ln * SP + (r0 - rbase) + i] = y[i];
             }
             __syncthreads();
 #pragma unroll
             for (int k = 0; k < PER; ++k) {
                 int const e = tid + k * NT;
-                int const l = e / kStageRows, u = rbase + e % kStageRows;
+                int const l = e / SR, u = rbase + e % SR;
                 if (l < nl && u < nq)
-                    __builtin_nontemporal_store(stg[l * kStagePitch + e % kStageRows],
+                    __builtin_nontemporal_store(stg[l * SP + e % SR],
                                                 &cwg[loff[LB + l] + uint32_t(tv.top + u + off)]);
             }
             __syncthreads();
@@ -310,7 +311,7 @@ void launch_cfg(BuilderPlan const& p, double* coef, Layout cl, const double* val
     constexpr size_t static_lds =
         size_t(KE * K * LB + kLamMax * LB + kNbMax * LB) * sizeof(double) + sizeof(BorderLds);
     bool const ic = vl.rs == 1 && cl.rs == 1;
-    size_t const stage = ic ? size_t(LB) * kStagePitch * sizeof(double) + 2 * LB * sizeof(uint32_t) : 0;
+    size_t const stage = ic ? size_t(LB) * stage_pitch(K * L) * sizeof(double) + 2 * LB * sizeof(uint32_t) : 0;
     constexpr int RSH = (KF + (FDIV ? 1 : 0)) > (KB + 1) ? (KF + (FDIV ? 1 : 0)) : (KB + 1);
     size_t const gam = gamma_lds_doubles(p.tview) * sizeof(double);
     size_t const table = size_t(K) * L * p.tiled.rs * sizeof(double) + gam;

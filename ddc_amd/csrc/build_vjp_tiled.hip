@@ -67,6 +67,7 @@ __global__ void __launch_bounds__(LB* K) build_vjp_tiled_kernel(TiledView tv, Bu
 {
     constexpr int KE = KF > KB ? KF : KB;
     constexpr int NT = LB * K;
+    constexpr int SR = stage_rows(K * L), SP = stage_pitch(K * L); // staging chunk rows, pitch (IC)
     __shared__ double ex[KE][K][LB];
     __shared__ double part[kMaxGBlocksVjp][kNbMax][LB];
     __shared__ double x2s[kNbMax][LB];
@@ -93,11 +94,11 @@ __global__ void __launch_bounds__(LB* K) build_vjp_tiled_kernel(TiledView tv, Bu
     uint32_t const crs = uint32_t(cl.rs), vrs = uint32_t(vl.rs);
 
     // dynamic LDS: the row table and the padded Q^-1 gamma blocks (TLDS),
-    // then the staging area (IC), LB x kStagePitch doubles, and the per-line
+    // then the staging area (IC), LB x stage_pitch(K * L) doubles, and the per-line
     // offsets of gcoef and gvals
     double* gpad = rt_lds + K * L * RS;
     double* stg = rt_lds + (TLDS ? K * L * RS + tv.ngb * L * kNbMax : 0);
-    uint32_t* loff = reinterpret_cast<uint32_t*>(stg + LB * kStagePitch);
+    uint32_t* loff = reinterpret_cast<uint32_t*>(stg + LB * SP);
     int const nl = int(min<int64_t>(LB, nbatch - line0));
     if constexpr (IC) {
         if (tid < LB) {
@@ -115,25 +116,25 @@ __global__ void __launch_bounds__(LB* K) build_vjp_tiled_kernel(TiledView tv, Bu
         uint32_t const c0 = clo + uint32_t(top + r0 + off) * crs;
         if constexpr (IC) {
             // each line's main-block rows as contiguous runs through LDS
-            constexpr int PER = LB * kStageRows / NT;
-            for (int rbase = 0; rbase < K * L; rbase += kStageRows) {
+            constexpr int PER = LB * SR / NT;
+            for (int rbase = 0; rbase < K * L; rbase += SR) {
                 double tmp[PER];
 #pragma unroll
                 for (int k = 0; k < PER; ++k) {
                     int const e = tid + k * NT;
-                    int const l = e / kStageRows, u = rbase + e % kStageRows;
+                    int const l = e / SR, u = rbase + e % SR;
                     tmp[k] = (l < nl && u < nq) ? __builtin_nontemporal_load(&cwg[loff[l] + uint32_t(top + u + off)])
                                                 : 0.0;
                 }
 #pragma unroll
                 for (int k = 0; k < PER; ++k) {
                     int const e = tid + k * NT;
-                    stg[(e / kStageRows) * kStagePitch + e % kStageRows] = tmp[k];
+                    stg[(e / SR) * SP + e % SR] = tmp[k];
                 }
                 __syncthreads();
-                if (r0 >= rbase && r0 < rbase + kStageRows) {
+                if (r0 >= rbase && r0 < rbase + SR) {
 #pragma unroll
-                    for (int i = 0; i < L; ++i) y[i] = stg[ln * kStagePitch + (r0 - rbase) + i];
+                    for (int i = 0; i < L; ++i) y[i] = stg[ln * SP + (r0 - rbase) + i];
                 }
                 __syncthreads();
             }
@@ -277,19 +278,19 @@ __global__ void __launch_bounds__(LB* K) build_vjp_tiled_kernel(TiledView tv, Bu
 
     // 6. main-block rows to the value gradients
     if constexpr (IC) {
-        constexpr int PER = LB * kStageRows / NT;
-        for (int rbase = 0; rbase < K * L; rbase += kStageRows) {
-            if (r0 >= rbase && r0 < rbase + kStageRows) {
+        constexpr int PER = LB * SR / NT;
+        for (int rbase = 0; rbase < K * L; rbase += SR) {
+            if (r0 >= rbase && r0 < rbase + SR) {
 #pragma unroll
-                for (int i = 0; i < L; ++i) stg[ln * kStagePitch + (r0 - rbase) + i] = y[i];
+                for (int i = 0; i < L; ++i) stg[ln * SP + (r0 - rbase) + i] = y[i];
             }
             __syncthreads();
 #pragma unroll
             for (int k = 0; k < PER; ++k) {
                 int const e = tid + k * NT;
-                int const l = e / kStageRows, u = rbase + e % kStageRows;
+                int const l = e / SR, u = rbase + e % SR;
                 if (l < nl && u < nq)
-                    __builtin_nontemporal_store(stg[l * kStagePitch + e % kStageRows],
+                    __builtin_nontemporal_store(stg[l * SP + e % SR],
                                                 &vwg[loff[LB + l] + uint32_t(tv.q0 + u)]);
             }
             __syncthreads();
@@ -306,19 +307,19 @@ __global__ void __launch_bounds__(LB* K) build_vjp_tiled_kernel(TiledView tv, Bu
     }
 }
 
-template <int K, int L, int KF, int KB, bool FDIV, bool BDIVF>
+template <int K, int L, int KF, int KB, bool FDIV, bool BDIVF, int NT>
 void launch_cfg(BuilderPlan const& p, const double* gcoef, Layout cl, double* gvals, Layout vl, double* gdl,
                 double* gdr, Layout dlay, int64_t nbatch, hipStream_t stream)
 {
     if (p.tiled_t.rs != KF + KB + 1 + (FDIV ? 1 : 0))
         throw Error(DDCX_ERR_INVALID, "row table record length does not match the band shape");
-    constexpr int NT = 1024, LB = NT / K;
+    constexpr int LB = NT / K;
     constexpr int KE = KF > KB ? KF : KB;
     constexpr size_t static_lds =
         size_t(KE * K * LB + kMaxGBlocksVjp * kNbMax * LB + kNbMax * LB) * sizeof(double) + sizeof(BorderLds);
     size_t const table = (size_t(K) * L * p.tiled_t.rs + gamma_lds_doubles(p.tview_t)) * sizeof(double);
     bool const ic = vl.rs == 1 && cl.rs == 1;
-    size_t const stage = ic ? size_t(LB) * kStagePitch * sizeof(double) + 2 * LB * sizeof(uint32_t) : 0;
+    size_t const stage = ic ? size_t(LB) * stage_pitch(K * L) * sizeof(double) + 2 * LB * sizeof(uint32_t) : 0;
     bool const tlds = static_lds + table + stage <= size_t(160 * 1024) && p.tiled_t.rs <= 10;
     size_t const lds = (tlds ? table : 0) + stage;
     int64_t const blocks = (nbatch + LB - 1) / LB;
@@ -341,13 +342,14 @@ void launch_cfg(BuilderPlan const& p, const double* gcoef, Layout cl, double* gv
     hip_check(hipGetLastError(), "build_vjp_tiled_kernel launch");
 }
 
-template <int K, int L>
+// NT threads per workgroup: NT / K lines of K segments of L rows
+template <int K, int L, int NT = 1024>
 bool launch_K(BuilderPlan const& p, const double* gcoef, Layout cl, double* gvals, Layout vl, double* gdl,
               double* gdr, Layout dlay, int64_t nbatch, hipStream_t stream)
 {
     TiledHost const& t = p.tiled_t;
     auto go = [&]<int KF, int KB, bool FD, bool BD>() {
-        launch_cfg<K, L, KF, KB, FD, BD>(p, gcoef, cl, gvals, vl, gdl, gdr, dlay, nbatch, stream);
+        launch_cfg<K, L, KF, KB, FD, BD, NT>(p, gcoef, cl, gvals, vl, gdl, gdr, dlay, nbatch, stream);
         return true;
     };
     if (t.bwd_divfirst) {
@@ -375,7 +377,12 @@ bool launch_build_vjp_tiled(BuilderPlan const& p, const double* gcoef, Layout cl
         return launch_K<32, 32>(p, gcoef, cl, gvals, vl, gdl, gdr, dlay, nbatch, stream);
     if (p.tiled_t.K == 64 && p.tiled_t.L == 32)
         return launch_K<64, 32>(p, gcoef, cl, gvals, vl, gdl, gdr, dlay, nbatch, stream);
-    if (p.tiled_t.K == 32 && p.tiled_t.L == 8) // short lines
+    // short lines
+    if (p.tiled_t.K == 16 && p.tiled_t.L == 8)
+        return launch_K<16, 8, 512>(p, gcoef, cl, gvals, vl, gdl, gdr, dlay, nbatch, stream);
+    if (p.tiled_t.K == 16 && p.tiled_t.L == 9)
+        return launch_K<16, 9, 512>(p, gcoef, cl, gvals, vl, gdl, gdr, dlay, nbatch, stream);
+    if (p.tiled_t.K == 32 && p.tiled_t.L == 8)
         return launch_K<32, 8>(p, gcoef, cl, gvals, vl, gdl, gdr, dlay, nbatch, stream);
     return false;
 }

@@ -124,6 +124,10 @@ __device__ __forceinline__ void getrs_small(int nb, const double* a, const int* 
 // as contiguous runs, and every thread takes its L rows from LDS.
 constexpr int kStageRows = 256;
 constexpr int kStagePitch = kStageRows + 1; // doubles; odd pitch spreads the LDS banks
+// the chunk of a K x L tiling: the whole padded line when it is shorter
+// (K * L rows; the short tilings have K * L = 128 or 144, so the pitch stays odd)
+constexpr int stage_rows(int kl) { return kl < kStageRows ? kl : kStageRows; }
+constexpr int stage_pitch(int kl) { return stage_rows(kl) + 1; }
 
 // The tiled kernels address a tile as a workgroup-uniform base plus 32-bit
 // per-lane offsets; true when every offset of the batch fits.

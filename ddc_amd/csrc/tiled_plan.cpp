@@ -263,11 +263,10 @@ void make_tiled(BuilderPlan& p)
     int const nq = p.prob.q.n;
     if (nq < 256) {
         // short lines (64 to 255 rows): 16 segments of 8 or 9 rows up to 144
-        // rows (512-thread workgroups), else 32 segments of 8 rows; the
-        // adjoint keeps 32 x 8
-        if (nq <= 144) tile_tables(p.prob, 16, nq <= 128 ? 8 : 9, kCarryTol, p.tiled, false, 64);
-        else tile_tables(p.prob, 32, 8, kCarryTol, p.tiled, false, 64);
-        tile_tables(p.prob, 32, 8, kCarryTol, p.tiled_t, true, 64);
+        // rows (512-thread workgroups), else 32 segments of 8 rows
+        int const K = nq <= 144 ? 16 : 32, L = nq <= 128 || nq > 144 ? 8 : 9;
+        tile_tables(p.prob, K, L, kCarryTol, p.tiled, false, 64);
+        tile_tables(p.prob, K, L, kCarryTol, p.tiled_t, true, 64);
     } else {
         int const K = nq <= 1024 ? 32 : 64;
         tile_tables(p.prob, K, 32, kCarryTol, p.tiled);
