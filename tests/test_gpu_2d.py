@@ -418,3 +418,26 @@ def test_eval2d_packed_nonuniform_both(n1, n2, cuda):
     out = torch.full((nb, npts), np.nan, dtype=torch.float64, device=cuda)
     ev(out, torch.from_numpy(coords).to(cuda), torch.from_numpy(coef).to(cuda), packed=True)
     assert rel_err(out.cpu().numpy(), ref) <= TOL
+
+
+@pytest.mark.parametrize("npts", [1, 37, 64, 257])
+def test_eval2d_packed_ragged(npts, cuda):
+    """The packed evaluation with fewer points than a wave or a workgroup, or
+    one more than a workgroup's 256: every point against the oracle, with
+    points outside the domain (Null x periodic)."""
+    import torch
+    import ddc_amd as D
+    import oracle as O
+
+    o1, o2, bs1, bs2, _, _ = _pair(12, 9, seed=3)
+    ev = D.SplineEvaluator2D(D.SplineEvaluator(bs1, D.NullExtrapolationRule(), D.NullExtrapolationRule()),
+                             D.SplineEvaluator(bs2, D.PeriodicExtrapolationRule(), D.PeriodicExtrapolationRule()))
+    nb = 5
+    rng = np.random.default_rng(npts)
+    coef = rng.standard_normal((nb, bs1.size(), bs2.size()))
+    coef[:, :, bs2.nbasis():] = coef[:, :, :3]
+    coords = np.stack([rng.uniform(-0.2, 1.2, (nb, npts)), rng.uniform(-0.5, 1.5, (nb, npts))], axis=-1)
+    ref = O.spline2d_eval(o1, o2, coords, coef)
+    out = torch.full((nb, npts), np.nan, dtype=torch.float64, device=cuda)
+    ev(out, torch.from_numpy(coords).to(cuda), torch.from_numpy(coef).to(cuda), packed=True)
+    assert rel_err(out.cpu().numpy(), ref) <= TOL
