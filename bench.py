@@ -492,6 +492,7 @@ def main():
         other = {}
         other["2a_fused_build_eval"] = measure_fused(D, bs, builder, B, timed, n_k, gen)
         torch.cuda.empty_cache()
+        other["2a_batch_contiguous"] = measure_config(D, bs, builder, B, "batch_contiguous", timed, n_k, gen)
         other["2a_interp_contiguous"] = measure_config(D, bs, builder, B, "interp_contiguous", timed, n_k, gen)
         bs5 = D.UniformBSplines(5, 0.0, 1.0, nc, periodic=True)
         other["2b_uniform_periodic_d5"] = measure_config(D, bs5, D.SplineBuilder(bs5), B, "batch_contiguous", timed,

@@ -200,7 +200,7 @@ __device__ __forceinline__ void buffer_store(double v, __amdgpu_buffer_rsrc_t r,
 // buffers (loads give 0, stores are dropped), rows past the end of the tile
 // repeat the last row, and lines past the batch repeat the last line.
 // Addresses are 32-bit byte offsets into buffers of at most 4 GiB.
-template <int D, int KIND, int NT, int U, int NST, int PR>
+template <int D, int ORDER, int KIND, int NT, int U, int NST, int PR>
 __global__ void __launch_bounds__(NT) eval_stream_kernel(BSplinesView b, Rules1D rules,
                                                            double* __restrict__ out,
                                                            const double* __restrict__ coords, uint32_t prs,
@@ -290,7 +290,7 @@ __global__ void __launch_bounds__(NT) eval_stream_kernel(BSplinesView b, Rules1D
             for (int j = s * PS; j < (s + 1) * PS && j < PR; ++j) pf[j] = buffer_load<kNonTemporal>(rkn, cr(j, ltn));
             double y[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) y[u] = point_value<D, 0, KIND>(bl, knots, rules, xa[u], tl, pitch);
+            for (int u = 0; u < U; ++u) y[u] = point_value<D, ORDER, KIND>(bl, knots, rules, xa[u], tl, pitch);
 #pragma unroll
             for (int u = 0; u < U; ++u) buffer_store<kNonTemporal>(y[u], ro, pt(s * U * ROWS + prow + u * ROWS, lt));
 #pragma unroll
@@ -428,8 +428,123 @@ __global__ void __launch_bounds__(NT) eval_vjp_tiled_kernel(BSplinesView b, Rule
 
 constexpr size_t kMaxLds = 160 * 1024;
 
+// Adjoint of the value form for batch-contiguous arrays on 16-line tiles (as
+// the evaluators; eval_vjp_tiled_kernel holds two tiles and so takes 8
+// lines), with one LDS tile used twice: first the coefficients, for
+// gx = w * f'(x) over all the tile's points; then the coefficient gradient,
+// accumulated over the same points read again and written once.  Same
+// arithmetic per point as eval_vjp_tiled_kernel.
+template <int D, int LB, int NT>
+__global__ void __launch_bounds__(NT) eval_vjp_2pass_kernel(BSplinesView b, Rules1D rules,
+                                                            const double* __restrict__ coords,
+                                                            const double* __restrict__ w, double* __restrict__ gx,
+                                                            Layout pl, int64_t npts, const double* __restrict__ coef,
+                                                            double* __restrict__ gcoef, Layout cl, int64_t nbatch,
+                                                            int ncoef, int pitch, int nknots)
+{
+    extern __shared__ double smem[];
+    double* tile = smem;
+    double* kn = tile + size_t(ncoef) * pitch;
+    uint32_t* cells = reinterpret_cast<uint32_t*>(kn + nknots);
+    constexpr int ROWS = NT / LB;
+    constexpr int U = 4;
+    constexpr int TC = 16;
+    int const tid = threadIdx.x;
+    int const ln = tid % LB;
+    int const prow = tid / LB;
+    int64_t const line0 = int64_t(blockIdx.x) * LB;
+    int64_t const line = line0 + ln;
+    bool const active = line < nbatch;
+    int64_t const lc = active ? line : 0;
+    int64_t const cb = lay(cl, 0, lc);
+    int64_t const pb = lay(pl, 0, lc);
+    for (int c0 = prow; c0 < ncoef; c0 += TC * ROWS) {
+        double tv[TC];
+#pragma unroll
+        for (int k = 0; k < TC; ++k) {
+            int const r = c0 + k * ROWS;
+            tv[k] = r < ncoef ? coef[cb + int64_t(r) * cl.rs] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < TC; ++k) {
+            int const r = c0 + k * ROWS;
+            if (r < ncoef) tile[r * pitch + ln] = tv[k];
+        }
+    }
+    if (!b.uniform) {
+        for (int k = tid; k < nknots; k += NT) kn[k] = b.knots[k];
+        for (int k = tid; k < b.nbucket; k += NT) cells[k] = b.cells[k];
+    }
+    __syncthreads();
+    const double* knots = b.uniform ? nullptr : kn;
+    BSplinesView bl = b;
+    bl.cells = b.nbucket > 0 ? cells : nullptr;
+    double* tl = tile + ln;
+    // pass 1: gx from the coefficient tile
+    if (active) {
+        for (int64_t i = prow; i < npts; i += U * ROWS) {
+            double xa[U], wa[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                int64_t const ii = i + int64_t(u) * ROWS;
+                xa[u] = ii < npts ? coords[pb + ii * pl.rs] : 0.0;
+                wa[u] = ii < npts ? w[pb + ii * pl.rs] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                int64_t const ii = i + int64_t(u) * ROWS;
+                if (ii >= npts) continue;
+                double x = xa[u];
+                bool outside;
+                if (!apply_bc(bl, rules, x, outside)) {
+                    gx[pb + ii * pl.rs] = 0.0;
+                    continue;
+                }
+                double der[(D + 1) * 2];
+                int const jmin = eval_basis_and_n_derivs<D, 1, true>(bl, knots, x, der);
+                double dy = 0.0;
+#pragma unroll
+                for (int k = 0; k < D + 1; ++k) dy += tl[(jmin + k) * pitch] * der[k * 2 + 1];
+                gx[pb + ii * pl.rs] = outside ? 0.0 : wa[u] * dy;
+            }
+        }
+    }
+    __syncthreads();
+    for (int k = tid; k < ncoef * pitch; k += NT) tile[k] = 0.0;
+    __syncthreads();
+    // pass 2: the coefficient gradient, the points read again
+    if (active) {
+        for (int64_t i = prow; i < npts; i += U * ROWS) {
+            double xa[U], wa[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                int64_t const ii = i + int64_t(u) * ROWS;
+                xa[u] = ii < npts ? coords[pb + ii * pl.rs] : 0.0;
+                wa[u] = ii < npts ? w[pb + ii * pl.rs] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                int64_t const ii = i + int64_t(u) * ROWS;
+                if (ii >= npts) continue;
+                double x = xa[u];
+                bool outside;
+                if (!apply_bc(bl, rules, x, outside)) continue;
+                double v[D + 1];
+                int const jmin = eval_basis<D, true>(bl, knots, x, v);
+#pragma unroll
+                for (int k = 0; k < D + 1; ++k) atomicAdd(&tl[(jmin + k) * pitch], wa[u] * v[k]);
+            }
+        }
+    }
+    __syncthreads();
+    if (active)
+        for (int r = prow; r < ncoef; r += ROWS) gcoef[cb + int64_t(r) * cl.rs] = tile[r * pitch + ln];
+}
 
-// eval_stream_kernel for the value form when its shape fits: plain
+
+
+// eval_stream_kernel for the value form (and the first derivative of uniform
+// bases up to degree 3) when its shape fits: plain
 // batch-contiguous points and coefficients (element (row, line) at row * rs +
 // line), npts <= NST * U * ROWS, size() <= PR * ROWS, buffers below 4 GiB
 // including the offsets of the point slots past the end of a line.  Uniform
@@ -437,7 +552,7 @@ constexpr size_t kMaxLds = 160 * 1024;
 // uniform bases (whose cell search wants more waves to hide its dependent
 // LDS reads): 768 threads, 4 points per thread and stage, 22 prefetch rows,
 // degrees up to 3.
-template <int D>
+template <int D, int ORDER>
 bool launch_stream(BSplinesPlan const& bs, Rules1D rules, double* out, const double* coords, Layout pl,
                    int64_t npts, const double* coef, Layout cl, int64_t nbatch, hipStream_t stream)
 {
@@ -466,7 +581,7 @@ bool launch_stream(BSplinesPlan const& bs, Rules1D rules, double* out, const dou
         if ((uint64_t(ncoef) * uint64_t(cl.rs) + uint64_t(nbatch)) * 8 >= lim) return false;
         uint32_t const pbytes = uint32_t((uint64_t(npts - 1) * uint64_t(pl.rs) + uint64_t(nbatch)) * 8);
         uint32_t const cbytes = uint32_t((uint64_t(ncoef - 1) * uint64_t(cl.rs) + uint64_t(nbatch)) * 8);
-        auto kern = eval_stream_kernel<D, KIND, NT, U, NST, PR>;
+        auto kern = eval_stream_kernel<D, ORDER, KIND, NT, U, NST, PR>;
         hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)),
                   "hipFuncSetAttribute");
@@ -475,12 +590,20 @@ bool launch_stream(BSplinesPlan const& bs, Rules1D rules, double* out, const dou
         hip_check(hipGetLastError(), "eval_stream_kernel launch");
         return true;
     };
-    if (b.uniform)
-        return npts <= 2 * 8 * 32 ? go.template operator()<kUniformBasis, 512, 8, 2, 33>()
-                                  : go.template operator()<kUniformBasis, 512, 8, 4, 33>();
-    if constexpr (D <= 3) // higher degrees spill in this form and keep eval_tiled_kernel
-        return npts <= 4 * 4 * 48 ? go.template operator()<kNonUniformBasis, 768, 4, 4, 22>()
-                                  : go.template operator()<kNonUniformBasis, 768, 4, 6, 22>();
+    if constexpr (ORDER == 0) {
+        if (b.uniform)
+            return npts <= 2 * 8 * 32 ? go.template operator()<kUniformBasis, 512, 8, 2, 33>()
+                                      : go.template operator()<kUniformBasis, 512, 8, 4, 33>();
+        if constexpr (D <= 3) // higher degrees spill in this form and keep eval_tiled_kernel
+            return npts <= 4 * 4 * 48 ? go.template operator()<kNonUniformBasis, 768, 4, 4, 22>()
+                                      : go.template operator()<kNonUniformBasis, 768, 4, 6, 22>();
+    } else if constexpr (D <= 3) {
+        // the first derivative's basis takes more registers: 4 points per
+        // thread and stage; uniform bases only (the non-uniform form spills)
+        if (b.uniform)
+            return npts <= 4 * 4 * 32 ? go.template operator()<kUniformBasis, 512, 4, 4, 33>()
+                                      : go.template operator()<kUniformBasis, 512, 4, 8, 33>();
+    }
     return false;
 }
 
@@ -513,8 +636,9 @@ bool launch_orders(BSplinesPlan const& bs, Rules1D rules, double* out, const dou
     auto same = [](Layout const& a, Layout const& c) {
         return a.rs == c.rs && a.inner == c.inner && a.is == c.is && a.os == c.os;
     };
-    if constexpr (ORDER == 0) {
-        if (same(xl, pl) && launch_stream<D>(bs, rules, out, coords, pl, npts, coef, cl, nbatch, stream)) return true;
+    if constexpr (ORDER <= 1) {
+        if (same(xl, pl) && launch_stream<D, ORDER>(bs, rules, out, coords, pl, npts, coef, cl, nbatch, stream))
+            return true;
     }
     if (fits(16, 17)) return go.template operator()<16, 1024>(17);
     if (fits(16, 16)) return go.template operator()<16, 1024>(16);
@@ -583,6 +707,30 @@ bool launch_eval_vjp_tiled(BSplinesPlan const& bs, Rules1D rules, const double* 
     else return false;
     bool const line_major = pl.rs == 1;
     bool launched = false;
+    if (!line_major && nbatch >= 16 * 2 * int64_t(cu_count()) && b.uniform && b.degree <= 3) {
+        // batch-contiguous, uniform bases up to degree 3, at least two tiles
+        // per CU: 16-line tiles, one LDS tile used for both passes (measured
+        // faster than the 8-line kernel there; with the dearer basis of
+        // non-uniform or quintic splines, computing it twice costs more than
+        // the wider tile gains, and small batches want the 8-line kernel's
+        // twice as many workgroups)
+        size_t const lds2 = size_t(ncoef) * 17 * sizeof(double) + extra;
+        if (lds2 <= kMaxLds) {
+            dispatch_degree(b.degree, [&]<int D>() {
+                constexpr int LB = 16, NT = 1024;
+                auto kern = eval_vjp_2pass_kernel<D, LB, NT>;
+                hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, int(lds2)),
+                          "hipFuncSetAttribute");
+                int64_t const blocks = (nbatch + LB - 1) / LB;
+                hipLaunchKernelGGL(kern, dim3(unsigned(blocks)), dim3(NT), lds2, stream, b, rules, coords, w, gx, pl,
+                                   npts, coef, gcoef, cl, nbatch, ncoef, 17, nknots);
+                hip_check(hipGetLastError(), "eval_vjp_2pass_kernel launch");
+                launched = true;
+            });
+            if (launched) return true;
+        }
+    }
     dispatch_degree(b.degree, [&]<int D>() {
         constexpr int LB = 8, NT = 1024;
         auto kern = line_major ? eval_vjp_tiled_kernel<D, LB, NT, true> : eval_vjp_tiled_kernel<D, LB, NT, false>;

@@ -158,10 +158,10 @@ def test_eval_tiled(case, layout, cuda):
 @pytest.mark.parametrize("npts", [300, 1000])
 def test_eval_stream(case, npts, cuda):
     """Persistent evaluator for uniform bases on large batch-contiguous batches
-    (at least two 16-line tiles per CU): 8200 lines, so the last tile is
-    ragged, and point counts that leave the last stage of each tile partly
-    empty; extrapolated points on both sides.  Checked on 192 lines spread
-    over the batch (both ends included)."""
+    (at least two 16-line tiles per CU), values and deriv(1): 8200 lines, so
+    the last tile is ragged, and point counts that leave the last stage of
+    each tile partly empty; extrapolated points on both sides.  Checked on
+    192 lines spread over the batch (both ends included)."""
     import torch
     import ddc_amd as D
 
@@ -183,6 +183,16 @@ def test_eval_stream(case, npts, cuda):
     lines = np.unique(np.concatenate([np.arange(64), np.arange(nbatch - 64, nbatch),
                                       rng.choice(nbatch, 64, replace=False)]))
     ref = orc.eval(x[:, lines], coef[:, lines], extrap=("constant", "null"))
+    assert not np.isnan(got).any()
+    assert rel_err(got[:, lines], ref) <= TOL
+    # deriv(1) takes the same persistent form for degrees up to 3 (no rule
+    # applies: points inside the domain)
+    xin = np.clip(x, orc.rmin, orc.rmax)
+    full[:npts] = np.nan
+    ev.deriv(1, out, torch.from_numpy(xin).to(cuda), torch.from_numpy(coef).to(cuda))
+    assert bool((full[npts:] == -7.0).all())
+    got = out.cpu().numpy()
+    ref = orc.eval(xin[:, lines], coef[:, lines], deriv=1)
     assert not np.isnan(got).any()
     assert rel_err(got[:, lines], ref) <= TOL
 
