@@ -147,6 +147,43 @@ def _timer(stream):
     return timed
 
 
+def pattern_ceiling(src, dst, lb, timed, n):
+    """GB/s of a plain copy of src -> dst ((rows, lines) batch-contiguous
+    FP64, same shape) in tiles of lb lines by all rows: the access pattern of
+    the tiled kernels, measured by the probe library built beside libddcx
+    (None when it is not built)."""
+    import ctypes
+
+    import torch
+
+    path = os.path.join(ROOT, "ddc_amd", "libddcx_probe.so")
+    if not os.path.exists(path):
+        return None
+    fn = ctypes.CDLL(path).ddcx_probe_tile_copy
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ctypes.c_int,
+                   ctypes.c_int, ctypes.c_void_p]
+    rows, lines = src.shape
+    assert dst.shape == src.shape and src.is_contiguous() and dst.is_contiguous()
+    grid = min((lines + lb - 1) // lb, 2 * torch.cuda.get_device_properties(src.device).multi_processor_count)
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def run():
+        if fn(src.data_ptr(), dst.data_ptr(), rows, lines, lines, lb, grid, stream) != 0:
+            raise RuntimeError("ddcx_probe_tile_copy failed")
+
+    run()
+    return 2 * src.numel() * 8 / timed(run, n) / 1e9
+
+
+def _ceiling(gbps, lb, achieved):
+    if gbps is None:
+        return None
+    return {"pattern": f"copy in batch-contiguous tiles of {lb} lines x all rows (row stride = the batch), "
+                       "the dominant kernel's access pattern", "GBps": round(gbps, 1),
+            "frac": round(achieved / gbps, 4)}
+
+
 def _line_values(xg, B, gen, axis_interp_first=True):
     """f(x, y) = sin(2 pi x k_y) + 0.1 N(0, 1), k_y in {1..8}."""
     import numpy as np
@@ -455,6 +492,9 @@ def main():
     # calibration on this box: a plain device copy of the same size as vals
     t_copy = timed(lambda: out.copy_(vals), n_k)
     copy_gbps = 2 * vals.numel() * 8 / t_copy / 1e9
+    # the tiled kernels' access pattern (16-line tiles for the evaluator,
+    # 32-line tiles for the build) in a plain copy of the same array
+    tile_gbps = {lb: pattern_ceiling(vals, out, lb, timed, n_k) for lb in (16, 32)}
     b_build, b_eval = algorithmic_bytes(m, B, ncoef, P)
     dominant = "build" if t_build >= t_eval else "eval"
     t_dom = max(t_build, t_eval)
@@ -479,9 +519,13 @@ def main():
                    "eval_points": P, "precision": "fp64"},
         "kernels": {"build_us": round(t_build * 1e6, 2), "eval_us": round(t_eval * 1e6, 2),
                     "build_GBps": round(b_build / t_build / 1e9, 1), "eval_GBps": round(b_eval / t_eval / 1e9, 1),
-                    "device_copy_GBps": round(copy_gbps, 1)},
+                    "device_copy_GBps": round(copy_gbps, 1),
+                    "tile_copy_16_lines_GBps": None if tile_gbps[16] is None else round(tile_gbps[16], 1),
+                    "tile_copy_32_lines_GBps": None if tile_gbps[32] is None else round(tile_gbps[32], 1)},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic},
+                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "pattern_ceiling": _ceiling(tile_gbps[16 if dominant == "eval" else 32],
+                                                 16 if dominant == "eval" else 32, ach)},
         "cpu_baseline": None,
     }
     if not args.headline_only:
