@@ -641,33 +641,118 @@ public:
     }
 };
 
+// The derivative inputs of SplineBuilder3D::operator() with Hermite closures
+// (spline_builder_3d.hpp:779-1090): layout_right device arrays named as the
+// reference's arguments, nullptr where the closures take none.  With e_k =
+// max(nbv_lower, nbv_upper) of dimension k's builder: derivs_*1 (nbatch, e1,
+// n2, n3), derivs_*2 (nbatch, n1, e2, n3), derivs_*3 (nbatch, n1, n2, e3);
+// mixed12[a][b] = mixed_derivs_{a}1_{b}2 (nbatch, e1, e2, n3), mixed23[a][b]
+// = mixed_derivs_{a}2_{b}3 (nbatch, n1, e2, e3), mixed13[a][b] =
+// mixed_derivs_{a}1_{b}3 (nbatch, e1, n2, e3), mixed123[a][b][c] =
+// mixed_derivs_{a}1_{b}2_{c}3 (nbatch, e1, e2, e3); index 0 is min, 1 max.
+struct SplineBuilder3DDerivs {
+    const double* derivs_min1 = nullptr;
+    const double* derivs_max1 = nullptr;
+    const double* derivs_min2 = nullptr;
+    const double* derivs_max2 = nullptr;
+    const double* derivs_min3 = nullptr;
+    const double* derivs_max3 = nullptr;
+    const double* mixed12[2][2] = {};
+    const double* mixed23[2][2] = {};
+    const double* mixed13[2][2] = {};
+    const double* mixed123[2][2][2] = {};
+};
+
 // SplineBuilder3D<...> (spline_builder_3d.hpp:45): one batched 1D build per
 // dimension, dimension 1 first, on layout_right arrays vals (nbatch, n1, n2,
 // n3) and spline (nbatch, s1, s2, s3), through two caller-provided device
 // workspaces of work1_size() and work2_size() doubles (nothing is allocated).
-// Hermite closures are not part of this build.
+// With Hermite closures the call takes the derivative inputs and one
+// workspace of hermite_workspace_size() doubles, and composes the builds as
+// the reference does: the dimension-2 and dimension-3 derivatives and the
+// 2-3 cross derivatives are built along dimension 1 (with the mixed
+// derivatives involving dimension 1 as their Hermite data), the values along
+// dimension 1, then SplineBuilder2D over dimensions 2 and 3.
 class SplineBuilder3D {
 public:
     SplineBuilder3D(SplineBuilder const& b1, SplineBuilder const& b2, SplineBuilder const& b3, int64_t size1,
                     int64_t size2, int64_t size3)
         : b_{&b1, &b2, &b3}, s_{size1, size2, size3}
     {
-        for (auto const* b : b_)
-            if (b->info().nbv_lower || b->info().nbv_upper)
-                throw std::runtime_error("ddcx: 3D Hermite closures are not part of this build");
     }
     int64_t work1_size(int64_t nbatch) const { return nbatch * s_[0] * n(1) * n(2); }
     int64_t work2_size(int64_t nbatch) const { return nbatch * s_[0] * s_[1] * n(2); }
+    // closures without derivative inputs
     void operator()(double* spline, const double* vals, int64_t nbatch, double* work1, double* work2,
                     void* stream = nullptr) const
     {
+        if (hermite()) throw std::runtime_error("ddcx: Hermite closures take the derivative inputs");
         pass(0, work1, {nbatch, s_[0], n(1), n(2)}, vals, {nbatch, n(0), n(1), n(2)}, stream);
         pass(1, work2, {nbatch, s_[0], s_[1], n(2)}, work1, {nbatch, s_[0], n(1), n(2)}, stream);
         pass(2, spline, {nbatch, s_[0], s_[1], s_[2]}, work2, {nbatch, s_[0], s_[1], n(2)}, stream);
     }
+    int64_t hermite_workspace_size(int64_t nbatch) const
+    {
+        int64_t const e2 = e(1), e3 = e(2), nb1 = nbatch * s_[0];
+        int64_t const w = nb1 * n(1) * n(2) + 2 * nb1 * e2 * n(2) + 2 * nb1 * n(1) * e3 + 4 * nb1 * e2 * e3;
+        return w + SplineBuilder2D(*b_[1], *b_[2], s_[1]).workspace_size(nb1);
+    }
+    // any closures, with the derivative inputs they take
+    void operator()(double* spline, const double* vals, int64_t nbatch, double* work, SplineBuilder3DDerivs const& d,
+                    void* stream = nullptr) const
+    {
+        int64_t const n1 = n(0), n2 = n(1), n3 = n(2), s1 = s_[0], e1 = e(0), e2 = e(1), e3 = e(2);
+        bool const h2[2] = {b_[1]->info().nbv_lower > 0, b_[1]->info().nbv_upper > 0};
+        bool const h3[2] = {b_[2]->info().nbv_lower > 0, b_[2]->info().nbv_upper > 0};
+        double* p = work;
+        auto take = [&](int64_t count) {
+            double* q = p;
+            p += count;
+            return q;
+        };
+        double* spline1 = take(nbatch * s1 * n2 * n3);
+        double* d2[2] = {take(nbatch * s1 * e2 * n3), take(nbatch * s1 * e2 * n3)};
+        double* d3[2] = {take(nbatch * s1 * n2 * e3), take(nbatch * s1 * n2 * e3)};
+        double* m23[2][2];
+        for (auto& row : m23)
+            for (auto& m : row) m = take(nbatch * s1 * e2 * e3);
+        // one dimension-1 build: in (nbatch, n1, a, b) -> out (nbatch, s1, a, b)
+        auto along1 = [&](double* out, const double* in, int64_t a, int64_t b, const double* lo, const double* hi) {
+            if (!in) throw std::runtime_error("ddcx: a derivative input the Hermite closures take is missing");
+            int64_t nb = 0;
+            ddcx_layout const l_out = batched_layout({nbatch, s1, a, b}, 1, &nb);
+            ddcx_layout const l_in = batched_layout({nbatch, n1, a, b}, 1);
+            ddcx_layout const l_d = batched_layout({nbatch, std::max<int64_t>(e1, 1), a, b}, 1);
+            check(ddcx_builder_build(b_[0]->handle(), out, l_out, in, l_in, lo, hi, l_d, nb, stream));
+        };
+        const double* dk2[2] = {d.derivs_min2, d.derivs_max2};
+        const double* dk3[2] = {d.derivs_min3, d.derivs_max3};
+        for (int sd = 0; sd < 2; ++sd)
+            if (h2[sd]) along1(d2[sd], dk2[sd], e2, n3, d.mixed12[0][sd], d.mixed12[1][sd]);
+        for (int sd = 0; sd < 2; ++sd)
+            if (h3[sd]) along1(d3[sd], dk3[sd], n2, e3, d.mixed13[0][sd], d.mixed13[1][sd]);
+        for (int a2 = 0; a2 < 2; ++a2)
+            for (int a3 = 0; a3 < 2; ++a3)
+                if (h2[a2] && h3[a3])
+                    along1(m23[a2][a3], d.mixed23[a2][a3], e2, e3, d.mixed123[0][a2][a3], d.mixed123[1][a2][a3]);
+        along1(spline1, vals, n2, n3, d.derivs_min1, d.derivs_max1);
+        auto opt = [](bool on, double* q) { return on ? static_cast<const double*>(q) : nullptr; };
+        SplineBuilder2D(*b_[1], *b_[2], s_[1])(spline, spline1, nbatch * s1, p, opt(h2[0], d2[0]), opt(h2[1], d2[1]),
+                                               opt(h3[0], d3[0]), opt(h3[1], d3[1]),
+                                               opt(h2[0] && h3[0], m23[0][0]), opt(h2[1] && h3[0], m23[1][0]),
+                                               opt(h2[0] && h3[1], m23[0][1]), opt(h2[1] && h3[1], m23[1][1]),
+                                               stream);
+    }
 
 private:
     int64_t n(int k) const { return b_[k]->npoints(); }
+    int64_t e(int k) const { return std::max(b_[k]->info().nbv_lower, b_[k]->info().nbv_upper); }
+    bool hermite() const
+    {
+        for (auto const* b : b_)
+            if (b->info().nbv_lower || b->info().nbv_upper) return true;
+        return false;
+    }
     void pass(int k, double* out, std::vector<int64_t> const& eo, const double* in, std::vector<int64_t> const& ei,
               void* stream) const
     {

@@ -5,9 +5,11 @@
 // test test_gpu_cpp_facade.py checks them against the oracle.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -273,6 +275,60 @@ void case_2d_hermite()
     for (int k = 0; k < 9; ++k) dump(gn[k], g9[k]->get());
 }
 
+// 3D with Hermite closures in every dimension: non-uniform cubic (9 cells) x
+// uniform quintic (8 cells) x uniform quartic (7 cells), 2 batch entries;
+// SplineBuilder3D with every derivative, edge and corner input
+void case_3d_hermite()
+{
+    int const NB = 2;
+    Lcg r{6};
+    std::vector<double> br(10);
+    br[0] = 0.0;
+    for (int i = 1; i <= 9; ++i) br[i] = br[i - 1] + 0.5 + r.next();
+    for (auto& b : br) b /= br[9];
+    auto const H = ddcx::SplineBuilderClosure::HERMITE;
+    ddcx::NonUniformBSplines bs1(3, br, false);
+    ddcx::UniformBSplines bs2(5, 0.0, 1.0, 8, false);
+    ddcx::UniformBSplines bs3(4, 0.0, 1.0, 7, false);
+    ddcx::SplineBuilder b1(bs1, H, H), b2(bs2, H, H), b3(bs3, H, H);
+    ddcx::SplineBuilder3D builder(b1, b2, b3, bs1.size(), bs2.size(), bs3.size());
+    int64_t const n1 = b1.npoints(), n2 = b2.npoints(), n3 = b3.npoints();
+    int64_t const e1 = std::max(b1.info().nbv_lower, b1.info().nbv_upper);
+    int64_t const e2 = std::max(b2.info().nbv_lower, b2.info().nbv_upper);
+    int64_t const e3 = std::max(b3.info().nbv_lower, b3.info().nbv_upper);
+    std::vector<std::unique_ptr<DeviceArray>> keep;
+    auto input = [&](std::string const& name, int64_t a, int64_t b, int64_t c) {
+        std::vector<double> v(static_cast<size_t>(NB * a * b * c));
+        for (auto& x : v) x = r.next() - 0.5;
+        dump("3dh_" + name, v);
+        keep.push_back(std::make_unique<DeviceArray>(v));
+        return static_cast<const double*>(keep.back()->p);
+    };
+    char const* sd[2] = {"min", "max"};
+    const double* vals = input("vals", n1, n2, n3);
+    ddcx::SplineBuilder3DDerivs d;
+    d.derivs_min1 = input("derivs_min1", e1, n2, n3);
+    d.derivs_max1 = input("derivs_max1", e1, n2, n3);
+    d.derivs_min2 = input("derivs_min2", n1, e2, n3);
+    d.derivs_max2 = input("derivs_max2", n1, e2, n3);
+    d.derivs_min3 = input("derivs_min3", n1, n2, e3);
+    d.derivs_max3 = input("derivs_max3", n1, n2, e3);
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) {
+            d.mixed12[a][b] = input(std::string("mixed_derivs_") + sd[a] + "1_" + sd[b] + "2", e1, e2, n3);
+            d.mixed23[a][b] = input(std::string("mixed_derivs_") + sd[a] + "2_" + sd[b] + "3", n1, e2, e3);
+            d.mixed13[a][b] = input(std::string("mixed_derivs_") + sd[a] + "1_" + sd[b] + "3", e1, n2, e3);
+            for (int c = 0; c < 2; ++c)
+                d.mixed123[a][b][c] =
+                    input(std::string("mixed_derivs_") + sd[a] + "1_" + sd[b] + "2_" + sd[c] + "3", e1, e2, e3);
+        }
+    DeviceArray coef(static_cast<size_t>(NB * bs1.size() * bs2.size() * bs3.size()));
+    DeviceArray work(static_cast<size_t>(builder.hermite_workspace_size(NB)));
+    builder(coef.p, vals, NB, work.p, d);
+    dump("3dh_breaks", br);
+    dump("3dh_coef", coef.get());
+}
+
 } // namespace
 
 int main(int argc, char** argv)
@@ -298,6 +354,9 @@ int main(int argc, char** argv)
         std::printf("facade_device: 2D Hermite\n");
         std::fflush(stdout);
         case_2d_hermite();
+        std::printf("facade_device: 3D Hermite\n");
+        std::fflush(stdout);
+        case_3d_hermite();
     } catch (std::exception const& e) {
         std::fprintf(stderr, "facade_device: %s\n", e.what());
         return 1;

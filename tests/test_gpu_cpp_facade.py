@@ -2,7 +2,8 @@
 a compiled C++ program, builds and evaluates through SplineBuilder /
 SplineEvaluator (both layouts, periodic and constant rules with eval_pos)
 and SplineBuilder2D / SplineEvaluator2D (values, deriv_dim_1, adjoint, and
-the builder with Hermite closures and its adjoint), and
+the builder with Hermite closures and its adjoint) and SplineBuilder3D with
+Hermite closures, and
 writes its inputs and outputs; they are checked here against the oracle at
 the parity bars (1e-13; 1e-12 for accumulated coefficient gradients)."""
 import os
@@ -117,3 +118,49 @@ def test_facade_2d_hermite(outputs):
         J = O.spline2d_build(o1, o2, *[args[j] for j in order]).reshape(m, -1)
         got = outputs("2dh_g" + k).reshape(NB, m)
         assert rel_err(got, gs @ J.T) <= (1e-13 if k == "vals" else 1e-12), k
+
+
+def test_facade_3d_hermite(outputs, cuda):
+    """SplineBuilder3D with Hermite closures in all three dimensions through
+    the facade (every derivative, edge and corner input): the coefficients
+    equal the Python builder's on the same inputs bit for bit (the same 1D
+    builds on the same C ABI) and the oracle's 3D restatement to 1e-13."""
+    import torch
+    import ddc_amd as D
+
+    NB = 2
+    H = O.HERMITE
+    ss = [O.Spline(3, False, H, H, breaks=outputs("3dh_breaks")),
+          O.Spline(5, True, H, H, ncells=8, rmin=0.0, rmax=1.0),
+          O.Spline(4, True, H, H, ncells=7, rmin=0.0, rmax=1.0)]
+    n = [s.npts for s in ss]
+    e = [max(s.nbv_l, s.nbv_r) for s in ss]
+    shape = {"vals": n}
+    for k in range(3):
+        for sd in ("min", "max"):
+            sh = list(n)
+            sh[k] = e[k]
+            shape[f"derivs_{sd}{k + 1}"] = sh
+    for (a, b) in ((0, 1), (1, 2), (0, 2)):
+        for sa in ("min", "max"):
+            for sb in ("min", "max"):
+                sh = list(n)
+                sh[a], sh[b] = e[a], e[b]
+                shape[f"mixed_derivs_{sa}{a + 1}_{sb}{b + 1}"] = sh
+    for s1 in ("min", "max"):
+        for s2 in ("min", "max"):
+            for s3 in ("min", "max"):
+                shape[f"mixed_derivs_{s1}1_{s2}2_{s3}3"] = e
+    x = {k: outputs("3dh_" + k).reshape([NB] + list(sh)) for k, sh in shape.items()}
+    got = outputs("3dh_coef").reshape([NB] + [s.size for s in ss])
+    derivs = {k: v for k, v in x.items() if k != "vals"}
+    ref = O.spline3d_build(*ss, x["vals"], derivs)
+    assert rel_err(got, ref) <= 1e-13
+    G = D.SplineBuilderClosure.HERMITE
+    bl = [D.SplineBuilder(D.NonUniformBSplines(3, outputs("3dh_breaks")), G, G),
+          D.SplineBuilder(D.UniformBSplines(5, 0.0, 1.0, 8), G, G),
+          D.SplineBuilder(D.UniformBSplines(4, 0.0, 1.0, 7), G, G)]
+    coef = torch.full(got.shape, np.nan, dtype=torch.float64, device=cuda)
+    t = {k: torch.from_numpy(v).to(cuda) for k, v in derivs.items()}
+    D.SplineBuilder3D(*bl)(coef, torch.from_numpy(x["vals"]).to(cuda), **t)
+    assert np.array_equal(coef.cpu().numpy(), got)
