@@ -650,6 +650,22 @@ public:
 // = mixed_derivs_{a}2_{b}3 (nbatch, n1, e2, e3), mixed13[a][b] =
 // mixed_derivs_{a}1_{b}3 (nbatch, e1, n2, e3), mixed123[a][b][c] =
 // mixed_derivs_{a}1_{b}2_{c}3 (nbatch, e1, e2, e3); index 0 is min, 1 max.
+// The gradients SplineBuilder3D::build_vjp returns with Hermite closures,
+// one per derivative input of SplineBuilder3DDerivs and with the same extents
+// (overwritten; nullptr where the closures take no such input).
+struct SplineBuilder3DGrads {
+    double* derivs_min1 = nullptr;
+    double* derivs_max1 = nullptr;
+    double* derivs_min2 = nullptr;
+    double* derivs_max2 = nullptr;
+    double* derivs_min3 = nullptr;
+    double* derivs_max3 = nullptr;
+    double* mixed12[2][2] = {};
+    double* mixed23[2][2] = {};
+    double* mixed13[2][2] = {};
+    double* mixed123[2][2][2] = {};
+};
+
 struct SplineBuilder3DDerivs {
     const double* derivs_min1 = nullptr;
     const double* derivs_max1 = nullptr;
@@ -744,9 +760,78 @@ public:
                                                stream);
     }
 
+    // adjoint of the form without derivative inputs: the three transposed
+    // passes in reverse order, through the same two workspaces
+    void build_vjp(const double* gspline, double* gvals, int64_t nbatch, double* work1, double* work2,
+                   void* stream = nullptr) const
+    {
+        if (hermite()) throw std::runtime_error("ddcx: Hermite closures return the derivative gradients");
+        pass_vjp(2, gspline, {nbatch, s_[0], s_[1], s_[2]}, work2, {nbatch, s_[0], s_[1], n(2)}, stream);
+        pass_vjp(1, work2, {nbatch, s_[0], s_[1], n(2)}, work1, {nbatch, s_[0], n(1), n(2)}, stream);
+        pass_vjp(0, work1, {nbatch, s_[0], n(1), n(2)}, gvals, {nbatch, n(0), n(1), n(2)}, stream);
+    }
+    // adjoint of the Hermite form: gvals and the derivative gradients (every
+    // one the closures take must be given), through a workspace of
+    // hermite_workspace_size() doubles; operator()'s builds in reverse order
+    void build_vjp(const double* gspline, double* gvals, int64_t nbatch, double* work, SplineBuilder3DGrads const& g,
+                   void* stream = nullptr) const
+    {
+        int64_t const n1 = n(0), n2 = n(1), n3 = n(2), s1 = s_[0], e1 = e(0), e2 = e(1), e3 = e(2);
+        bool const h2[2] = {b_[1]->info().nbv_lower > 0, b_[1]->info().nbv_upper > 0};
+        bool const h3[2] = {b_[2]->info().nbv_lower > 0, b_[2]->info().nbv_upper > 0};
+        double* p = work;
+        auto take = [&](int64_t count) {
+            double* q = p;
+            p += count;
+            return q;
+        };
+        double* gs1 = take(nbatch * s1 * n2 * n3);
+        double* gd2[2] = {take(nbatch * s1 * e2 * n3), take(nbatch * s1 * e2 * n3)};
+        double* gd3[2] = {take(nbatch * s1 * n2 * e3), take(nbatch * s1 * n2 * e3)};
+        double* gm23[2][2];
+        for (auto& row : gm23)
+            for (auto& m : row) m = take(nbatch * s1 * e2 * e3);
+        auto opt = [](bool on, double* q) { return on ? q : nullptr; };
+        SplineBuilder2D(*b_[1], *b_[2], s_[1]).build_vjp(gspline, gs1, nbatch * s1, p, opt(h2[0], gd2[0]),
+                                                         opt(h2[1], gd2[1]), opt(h3[0], gd3[0]), opt(h3[1], gd3[1]),
+                                                         opt(h2[0] && h3[0], gm23[0][0]),
+                                                         opt(h2[1] && h3[0], gm23[1][0]),
+                                                         opt(h2[0] && h3[1], gm23[0][1]),
+                                                         opt(h2[1] && h3[1], gm23[1][1]), stream);
+        // one transposed dimension-1 build: gin (nbatch, s1, a, b) -> gout (nbatch, n1, a, b)
+        auto along1 = [&](const double* gin, double* gout, int64_t a, int64_t b, double* glo, double* ghi) {
+            if (!gout) throw std::runtime_error("ddcx: a gradient of a derivative input the closures take is missing");
+            int64_t nb = 0;
+            ddcx_layout const l_in = batched_layout({nbatch, s1, a, b}, 1, &nb);
+            ddcx_layout const l_out = batched_layout({nbatch, n1, a, b}, 1);
+            ddcx_layout const l_d = batched_layout({nbatch, std::max<int64_t>(e1, 1), a, b}, 1);
+            check(ddcx_builder_build_vjp(b_[0]->handle(), gin, l_in, gout, l_out, glo, ghi, l_d, nb, stream));
+        };
+        along1(gs1, gvals, n2, n3, g.derivs_min1, g.derivs_max1);
+        double* gk2[2] = {g.derivs_min2, g.derivs_max2};
+        double* gk3[2] = {g.derivs_min3, g.derivs_max3};
+        for (int sd = 0; sd < 2; ++sd)
+            if (h2[sd]) along1(gd2[sd], gk2[sd], e2, n3, g.mixed12[0][sd], g.mixed12[1][sd]);
+        for (int sd = 0; sd < 2; ++sd)
+            if (h3[sd]) along1(gd3[sd], gk3[sd], n2, e3, g.mixed13[0][sd], g.mixed13[1][sd]);
+        for (int a2 = 0; a2 < 2; ++a2)
+            for (int a3 = 0; a3 < 2; ++a3)
+                if (h2[a2] && h3[a3])
+                    along1(gm23[a2][a3], g.mixed23[a2][a3], e2, e3, g.mixed123[0][a2][a3], g.mixed123[1][a2][a3]);
+    }
+
 private:
     int64_t n(int k) const { return b_[k]->npoints(); }
     int64_t e(int k) const { return std::max(b_[k]->info().nbv_lower, b_[k]->info().nbv_upper); }
+    void pass_vjp(int k, const double* gin, std::vector<int64_t> const& ei, double* gout,
+                  std::vector<int64_t> const& eo, void* stream) const
+    {
+        int64_t nb = 0;
+        ddcx_layout const li = batched_layout(ei, k + 1, &nb);
+        ddcx_layout const lo = batched_layout(eo, k + 1);
+        b_[k]->build_vjp(Span<double const>{gin, li, nb}, Span<double>{gout, lo, nb}, std::nullopt, std::nullopt,
+                         stream);
+    }
     bool hermite() const
     {
         for (auto const* b : b_)

@@ -11,6 +11,7 @@
 #include <cstdlib>
 #include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "ddcx.hpp"
@@ -327,6 +328,35 @@ void case_3d_hermite()
     builder(coef.p, vals, NB, work.p, d);
     dump("3dh_breaks", br);
     dump("3dh_coef", coef.get());
+    // the adjoint: a random coefficient gradient, every input's gradient
+    std::vector<double> gs(static_cast<size_t>(NB * bs1.size() * bs2.size() * bs3.size()));
+    for (auto& x : gs) x = r.next() - 0.5;
+    dump("3dh_gs", gs);
+    DeviceArray dgs(gs);
+    std::vector<std::pair<std::string, std::unique_ptr<DeviceArray>>> grads;
+    auto grad = [&](std::string const& name, int64_t a, int64_t b, int64_t c) {
+        grads.emplace_back(name, std::make_unique<DeviceArray>(static_cast<size_t>(NB * a * b * c)));
+        return grads.back().second->p;
+    };
+    double* gvals = grad("vals", n1, n2, n3);
+    ddcx::SplineBuilder3DGrads g;
+    g.derivs_min1 = grad("derivs_min1", e1, n2, n3);
+    g.derivs_max1 = grad("derivs_max1", e1, n2, n3);
+    g.derivs_min2 = grad("derivs_min2", n1, e2, n3);
+    g.derivs_max2 = grad("derivs_max2", n1, e2, n3);
+    g.derivs_min3 = grad("derivs_min3", n1, n2, e3);
+    g.derivs_max3 = grad("derivs_max3", n1, n2, e3);
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) {
+            g.mixed12[a][b] = grad(std::string("mixed_derivs_") + sd[a] + "1_" + sd[b] + "2", e1, e2, n3);
+            g.mixed23[a][b] = grad(std::string("mixed_derivs_") + sd[a] + "2_" + sd[b] + "3", n1, e2, e3);
+            g.mixed13[a][b] = grad(std::string("mixed_derivs_") + sd[a] + "1_" + sd[b] + "3", e1, n2, e3);
+            for (int c = 0; c < 2; ++c)
+                g.mixed123[a][b][c] =
+                    grad(std::string("mixed_derivs_") + sd[a] + "1_" + sd[b] + "2_" + sd[c] + "3", e1, e2, e3);
+        }
+    builder.build_vjp(dgs.p, gvals, NB, work.p, g);
+    for (auto const& [name, arr] : grads) dump("3dh_g" + name, arr->get());
 }
 
 } // namespace

@@ -162,5 +162,23 @@ def test_facade_3d_hermite(outputs, cuda):
           D.SplineBuilder(D.UniformBSplines(4, 0.0, 1.0, 7), G, G)]
     coef = torch.full(got.shape, np.nan, dtype=torch.float64, device=cuda)
     t = {k: torch.from_numpy(v).to(cuda) for k, v in derivs.items()}
-    D.SplineBuilder3D(*bl)(coef, torch.from_numpy(x["vals"]).to(cuda), **t)
+    b3 = D.SplineBuilder3D(*bl)
+    b3(coef, torch.from_numpy(x["vals"]).to(cuda), **t)
     assert np.array_equal(coef.cpu().numpy(), got)
+    # build_vjp: every gradient equals the Python adjoint's bit for bit, and
+    # satisfies <gs, build(input)> = <input, gradient> block by block with the
+    # oracle's build
+    gs = outputs("3dh_gs").reshape(got.shape)
+    gv = torch.full([NB] + n, np.nan, dtype=torch.float64, device=cuda)
+    gd = {"g" + k: torch.full(v.shape, np.nan, dtype=torch.float64, device=cuda) for k, v in derivs.items()}
+    b3.build_vjp(torch.from_numpy(gs).to(cuda), gv, **gd)
+    blocks = {"vals": gv.cpu().numpy()}
+    blocks.update({k: gd["g" + k].cpu().numpy() for k in derivs})
+    for k, py in blocks.items():
+        cpp = outputs("3dh_g" + k).reshape(py.shape)
+        assert np.array_equal(cpp, py), k
+        v0 = x["vals"] if k == "vals" else np.zeros_like(x["vals"])
+        dk = {j: (x[j] if j == k else np.zeros_like(u)) for j, u in derivs.items()}
+        c = O.spline3d_build(*ss, v0, dk)
+        lhs, rhs = float((gs * c).sum()), float((x[k] * cpp).sum())
+        assert abs(lhs - rhs) <= 1e-12 * max(np.abs(gs * c).sum(), 1e-300), k
