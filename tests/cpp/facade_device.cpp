@@ -276,6 +276,39 @@ void case_2d_hermite()
     for (int k = 0; k < 9; ++k) dump(gn[k], g9[k]->get());
 }
 
+// 3D without derivative inputs: non-uniform cubic Greville (9 cells) x
+// uniform periodic quintic (8 cells) x uniform quartic Greville (7 cells), 2
+// batch entries; SplineBuilder3D through its two workspaces, and its adjoint
+void case_3d()
+{
+    int const NB = 2;
+    Lcg r{7};
+    std::vector<double> br(10);
+    br[0] = 0.0;
+    for (int i = 1; i <= 9; ++i) br[i] = br[i - 1] + 0.5 + r.next();
+    for (auto& b : br) b /= br[9];
+    auto const G = ddcx::SplineBuilderClosure::GREVILLE, P = ddcx::SplineBuilderClosure::PERIODIC;
+    ddcx::NonUniformBSplines bs1(3, br, false);
+    ddcx::UniformBSplines bs2(5, 0.0, 1.0, 8, true);
+    ddcx::UniformBSplines bs3(4, 0.0, 1.0, 7, false);
+    ddcx::SplineBuilder b1(bs1, G, G), b2(bs2, P, P), b3(bs3, G, G);
+    ddcx::SplineBuilder3D builder(b1, b2, b3, bs1.size(), bs2.size(), bs3.size());
+    size_t const nv = size_t(NB) * b1.npoints() * b2.npoints() * b3.npoints();
+    size_t const nc = size_t(NB) * bs1.size() * bs2.size() * bs3.size();
+    std::vector<double> vals(nv), gs(nc);
+    for (auto& x : vals) x = r.next() - 0.5;
+    for (auto& x : gs) x = r.next() - 0.5;
+    DeviceArray dv(vals), dc(nc), dgs(gs), dgv(nv);
+    DeviceArray w1(static_cast<size_t>(builder.work1_size(NB))), w2(static_cast<size_t>(builder.work2_size(NB)));
+    builder(dc.p, dv.p, NB, w1.p, w2.p);
+    builder.build_vjp(dgs.p, dgv.p, NB, w1.p, w2.p);
+    dump("3d_breaks", br);
+    dump("3d_vals", vals);
+    dump("3d_gs", gs);
+    dump("3d_coef", dc.get());
+    dump("3d_gvals", dgv.get());
+}
+
 // 3D with Hermite closures in every dimension: non-uniform cubic (9 cells) x
 // uniform quintic (8 cells) x uniform quartic (7 cells), 2 batch entries;
 // SplineBuilder3D with every derivative, edge and corner input
@@ -384,6 +417,9 @@ int main(int argc, char** argv)
         std::printf("facade_device: 2D Hermite\n");
         std::fflush(stdout);
         case_2d_hermite();
+        std::printf("facade_device: 3D\n");
+        std::fflush(stdout);
+        case_3d();
         std::printf("facade_device: 3D Hermite\n");
         std::fflush(stdout);
         case_3d_hermite();

@@ -182,3 +182,31 @@ def test_facade_3d_hermite(outputs, cuda):
         c = O.spline3d_build(*ss, v0, dk)
         lhs, rhs = float((gs * c).sum()), float((x[k] * cpp).sum())
         assert abs(lhs - rhs) <= 1e-12 * max(np.abs(gs * c).sum(), 1e-300), k
+
+
+def test_facade_3d(outputs, cuda):
+    """SplineBuilder3D without derivative inputs through the facade (its two
+    workspaces) and its adjoint: coefficients against the oracle (1e-13), and
+    both coefficients and value gradients bit for bit equal to the Python
+    builder's and its adjoint's on the same inputs."""
+    import torch
+    import ddc_amd as D
+
+    NB = 2
+    br = outputs("3d_breaks")
+    ss = [O.Spline(3, False, O.GREVILLE, O.GREVILLE, breaks=br),
+          O.Spline(5, True, O.PERIODIC, O.PERIODIC, ncells=8, rmin=0.0, rmax=1.0),
+          O.Spline(4, True, O.GREVILLE, O.GREVILLE, ncells=7, rmin=0.0, rmax=1.0)]
+    vals = outputs("3d_vals").reshape([NB] + [s.npts for s in ss])
+    got = outputs("3d_coef").reshape([NB] + [s.size for s in ss])
+    assert rel_err(got, O.splinend_build(ss, vals)) <= 1e-13
+    C = D.SplineBuilderClosure
+    b3 = D.SplineBuilder3D(D.SplineBuilder(D.NonUniformBSplines(3, br), C.GREVILLE, C.GREVILLE),
+                           D.SplineBuilder(D.UniformBSplines(5, 0.0, 1.0, 8, periodic=True)),
+                           D.SplineBuilder(D.UniformBSplines(4, 0.0, 1.0, 7), C.GREVILLE, C.GREVILLE))
+    coef = torch.full(got.shape, np.nan, dtype=torch.float64, device=cuda)
+    b3(coef, torch.from_numpy(vals).to(cuda))
+    assert np.array_equal(coef.cpu().numpy(), got)
+    gv = torch.full(vals.shape, np.nan, dtype=torch.float64, device=cuda)
+    b3.build_vjp(torch.from_numpy(outputs("3d_gs").reshape(got.shape)).to(cuda), gv)
+    assert np.array_equal(gv.cpu().numpy(), outputs("3d_gvals").reshape(vals.shape))
