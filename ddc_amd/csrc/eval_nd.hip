@@ -31,29 +31,27 @@ struct NdArgs {
 
 constexpr int kM = 6; // basis values per dimension (degree <= 5)
 
-// y += coef(jmin + idx) * (v_1[idx_1] * (v_2[idx_2] * (...))) over the
-// multi-index in row-major order, every index a compile-time constant (so
-// the basis tables stay in registers) behind a wave-uniform bound check.
-template <int N, int... I>
-__device__ __forceinline__ void tensor_acc(double const (&v)[N][kM], int const (&n)[N], int64_t const (&st)[N],
-                                           const double* c, double& y)
+// sum over the multi-index of coef(jmin + idx) * v_1[idx_1] * ... * v_N[idx_N],
+// contracted one dimension at a time from the last: for each index of the
+// leading dimensions, s = c_0 v_0, then s = fma(c_i, v_i, s) for i = 1..n-1,
+// and the same over the next dimension up with those sums as coefficients.
+// A tricubic point takes 84 multiply-adds in chains of 4 instead of 208
+// operations behind one 64-term addition chain. Every index is a
+// compile-time constant (the basis tables stay in registers); the bounds
+// n[K] are wave-uniform.
+template <int N, int K = 0>
+__device__ __forceinline__ double tensor_contract(double const (&v)[N][kM], int const (&n)[N], int64_t const (&st)[N],
+                                                  const double* c)
 {
-    constexpr int K = sizeof...(I);
-    if constexpr (K == N) {
-        constexpr int ix[N] = {I...};
-        double p = v[N - 1][ix[N - 1]];
-        int64_t off = ix[N - 1] * st[N - 1];
+    auto term = [&](int i) {
+        if constexpr (K == N - 1) return c[i * st[K]];
+        else return tensor_contract<N, K + 1>(v, n, st, c + i * st[K]);
+    };
+    double s = term(0) * v[K][0];
 #pragma unroll
-        for (int k = N - 2; k >= 0; --k) {
-            p = v[k][ix[k]] * p;
-            off += ix[k] * st[k];
-        }
-        y += c[off] * p;
-    } else {
-        [&]<int... J>(std::integer_sequence<int, J...>) {
-            ((J < n[K] ? tensor_acc<N, I..., J>(v, n, st, c, y) : void()), ...);
-        }(std::make_integer_sequence<int, kM>{});
-    }
+    for (int i = 1; i < kM; ++i)
+        if (i < n[K]) s = fma(term(i), v[K][i], s);
+    return s;
 }
 
 template <int D>
@@ -135,35 +133,25 @@ __global__ void __launch_bounds__(256) evalnd_kernel(NdArgs a, double* __restric
         base += int64_t(j) * st[k];
         n[k] = a.b[k].degree + 1;
     }
-    double y = 0.0;
-    tensor_acc<N>(v, n, st, coef + l * cbs + base, y);
-    out[t] = y;
+    out[t] = tensor_contract<N>(v, n, st, coef + l * cbs + base);
 }
 
 // Same degree D in every dimension: the basis routines inline with the
-// degree fixed, the tables stay in registers and the patch loop has exactly
-// (D+1)^N terms, in the same row-major order and right-folded products as
-// tensor_acc, so the values are bit-identical to evalnd_kernel's.
-template <int N, int D, int... I>
-__device__ __forceinline__ void tensor_acc_fixed(double const (&v)[N][kM], int64_t const (&st)[N], const double* c,
-                                                 double& y)
+// degree fixed and the contraction has exactly D+1 terms per dimension, in
+// the order and with the operations of tensor_contract, so the values are
+// bit-identical to evalnd_kernel's.
+template <int N, int D, int K = 0>
+__device__ __forceinline__ double tensor_contract_fixed(double const (&v)[N][kM], int64_t const (&st)[N],
+                                                        const double* c)
 {
-    constexpr int K = sizeof...(I);
-    if constexpr (K == N) {
-        constexpr int ix[N] = {I...};
-        double p = v[N - 1][ix[N - 1]];
-        int64_t off = ix[N - 1] * st[N - 1];
+    auto term = [&](int i) {
+        if constexpr (K == N - 1) return c[i * st[K]];
+        else return tensor_contract_fixed<N, D, K + 1>(v, st, c + i * st[K]);
+    };
+    double s = term(0) * v[K][0];
 #pragma unroll
-        for (int k = N - 2; k >= 0; --k) {
-            p = v[k][ix[k]] * p;
-            off += ix[k] * st[k];
-        }
-        y += c[off] * p;
-    } else {
-        [&]<int... J>(std::integer_sequence<int, J...>) {
-            (tensor_acc_fixed<N, D, I..., J>(v, st, c, y), ...);
-        }(std::make_integer_sequence<int, D + 1>{});
-    }
+    for (int i = 1; i <= D; ++i) s = fma(term(i), v[K][i], s);
+    return s;
 }
 
 template <int D, bool ANY>
@@ -233,9 +221,7 @@ __global__ void __launch_bounds__(256) evalnd_fixed_kernel(NdArgs a, double* __r
         st[k] = a.st[k];
         base += int64_t(j) * st[k];
     }
-    double y = 0.0;
-    tensor_acc_fixed<N, D>(v, st, coef + l * cbs + base, y);
-    out[t] = y;
+    out[t] = tensor_contract_fixed<N, D>(v, st, coef + l * cbs + base);
 }
 
 constexpr int kIntThreads = 1024;
@@ -534,9 +520,7 @@ __global__ void __launch_bounds__(kNdTileThreads) ndbin_tile_kernel(NdArgs a, Bi
                 int const j = basis_fixed<D, ANY>(a.b[d], x[d], ANY ? a.order[d] : 0, v[d]);
                 base += int64_t(j - a0[d]) * pst[d];
             }
-            double y = 0.0;
-            tensor_acc_fixed<N, D>(v, pst, cp + base, y);
-            out[l * npts + cpn[u]] = y;
+            out[l * npts + cpn[u]] = tensor_contract_fixed<N, D>(v, pst, cp + base);
         }
     }
 }
