@@ -442,7 +442,7 @@ __global__ void __launch_bounds__(kNdBinT) ndbin_count_kernel(NdArgs a, Bin3D g,
 }
 
 template <int D, bool ANY>
-__global__ void __launch_bounds__(kNdTileThreads) ndbin_tile_kernel(NdArgs a, Bin3D g,
+__global__ void __launch_bounds__(kNdTileThreads, D <= 3 ? 4 : 1) ndbin_tile_kernel(NdArgs a, Bin3D g,
                                                                     const double* __restrict__ recs,
                                                                     double* __restrict__ out, int64_t npts,
                                                                     const double* __restrict__ coef, int64_t cbs,
@@ -480,8 +480,10 @@ __global__ void __launch_bounds__(kNdTileThreads) ndbin_tile_kernel(NdArgs a, Bi
     int64_t const pst[N] = {int64_t(P) * P, P, 1};
     int const k0 = starts[l * (g.nbins + 1) + bin], k1 = starts[l * (g.nbins + 1) + bin + 1];
     // U records per thread and step, the next step's in flight while the
-    // current ones are evaluated
-    constexpr int U = 4;
+    // current ones are evaluated; 2 for derivatives, whose tables take more
+    // registers: at most 128 VGPRs (the launch bound for D <= 3) keep two
+    // workgroups per CU, as the LDS allows
+    constexpr int U = ANY ? 2 : 4;
     double nx[U][N];
     int np[U];
     auto fetch = [&](int kb) {
