@@ -221,14 +221,65 @@ __host__ __device__ inline void uniform_basis_from_offset(double offset, double*
 
 // de Boor with left/right arrays, NonUniformBSplines eval_basis,
 // bsplines_non_uniform.hpp:461-497; `icell` is the knot index from
-// nonuniform_cell.  FAST (device evaluation): the divisor is inverted by the
-// refined hardware reciprocal and the update is one fma.
+// nonuniform_cell.  FAST (device evaluation): the update is one fma, and the
+// D(D+1)/2 divisors right[r] + left[j - r], which do not depend on the
+// recurrence's values, are inverted together: one refined hardware
+// reciprocal of their product, then each inverse as that reciprocal times
+// the product of the others (prefix products forward, the running inverse
+// backward), 3 multiplications per divisor instead of a reciprocal and four
+// fmas each. Every divisor is at least the cell's width, so the product is
+// positive; outside [2^-960, 2^960] each divisor takes its own reciprocal.
 template <int D, bool FAST = false>
 __host__ __device__ inline void nonuniform_basis(const double* knots, int icell, double x, double* values)
 {
     double left[D > 0 ? D : 1];
     double right[D > 0 ? D : 1];
     values[0] = 1.0;
+    if constexpr (FAST && D > 0) {
+        constexpr int M = D * (D + 1) / 2;
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            left[j] = x - knots[icell - j];
+            right[j] = knots[icell + j + 1] - x;
+        }
+        double den[M], pre[M], inv[M];
+        {
+            int k = 0;
+#pragma unroll
+            for (int j = 0; j < D; ++j)
+#pragma unroll
+                for (int r = 0; r < j + 1; ++r) den[k++] = right[r] + left[j - r];
+        }
+        pre[0] = den[0];
+#pragma unroll
+        for (int k = 1; k < M; ++k) pre[k] = pre[k - 1] * den[k];
+        double const mag = fabs(pre[M - 1]);
+        if (mag >= 0x1p-960 && mag <= 0x1p960) {
+            double rv = fast_rcp(pre[M - 1]);
+#pragma unroll
+            for (int k = M - 1; k > 0; --k) {
+                inv[k] = rv * pre[k - 1];
+                rv *= den[k];
+            }
+            inv[0] = rv;
+        } else {
+#pragma unroll
+            for (int k = 0; k < M; ++k) inv[k] = fast_rcp(den[k]);
+        }
+        int k = 0;
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            double saved = 0.0;
+#pragma unroll
+            for (int r = 0; r < j + 1; ++r) {
+                double const temp = values[r] * inv[k++];
+                values[r] = fma(right[r], temp, saved);
+                saved = left[j - r] * temp;
+            }
+            values[j + 1] = saved;
+        }
+        return;
+    }
 #pragma unroll
     for (int j = 0; j < D; ++j) {
         left[j] = x - knots[icell - j];
@@ -236,14 +287,8 @@ __host__ __device__ inline void nonuniform_basis(const double* knots, int icell,
         double saved = 0.0;
 #pragma unroll
         for (int r = 0; r < j + 1; ++r) {
-            double temp;
-            if constexpr (FAST) {
-                temp = values[r] * fast_rcp(right[r] + left[j - r]);
-                values[r] = fma(right[r], temp, saved);
-            } else {
-                temp = values[r] / (right[r] + left[j - r]);
-                values[r] = saved + right[r] * temp;
-            }
+            double const temp = values[r] / (right[r] + left[j - r]);
+            values[r] = saved + right[r] * temp;
             saved = left[j - r] * temp;
         }
         values[j + 1] = saved;
