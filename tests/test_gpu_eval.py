@@ -304,3 +304,44 @@ def test_eval_at_domain_points(case, layout, nbatch, cuda):
         got = out.cpu().numpy() if bc else out.cpu().numpy().T
         ref = orc.eval(full, coef, deriv=order, extrap=("constant", "null"))
         assert np.abs(got - ref).max() / max(1.0, np.abs(ref).max()) <= TOL * (10 ** order)
+
+
+@pytest.mark.parametrize("degree", [3, 5])
+@pytest.mark.parametrize("scale", [1e-100, 1e-40, 1e100])
+@pytest.mark.parametrize("layout", ["batch_contiguous", "interp_contiguous"])
+def test_nonuniform_values_at_extreme_scales(degree, scale, layout, cuda):
+    """A non-uniform basis inverts a point's d(d+1)/2 knot-difference divisors
+    together, from one reciprocal of their product; a product outside
+    [2^-960, 2^960] takes one reciprocal per divisor. Break points scaled to
+    [0, 1e-100], [0, 1e-40] and [0, 1e100] send the cubic and quintic
+    products below and above that range (1e-40 keeps the cubic one inside):
+    both branches against the oracle's divisions."""
+    import torch
+    import ddc_amd as D
+    from cases import breaks_for
+
+    import oracle as O
+
+    br = breaks_for("random", 200, seed=12) * scale
+    orc = O.Spline(degree, False, G, G, breaks=br)
+    bs = D.NonUniformBSplines(degree, br)
+    rng = np.random.default_rng(13)
+    nbatch, npts = 37, 400
+    coef = rng.standard_normal((orc.size, nbatch))
+    x = rng.uniform(orc.rmin, orc.rmax, (npts, nbatch))
+    x[:201, 0] = br  # the break points themselves
+    ref = orc.eval(x, coef)
+    ev = D.SplineEvaluator(bs, D.NullExtrapolationRule(), D.NullExtrapolationRule())
+    if layout == "batch_contiguous":
+        tx, tc = torch.from_numpy(x).to(cuda), torch.from_numpy(coef).to(cuda)
+        out = torch.empty_like(tx)
+        ev(out, tx, tc, axis=0)
+        got = out.cpu().numpy()
+    else:
+        tx = torch.from_numpy(np.ascontiguousarray(x.T)).to(cuda)
+        tc = torch.from_numpy(np.ascontiguousarray(coef.T)).to(cuda)
+        out = torch.empty_like(tx)
+        ev(out, tx, tc, axis=1)
+        got = out.cpu().numpy().T
+    assert np.all(np.isfinite(got))
+    assert rel_err(got, ref) <= TOL
